@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain physics path on one MI355X.
+
+Workload: 32,768 bodies (half boxes with half-extents U[0.2,0.6] m, half
+spheres with radius U[0.2,0.5] m, 16 segments) dropped from 1-60 m into a
+30x30 m container (ground + 4 static walls), density 1000, friction 0.5,
+restitution 0.1, gravity -9.81, dt 1/60 s, 10 substeps, 30 PGS iterations,
+FP32.  A step is one Physics.step call: one frame of 10 substeps plus the
+render-vertex update.  The metric is body-substeps/s over the timed frames.
+
+Prints one JSON line (see the keys below).  Usage:
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-sample-substeps S]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+SUBSTEPS = 10
+DT = 1.0 / 60.0
+GRAVITY = (0.0, -9.81, 0.0)
+PGS_ITERS = 30
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(count: int, seed: int):
+    from seevr_amd import Scene, default_caps
+    n = count + 5
+    caps = default_caps(max_bodies=n, max_hull_verts=n * 289, max_vertices=n * 289, max_triangles=n * 512,
+                        max_support_entries=n * 192 + 1)
+    sc = Scene(caps)
+    sc.build_workload(2, count, 16, seed)
+    return sc
+
+
+def algorithmic_bytes(stage: str, n: int, n_hull: int, n_dyn: int) -> float:
+    """Bytes each stage must move at minimum (per launch)."""
+    if stage == "phys/aabb":
+        return 36.0 * n + 12.0 * n_hull + 24.0 * n
+    if stage == "phys/gravity":
+        return 52.0 * n_dyn
+    if stage == "phys/integrate":
+        return 84.0 * n_dyn
+    return 0.0
+
+
+def cpu_baseline(ctx, scene, n, substeps: int) -> dict:
+    """Time the scalar CPU oracle on `substeps` substeps from the device state."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from harness import World
+    w = World.from_scene(scene, max_pairs=ctx.caps.max_pairs, max_contacts=ctx.caps.max_contacts,
+                         pgs_iterations=PGS_ITERS)
+    st = w.download_state(ctx)
+    for k, v in st.items():
+        w.a[k][:] = v
+    t0 = time.perf_counter()
+    w.oracle_step(substeps, DT * substeps / SUBSTEPS, GRAVITY)
+    el = time.perf_counter() - t0
+    return {"value": n * substeps / el, "unit": "body-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"{substeps} substeps of the same scene from the state at the start of the timed "
+                      f"frames (N={n}), scalar C oracle, 1 core, {el:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=240)
+    ap.add_argument("--bodies", type=int, default=32768)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-sample-substeps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the scene is stepped on a single device); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from seevr_amd import Context, default_params
+    t0 = time.perf_counter()
+    scene = build_workload(args.bodies, args.seed)
+    counts = scene.counts()
+    n, nv, nh = counts["num_bodies"], counts["num_vertices"], counts["num_hull_verts"]
+    n_dyn = args.bodies
+    log(f"scene: {n} bodies, {nh} hull verts, built in {time.perf_counter() - t0:.1f} s")
+    caps = scene.caps
+    caps.max_pairs = 8 * n + 1024
+    caps.max_contacts = 6 * n + 1024
+    ctx = Context(caps, default_params(pgs_iterations=PGS_ITERS))
+    scene.upload(ctx)
+    name, arch = ctx.device_info()
+    log(f"device: {name} {arch}")
+
+    for i in range(args.warmup):
+        ctx.step(n, nv, nh, SUBSTEPS, DT, GRAVITY)
+        if i % 60 == 59:
+            log(f"warmup frame {i + 1}/{args.warmup}: counters {ctx.read_counters()}")
+    ctx.enable_stage_timing(True)
+    ctx.stage_times(reset=True)
+    ctx.step(n, nv, nh, SUBSTEPS, DT, GRAVITY)  # one timed-stage frame for the breakdown
+    stages = ctx.stage_times(reset=True)
+    ctx.enable_stage_timing(False)
+    log("stage breakdown (us/launch, launches): " +
+        ", ".join(f"{k}={v[0]:.1f}x{v[1]}" for k, v in sorted(stages.items(), key=lambda kv: -kv[1][0] * kv[1][1])))
+
+    # timed region: K frames, device synchronized on both sides (each call is synchronous)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ctx.step(n, nv, nh, SUBSTEPS, DT, GRAVITY)
+    el = time.perf_counter() - t0
+    pairs, contacts = ctx.read_counters()
+    ms = 1000.0 * el / args.steps
+    value = n * SUBSTEPS * args.steps / el
+    log(f"timed {args.steps} frames in {el:.2f} s: {ms:.2f} ms/frame, pairs {pairs}, contacts {contacts}")
+
+    # roofline of the dominant HBM-bound stage, measured with HIP events on the library stream
+    ctx.enable_stage_timing(True)
+    ctx.stage_times(reset=True)
+    for i in range(3):
+        ctx.step(n, nv, nh, SUBSTEPS, DT, GRAVITY)
+    stages = ctx.stage_times(reset=True)
+    ctx.enable_stage_timing(False)
+    dom = "phys/aabb"
+    us = stages[dom][0]
+    alg = algorithmic_bytes(dom, n, nh, n_dyn)
+    achieved = alg / (us * 1e-6) / 1e9
+    roof = {"bound": "hbm", "kernel": "k_aabb (phys/aabb)", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_us": round(us, 2), "alg_bytes": int(alg)}
+    total_us = sum(v[0] * v[1] for v in stages.values())
+    breakdown = {k: round(100.0 * v[0] * v[1] / total_us, 1) for k, v in stages.items()}
+
+    cpu = None
+    if not args.no_cpu:
+        log("timing the CPU oracle baseline ...")
+        cpu = cpu_baseline(ctx, scene, n, args.cpu_sample_substeps)
+        log(f"cpu baseline: {cpu['value']:.0f} body-substeps/s")
+
+    out = {
+        "metric": "body-substeps/s @32k bodies", "value": round(value, 1), "unit": "body-substeps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded scene generator)",
+        "config": {"workload": "32,768 bodies (+5 static) in a 30x30 m walled container, 10 substeps, "
+                               "30 PGS iterations, dt 1/60",
+                   "bodies": n, "hull_verts": nh, "substeps": SUBSTEPS, "pgs_iterations": PGS_ITERS,
+                   "frames_before_timing": args.warmup, "pairs_last_substep": pairs,
+                   "contacts_last_substep": contacts},
+        "roofline": roof, "cpu_baseline": cpu, "stage_share_pct": breakdown,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,267 @@
+/*
+ * seevr.h - C ABI of libseevr_hip.so, a gfx950 (MI355X) implementation of
+ * seeVR's per-substep rigid-body physics (physics.comp) and LBVH build
+ * (bvh.comp).
+ *
+ * The library owns the device buffers that seeVR's `Data` struct owns
+ * (src/data.zig:36-121) and keeps their tightly packed std430 layouts:
+ * float arrays with vec3 stride 3 and quaternion stride 4 (x, y, z, w),
+ * uint arrays for indices.  A host replaces its Vulkan dispatches with the
+ * calls below and its staging copies with sv_upload / sv_download.
+ *
+ * Conventions (every entry point):
+ *   - return 0 on success, a negative SV_E_* code on failure; the message of
+ *     the last failure on the calling thread is sv_last_error();
+ *   - calls are synchronous (they return after the device work finished),
+ *     like the reference's vkQueueWaitIdle (src/physics.zig:200);
+ *   - one context is not thread-safe, like one Vulkan queue;
+ *   - list overflow keeps the first `cap` entries in canonical order and the
+ *     counters keep the raw totals, as the reference's counters do
+ *     (src/shaders/physics.comp:404-411,465-469).
+ *
+ * Canonical order.  The reference appends pairs and contacts with atomicAdd,
+ * so its list order is whatever order threads won the atomic.  This library
+ * fixes the one schedule in which invocations run in index order: pairs in
+ * lexicographic (i, j) order, contacts in pair order, and the Gauss-Seidel
+ * sweep in contact order.
+ */
+#ifndef SEEVR_H
+#define SEEVR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SV_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+    SV_OK = 0,
+    SV_E_INVALID = -1,   /* bad argument (null pointer, unknown id, ...)     */
+    SV_E_CAPACITY = -2,  /* a count exceeds the capacity fixed at creation   */
+    SV_E_DEVICE = -3,    /* a HIP runtime call failed                        */
+    SV_E_NOMEM = -4,     /* device or host allocation failed                 */
+    SV_E_TIMEOUT = -5,   /* the ordered solver's bounded wait expired        */
+    SV_E_RANGE = -6      /* offset/size outside a buffer                     */
+};
+
+/* Bits of counters[2] (the reference leaves counters[2..3] unused,
+ * src/data.zig:101). */
+enum {
+    SV_STATUS_PAIR_OVERFLOW = 1u << 0,    /* pairs found > max_pairs        */
+    SV_STATUS_CONTACT_OVERFLOW = 1u << 1, /* contacts found > max_contacts  */
+    SV_STATUS_SOLVER_TIMEOUT = 1u << 2    /* ordered PGS wait expired       */
+};
+
+/* ---- buffer ids --------------------------------------------------------
+ * 0..29 are the physics binding table, in the order of
+ * src/physics.zig:37-68 (== src/shaders/physics.comp:16-57).
+ * 30..45 are the BVH bindings of src/bvh.zig:32-51 (binding 0 `vertices`
+ * is SV_VERTICES; bindings 16/17 are the float/uint views of one temp
+ * buffer, SV_BVH_TEMP).  46+ are extensions of this library.            */
+typedef enum sv_buf {
+    SV_BODY_POS = 0,           /* float[3N]                                  */
+    SV_BODY_QUAT = 1,          /* float[4N] (x,y,z,w)                        */
+    SV_BODY_VEL = 2,           /* float[3N]                                  */
+    SV_BODY_OMEGA = 3,         /* float[3N] world frame                      */
+    SV_BODY_INV_MASS = 4,      /* float[N], 0 = static                       */
+    SV_BODY_INV_INERTIA = 5,   /* float[3N] body-frame diagonal              */
+    SV_BODY_VERT_START = 6,    /* uint[N] first render vertex of the body    */
+    SV_BODY_VERT_COUNT = 7,    /* uint[N]                                    */
+    SV_GEOM_BODY_IDX = 8,      /* uint[N] (geoms are 1:1 with bodies)        */
+    SV_GEOM_DATA = 9,          /* float[8N]: [0]=hull start, [1]=hull count  */
+    SV_GEOM_FRICTION = 10,     /* float[N]                                   */
+    SV_GEOM_RESTITUTION = 11,  /* float[N]                                   */
+    SV_HULL_VERTS = 12,        /* float[3H] local space                      */
+    SV_VERTICES = 13,          /* float[3V] world-space render vertices      */
+    SV_ORIGINAL_VERTICES = 14, /* float[3V] local-space render vertices      */
+    SV_CONTACT_POS = 15,       /* float[3C]                                  */
+    SV_CONTACT_NORMAL = 16,    /* float[3C], points from body A toward B     */
+    SV_CONTACT_PENETRATION = 17, /* float[C]                                 */
+    SV_CONTACT_GEOM_A = 18,    /* uint[C]                                    */
+    SV_CONTACT_GEOM_B = 19,    /* uint[C]                                    */
+    SV_CONTACT_LAMBDA_N = 20,  /* float[C] accumulated normal impulse        */
+    SV_ATOMIC_COUNTERS = 21,   /* uint[4]: [0]=pairs found, [1]=contacts found,
+                                  [2]=SV_STATUS_* bits                       */
+    SV_BODY_AABB_MIN = 22,     /* float[3N]                                  */
+    SV_BODY_AABB_MAX = 23,     /* float[3N]                                  */
+    SV_COLLISION_PAIRS = 24,   /* uint[2P], (i, j) with i < j                */
+    SV_AABB_TEMP_MIN = 25,     /* unused here (size 0): the per-body AABB is */
+    SV_AABB_TEMP_MAX = 26,     /*   reduced without the reference's cascade  */
+    SV_AABB_TEMP_BODY = 27,    /*   partials                                 */
+    SV_SUPPORT_LOOKUP_OFFSET = 28, /* uint[N], 0 = brute-force support       */
+    SV_SUPPORT_LOOKUP = 29,    /* uint[1 + 192 * lookups]                    */
+
+    SV_INDICES = 30,           /* uint[3T] triangle vertex indices           */
+    SV_BVH_CENTROIDS = 31,     /* float[3n]                                  */
+    SV_BVH_MORTON = 32,        /* uint[n] sorted Morton codes after a build  */
+    SV_BVH_MORTON_TEMP = 33,   /* uint[n] sort scratch                       */
+    SV_BVH_SORT_INDICES = 34,  /* uint[n] primitive of each sorted position  */
+    SV_BVH_SORT_TEMP = 35,     /* uint[n] sort scratch                       */
+    SV_BVH_SCENE_BOUNDS = 36,  /* float[6] centroid bounds min xyz, max xyz  */
+    SV_BVH_AABB_MIN = 37,      /* float[3(2n-1)] node boxes                  */
+    SV_BVH_AABB_MAX = 38,      /* float[3(2n-1)]                             */
+    SV_BVH_LEFT = 39,          /* uint[2n-1] left child / leaf first prim    */
+    SV_BVH_RIGHT = 40,         /* uint[2n-1] right child (internal nodes)    */
+    SV_BVH_COUNT = 41,         /* uint[2n-1] 0 = internal, 1 = leaf          */
+    SV_BVH_PARENT = 42,        /* uint[2n-1], root = 0xFFFFFFFF              */
+    SV_BVH_PRIM_INDICES = 43,  /* uint[n]                                    */
+    SV_BVH_FLAGS = 44,         /* uint[n-1] refit arrival flags              */
+    SV_BVH_TEMP = 45,          /* uint/float scratch (histograms, partials)  */
+
+    SV_GEOM_HULL_RANGE = 46,   /* extension: uint[2N] (start, count) of each
+                                  geom's hull, exact beyond 2^24 hull verts.
+                                  Read instead of geom_data[0..1] when
+                                  sv_params.hull_range_u32 is set.           */
+    SV_BUF_COUNT = 47
+} sv_buf;
+
+/* ---- capacities and parameters ----------------------------------------
+ * The reference fixes these at compile time (src/data.zig:11-29,
+ * src/shaders/physics.comp:66-70); here they are chosen at creation.     */
+typedef struct sv_caps {
+    uint32_t max_bodies;      /* MAX_BODIES == MAX_GEOMS (data.zig:13-14)    */
+    uint32_t max_hull_verts;  /* MAX_HULL_VERTS (data.zig:17)                */
+    uint32_t max_vertices;    /* MAX_VERTICES (data.zig:11)                  */
+    uint32_t max_triangles;   /* MAX_TRIANGLES (data.zig:12)                 */
+    uint32_t max_pairs;       /* MAX_COLLISION_PAIRS (data.zig:16)           */
+    uint32_t max_contacts;    /* MAX_CONTACTS (data.zig:15)                  */
+    uint32_t max_support_entries; /* MAX_GEOMS * 192 (data.zig:208)          */
+} sv_caps;
+
+typedef struct sv_params {
+    uint32_t pgs_iterations;  /* PGS_ITERATIONS, 4 in physics.comp:68        */
+    float baumgarte;          /* BAUMGARTE 0.2 (physics.comp:66)             */
+    float slop;               /* SLOP 0.005 (physics.comp:67)                */
+    float linear_damping;     /* 0.999 per substep (physics.comp:422)        */
+    float angular_damping;    /* 0.998 per substep (physics.comp:425)        */
+    uint32_t hull_range_u32;  /* 1: read SV_GEOM_HULL_RANGE, 0: geom_data    */
+} sv_params;
+
+/* Defaults equal the reference constants, with the capacities of
+ * src/data.zig:11-18. */
+void sv_default_caps(sv_caps *caps);
+void sv_default_params(sv_params *params);
+
+typedef struct sv_ctx sv_ctx;
+
+/* Physics.init + BVH.init + Data.init buffer creation
+ * (src/physics.zig:34-78, src/bvh.zig:29-61, src/data.zig:179-250).
+ * Uses the current HIP device. */
+int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out);
+/* Physics.deinit / BVH.deinit / Data.deinit (physics.zig:219, bvh.zig:208,
+ * data.zig:667). Null is accepted. */
+int sv_destroy(sv_ctx *ctx);
+
+int sv_get_params(sv_ctx *ctx, sv_params *out);
+int sv_set_params(sv_ctx *ctx, const sv_params *params);
+
+/* Device address and size of a buffer (descriptor binding, vulkan.zig:465). */
+int sv_buffer_ptr(sv_ctx *ctx, sv_buf id, void **dev, size_t *bytes);
+/* Staging upload / readback (vulkan.zig:304-328). Offsets in bytes. */
+int sv_upload(sv_ctx *ctx, sv_buf id, const void *src, size_t offset, size_t nbytes);
+int sv_download(sv_ctx *ctx, sv_buf id, void *dst, size_t offset, size_t nbytes);
+
+/* Physics.step (src/physics.zig:80-165): `substeps` substeps of
+ * clear counters, gravity, AABB, broadphase, narrowphase, clear impulses,
+ * PGS, integrate, then render vertices once.  No-op when num_bodies == 0
+ * or substeps == 0.  sub_dt = dt / substeps. */
+int sv_physics_step(sv_ctx *ctx, uint32_t num_bodies, uint32_t num_vertices,
+                    uint32_t num_hull_verts, uint32_t substeps, float dt,
+                    const float gravity[3]);
+
+/* Physics.readCounters (src/physics.zig:213-217): raw pair and contact
+ * totals of the last substep. */
+int sv_read_counters(sv_ctx *ctx, uint32_t out[2]);
+
+/* One physics stage, addressed like the reference's push constants
+ * {step, count, dt, gravity, num_bodies} (src/physics.zig:7-15,
+ * src/shaders/physics.comp:846-861).  Steps:
+ *   0 gravity (count = bodies)            1, 9 per-body AABB (count = bodies)
+ *   2 broadphase (count = bodies)         3 narrowphase over counters[0] pairs
+ *   4 clear impulses                      5 PGS over counters[1] contacts
+ *   6 integrate (count = bodies)          7 render vertices (count = verts)
+ *   8 clear counters                      10, 11 accepted, no work (the
+ *                                            AABB needs no cascade here)
+ * Used to test each stage from identical inputs. */
+int sv_physics_dispatch(sv_ctx *ctx, uint32_t step, uint32_t count, float dt,
+                        const float gravity[3], uint32_t num_bodies);
+
+/* BVH.build (src/bvh.zig:63-172) over `num_tris` triangles of
+ * SV_VERTICES / SV_INDICES.  No-op when num_tris == 0. */
+int sv_bvh_build(sv_ctx *ctx, uint32_t num_tris);
+/* The same LBVH over body AABBs (primitive i = body i, centroid =
+ * (min + max) / 2, leaf box = body AABB); reads SV_BODY_AABB_MIN/MAX. */
+int sv_bvh_build_bodies(sv_ctx *ctx, uint32_t num_bodies);
+/* One BVH stage, addressed like bvh.comp's push constants {step, num_tris}
+ * (src/shaders/bvh.comp:401-416): 0 centroids, 1 scene bounds, 2 Morton,
+ * 3 whole radix sort, 4 hierarchy, 5 leaf boxes, 6 clear flags,
+ * 7 bottom-up refit; 8-11 accepted, no work. */
+int sv_bvh_dispatch(sv_ctx *ctx, uint32_t step, uint32_t num_tris);
+
+/* Evaluate the library's device atan2f on n host values (a numerics
+ * check of the function the support lookup bins with). */
+int sv_debug_atan2(sv_ctx *ctx, const float *y, const float *x, float *out, uint32_t n);
+
+/* Thread-local message of the last failure ("" if none). */
+const char *sv_last_error(void);
+/* Device name and gfx arch of the context's device. */
+int sv_device_info(sv_ctx *ctx, char *name, size_t name_len, char *arch, size_t arch_len);
+/* Average duration in microseconds of each stage over the calls since the
+ * last reset, measured with HIP events on the library stream.  `names`
+ * receives up to `cap` entries; returns the number of stages. */
+int sv_stage_times(sv_ctx *ctx, int reset, const char **names, double *avg_us,
+                   uint64_t *calls, int cap);
+int sv_enable_stage_timing(sv_ctx *ctx, int enable);
+
+/* ---- host scene construction (src/data.zig) ------------------------- */
+typedef struct sv_scene sv_scene;
+
+typedef struct sv_scene_counts {
+    uint32_t num_vertices, num_triangles, num_bodies, num_geoms;
+    uint32_t num_hull_verts, num_materials, num_support_entries;
+} sv_scene_counts;
+
+int sv_scene_create(const sv_caps *caps, sv_scene **out);
+int sv_scene_destroy(sv_scene *scene);
+/* Data.addMaterial (data.zig:356-370); returns the material index. */
+int sv_scene_add_material(sv_scene *scene, const float albedo[3], float roughness,
+                          float metallic, const float emission[3], float ior,
+                          float density, uint32_t *material_out);
+/* Data.addBox (data.zig:392-485). */
+int sv_scene_add_box(sv_scene *scene, const float center[3], const float half[3],
+                     const float color[3], float friction, float restitution,
+                     uint32_t material, uint32_t *body_out);
+/* Data.addSphere (data.zig:487-597). */
+int sv_scene_add_sphere(sv_scene *scene, const float center[3], float radius,
+                        const float color[3], uint32_t segments, float friction,
+                        float restitution, uint32_t material, uint32_t *body_out);
+/* Overwrite a body's orientation and velocities (seeded scenes start with
+ * random orientations; the reference's addBox/addSphere set identity). */
+int sv_scene_set_body_state(sv_scene *scene, uint32_t body, const float quat[4],
+                            const float vel[3], const float omega[3]);
+int sv_scene_get_counts(const sv_scene *scene, sv_scene_counts *out);
+/* Host staging array of a buffer id (the s_* slices of data.zig:123-170),
+ * valid until the next add_* call. */
+int sv_scene_array(const sv_scene *scene, sv_buf id, const void **host, size_t *bytes);
+/* Data.upload (data.zig:599-655), including the derived body_vert_start /
+ * body_vert_count and the exact SV_GEOM_HULL_RANGE extension. */
+int sv_scene_upload(sv_scene *scene, sv_ctx *ctx);
+
+/* Seeded benchmark scenes (library extension, no reference counterpart):
+ *   1: 1,000 bodies in a 10x10 m column over a 50x1x50 m ground
+ *   2: 32,768 bodies in a 30x30 m walled container (ground + 4 walls)
+ *   4: `count` bodies (64 for the differentiable workload) over a ground
+ *   5: 1,048,576 bodies (70% spheres) over a 220x1x220 m ground
+ * `count` = 0 picks the workload's own body count; `sphere_segments` = 0
+ * picks 16 (8 for workload 5). */
+int sv_scene_build_workload(sv_scene *scene, int workload, uint32_t count,
+                            uint32_t sphere_segments, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEEVR_H */

@@ -1,0 +1,984 @@
+/*
+ * sv_oracle.c - CPU oracle (test infrastructure only) for seeVR's
+ * physics.comp / bvh.comp path and the data.zig scene construction.
+ *
+ * Serial restatement in the canonical schedule (invocations in index order).
+ * Build with -ffp-contract=off so that every a*b+c rounds twice, as GLSL
+ * without `precise`-driven fusion is modelled here.  GLSL operator semantics
+ * used throughout:
+ *   min(x, y) = y < x ? y : x      max(x, y) = x < y ? y : x   (GLSL 4.50 8.3)
+ *   dot(a, b) = (a.x*b.x + a.y*b.y) + a.z*b.z
+ *   normalize(v) = v * (1 / sqrt(dot(v, v)))
+ *   mat3 * v = col0*v.x + col1*v.y + col2*v.z, transpose(R)*v = (dot(col_i, v))
+ *
+ * Parity pins (tests/test_oracle_*.py): closed forms for gravity/integrate,
+ * brute force for AABB/broadphase/support/Karras/refit, std sort for the
+ * radix sort, KATs for Morton, closed-form sphere/box MPR depths, momentum
+ * and single-contact impulse identities for PGS, libm for atan2f.
+ */
+#include "sv_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef or_real R;
+typedef struct { R x, y, z; } v3;
+
+#ifdef OR_DOUBLE
+#define RSQRT(x) sqrt(x)
+#define RABS(x) fabs(x)
+#else
+#define RSQRT(x) sqrtf(x)
+#define RABS(x) fabsf(x)
+#endif
+
+static inline R gmin(R x, R y) { return y < x ? y : x; }
+static inline R gmax(R x, R y) { return x < y ? y : x; }
+static inline v3 V(R x, R y, R z) { v3 r = {x, y, z}; return r; }
+static inline v3 add(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 scl(R s, v3 a) { return V(s * a.x, s * a.y, s * a.z); }
+static inline v3 neg(v3 a) { return V(-a.x, -a.y, -a.z); }
+static inline v3 vmul(v3 a, v3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline R dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 cross(v3 a, v3 b) {
+    return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline R length3(v3 a) { return RSQRT(dot(a, a)); }
+static inline v3 normalize3(v3 a) { R inv = (R)1 / RSQRT(dot(a, a)); return scl(inv, a); }
+static inline v3 vmin(v3 a, v3 b) { return V(gmin(a.x, b.x), gmin(a.y, b.y), gmin(a.z, b.z)); }
+static inline v3 vmax(v3 a, v3 b) { return V(gmax(a.x, b.x), gmax(a.y, b.y), gmax(a.z, b.z)); }
+
+/* Columns of the rotation matrix of quaternion (x, y, z, w):
+ * physics.comp:74-81 (GLSL mat3 constructor is column-major). */
+typedef struct { v3 c0, c1, c2; } m3;
+static m3 quat_to_mat(const R *q) {
+    R x = q[0], y = q[1], z = q[2], w = q[3];
+    m3 m;
+    m.c0 = V((R)1 - (R)2 * (y * y + z * z), (R)2 * (x * y + w * z), (R)2 * (x * z - w * y));
+    m.c1 = V((R)2 * (x * y - w * z), (R)1 - (R)2 * (x * x + z * z), (R)2 * (y * z + w * x));
+    m.c2 = V((R)2 * (x * z + w * y), (R)2 * (y * z - w * x), (R)1 - (R)2 * (x * x + y * y));
+    return m;
+}
+static inline v3 mmul(m3 m, v3 v) {
+    return V(m.c0.x * v.x + m.c1.x * v.y + m.c2.x * v.z,
+             m.c0.y * v.x + m.c1.y * v.y + m.c2.y * v.z,
+             m.c0.z * v.x + m.c1.z * v.y + m.c2.z * v.z);
+}
+static inline v3 mtmul(m3 m, v3 v) { return V(dot(m.c0, v), dot(m.c1, v), dot(m.c2, v)); }
+
+static inline v3 ld3(const R *p, uint32_t i) { return V(p[i * 3], p[i * 3 + 1], p[i * 3 + 2]); }
+static inline void st3(R *p, uint32_t i, v3 v) { p[i * 3] = v.x; p[i * 3 + 1] = v.y; p[i * 3 + 2] = v.z; }
+
+/* ---------------------------------------------------------------------
+ * atan2f: the library's own definition (the reference calls GLSL atan,
+ * physics.comp:137-139, whose precision is driver-defined).  Reduction
+ * atan t = pi/2 - atan(1/t) and atan t = pi/4 + atan((t-1)/(t+1)) for
+ * t > tan(pi/8), then an odd degree-11 polynomial fitted by
+ * seevr_amd/csrc/gen_atan_poly.py.  Same constants and operation order as
+ * the device version, so both round identically.
+ * ------------------------------------------------------------------- */
+#define A_C0 -0x1.55554ap-2f
+#define A_C1 0x1.999198p-3f
+#define A_C2 -0x1.23b532p-3f
+#define A_C3 0x1.b1ecd2p-4f
+#define A_C4 -0x1.f1efd6p-5f
+#define A_PI 0x1.921fb6p+1f
+#define A_PIO2 0x1.921fb6p+0f
+#define A_PIO4 0x1.921fb6p-1f
+#define A_TAN_PI8 0x1.a827dap-2f
+
+static float atan_reduced(float z) {
+    float u = z * z;
+    float p = A_C4;
+    p = p * u + A_C3;
+    p = p * u + A_C2;
+    p = p * u + A_C1;
+    p = p * u + A_C0;
+    return z + (z * u) * p;
+}
+
+float OR(atan2f)(float y, float x) {
+    float ax = fabsf(x), ay = fabsf(y);
+    if (isnan(x) || isnan(y)) return x + y;
+    float r;
+    int swap = ay > ax;
+    float mx = swap ? ay : ax, mn = swap ? ax : ay;
+    if (mx == 0.0f) {
+        r = 0.0f;
+    } else if (isinf(mx)) {
+        r = isinf(mn) ? A_PIO4 : 0.0f;
+        if (isinf(mn)) swap = 0;
+    } else {
+        float t = mn / mx;
+        if (t > A_TAN_PI8) {
+            r = A_PIO4 + atan_reduced((t - 1.0f) / (t + 1.0f));
+        } else {
+            r = atan_reduced(t);
+        }
+    }
+    if (swap) r = A_PIO2 - r;
+    if (signbit(x)) r = A_PI - r;
+    return copysignf(r, y);
+}
+
+static inline R atan2r(R y, R x) {
+#ifdef OR_DOUBLE
+    return atan2(y, x);
+#else
+    return OR(atan2f)(y, x);
+#endif
+}
+
+/* ---------------------------------------------------------------------
+ * Physics
+ * ------------------------------------------------------------------- */
+static inline void hull_info(const or_bufs *b, uint32_t bi, uint32_t *vs, uint32_t *vc) {
+    /* physics.comp:98-102: gi = bi, start/count stored as float */
+    if (b->hull_range) {
+        *vs = b->hull_range[bi * 2];
+        *vc = b->hull_range[bi * 2 + 1];
+    } else {
+        *vs = (uint32_t)b->geom_data[bi * 8 + 0];
+        *vc = (uint32_t)b->geom_data[bi * 8 + 1];
+    }
+}
+
+/* physics.comp:418-428 */
+void OR(step_gravity)(or_bufs *b, uint32_t count, R dt, const R g[3]) {
+    for (uint32_t i = 0; i < count; i++) {
+        if (b->body_inv_mass[i] <= (R)0) continue;
+        for (int k = 0; k < 3; k++)
+            b->body_vel[i * 3 + k] = b->body_vel[i * 3 + k] * b->lin_damping + g[k] * dt;
+        for (int k = 0; k < 3; k++) b->body_omega[i * 3 + k] *= b->ang_damping;
+    }
+}
+
+/* physics.comp:431-448 (step 1, the per-body semantics the 9/10/11
+ * cascade computes in pieces). */
+void OR(step_compute_aabb)(or_bufs *b, uint32_t count) {
+    for (uint32_t i = 0; i < count; i++) {
+        m3 Rm = quat_to_mat(b->body_quat + i * 4);
+        v3 p = ld3(b->body_pos, i);
+        uint32_t vs, vc;
+        hull_info(b, i, &vs, &vc);
+        v3 amin = V((R)1e10, (R)1e10, (R)1e10), amax = V((R)-1e10, (R)-1e10, (R)-1e10);
+        for (uint32_t v = 0; v < vc; v++) {
+            v3 wv = add(p, mmul(Rm, ld3(b->hull_verts, vs + v)));
+            amin = vmin(amin, wv);
+            amax = vmax(amax, wv);
+        }
+        st3(b->aabb_min, i, amin);
+        st3(b->aabb_max, i, amax);
+    }
+}
+
+/* physics.comp:451-472, rows i in order, j ascending. */
+void OR(step_broad_phase)(or_bufs *b, uint32_t count) {
+    for (uint32_t i = 0; i < count; i++) {
+        v3 mna = ld3(b->aabb_min, i), mxa = ld3(b->aabb_max, i);
+        for (uint32_t j = i + 1; j < count; j++) {
+            if (b->body_inv_mass[i] <= (R)0 && b->body_inv_mass[j] <= (R)0) continue;
+            v3 mnb = ld3(b->aabb_min, j), mxb = ld3(b->aabb_max, j);
+            if (!(mxa.x < mnb.x || mxb.x < mna.x || mxa.y < mnb.y || mxb.y < mna.y ||
+                  mxa.z < mnb.z || mxb.z < mna.z)) {
+                uint32_t idx = b->counters[0]++;
+                if (idx < b->max_pairs) {
+                    b->pairs[idx * 2] = i;
+                    b->pairs[idx * 2 + 1] = j;
+                }
+            }
+        }
+    }
+}
+
+/* physics.comp:107-162 */
+static v3 support(const or_bufs *b, uint32_t bi, v3 d) {
+    m3 Rm = quat_to_mat(b->body_quat + bi * 4);
+    v3 p = ld3(b->body_pos, bi);
+    v3 ld = mtmul(Rm, d);
+    uint32_t vs, vc;
+    hull_info(b, bi, &vs, &vc);
+    uint32_t lk = b->support_offset[bi];
+    R best = (R)-1e10;
+    v3 bv = V(0, 0, 0);
+    if (lk == 0) {
+        for (uint32_t i = 0; i < vc; i++) {
+            v3 v = ld3(b->hull_verts, vs + i);
+            R dd = dot(v, ld);
+            if (dd > best) { best = dd; bv = v; }
+        }
+        return add(p, mmul(Rm, bv));
+    }
+    R ang[3] = {atan2r(ld.z, ld.y), atan2r(ld.z, ld.x), atan2r(ld.y, ld.x)};
+    for (uint32_t ax = 0; ax < 3; ax++) {
+        R a = ang[ax];
+        if (a < (R)0) a += (R)6.2831853;
+        R bin_f = a / (R)6.2831853 * (R)64;
+        uint32_t b0 = ((uint32_t)bin_f) % 64u;
+        uint32_t b1 = (b0 + 1u) % 64u;
+        uint32_t i0 = b->support_lookup[lk + ax * 64u + b0];
+        v3 v0 = ld3(b->hull_verts, vs + i0);
+        R d0 = dot(v0, ld);
+        if (d0 > best) { best = d0; bv = v0; }
+        uint32_t i1 = b->support_lookup[lk + ax * 64u + b1];
+        v3 v1 = ld3(b->hull_verts, vs + i1);
+        R d1 = dot(v1, ld);
+        if (d1 > best) { best = d1; bv = v1; }
+    }
+    return add(p, mmul(Rm, bv));
+}
+
+void OR(support)(const or_bufs *b, uint32_t bi, const R d[3], R out[3]) {
+    v3 r = support(b, bi, V(d[0], d[1], d[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* physics.comp:165-169 */
+static inline void mink(const or_bufs *b, uint32_t a, uint32_t bb, v3 d, v3 *v, v3 *sa, v3 *sb) {
+    *sa = support(b, a, d);
+    *sb = support(b, bb, neg(d));
+    *v = sub(*sa, *sb);
+}
+
+#define CCD_EPS ((R)1e-6)
+#define CCD_TOL ((R)1e-6)
+#define CCD_ITERS 50
+
+/* physics.comp:178-197 */
+static inline v3 portal_dir(v3 v1, v3 v2, v3 v3_) { return normalize3(cross(sub(v2, v1), sub(v3_, v1))); }
+static inline int reach_tol(v3 v1, v3 v2, v3 v3_, v3 v4, v3 dir) {
+    R dv1 = dot(v1, dir), dv2 = dot(v2, dir), dv3 = dot(v3_, dir), dv4 = dot(v4, dir);
+    R d1 = gmin(gmin(dv4 - dv1, dv4 - dv2), dv4 - dv3);
+    return d1 < CCD_TOL + CCD_EPS * gmax((R)1, d1);
+}
+
+/* Simplex state: v[0..3] Minkowski points, sa/sb the support points on A/B. */
+typedef struct { v3 v[4], sa[4], sb[4]; } simplex;
+
+/* physics.comp:200-221 */
+static void expand_portal(simplex *s, v3 v4, v3 s4a, v3 s4b) {
+    v3 v4v0 = cross(v4, s->v[0]);
+    R d = dot(s->v[1], v4v0);
+    int slot;
+    if (d > (R)0) {
+        d = dot(s->v[2], v4v0);
+        slot = d > (R)0 ? 1 : 3;
+    } else {
+        d = dot(s->v[3], v4v0);
+        slot = d > (R)0 ? 2 : 1;
+    }
+    s->v[slot] = v4; s->sa[slot] = s4a; s->sb[slot] = s4b;
+}
+
+/* physics.comp:224-247 */
+static v3 find_pos(const simplex *s) {
+    v3 v0 = s->v[0], v1 = s->v[1], v2 = s->v[2], v3_ = s->v[3];
+    R b0 = dot(cross(v1, v2), v3_);
+    R b1 = -dot(cross(v0, v2), v3_);
+    R b2 = dot(cross(v0, v1), v3_);
+    R b3 = -dot(cross(v0, v1), v2);
+    R sum = b0 + b1 + b2 + b3;
+    if (RABS(sum) < CCD_EPS) {
+        v3 dir = portal_dir(v1, v2, v3_);
+        b0 = (R)0;
+        b1 = dot(cross(v2, v3_), dir);
+        b2 = dot(cross(v3_, v1), dir);
+        b3 = dot(cross(v1, v2), dir);
+        sum = b0 + b1 + b2 + b3;
+    }
+    v3 p1 = add(add(add(scl(b0, s->sa[0]), scl(b1, s->sa[1])), scl(b2, s->sa[2])), scl(b3, s->sa[3]));
+    v3 p2 = add(add(add(scl(b0, s->sb[0]), scl(b1, s->sb[1])), scl(b2, s->sb[2])), scl(b3, s->sb[3]));
+    return scl((R)0.5 / sum, add(p1, p2));
+}
+
+/* physics.comp:250-320; returns -1 miss, 0 portal, 1 touching, 2 collinear */
+static int discover_portal(const or_bufs *b, uint32_t a, uint32_t bb, simplex *s) {
+    s->sa[0] = ld3(b->body_pos, a);
+    s->sb[0] = ld3(b->body_pos, bb);
+    s->v[0] = sub(s->sa[0], s->sb[0]);
+    if (RABS(s->v[0].x) < CCD_EPS && RABS(s->v[0].y) < CCD_EPS && RABS(s->v[0].z) < CCD_EPS)
+        s->v[0].x += (R)10.0 * CCD_EPS;
+    v3 dir = normalize3(neg(s->v[0]));
+    mink(b, a, bb, dir, &s->v[1], &s->sa[1], &s->sb[1]);
+    R d = dot(s->v[1], dir);
+    if (d < CCD_EPS) return -1;
+    dir = cross(s->v[0], s->v[1]);
+    if (dot(dir, dir) < CCD_EPS) {
+        if (RABS(s->v[1].x) < CCD_EPS && RABS(s->v[1].y) < CCD_EPS && RABS(s->v[1].z) < CCD_EPS)
+            return 1;
+        return 2;
+    }
+    dir = normalize3(dir);
+    mink(b, a, bb, dir, &s->v[2], &s->sa[2], &s->sb[2]);
+    d = dot(s->v[2], dir);
+    if (d < CCD_EPS) return -1;
+    v3 va = sub(s->v[1], s->v[0]);
+    v3 vb = sub(s->v[2], s->v[0]);
+    dir = normalize3(cross(va, vb));
+    if (dot(dir, s->v[0]) > (R)0) {
+        v3 t;
+        t = s->v[1]; s->v[1] = s->v[2]; s->v[2] = t;
+        t = s->sa[1]; s->sa[1] = s->sa[2]; s->sa[2] = t;
+        t = s->sb[1]; s->sb[1] = s->sb[2]; s->sb[2] = t;
+        dir = neg(dir);
+    }
+    for (int trials = 0; trials < 15;) {
+        v3 v3c, s3a, s3b;
+        mink(b, a, bb, dir, &v3c, &s3a, &s3b);
+        d = dot(v3c, dir);
+        if (d < CCD_EPS) return -1;
+        va = cross(s->v[1], v3c);
+        d = dot(va, s->v[0]);
+        if (d < -CCD_EPS) {
+            s->v[2] = v3c; s->sa[2] = s3a; s->sb[2] = s3b;
+            va = sub(s->v[1], s->v[0]); vb = sub(s->v[2], s->v[0]);
+            dir = normalize3(cross(va, vb));
+            trials++;
+            continue;
+        }
+        va = cross(v3c, s->v[2]);
+        d = dot(va, s->v[0]);
+        if (d < -CCD_EPS) {
+            s->v[1] = v3c; s->sa[1] = s3a; s->sb[1] = s3b;
+            va = sub(s->v[1], s->v[0]); vb = sub(s->v[2], s->v[0]);
+            dir = normalize3(cross(va, vb));
+            trials++;
+            continue;
+        }
+        s->v[3] = v3c; s->sa[3] = s3a; s->sb[3] = s3b;
+        return 0;
+    }
+    return -1;
+}
+
+/* physics.comp:323-343 */
+static int refine_portal(const or_bufs *b, uint32_t a, uint32_t bb, simplex *s) {
+    for (int it = 0; it < CCD_ITERS; it++) {
+        v3 dir = portal_dir(s->v[1], s->v[2], s->v[3]);
+        if (dot(s->v[1], dir) > -CCD_EPS) return 0;
+        v3 v4, s4a, s4b;
+        mink(b, a, bb, dir, &v4, &s4a, &s4b);
+        if (!(dot(v4, dir) > -CCD_EPS) || reach_tol(s->v[1], s->v[2], s->v[3], v4, dir)) return -1;
+        expand_portal(s, v4, s4a, s4b);
+    }
+    return 1;
+}
+
+/* physics.comp:346-365 */
+static void find_penetration(const or_bufs *b, uint32_t a, uint32_t bb, simplex *s, v3 *normal, R *depth) {
+    for (int it = 0; it < CCD_ITERS; it++) {
+        v3 dir = portal_dir(s->v[1], s->v[2], s->v[3]);
+        v3 v4, s4a, s4b;
+        mink(b, a, bb, dir, &v4, &s4a, &s4b);
+        if (reach_tol(s->v[1], s->v[2], s->v[3], v4, dir) || it >= CCD_ITERS - 1) {
+            *depth = dot(dir, s->v[1]);
+            *normal = dir;
+            return;
+        }
+        expand_portal(s, v4, s4a, s4b);
+    }
+}
+
+/* physics.comp:368-399 */
+static int mpr(const or_bufs *b, uint32_t a, uint32_t bb, v3 *normal, R *depth, v3 *cp) {
+    simplex s;
+    memset(&s, 0, sizeof(s));
+    int res = discover_portal(b, a, bb, &s);
+    if (res == 1) {
+        *normal = normalize3(s.v[0]);
+        *depth = (R)0;
+        *cp = scl((R)0.5, add(s.sa[1], s.sb[1]));
+        return 1;
+    }
+    if (res == 2) {
+        *depth = length3(s.v[1]);
+        *normal = normalize3(s.v[1]);
+        *cp = scl((R)0.5, add(s.sa[1], s.sb[1]));
+        return 1;
+    }
+    if (res != 0) return 0;
+    res = refine_portal(b, a, bb, &s);
+    if (res < 0) return 0;
+    find_penetration(b, a, bb, &s, normal, depth);
+    *cp = find_pos(&s);
+    return 1;
+}
+
+int OR(mpr)(const or_bufs *b, uint32_t a, uint32_t bb, R normal[3], R *depth, R contact[3]) {
+    v3 n = V(0, 0, 0), c = V(0, 0, 0);
+    R d = 0;
+    int hit = mpr(b, a, bb, &n, &d, &c);
+    normal[0] = n.x; normal[1] = n.y; normal[2] = n.z;
+    contact[0] = c.x; contact[1] = c.y; contact[2] = c.z;
+    *depth = d;
+    return hit;
+}
+
+/* physics.comp:403-412 */
+static void add_contact(or_bufs *b, v3 cp, v3 n, R pen, uint32_t ba, uint32_t bb) {
+    uint32_t idx = b->counters[1]++;
+    if (idx < b->max_contacts) {
+        st3(b->contact_pos, idx, cp);
+        st3(b->contact_normal, idx, n);
+        b->contact_pen[idx] = pen;
+        b->contact_geom_a[idx] = ba;
+        b->contact_geom_b[idx] = bb;
+    }
+}
+
+/* physics.comp:475-491 */
+void OR(step_narrow_phase)(or_bufs *b) {
+    uint32_t np = b->counters[0] < b->max_pairs ? b->counters[0] : b->max_pairs;
+    for (uint32_t pi = 0; pi < np; pi++) {
+        uint32_t a = b->pairs[pi * 2], bb = b->pairs[pi * 2 + 1];
+        v3 n, cp;
+        R depth;
+        if (mpr(b, a, bb, &n, &depth, &cp) && depth > (R)0) add_contact(b, cp, n, depth, a, bb);
+    }
+}
+
+/* physics.comp:494-497 */
+void OR(step_clear)(or_bufs *b) {
+    for (uint32_t i = 0; i < b->max_contacts; i++) b->lambda_n[i] = (R)0;
+}
+
+/* physics.comp:90-94 */
+static inline v3 apply_inv_I(const or_bufs *b, uint32_t bi, v3 v) {
+    m3 Rm = quat_to_mat(b->body_quat + bi * 4);
+    v3 inv_i = ld3(b->body_inv_inertia, bi);
+    return mmul(Rm, vmul(inv_i, mtmul(Rm, v)));
+}
+
+/* physics.comp:500-583: serial projected Gauss-Seidel. */
+void OR(step_solve)(or_bufs *b, R dt) {
+    uint32_t nc = b->counters[1] < b->max_contacts ? b->counters[1] : b->max_contacts;
+    R *bias = (R *)malloc(sizeof(R) * (nc ? nc : 1));
+    for (uint32_t ci = 0; ci < nc; ci++) {
+        uint32_t ga = b->contact_geom_a[ci], gb = b->contact_geom_b[ci];
+        uint32_t a = b->geom_body_idx[ga], bb = b->geom_body_idx[gb];
+        v3 n = ld3(b->contact_normal, ci), cp = ld3(b->contact_pos, ci);
+        v3 ra = sub(cp, ld3(b->body_pos, a)), rb = sub(cp, ld3(b->body_pos, bb));
+        v3 va = add(ld3(b->body_vel, a), cross(ld3(b->body_omega, a), ra));
+        v3 vb = add(ld3(b->body_vel, bb), cross(ld3(b->body_omega, bb), rb));
+        R vn = dot(sub(vb, va), n);
+        R pen = b->contact_pen[ci];
+        R e = gmax(b->geom_restitution[ga], b->geom_restitution[gb]);
+        bias[ci] = b->baumgarte * gmax(pen - b->slop, (R)0) / dt + e * gmax(-vn, (R)0);
+    }
+    for (uint32_t it = 0; it < b->pgs_iterations; it++) {
+        for (uint32_t ci = 0; ci < nc; ci++) {
+            uint32_t ga = b->contact_geom_a[ci], gb = b->contact_geom_b[ci];
+            uint32_t a = b->geom_body_idx[ga], bb = b->geom_body_idx[gb];
+            R ima = b->body_inv_mass[a], imb = b->body_inv_mass[bb];
+            if (ima == (R)0 && imb == (R)0) continue;
+            v3 n = ld3(b->contact_normal, ci), cp = ld3(b->contact_pos, ci);
+            v3 ra = sub(cp, ld3(b->body_pos, a)), rb = sub(cp, ld3(b->body_pos, bb));
+            v3 va = add(ld3(b->body_vel, a), cross(ld3(b->body_omega, a), ra));
+            v3 vb = add(ld3(b->body_vel, bb), cross(ld3(b->body_omega, bb), rb));
+            R vn = dot(sub(vb, va), n);
+            v3 ran = cross(ra, n), rbn = cross(rb, n);
+            R k = ima + imb + dot(ran, apply_inv_I(b, a, ran)) + dot(rbn, apply_inv_I(b, bb, rbn));
+            R dln = (-vn + bias[ci]) / k;
+            R old = b->lambda_n[ci];
+            R nl = gmax(old + dln, (R)0);
+            dln = nl - old;
+            b->lambda_n[ci] = nl;
+            v3 imp = scl(dln, n);
+            st3(b->body_vel, a, sub(ld3(b->body_vel, a), scl(ima, imp)));
+            st3(b->body_vel, bb, add(ld3(b->body_vel, bb), scl(imb, imp)));
+            st3(b->body_omega, a, sub(ld3(b->body_omega, a), apply_inv_I(b, a, cross(ra, imp))));
+            st3(b->body_omega, bb, add(ld3(b->body_omega, bb), apply_inv_I(b, bb, cross(rb, imp))));
+            R mu = RSQRT(b->geom_friction[ga] * b->geom_friction[gb]);
+            va = add(ld3(b->body_vel, a), cross(ld3(b->body_omega, a), ra));
+            vb = add(ld3(b->body_vel, bb), cross(ld3(b->body_omega, bb), rb));
+            v3 dv = sub(vb, va);
+            v3 vt = sub(dv, scl(dot(dv, n), n));
+            R vtl = length3(vt);
+            if (vtl > (R)1e-6) {
+                v3 td = V(vt.x / vtl, vt.y / vtl, vt.z / vtl);
+                v3 rat = cross(ra, td), rbt = cross(rb, td);
+                R kt = ima + imb + dot(rat, apply_inv_I(b, a, rat)) + dot(rbt, apply_inv_I(b, bb, rbt));
+                R dlt = gmin(vtl / kt, mu * b->lambda_n[ci]);
+                v3 impt = scl(dlt, td);
+                st3(b->body_vel, a, add(ld3(b->body_vel, a), scl(ima, impt)));
+                st3(b->body_vel, bb, sub(ld3(b->body_vel, bb), scl(imb, impt)));
+                st3(b->body_omega, a, add(ld3(b->body_omega, a), apply_inv_I(b, a, cross(ra, impt))));
+                st3(b->body_omega, bb, sub(ld3(b->body_omega, bb), apply_inv_I(b, bb, cross(rb, impt))));
+            }
+        }
+    }
+    free(bias);
+}
+
+/* physics.comp:586-604 */
+void OR(step_integrate)(or_bufs *b, uint32_t count, R dt) {
+    for (uint32_t i = 0; i < count; i++) {
+        if (b->body_inv_mass[i] <= (R)0) continue;
+        for (int k = 0; k < 3; k++) b->body_pos[i * 3 + k] += b->body_vel[i * 3 + k] * dt;
+        R ox = b->body_omega[i * 3], oy = b->body_omega[i * 3 + 1], oz = b->body_omega[i * 3 + 2];
+        R *q = b->body_quat + i * 4;
+        R qx = q[0], qy = q[1], qz = q[2], qw = q[3];
+        R dx = ox * qw + oy * qz - oz * qy;
+        R dy = -ox * qz + oy * qw + oz * qx;
+        R dz = ox * qy - oy * qx + oz * qw;
+        R dw = -ox * qx - oy * qy - oz * qz;
+        qx += (R)0.5 * dx * dt; qy += (R)0.5 * dy * dt; qz += (R)0.5 * dz * dt; qw += (R)0.5 * dw * dt;
+        R l = RSQRT(qx * qx + qy * qy + qz * qz + qw * qw);
+        if (l > (R)1e-8) { qx /= l; qy /= l; qz /= l; qw /= l; }
+        q[0] = qx; q[1] = qy; q[2] = qz; q[3] = qw;
+    }
+}
+
+/* physics.comp:607-627 */
+void OR(step_update_verts)(or_bufs *b, uint32_t count, uint32_t num_bodies) {
+    for (uint32_t vid = 0; vid < count; vid++) {
+        uint32_t lo = 0, hi = num_bodies;
+        while (lo + 1 < hi) {
+            uint32_t mid = (lo + hi) / 2;
+            if (b->body_vert_start[mid] <= vid) lo = mid; else hi = mid;
+        }
+        m3 Rm = quat_to_mat(b->body_quat + lo * 4);
+        v3 wp = add(ld3(b->body_pos, lo), mmul(Rm, ld3(b->orig_vertices, vid)));
+        st3(b->vertices, vid, wp);
+    }
+}
+
+/* physics.comp:630-634 */
+void OR(step_clear_counters)(or_bufs *b) { b->counters[0] = 0; b->counters[1] = 0; }
+
+/* physics.zig:80-117 (profiling-mode order; the AABB cascade 9/10/11
+ * computes the step-1 per-body box). */
+void OR(physics_step)(or_bufs *b, uint32_t num_bodies, uint32_t num_vertices,
+                      uint32_t num_hull_verts, uint32_t substeps, float dt, const float g[3]) {
+    (void)num_hull_verts;
+    if (num_bodies == 0 || substeps == 0) return;
+    float sub_dt = dt / (float)substeps;
+    R gr[3] = {(R)g[0], (R)g[1], (R)g[2]};
+    for (uint32_t s = 0; s < substeps; s++) {
+        OR(step_clear_counters)(b);
+        OR(step_gravity)(b, num_bodies, (R)sub_dt, gr);
+        OR(step_compute_aabb)(b, num_bodies);
+        OR(step_broad_phase)(b, num_bodies);
+        OR(step_narrow_phase)(b);
+        OR(step_clear)(b);
+        OR(step_solve)(b, (R)sub_dt);
+        OR(step_integrate)(b, num_bodies, (R)sub_dt);
+    }
+    OR(step_update_verts)(b, num_vertices, num_bodies);
+}
+
+/* ---------------------------------------------------------------------
+ * LBVH
+ * ------------------------------------------------------------------- */
+/* bvh.comp:32-39 */
+uint32_t OR(expand_bits)(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+static inline R clampr(R x, R lo, R hi) { return gmin(gmax(x, lo), hi); }
+
+/* bvh.comp:41-46 */
+uint32_t OR(morton3d)(R x, R y, R z) {
+    uint32_t xi = (uint32_t)clampr(x * (R)1024, (R)0, (R)1023);
+    uint32_t yi = (uint32_t)clampr(y * (R)1024, (R)0, (R)1023);
+    uint32_t zi = (uint32_t)clampr(z * (R)1024, (R)0, (R)1023);
+    return (OR(expand_bits)(xi) << 2) | (OR(expand_bits)(yi) << 1) | OR(expand_bits)(zi);
+}
+
+/* bvh.comp:48-57 */
+int OR(clz32)(uint32_t x) {
+    if (x == 0u) return 32;
+    int n = 0;
+    if ((x & 0xFFFF0000u) == 0u) { n += 16; x <<= 16; }
+    if ((x & 0xFF000000u) == 0u) { n += 8; x <<= 8; }
+    if ((x & 0xF0000000u) == 0u) { n += 4; x <<= 4; }
+    if ((x & 0xC0000000u) == 0u) { n += 2; x <<= 2; }
+    if ((x & 0x80000000u) == 0u) { n += 1; }
+    return n;
+}
+
+/* bvh.comp:59-65 */
+static int delta_f(const uint32_t *morton, int i, int j, int n) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = morton[i], b = morton[j];
+    if (a == b) return 32 + OR(clz32)((uint32_t)i ^ (uint32_t)j);
+    return OR(clz32)(a ^ b);
+}
+
+/* bvh.comp:69-79 */
+void OR(bvh_centroids)(or_bvh *t, uint32_t n) {
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t i0 = t->indices[i * 3], i1 = t->indices[i * 3 + 1], i2 = t->indices[i * 3 + 2];
+        for (int k = 0; k < 3; k++)
+            t->centroids[i * 3 + k] = (t->vertices[i0 * 3 + k] + t->vertices[i1 * 3 + k] +
+                                       t->vertices[i2 * 3 + k]) / (R)3.0;
+        t->sort_idx[i] = i;
+        t->prim_idx[i] = i;
+    }
+}
+
+/* bvh.comp:88-165: min/max over centroids with the +-1e30 padding of idle
+ * lanes (exact; order independent). */
+void OR(bvh_bounds)(or_bvh *t, uint32_t n) {
+    v3 mn = V((R)1e30, (R)1e30, (R)1e30), mx = V((R)-1e30, (R)-1e30, (R)-1e30);
+    for (uint32_t i = 0; i < n; i++) {
+        v3 c = ld3(t->centroids, i);
+        mn = vmin(mn, c);
+        mx = vmax(mx, c);
+    }
+    t->bounds[0] = mn.x; t->bounds[1] = mn.y; t->bounds[2] = mn.z;
+    t->bounds[3] = mx.x; t->bounds[4] = mx.y; t->bounds[5] = mx.z;
+}
+
+/* bvh.comp:169-183 */
+void OR(bvh_morton)(or_bvh *t, uint32_t n) {
+    v3 bmin = V(t->bounds[0], t->bounds[1], t->bounds[2]);
+    v3 ext = sub(V(t->bounds[3], t->bounds[4], t->bounds[5]), bmin);
+    v3 inv = V(ext.x > (R)1e-6 ? (R)1 / ext.x : (R)0, ext.y > (R)1e-6 ? (R)1 / ext.y : (R)0,
+               ext.z > (R)1e-6 ? (R)1 / ext.z : (R)0);
+    for (uint32_t i = 0; i < n; i++) {
+        v3 c = vmul(sub(ld3(t->centroids, i), bmin), inv);
+        t->morton[i] = OR(morton3d)(c.x, c.y, c.z);
+    }
+}
+
+/* bvh.comp:192-295 with bvh.zig:141: eight stable counting-sort passes on
+ * 4-bit digits (histogram, exclusive scan, stable scatter). */
+void OR(bvh_radix_sort)(or_bvh *t, uint32_t n) {
+    uint32_t *k2 = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t *v2 = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t *ka = t->morton, *va = t->sort_idx, *kb = k2, *vb = v2;
+    for (uint32_t shift = 0; shift < 32; shift += 4) {
+        uint32_t hist[16] = {0}, off[16];
+        for (uint32_t i = 0; i < n; i++) hist[(ka[i] >> shift) & 15u]++;
+        uint32_t run = 0;
+        for (int d = 0; d < 16; d++) { off[d] = run; run += hist[d]; }
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t d = (ka[i] >> shift) & 15u;
+            kb[off[d]] = ka[i];
+            vb[off[d]] = va[i];
+            off[d]++;
+        }
+        uint32_t *tk = ka; ka = kb; kb = tk;
+        uint32_t *tv = va; va = vb; vb = tv;
+    }
+    /* 8 passes: the result is back in morton / sort_idx */
+    free(k2);
+    free(v2);
+}
+
+/* bvh.comp:299-348 (Karras 2012) */
+void OR(bvh_hierarchy)(or_bvh *t, uint32_t nt) {
+    int n = (int)nt;
+    for (uint32_t i = 0; i < nt; i++) {
+        uint32_t leaf = nt - 1 + i;
+        t->node_left[leaf] = i;
+        t->node_count[leaf] = 1;
+    }
+    if (nt == 1) t->node_parent[0] = 0xFFFFFFFFu; /* bvh.comp leaves it unset */
+    for (uint32_t i = 0; i + 1 < nt; i++) {
+        int ii = (int)i;
+        int dl = delta_f(t->morton, ii, ii - 1, n);
+        int dr = delta_f(t->morton, ii, ii + 1, n);
+        int d = dr > dl ? 1 : -1;
+        int dmin = delta_f(t->morton, ii, ii - d, n);
+        int lmax = 2;
+        while (delta_f(t->morton, ii, ii + lmax * d, n) > dmin) lmax *= 2;
+        int l = 0, s = lmax / 2;
+        while (s >= 1) {
+            if (delta_f(t->morton, ii, ii + (l + s) * d, n) > dmin) l += s;
+            s /= 2;
+        }
+        int j = ii + l * d;
+        int first = ii < j ? ii : j, last = ii < j ? j : ii;
+        int dn = delta_f(t->morton, first, last, n);
+        int split = first, stride = last - first;
+        while (stride > 1) {
+            stride = (stride + 1) / 2;
+            int mid = split + stride;
+            if (mid < last && delta_f(t->morton, first, mid, n) > dn) split = mid;
+        }
+        uint32_t lc = (uint32_t)split, rc = (uint32_t)(split + 1);
+        if (first == split) lc = (uint32_t)split + nt - 1;
+        if (last == split + 1) rc = (uint32_t)(split + 1) + nt - 1;
+        t->node_left[i] = lc;
+        t->node_right[i] = rc;
+        t->node_count[i] = 0;
+        t->node_parent[lc] = i;
+        t->node_parent[rc] = i;
+        if (i == 0) t->node_parent[0] = 0xFFFFFFFFu;
+    }
+}
+
+/* bvh.comp:352-368 */
+void OR(bvh_leaf_aabb)(or_bvh *t, uint32_t n) {
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t leaf = n - 1 + i, pi = t->sort_idx[i];
+        t->prim_idx[i] = pi;
+        v3 a = ld3(t->vertices, t->indices[pi * 3]);
+        v3 b = ld3(t->vertices, t->indices[pi * 3 + 1]);
+        v3 c = ld3(t->vertices, t->indices[pi * 3 + 2]);
+        st3(t->node_min, leaf, vmin(vmin(a, b), c));
+        st3(t->node_max, leaf, vmax(vmax(a, b), c));
+    }
+}
+
+/* bvh.comp:372-397: each leaf climbs; the first arrival at a node stops,
+ * the second writes the union of the children and continues. */
+void OR(bvh_propagate)(or_bvh *t, uint32_t n) {
+    for (uint32_t i = 0; i + 1 < n; i++) t->flags[i] = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t parent = t->node_parent[n - 1 + i];
+        while (parent != 0xFFFFFFFFu) {
+            uint32_t old = t->flags[parent];
+            t->flags[parent] |= 1u;
+            if (old == 0u) break;
+            uint32_t li = t->node_left[parent], ri = t->node_right[parent];
+            st3(t->node_min, parent, vmin(ld3(t->node_min, li), ld3(t->node_min, ri)));
+            st3(t->node_max, parent, vmax(ld3(t->node_max, li), ld3(t->node_max, ri)));
+            parent = t->node_parent[parent];
+        }
+    }
+}
+
+/* bvh.zig:63-172 pass order */
+void OR(bvh_build)(or_bvh *t, uint32_t n) {
+    if (n == 0) return;
+    OR(bvh_centroids)(t, n);
+    OR(bvh_bounds)(t, n);
+    OR(bvh_morton)(t, n);
+    OR(bvh_radix_sort)(t, n);
+    OR(bvh_hierarchy)(t, n);
+    OR(bvh_leaf_aabb)(t, n);
+    OR(bvh_propagate)(t, n);
+}
+
+void OR(bvh_build_bodies)(or_bvh *t, const R *amin, const R *amax, uint32_t n) {
+    if (n == 0) return;
+    for (uint32_t i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++)
+            t->centroids[i * 3 + k] = (amin[i * 3 + k] + amax[i * 3 + k]) * (R)0.5;
+        t->sort_idx[i] = i;
+        t->prim_idx[i] = i;
+    }
+    OR(bvh_bounds)(t, n);
+    OR(bvh_morton)(t, n);
+    OR(bvh_radix_sort)(t, n);
+    OR(bvh_hierarchy)(t, n);
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t leaf = n - 1 + i, pi = t->sort_idx[i];
+        t->prim_idx[i] = pi;
+        st3(t->node_min, leaf, ld3(amin, pi));
+        st3(t->node_max, leaf, ld3(amax, pi));
+    }
+    OR(bvh_propagate)(t, n);
+}
+
+/* ---------------------------------------------------------------------
+ * Scene construction (data.zig). Host-side float math exactly as Zig's
+ * strict float mode evaluates it: left to right, no contraction.  sinf /
+ * cosf are glibc's (Zig's std.math versions can differ in the last ulp).
+ * ------------------------------------------------------------------- */
+int OR(scene_init)(or_scene *s, uint32_t cb, uint32_t cv, uint32_t ct, uint32_t ch) {
+    memset(s, 0, sizeof(*s));
+    s->cap_bodies = cb; s->cap_verts = cv; s->cap_tris = ct; s->cap_hull = ch;
+    s->cap_lookup = cb * 192u + 1u; s->cap_mat = 256;
+    s->vertices = calloc((size_t)cv * 3, sizeof(float));
+    s->orig_vertices = calloc((size_t)cv * 3, sizeof(float));
+    s->indices = calloc((size_t)ct * 3, sizeof(uint32_t));
+    s->body_pos = calloc((size_t)cb * 3, sizeof(float));
+    s->body_quat = calloc((size_t)cb * 4, sizeof(float));
+    s->body_vel = calloc((size_t)cb * 3, sizeof(float));
+    s->body_omega = calloc((size_t)cb * 3, sizeof(float));
+    s->body_inv_mass = calloc(cb, sizeof(float));
+    s->body_inertia = calloc((size_t)cb * 3, sizeof(float));
+    s->body_inv_inertia = calloc((size_t)cb * 3, sizeof(float));
+    s->body_vert_start = calloc(cb, sizeof(uint32_t));
+    s->body_vert_count = calloc(cb, sizeof(uint32_t));
+    s->geom_body_idx = calloc(cb, sizeof(uint32_t));
+    s->geom_data = calloc((size_t)cb * 8, sizeof(float));
+    s->geom_friction = calloc(cb, sizeof(float));
+    s->geom_restitution = calloc(cb, sizeof(float));
+    s->geom_vert_start = calloc(cb, sizeof(uint32_t));
+    s->geom_vert_count = calloc(cb, sizeof(uint32_t));
+    s->hull_verts = calloc((size_t)ch * 3, sizeof(float));
+    s->support_offset = calloc(cb, sizeof(uint32_t));
+    s->support_lookup = calloc(s->cap_lookup, sizeof(uint32_t));
+    s->material_density = calloc(s->cap_mat, sizeof(float));
+    return (s->vertices && s->indices && s->hull_verts && s->support_lookup) ? 0 : -1;
+}
+
+void OR(scene_free)(or_scene *s) {
+    void *ptrs[] = {s->vertices, s->orig_vertices, s->indices, s->body_pos, s->body_quat,
+                    s->body_vel, s->body_omega, s->body_inv_mass, s->body_inertia,
+                    s->body_inv_inertia, s->body_vert_start, s->body_vert_count,
+                    s->geom_body_idx, s->geom_data, s->geom_friction, s->geom_restitution,
+                    s->geom_vert_start, s->geom_vert_count, s->hull_verts, s->support_offset,
+                    s->support_lookup, s->material_density};
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+    memset(s, 0, sizeof(*s));
+}
+
+/* data.zig:356-370 (only the density reaches physics) */
+uint32_t OR(scene_add_material)(or_scene *s, float density) {
+    uint32_t mi = s->num_materials++;
+    s->material_density[mi] = density;
+    return mi;
+}
+
+/* data.zig:299-353 */
+static void build_support_lookup(or_scene *s, uint32_t gi, uint32_t hs, uint32_t hc) {
+    if (hc <= 32u) { s->support_offset[gi] = 0; return; }
+    uint32_t start = s->num_support_entries == 0 ? 1u : s->num_support_entries;
+    s->support_offset[gi] = start;
+    static const float perps[3][2][3] = {{{0, 1, 0}, {0, 0, 1}},
+                                         {{1, 0, 0}, {0, 0, 1}},
+                                         {{1, 0, 0}, {0, 1, 0}}};
+    const float pi = 3.14159265358979323846f; /* std.math.pi coerced to f32 */
+    for (uint32_t ax = 0; ax < 3; ax++) {
+        const float *p1 = perps[ax][0], *p2 = perps[ax][1];
+        for (uint32_t bin = 0; bin < 64; bin++) {
+            float angle = (float)bin / 64.0f * 2.0f * pi;
+            float ca = cosf(angle), sa = sinf(angle);
+            float dx = ca * p1[0] + sa * p2[0];
+            float dy = ca * p1[1] + sa * p2[1];
+            float dz = ca * p1[2] + sa * p2[2];
+            float best = -1e10f;
+            uint32_t bi = 0;
+            for (uint32_t v = 0; v < hc; v++) {
+                const float *h = s->hull_verts + (size_t)(hs + v) * 3;
+                float d = dx * h[0] + dy * h[1] + dz * h[2];
+                if (d > best) { best = d; bi = v; }
+            }
+            s->support_lookup[start + ax * 64u + bin] = bi;
+        }
+    }
+    s->num_support_entries = start + 192u;
+}
+
+static void set_geom(or_scene *s, uint32_t gi, uint32_t bi, uint32_t hv, uint32_t hc,
+                     float fr, float re, uint32_t vs, uint32_t vc) {
+    s->geom_body_idx[gi] = bi;
+    float *g = s->geom_data + (size_t)gi * 8;
+    g[0] = (float)hv; g[1] = (float)hc;
+    for (int k = 2; k < 8; k++) g[k] = 0.0f;
+    s->geom_friction[gi] = fr;
+    s->geom_restitution[gi] = re;
+    s->geom_vert_start[gi] = vs;
+    s->geom_vert_count[gi] = vc;
+}
+
+/* data.zig:392-485 */
+uint32_t OR(scene_add_box)(or_scene *s, const float c[3], const float h[3], float fr,
+                           float re, uint32_t mat) {
+    uint32_t vs = s->num_vertices, ts = s->num_triangles, bi = s->num_bodies, gi = s->num_geoms;
+    const float bv[8][3] = {{-h[0], -h[1], -h[2]}, {h[0], -h[1], -h[2]}, {h[0], h[1], -h[2]},
+                            {-h[0], h[1], -h[2]},  {-h[0], -h[1], h[2]}, {h[0], -h[1], h[2]},
+                            {h[0], h[1], h[2]},    {-h[0], h[1], h[2]}};
+    static const uint32_t bf[12][3] = {{0, 2, 1}, {0, 3, 2}, {5, 7, 4}, {5, 6, 7},
+                                       {4, 3, 0}, {4, 7, 3}, {1, 6, 5}, {1, 2, 6},
+                                       {3, 6, 2}, {3, 7, 6}, {4, 1, 5}, {4, 0, 1}};
+    for (uint32_t i = 0; i < 8; i++)
+        for (int j = 0; j < 3; j++) {
+            s->orig_vertices[(size_t)(vs + i) * 3 + j] = bv[i][j];
+            s->vertices[(size_t)(vs + i) * 3 + j] = c[j] + bv[i][j];
+        }
+    for (uint32_t i = 0; i < 12; i++)
+        for (int j = 0; j < 3; j++) s->indices[(size_t)(ts + i) * 3 + j] = vs + bf[i][j];
+    s->num_vertices += 8;
+    s->num_triangles += 12;
+    float density = s->material_density[mat];
+    float volume = h[0] * 2 * h[1] * 2 * h[2] * 2;
+    float mass = density * volume;
+    for (int k = 0; k < 3; k++) { s->body_pos[bi * 3 + k] = c[k]; s->body_vel[bi * 3 + k] = 0; s->body_omega[bi * 3 + k] = 0; }
+    s->body_quat[bi * 4 + 0] = 0; s->body_quat[bi * 4 + 1] = 0; s->body_quat[bi * 4 + 2] = 0; s->body_quat[bi * 4 + 3] = 1;
+    s->body_inv_mass[bi] = mass > 0 ? 1.0f / mass : 0.0f;
+    if (mass > 0) {
+        float wx = h[0] * 2, wy = h[1] * 2, wz = h[2] * 2;
+        s->body_inertia[bi * 3 + 0] = mass / 12.0f * (wy * wy + wz * wz);
+        s->body_inertia[bi * 3 + 1] = mass / 12.0f * (wx * wx + wz * wz);
+        s->body_inertia[bi * 3 + 2] = mass / 12.0f * (wx * wx + wy * wy);
+        for (int k = 0; k < 3; k++) s->body_inv_inertia[bi * 3 + k] = 1.0f / s->body_inertia[bi * 3 + k];
+    } else {
+        for (int k = 0; k < 3; k++) { s->body_inertia[bi * 3 + k] = 0; s->body_inv_inertia[bi * 3 + k] = 0; }
+    }
+    s->num_bodies++;
+    uint32_t hv = s->num_hull_verts;
+    for (uint32_t i = 0; i < 8; i++)
+        for (int j = 0; j < 3; j++) s->hull_verts[(size_t)(hv + i) * 3 + j] = bv[i][j];
+    s->num_hull_verts += 8;
+    set_geom(s, gi, bi, hv, 8, fr, re, vs, 8);
+    build_support_lookup(s, gi, hv, 8);
+    s->num_geoms++;
+    return bi;
+}
+
+/* data.zig:487-597 */
+uint32_t OR(scene_add_sphere)(or_scene *s, const float c[3], float r, uint32_t seg, float fr,
+                              float re, uint32_t mat) {
+    uint32_t vs = s->num_vertices, ts = s->num_triangles, bi = s->num_bodies, gi = s->num_geoms;
+    const float pi = 3.14159265f;
+    uint32_t nv = (seg + 1) * (seg + 1), nt = seg * seg * 2;
+    float segs = (float)seg;
+    for (uint32_t i = 0; i <= seg; i++) {
+        float fi = (float)i;
+        float lat = pi * fi / segs;
+        float sl = sinf(lat), cl = cosf(lat);
+        for (uint32_t j = 0; j <= seg; j++) {
+            float fj = (float)j;
+            float lon = 2.0f * pi * fj / segs;
+            size_t idx = (size_t)(vs + i * (seg + 1) + j) * 3;
+            float lx = r * cosf(lon) * sl;
+            float ly = r * cl;
+            float lz = r * sinf(lon) * sl;
+            s->orig_vertices[idx] = lx; s->orig_vertices[idx + 1] = ly; s->orig_vertices[idx + 2] = lz;
+            s->vertices[idx] = c[0] + lx; s->vertices[idx + 1] = c[1] + ly; s->vertices[idx + 2] = c[2] + lz;
+        }
+    }
+    s->num_vertices += nv;
+    uint32_t ti = 0;
+    for (uint32_t i = 0; i < seg; i++)
+        for (uint32_t j = 0; j < seg; j++) {
+            uint32_t cur = vs + i * (seg + 1) + j, nxt = cur + seg + 1;
+            size_t b = (size_t)(ts + ti) * 3;
+            s->indices[b] = cur; s->indices[b + 1] = nxt; s->indices[b + 2] = cur + 1;
+            s->indices[b + 3] = cur + 1; s->indices[b + 4] = nxt; s->indices[b + 5] = nxt + 1;
+            ti += 2;
+        }
+    s->num_triangles += nt;
+    float density = s->material_density[mat];
+    float volume = (4.0f / 3.0f) * pi * r * r * r;
+    float mass = density * volume;
+    for (int k = 0; k < 3; k++) { s->body_pos[bi * 3 + k] = c[k]; s->body_vel[bi * 3 + k] = 0; s->body_omega[bi * 3 + k] = 0; }
+    s->body_quat[bi * 4 + 0] = 0; s->body_quat[bi * 4 + 1] = 0; s->body_quat[bi * 4 + 2] = 0; s->body_quat[bi * 4 + 3] = 1;
+    s->body_inv_mass[bi] = mass > 0 ? 1.0f / mass : 0.0f;
+    if (mass > 0) {
+        float I = 0.4f * mass * r * r;
+        for (int k = 0; k < 3; k++) { s->body_inertia[bi * 3 + k] = I; s->body_inv_inertia[bi * 3 + k] = 1.0f / I; }
+    } else {
+        for (int k = 0; k < 3; k++) { s->body_inertia[bi * 3 + k] = 0; s->body_inv_inertia[bi * 3 + k] = 0; }
+    }
+    s->num_bodies++;
+    uint32_t hv = s->num_hull_verts;
+    memcpy(s->hull_verts + (size_t)hv * 3, s->orig_vertices + (size_t)vs * 3, sizeof(float) * 3 * nv);
+    s->num_hull_verts += nv;
+    set_geom(s, gi, bi, hv, nv, fr, re, vs, nv);
+    build_support_lookup(s, gi, hv, nv);
+    s->num_geoms++;
+    return bi;
+}
+
+/* data.zig:633-640 */
+void OR(scene_derive)(or_scene *s) {
+    for (uint32_t gi = 0; gi < s->num_geoms; gi++) {
+        uint32_t bi = s->geom_body_idx[gi];
+        s->body_vert_start[bi] = s->geom_vert_start[gi];
+        s->body_vert_count[bi] = s->geom_vert_count[gi];
+    }
+}

@@ -1,0 +1,241 @@
+"""Python view of the C ABI: a device context (seeVR's Physics + BVH + the
+device side of Data) and a host scene builder (the host side of Data).
+
+Method names follow the reference: Physics.step / readCounters
+(src/physics.zig:80,213), BVH.build (src/bvh.zig:63), Data.addMaterial /
+addBox / addSphere / upload (src/data.zig:356,392,487,599).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import BUF, BUF_DTYPE, Caps, Params, SceneCounts, check
+
+
+def _fvec(vals: Iterable[float], n: int):
+    arr = (C.c_float * n)(*[float(v) for v in vals])
+    return arr
+
+
+def default_caps(**over) -> Caps:
+    lib = _lib.load()
+    c = Caps()
+    lib.sv_default_caps(C.byref(c))
+    for k, v in over.items():
+        setattr(c, k, int(v))
+    return c
+
+
+def default_params(**over) -> Params:
+    lib = _lib.load()
+    p = Params()
+    lib.sv_default_params(C.byref(p))
+    for k, v in over.items():
+        setattr(p, k, type(getattr(p, k))(v))
+    return p
+
+
+def _np_dtype(name: str):
+    return np.float32 if BUF_DTYPE[name] == "f" else np.uint32
+
+
+class Context:
+    """One sv_ctx: device buffers at fixed capacities plus the stages."""
+
+    def __init__(self, caps: Caps | None = None, params: Params | None = None, **caps_over):
+        self.lib = _lib.load()
+        self.caps = caps if caps is not None else default_caps(**caps_over)
+        self.params = params if params is not None else default_params()
+        h = C.c_void_p()
+        check(self.lib.sv_create(C.byref(self.caps), C.byref(self.params), C.byref(h)))
+        self.h = h
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.sv_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- buffers ------------------------------------------------------------
+    def buffer_bytes(self, name: str) -> int:
+        p, n = C.c_void_p(), C.c_size_t()
+        check(self.lib.sv_buffer_ptr(self.h, BUF[name], C.byref(p), C.byref(n)))
+        return n.value
+
+    def buffer_ptr(self, name: str) -> int:
+        p, n = C.c_void_p(), C.c_size_t()
+        check(self.lib.sv_buffer_ptr(self.h, BUF[name], C.byref(p), C.byref(n)))
+        return p.value or 0
+
+    def upload(self, name: str, arr, offset_elems: int = 0) -> None:
+        a = np.ascontiguousarray(arr, dtype=_np_dtype(name))
+        check(self.lib.sv_upload(self.h, BUF[name], a.ctypes.data_as(C.c_void_p), offset_elems * 4, a.nbytes))
+
+    def download(self, name: str, count: int, offset_elems: int = 0) -> np.ndarray:
+        out = np.empty(count, dtype=_np_dtype(name))
+        if count:
+            check(self.lib.sv_download(self.h, BUF[name], out.ctypes.data_as(C.c_void_p), offset_elems * 4,
+                                       out.nbytes))
+        return out
+
+    def set_params(self, **kw) -> None:
+        for k, v in kw.items():
+            setattr(self.params, k, type(getattr(self.params, k))(v))
+        check(self.lib.sv_set_params(self.h, C.byref(self.params)))
+
+    # -- physics --------------------------------------------------------------
+    def step(self, num_bodies: int, num_vertices: int, num_hull_verts: int, substeps: int = 10,
+             dt: float = 1.0 / 60.0, gravity: Sequence[float] = (0.0, -9.81, 0.0)) -> None:
+        """Physics.step (src/physics.zig:80-165)."""
+        check(self.lib.sv_physics_step(self.h, num_bodies, num_vertices, num_hull_verts, substeps,
+                                       C.c_float(dt), _fvec(gravity, 3)))
+
+    def read_counters(self) -> tuple[int, int]:
+        """Physics.readCounters (src/physics.zig:213-217)."""
+        out = (C.c_uint32 * 2)()
+        check(self.lib.sv_read_counters(self.h, out))
+        return int(out[0]), int(out[1])
+
+    def status(self) -> int:
+        return int(self.download("ATOMIC_COUNTERS", 3)[2])
+
+    def dispatch(self, step: int, count: int, dt: float = 0.0, gravity: Sequence[float] = (0.0, 0.0, 0.0),
+                 num_bodies: int = 0) -> None:
+        """One physics.comp step by its reference step id."""
+        check(self.lib.sv_physics_dispatch(self.h, step, count, C.c_float(dt), _fvec(gravity, 3), num_bodies))
+
+    # -- LBVH -----------------------------------------------------------------
+    def bvh_build(self, num_tris: int) -> None:
+        """BVH.build (src/bvh.zig:63-172)."""
+        check(self.lib.sv_bvh_build(self.h, num_tris))
+
+    def bvh_build_bodies(self, num_bodies: int) -> None:
+        check(self.lib.sv_bvh_build_bodies(self.h, num_bodies))
+
+    def bvh_dispatch(self, step: int, n: int) -> None:
+        check(self.lib.sv_bvh_dispatch(self.h, step, n))
+
+    # -- diagnostics ----------------------------------------------------------
+    def debug_atan2(self, y: np.ndarray, x: np.ndarray) -> np.ndarray:
+        y = np.ascontiguousarray(y, np.float32)
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.empty_like(y)
+        fp = C.POINTER(C.c_float)
+        check(self.lib.sv_debug_atan2(self.h, y.ctypes.data_as(fp), x.ctypes.data_as(fp), out.ctypes.data_as(fp),
+                                      y.size))
+        return out
+
+    def enable_stage_timing(self, on: bool = True) -> None:
+        check(self.lib.sv_enable_stage_timing(self.h, int(on)))
+
+    def stage_times(self, reset: bool = False) -> dict:
+        cap = 64
+        names = (C.c_char_p * cap)()
+        avg = (C.c_double * cap)()
+        calls = (C.c_uint64 * cap)()
+        n = self.lib.sv_stage_times(self.h, int(reset), names, avg, calls, cap)
+        if n < 0:
+            check(n)
+        return {names[i].decode(): (avg[i], int(calls[i])) for i in range(min(n, cap))}
+
+    def device_info(self) -> tuple[str, str]:
+        name = C.create_string_buffer(256)
+        arch = C.create_string_buffer(64)
+        check(self.lib.sv_device_info(self.h, name, 256, arch, 64))
+        return name.value.decode(), arch.value.decode()
+
+
+class Scene:
+    """Host scene builder with data.zig's construction and layouts."""
+
+    def __init__(self, caps: Caps | None = None, **caps_over):
+        self.lib = _lib.load()
+        self.caps = caps if caps is not None else default_caps(**caps_over)
+        h = C.c_void_p()
+        check(self.lib.sv_scene_create(C.byref(self.caps), C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.sv_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_material(self, density: float, albedo=(0.8, 0.8, 0.8), roughness=0.5, metallic=0.0,
+                     emission=(0.0, 0.0, 0.0), ior=0.0) -> int:
+        out = C.c_uint32()
+        check(self.lib.sv_scene_add_material(self.h, _fvec(albedo, 3), roughness, metallic, _fvec(emission, 3),
+                                             ior, density, C.byref(out)))
+        return out.value
+
+    def add_box(self, center, half, friction=0.5, restitution=0.0, material=0, color=(0.8, 0.8, 0.8)) -> int:
+        out = C.c_uint32()
+        check(self.lib.sv_scene_add_box(self.h, _fvec(center, 3), _fvec(half, 3), _fvec(color, 3), friction,
+                                        restitution, material, C.byref(out)))
+        return out.value
+
+    def add_sphere(self, center, radius, segments=16, friction=0.5, restitution=0.0, material=0,
+                   color=(0.8, 0.8, 0.8)) -> int:
+        out = C.c_uint32()
+        check(self.lib.sv_scene_add_sphere(self.h, _fvec(center, 3), radius, _fvec(color, 3), segments, friction,
+                                           restitution, material, C.byref(out)))
+        return out.value
+
+    def set_body_state(self, body: int, quat=None, vel=None, omega=None) -> None:
+        q = _fvec(quat, 4) if quat is not None else None
+        v = _fvec(vel, 3) if vel is not None else None
+        w = _fvec(omega, 3) if omega is not None else None
+        check(self.lib.sv_scene_set_body_state(self.h, body, q, v, w))
+
+    def build_workload(self, workload: int, count: int = 0, segments: int = 0, seed: int = 1) -> None:
+        check(self.lib.sv_scene_build_workload(self.h, workload, count, segments, seed))
+
+    def counts(self) -> dict:
+        c = SceneCounts()
+        check(self.lib.sv_scene_get_counts(self.h, C.byref(c)))
+        return {k: int(getattr(c, k)) for k, _ in SceneCounts._fields_}
+
+    def array(self, name: str) -> np.ndarray:
+        p, n = C.c_void_p(), C.c_size_t()
+        check(self.lib.sv_scene_array(self.h, BUF[name], C.byref(p), C.byref(n)))
+        dt = _np_dtype(name)
+        cnt = n.value // 4
+        if cnt == 0:
+            return np.zeros(0, dtype=dt)
+        buf = (C.c_char * n.value).from_address(p.value)
+        return np.frombuffer(bytes(buf), dtype=dt).copy()
+
+    def upload(self, ctx: Context) -> None:
+        """Data.upload (src/data.zig:599-655)."""
+        check(self.lib.sv_scene_upload(self.h, ctx.h))
+
+
+def caps_for(scene: Scene, pairs: int, contacts: int, triangles: int | None = None) -> Caps:
+    """Capacities that fit a built scene."""
+    c = scene.counts()
+    return default_caps(max_bodies=max(c["num_bodies"], 1), max_hull_verts=max(c["num_hull_verts"], 1),
+                        max_vertices=max(c["num_vertices"], 1),
+                        max_triangles=max(c["num_triangles"] if triangles is None else triangles, 1),
+                        max_pairs=pairs, max_contacts=contacts,
+                        max_support_entries=max(c["num_support_entries"], 1))

@@ -1,0 +1,298 @@
+// sv_bvh.hip - LBVH build of bvh.comp on gfx950: centroids, centroid
+// bounds, 30-bit Morton codes, stable LSD radix sort (8-bit digits, wave
+// ballot ranking; any digit width gives the same stable order as the
+// reference's eight 4-bit passes), Karras 2012 hierarchy, leaf boxes and the
+// bottom-up refit with release/acquire arrival flags.
+#include "sv_internal.h"
+#include "sv_math.h"
+
+namespace sv {
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int BOUNDS_ITEMS = 8;
+
+template <typename T>
+inline T *B(sv_ctx *c, int id) { return reinterpret_cast<T *>(c->buf[id]); }
+
+// bvh.comp:69-79 (triangles) / body AABB centres (extension).
+__global__ void k_centroids(uint32_t n, const float *__restrict__ verts, const uint32_t *__restrict__ idx,
+                            float *__restrict__ cent, uint32_t *__restrict__ sort_idx, uint32_t *__restrict__ prim) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t i0 = idx[i * 3], i1 = idx[i * 3 + 1], i2 = idx[i * 3 + 2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) cent[i * 3 + k] = (verts[i0 * 3 + k] + verts[i1 * 3 + k] + verts[i2 * 3 + k]) / 3.0f;
+    sort_idx[i] = i;
+    prim[i] = i;
+}
+
+__global__ void k_centroids_bodies(uint32_t n, const float *__restrict__ amin, const float *__restrict__ amax,
+                                   float *__restrict__ cent, uint32_t *__restrict__ sort_idx,
+                                   uint32_t *__restrict__ prim) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int k = 0; k < 3; k++) cent[i * 3 + k] = (amin[i * 3 + k] + amax[i * 3 + k]) * 0.5f;
+    sort_idx[i] = i;
+    prim[i] = i;
+}
+
+// bvh.comp:88-165: min/max of centroids, idle lanes padded with +-1e30.
+__device__ void block_minmax(f3 &mn, f3 &mx) {
+    __shared__ float s[6][BLOCK / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        mn = vmin3(mn, mk(__shfl_xor(mn.x, off, 64), __shfl_xor(mn.y, off, 64), __shfl_xor(mn.z, off, 64)));
+        mx = vmax3(mx, mk(__shfl_xor(mx.x, off, 64), __shfl_xor(mx.y, off, 64), __shfl_xor(mx.z, off, 64)));
+    }
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        s[0][w] = mn.x; s[1][w] = mn.y; s[2][w] = mn.z;
+        s[3][w] = mx.x; s[4][w] = mx.y; s[5][w] = mx.z;
+    }
+    __syncthreads();
+    mn = mk(s[0][0], s[1][0], s[2][0]);
+    mx = mk(s[3][0], s[4][0], s[5][0]);
+#pragma unroll
+    for (int q = 1; q < BLOCK / 64; q++) {
+        mn = vmin3(mn, mk(s[0][q], s[1][q], s[2][q]));
+        mx = vmax3(mx, mk(s[3][q], s[4][q], s[5][q]));
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_bounds_partial(uint32_t n, const float *__restrict__ cent,
+                                                          float *__restrict__ part) {
+    f3 mn = mk(1e30f, 1e30f, 1e30f), mx = mk(-1e30f, -1e30f, -1e30f);
+    uint32_t base = blockIdx.x * (BLOCK * BOUNDS_ITEMS);
+#pragma unroll
+    for (int k = 0; k < BOUNDS_ITEMS; k++) {
+        uint32_t i = base + k * BLOCK + threadIdx.x;
+        if (i < n) {
+            f3 c = ld3(cent, i);
+            mn = vmin3(mn, c);
+            mx = vmax3(mx, c);
+        }
+    }
+    block_minmax(mn, mx);
+    if (threadIdx.x == 0) {
+        float *p = part + blockIdx.x * 6;
+        p[0] = mn.x; p[1] = mn.y; p[2] = mn.z; p[3] = mx.x; p[4] = mx.y; p[5] = mx.z;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_bounds_final(uint32_t nparts, const float *__restrict__ part,
+                                                        float *__restrict__ bounds) {
+    f3 mn = mk(1e30f, 1e30f, 1e30f), mx = mk(-1e30f, -1e30f, -1e30f);
+    for (uint32_t i = threadIdx.x; i < nparts; i += BLOCK) {
+        const float *p = part + i * 6;
+        mn = vmin3(mn, mk(p[0], p[1], p[2]));
+        mx = vmax3(mx, mk(p[3], p[4], p[5]));
+    }
+    block_minmax(mn, mx);
+    if (threadIdx.x == 0) {
+        bounds[0] = mn.x; bounds[1] = mn.y; bounds[2] = mn.z;
+        bounds[3] = mx.x; bounds[4] = mx.y; bounds[5] = mx.z;
+    }
+}
+
+// bvh.comp:32-46
+SVD uint32_t expand_bits(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+SVD uint32_t quant10(float x) { return (uint32_t)gmin(gmax(x * 1024.0f, 0.0f), 1023.0f); }
+
+// bvh.comp:169-183
+__global__ void k_morton(uint32_t n, const float *__restrict__ bounds, const float *__restrict__ cent,
+                         uint32_t *__restrict__ morton) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 bmin = mk(bounds[0], bounds[1], bounds[2]);
+    f3 ext = mk(bounds[3], bounds[4], bounds[5]) - bmin;
+    f3 inv = mk(ext.x > 1e-6f ? 1.0f / ext.x : 0.0f, ext.y > 1e-6f ? 1.0f / ext.y : 0.0f,
+                ext.z > 1e-6f ? 1.0f / ext.z : 0.0f);
+    f3 c = hmul(ld3(cent, i) - bmin, inv);
+    morton[i] = (expand_bits(quant10(c.x)) << 2) | (expand_bits(quant10(c.y)) << 1) | expand_bits(quant10(c.z));
+}
+
+// bvh.comp:48-65: clz with clz(0) = 32; equal keys fall back to the index bits.
+SVD int clz32(uint32_t x) { return x == 0u ? 32 : __builtin_clz(x); }
+SVD int delta(const uint32_t *__restrict__ m, int i, int j, int n) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = m[i], b = m[j];
+    if (a == b) return 32 + clz32((uint32_t)i ^ (uint32_t)j);
+    return clz32(a ^ b);
+}
+
+// bvh.comp:299-348 (Karras 2012): leaves n-1+i, internal nodes 0..n-2.
+__global__ void k_hierarchy(uint32_t nt, const uint32_t *__restrict__ m, uint32_t *__restrict__ left,
+                            uint32_t *__restrict__ right, uint32_t *__restrict__ count,
+                            uint32_t *__restrict__ parent) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = (int)nt;
+    if (i < nt) {
+        uint32_t leaf = nt - 1 + i;
+        left[leaf] = i;
+        count[leaf] = 1;
+        if (nt == 1) parent[0] = 0xFFFFFFFFu;  // single leaf is the root
+    }
+    if (i + 1 >= nt) return;
+    int ii = (int)i;
+    int dl = delta(m, ii, ii - 1, n), dr = delta(m, ii, ii + 1, n);
+    int d = dr > dl ? 1 : -1;
+    int dmin = delta(m, ii, ii - d, n);
+    int lmax = 2;
+    while (delta(m, ii, ii + lmax * d, n) > dmin) lmax *= 2;
+    int l = 0, t = lmax / 2;
+    while (t >= 1) {
+        if (delta(m, ii, ii + (l + t) * d, n) > dmin) l += t;
+        t /= 2;
+    }
+    int j = ii + l * d;
+    int first = ii < j ? ii : j, last = ii < j ? j : ii;
+    int dn = delta(m, first, last, n);
+    int split = first, stride = last - first;
+    while (stride > 1) {
+        stride = (stride + 1) / 2;
+        int mid = split + stride;
+        if (mid < last && delta(m, first, mid, n) > dn) split = mid;
+    }
+    uint32_t lc = (uint32_t)split, rc = (uint32_t)(split + 1);
+    if (first == split) lc = (uint32_t)split + nt - 1;
+    if (last == split + 1) rc = (uint32_t)(split + 1) + nt - 1;
+    left[i] = lc;
+    right[i] = rc;
+    count[i] = 0;
+    parent[lc] = i;
+    parent[rc] = i;
+    if (i == 0) parent[0] = 0xFFFFFFFFu;
+}
+
+// bvh.comp:352-368
+__global__ void k_leaf_tris(uint32_t n, const uint32_t *__restrict__ sort_idx, const float *__restrict__ verts,
+                            const uint32_t *__restrict__ idx, uint32_t *__restrict__ prim,
+                            float *__restrict__ nmin, float *__restrict__ nmax) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t leaf = n - 1 + i, pi = sort_idx[i];
+    prim[i] = pi;
+    f3 a = ld3(verts, idx[pi * 3]), b = ld3(verts, idx[pi * 3 + 1]), c = ld3(verts, idx[pi * 3 + 2]);
+    st3(nmin, leaf, vmin3(vmin3(a, b), c));
+    st3(nmax, leaf, vmax3(vmax3(a, b), c));
+}
+
+__global__ void k_leaf_bodies(uint32_t n, const uint32_t *__restrict__ sort_idx, const float *__restrict__ amin,
+                              const float *__restrict__ amax, uint32_t *__restrict__ prim,
+                              float *__restrict__ nmin, float *__restrict__ nmax) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t leaf = n - 1 + i, pi = sort_idx[i];
+    prim[i] = pi;
+    st3(nmin, leaf, ld3(amin, pi));
+    st3(nmax, leaf, ld3(amax, pi));
+}
+
+// bvh.comp:372-375
+__global__ void k_clear_flags(uint32_t n, uint32_t *flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + 1 < n) flags[i] = 0;
+}
+
+// bvh.comp:379-397.  The flag update is an acquire-release RMW at device
+// scope: the first arrival's writes of its child box are published before
+// its flag, and the second arrival reads both children only after seeing it.
+__global__ void k_propagate(uint32_t n, const uint32_t *__restrict__ parent_of, const uint32_t *__restrict__ left,
+                            const uint32_t *__restrict__ right, uint32_t *flags, float *nmin, float *nmax) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = parent_of[n - 1 + i];
+    // Node prefix lengths strictly decrease toward the root and lie in
+    // [0, 64], so a climb takes at most 65 steps; the bound only guards a
+    // corrupt parent array.
+    for (int depth = 0; p != 0xFFFFFFFFu && depth < 80; depth++) {
+        uint32_t old = __hip_atomic_fetch_or(flags + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0u) return;
+        uint32_t li = left[p], ri = right[p];
+        st3(nmin, p, vmin3(ld3(nmin, li), ld3(nmin, ri)));
+        st3(nmax, p, vmax3(ld3(nmax, li), ld3(nmax, ri)));
+        p = parent_of[p];
+    }
+}
+
+__global__ void k_copy_u32(uint32_t n, const uint32_t *src, uint32_t *dst) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+}  // namespace
+
+void bvh_stage(sv_ctx *c, uint32_t step, uint32_t n, bool bodies) {
+    if (n == 0) return;
+    uint32_t g = div_up(n, BLOCK);
+    hipStream_t s = c->stream;
+    float *cent = B<float>(c, SV_BVH_CENTROIDS);
+    uint32_t *morton = B<uint32_t>(c, SV_BVH_MORTON), *sidx = B<uint32_t>(c, SV_BVH_SORT_INDICES);
+    uint32_t *prim = B<uint32_t>(c, SV_BVH_PRIM_INDICES);
+    float *bounds = B<float>(c, SV_BVH_SCENE_BOUNDS);
+    switch (step) {
+    case 0:
+        if (bodies)
+            hipLaunchKernelGGL(k_centroids_bodies, dim3(g), dim3(BLOCK), 0, s, n, B<float>(c, SV_BODY_AABB_MIN),
+                               B<float>(c, SV_BODY_AABB_MAX), cent, sidx, prim);
+        else
+            hipLaunchKernelGGL(k_centroids, dim3(g), dim3(BLOCK), 0, s, n, B<float>(c, SV_VERTICES),
+                               B<uint32_t>(c, SV_INDICES), cent, sidx, prim);
+        break;
+    case 1: {
+        uint32_t parts = div_up(n, BLOCK * BOUNDS_ITEMS);
+        float *part = B<float>(c, SV_BVH_TEMP);
+        hipLaunchKernelGGL(k_bounds_partial, dim3(parts), dim3(BLOCK), 0, s, n, cent, part);
+        hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(BLOCK), 0, s, parts, part, bounds);
+        break;
+    }
+    case 2:
+        hipLaunchKernelGGL(k_morton, dim3(g), dim3(BLOCK), 0, s, n, bounds, cent, morton);
+        break;
+    case 3: {
+        uint32_t *mt = B<uint32_t>(c, SV_BVH_MORTON_TEMP), *st = B<uint32_t>(c, SV_BVH_SORT_TEMP);
+        int where = radix_sort(c, morton, sidx, mt, st, nullptr, n, 32);
+        if (where == 1) {
+            hipLaunchKernelGGL(k_copy_u32, dim3(g), dim3(BLOCK), 0, s, n, mt, morton);
+            hipLaunchKernelGGL(k_copy_u32, dim3(g), dim3(BLOCK), 0, s, n, st, sidx);
+        }
+        break;
+    }
+    case 4:
+        hipLaunchKernelGGL(k_hierarchy, dim3(g), dim3(BLOCK), 0, s, n, morton, B<uint32_t>(c, SV_BVH_LEFT),
+                           B<uint32_t>(c, SV_BVH_RIGHT), B<uint32_t>(c, SV_BVH_COUNT), B<uint32_t>(c, SV_BVH_PARENT));
+        break;
+    case 5:
+        if (bodies)
+            hipLaunchKernelGGL(k_leaf_bodies, dim3(g), dim3(BLOCK), 0, s, n, sidx, B<float>(c, SV_BODY_AABB_MIN),
+                               B<float>(c, SV_BODY_AABB_MAX), prim, B<float>(c, SV_BVH_AABB_MIN),
+                               B<float>(c, SV_BVH_AABB_MAX));
+        else
+            hipLaunchKernelGGL(k_leaf_tris, dim3(g), dim3(BLOCK), 0, s, n, sidx, B<float>(c, SV_VERTICES),
+                               B<uint32_t>(c, SV_INDICES), prim, B<float>(c, SV_BVH_AABB_MIN),
+                               B<float>(c, SV_BVH_AABB_MAX));
+        break;
+    case 6:
+        hipLaunchKernelGGL(k_clear_flags, dim3(g), dim3(BLOCK), 0, s, n, B<uint32_t>(c, SV_BVH_FLAGS));
+        break;
+    case 7:
+        hipLaunchKernelGGL(k_propagate, dim3(g), dim3(BLOCK), 0, s, n, B<uint32_t>(c, SV_BVH_PARENT),
+                           B<uint32_t>(c, SV_BVH_LEFT), B<uint32_t>(c, SV_BVH_RIGHT), B<uint32_t>(c, SV_BVH_FLAGS),
+                           B<float>(c, SV_BVH_AABB_MIN), B<float>(c, SV_BVH_AABB_MAX));
+        break;
+    default:
+        break;  // 8..11: folded into the stages above
+    }
+}
+
+}  // namespace sv

@@ -1,0 +1,125 @@
+// sv_internal.h - context layout and the launchers each kernel file exports
+// to the C ABI layer (sv_capi.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/seevr.h"
+
+namespace sv {
+
+// Internal scratch buffers (not part of the reference binding table).
+struct Scratch {
+    uint32_t *hrange = nullptr;   // uint[2N] hull (start, count) read by the kernels
+    float *rot = nullptr;         // float[9N] per-body rotation columns of this substep
+    float4 *bpbox = nullptr;      // float4[2N] broadphase boxes: (min, static), (max, 0)
+    uint32_t *row_count = nullptr;  // uint[N + 1]
+    uint32_t *row_off = nullptr;    // uint[N + 1]
+    uint32_t *np_flag = nullptr;    // uint[P] narrowphase hit flags
+    uint32_t *np_idx = nullptr;     // uint[P] contact slot of each hit
+    float *np_res = nullptr;        // float[8P] pos, normal, depth per pair
+    float *cprep = nullptr;         // float[CPREP_WORDS * C] solver constants
+    uint32_t *inc_key[2] = {nullptr, nullptr};  // uint[2C] incidence sort keys
+    uint32_t *inc_val[2] = {nullptr, nullptr};  // uint[2C] incidence sort values
+    uint32_t *deg = nullptr;        // uint[N + 1] dynamic contacts per body
+    uint32_t *deg_off = nullptr;    // uint[N + 1]
+    uint32_t *rank = nullptr;       // uint[2C] rank of each incidence in its body
+    uint32_t *ticket = nullptr;     // uint[N] completed updates per body (solver)
+    uint32_t *solve_ctl = nullptr;  // uint[4]: [0] abort flag, [1] scratch
+    uint32_t *scan_part = nullptr;  // scan partials
+    uint32_t scan_part_cap = 0;
+    uint32_t *sort_hist = nullptr;  // radix histograms
+    uint32_t sort_hist_cap = 0;
+    uint32_t *misc = nullptr;       // uint[64] small device scalars
+};
+
+// Words per contact in Scratch::cprep (see sv_pgs.hip).
+constexpr int CPREP_WORDS = 20;
+
+}  // namespace sv
+
+struct sv_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    sv_caps caps{};
+    sv_params params{};
+    void *buf[SV_BUF_COUNT] = {};
+    size_t bytes[SV_BUF_COUNT] = {};
+    sv::Scratch s;
+    // stage timing
+    bool timing = false;
+    std::vector<std::string> stage_names;
+    std::vector<double> stage_ms;
+    std::vector<uint64_t> stage_calls;
+    std::vector<hipEvent_t> ev_pool;
+    struct Pending { int stage; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    size_t ev_next = 0;
+};
+
+namespace sv {
+
+// misc[] slots
+enum : int {
+    MISC_NP = 0,       // pairs to narrow (min(counters[0], max_pairs))
+    MISC_NC = 1,       // contacts to solve (min(counters[1], max_contacts))
+    MISC_NINC = 2,     // incidences (2 * NC)
+    MISC_TMP = 3,
+    MISC_SCAN_TOTAL = 4,
+    MISC_DEG_TOTAL = 5,
+};
+
+// ---- sv_prims.hip ----
+// Exclusive scan of n = min(*n_dev, n_max) (or n_max when n_dev == null)
+// uint32 values; writes the total to *total_dev when non-null.
+void scan_exclusive(sv_ctx *c, const uint32_t *in, uint32_t *out, const uint32_t *n_dev,
+                    uint32_t n_max, uint32_t *total_dev);
+// Stable LSD radix sort of (key, val) by the low `key_bits` bits of key, for
+// n = min(*n_dev, n_max) elements (n_max when n_dev == null).  Ping-pongs
+// between (k0, v0) and (k1, v1); returns 0 when the result ends in (k0, v0)
+// and 1 when in (k1, v1).
+int radix_sort(sv_ctx *c, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1,
+               const uint32_t *n_dev, uint32_t n_max, int key_bits);
+void fill_u32(sv_ctx *c, uint32_t *p, uint32_t value, size_t n);
+void ensure_u32(uint32_t *&p, uint32_t &cap, uint32_t need);
+// Allocate scan/sort scratch for the largest sizes a context can use.
+void prims_reserve(sv_ctx *c, uint32_t max_scan_n, uint32_t max_sort_n);
+
+// ---- sv_physics.hip ----
+void launch_hull_ranges(sv_ctx *c, uint32_t n);
+void launch_gravity(sv_ctx *c, uint32_t n, float dt, const float g[3]);
+void launch_aabb(sv_ctx *c, uint32_t n);
+void launch_broadphase(sv_ctx *c, uint32_t n);
+void launch_narrowphase(sv_ctx *c);
+void launch_clear_lambda(sv_ctx *c);
+void launch_integrate(sv_ctx *c, uint32_t n, float dt);
+void launch_update_verts(sv_ctx *c, uint32_t nv, uint32_t nb);
+void launch_clear_counters(sv_ctx *c);
+void launch_debug_atan2(sv_ctx *c, const float *y, const float *x, float *out, uint32_t n);
+void launch_rot(sv_ctx *c, uint32_t n);
+void launch_pack_boxes(sv_ctx *c, uint32_t n);
+
+// ---- sv_pgs.hip ----
+void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt);
+
+// ---- sv_bvh.hip ----
+void bvh_stage(sv_ctx *c, uint32_t step, uint32_t n, bool bodies);
+
+// ---- stage timing (sv_capi.cpp) ----
+int timing_begin(sv_ctx *c, const char *name);
+void timing_end(sv_ctx *c, int token);
+
+struct StageTimer {
+    sv_ctx *c;
+    int tok;
+    StageTimer(sv_ctx *ctx, const char *name) : c(ctx), tok(timing_begin(ctx, name)) {}
+    ~StageTimer() { timing_end(c, tok); }
+};
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace sv
+
+#define SV_LAUNCH_CHECK() (void)hipGetLastError()

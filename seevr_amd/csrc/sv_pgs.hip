@@ -1,0 +1,269 @@
+// sv_pgs.hip - projected Gauss-Seidel of physics.comp:500-583 in the
+// reference's sequential contact order, bit for bit, on many lanes.
+//
+// The serial sweep visits items (iteration t, contact c) in order.  Item
+// (t, c) reads and writes the velocities of its two bodies only; a body with
+// zero inverse mass and zero inverse inertia is never changed by an update
+// (v - 0*imp == v for finite imp), so it is read-only and orders nothing.
+// Two items therefore commute unless they share a dynamic body, and running
+// every item after its predecessors on each of its dynamic bodies yields
+// the exact serial result.  Each body keeps a ticket counter of completed
+// updates; item (t, c) may run when ticket[a] == t*deg(a) + rank_a(c) (and
+// the same for b), where deg(a) counts a's non-skipped contacts and
+// rank_a(c) is c's position among them.  rank/deg come from a stable radix
+// sort of the (body, contact) incidences.
+//
+// One workgroup of 1024 lanes executes the items in order, 64 per wave
+// round, so every data hand-off stays inside one CU (workgroup-scope
+// release/acquire).  The earliest unfinished item always has all its
+// predecessors done, so the schedule cannot deadlock; a wave that makes no
+// progress for 2 s (a broken invariant) raises an abort flag that ends the
+// kernel and the host reports SV_E_TIMEOUT.
+#include "sv_internal.h"
+#include "sv_math.h"
+
+namespace sv {
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int SOLVE_THREADS = 1024;
+constexpr int SOLVE_WAVES = SOLVE_THREADS / 64;
+constexpr uint64_t STALL_TICKS = 200000000ull;  // 2 s of the 100 MHz wall clock
+
+template <typename T>
+inline T *B(sv_ctx *c, int id) { return reinterpret_cast<T *>(c->buf[id]); }
+
+// cprep word offsets
+enum : int {
+    P_N = 0, P_RA = 3, P_RB = 6, P_KN = 9, P_BIAS = 10, P_MU = 11, P_IMA = 12, P_IMB = 13,
+    P_A = 14, P_B = 15, P_FLAGS = 16
+};
+enum : uint32_t { F_SKIP = 1u, F_DYN_A = 2u, F_DYN_B = 4u };
+
+SVD f3 apply_inv_I(const m3 &R, f3 inv_i, f3 v) { return mvec(R, hmul(inv_i, mtvec(R, v))); }
+
+__global__ void k_set_ninc(uint32_t *misc) { misc[MISC_NINC] = 2u * misc[MISC_NC]; }
+
+// contacts to solve: min(counters[1], max_contacts) (physics.comp:502)
+__global__ void k_solve_count(const uint32_t *counters, uint32_t *misc, uint32_t max_contacts) {
+    uint32_t t = counters[1];
+    misc[MISC_NC] = t < max_contacts ? t : max_contacts;
+}
+
+// Per-contact constants of the sweep.  The bias uses the velocities before
+// the sweep (physics.comp:504-521); k_n depends only on positions and
+// orientations, which the sweep does not change, so it equals the value the
+// serial loop recomputes every iteration (physics.comp:544-546).
+__global__ void __launch_bounds__(BLOCK) k_prep(
+    const uint32_t *__restrict__ misc, uint32_t n_bodies, float dt, float baumgarte, float slop,
+    const uint32_t *__restrict__ cga, const uint32_t *__restrict__ cgb, const uint32_t *__restrict__ gbody,
+    const float *__restrict__ cpos, const float *__restrict__ cnrm, const float *__restrict__ cpen,
+    const float *__restrict__ pos, const float *__restrict__ vel, const float *__restrict__ omega,
+    const float *__restrict__ inv_mass, const float *__restrict__ inv_i, const float *__restrict__ rot,
+    const float *__restrict__ fric, const float *__restrict__ rest, float *__restrict__ prep,
+    uint32_t *__restrict__ key, uint32_t *__restrict__ val, uint32_t *__restrict__ deg) {
+    uint32_t nc = misc[MISC_NC];
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        uint32_t ga = cga[c], gb = cgb[c];
+        uint32_t a = gbody[ga], b = gbody[gb];
+        f3 n = ld3(cnrm, c), cp = ld3(cpos, c);
+        f3 ra = cp - ld3(pos, a), rb = cp - ld3(pos, b);
+        f3 va = ld3(vel, a) + cross3(ld3(omega, a), ra);
+        f3 vb = ld3(vel, b) + cross3(ld3(omega, b), rb);
+        float vn = dot3(vb - va, n);
+        float e = gmax(rest[ga], rest[gb]);
+        float bias = baumgarte * gmax(cpen[c] - slop, 0.0f) / dt + e * gmax(-vn, 0.0f);
+        float ima = inv_mass[a], imb = inv_mass[b];
+        f3 iia = ld3(inv_i, a), iib = ld3(inv_i, b);
+        m3 Ra = ld_rot(rot, a), Rb = ld_rot(rot, b);
+        f3 ran = cross3(ra, n), rbn = cross3(rb, n);
+        float kn = ima + imb + dot3(ran, apply_inv_I(Ra, iia, ran)) + dot3(rbn, apply_inv_I(Rb, iib, rbn));
+        float mu = __builtin_sqrtf(fric[ga] * fric[gb]);
+        bool skip = (ima == 0.0f && imb == 0.0f) || a == b;
+        bool dyn_a = !(ima == 0.0f && iia.x == 0.0f && iia.y == 0.0f && iia.z == 0.0f);
+        bool dyn_b = !(imb == 0.0f && iib.x == 0.0f && iib.y == 0.0f && iib.z == 0.0f);
+        uint32_t flags = (skip ? F_SKIP : 0u) | (dyn_a ? F_DYN_A : 0u) | (dyn_b ? F_DYN_B : 0u);
+        float *P = prep + (size_t)c * CPREP_WORDS;
+        P[P_N + 0] = n.x; P[P_N + 1] = n.y; P[P_N + 2] = n.z;
+        P[P_RA + 0] = ra.x; P[P_RA + 1] = ra.y; P[P_RA + 2] = ra.z;
+        P[P_RB + 0] = rb.x; P[P_RB + 1] = rb.y; P[P_RB + 2] = rb.z;
+        P[P_KN] = kn;
+        P[P_BIAS] = bias;
+        P[P_MU] = mu;
+        P[P_IMA] = ima;
+        P[P_IMB] = imb;
+        P[P_A] = __uint_as_float(a);
+        P[P_B] = __uint_as_float(b);
+        P[P_FLAGS] = __uint_as_float(flags);
+        bool ia = !skip && dyn_a, ib = !skip && dyn_b;
+        key[2 * c] = ia ? a : n_bodies;
+        val[2 * c] = 2 * c;
+        key[2 * c + 1] = ib ? b : n_bodies;
+        val[2 * c + 1] = 2 * c + 1;
+        if (ia) atomicAdd(&deg[a], 1u);
+        if (ib) atomicAdd(&deg[b], 1u);
+    }
+}
+
+// rank of each incidence inside its body's segment of the sorted list.
+__global__ void __launch_bounds__(BLOCK) k_rank(const uint32_t *__restrict__ misc, uint32_t n_bodies,
+                                                const uint32_t *__restrict__ key, const uint32_t *__restrict__ val,
+                                                const uint32_t *__restrict__ deg_off, uint32_t *__restrict__ rank) {
+    uint32_t ninc = misc[MISC_NINC];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < ninc; p += gridDim.x * blockDim.x) {
+        uint32_t k = key[p];
+        if (k < n_bodies) rank[val[p]] = p - deg_off[k];
+    }
+}
+
+SVD uint32_t ld_ticket(const uint32_t *t) {
+    return __hip_atomic_load(t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One item of the sweep: physics.comp:525-580 for contact c.
+SVD void pgs_item(const float *__restrict__ P, uint32_t c, float *__restrict__ vel, float *__restrict__ omega,
+                  const float *__restrict__ rot, const float *__restrict__ inv_i, float *__restrict__ lambda,
+                  uint32_t a, uint32_t b, bool dyn_a, bool dyn_b) {
+    f3 n = mk(P[P_N], P[P_N + 1], P[P_N + 2]);
+    f3 ra = mk(P[P_RA], P[P_RA + 1], P[P_RA + 2]);
+    f3 rb = mk(P[P_RB], P[P_RB + 1], P[P_RB + 2]);
+    float kn = P[P_KN], bias = P[P_BIAS], mu = P[P_MU], ima = P[P_IMA], imb = P[P_IMB];
+    m3 Ra = ld_rot(rot, a), Rb = ld_rot(rot, b);
+    f3 iia = ld3(inv_i, a), iib = ld3(inv_i, b);
+    f3 la = ld3(vel, a), wa = ld3(omega, a);
+    f3 lb = ld3(vel, b), wb = ld3(omega, b);
+
+    f3 va = la + cross3(wa, ra);
+    f3 vb = lb + cross3(wb, rb);
+    float vn = dot3(vb - va, n);
+    float dln = (-vn + bias) / kn;
+    float old = lambda[c];
+    float nl = gmax(old + dln, 0.0f);
+    dln = nl - old;
+    lambda[c] = nl;
+    f3 imp = dln * n;
+    la = la - ima * imp;
+    lb = lb + imb * imp;
+    wa = wa - apply_inv_I(Ra, iia, cross3(ra, imp));
+    wb = wb + apply_inv_I(Rb, iib, cross3(rb, imp));
+
+    va = la + cross3(wa, ra);
+    vb = lb + cross3(wb, rb);
+    f3 dv = vb - va;
+    f3 vt = dv - dot3(dv, n) * n;
+    float vtl = len3(vt);
+    if (vtl > 1e-6f) {
+        f3 td = mk(vt.x / vtl, vt.y / vtl, vt.z / vtl);
+        f3 rat = cross3(ra, td), rbt = cross3(rb, td);
+        float kt = ima + imb + dot3(rat, apply_inv_I(Ra, iia, rat)) + dot3(rbt, apply_inv_I(Rb, iib, rbt));
+        float dlt = gmin(vtl / kt, mu * nl);
+        f3 impt = dlt * td;
+        la = la + ima * impt;
+        lb = lb - imb * impt;
+        wa = wa + apply_inv_I(Ra, iia, cross3(ra, impt));
+        wb = wb - apply_inv_I(Rb, iib, cross3(rb, impt));
+    }
+    if (dyn_a) { st3(vel, a, la); st3(omega, a, wa); }
+    if (dyn_b) { st3(vel, b, lb); st3(omega, b, wb); }
+}
+
+__global__ void __launch_bounds__(SOLVE_THREADS) k_solve(
+    const uint32_t *__restrict__ misc, uint32_t iters, const float *__restrict__ prep,
+    const uint32_t *__restrict__ rank, const uint32_t *__restrict__ deg, uint32_t *ticket, float *vel, float *omega,
+    const float *__restrict__ rot, const float *__restrict__ inv_i, float *lambda, uint32_t *ctl,
+    uint32_t *counters) {
+    const uint32_t nc = misc[MISC_NC];
+    const uint64_t total = (uint64_t)iters * nc;
+    const int lane = threadIdx.x & 63;
+    const uint32_t w = threadIdx.x >> 6;
+    for (uint64_t chunk = w; chunk * 64 < total; chunk += SOLVE_WAVES) {
+        uint64_t q = chunk * 64 + lane;
+        bool pending = q < total;
+        uint32_t c = 0, a = 0, b = 0, ta = 0, tb = 0;
+        bool dyn_a = false, dyn_b = false;
+        const float *P = prep;
+        if (pending) {
+            uint32_t t = (uint32_t)(q / nc);
+            c = (uint32_t)(q - (uint64_t)t * nc);
+            P = prep + (size_t)c * CPREP_WORDS;
+            uint32_t flags = __float_as_uint(P[P_FLAGS]);
+            if (flags & F_SKIP) {
+                pending = false;
+            } else {
+                a = __float_as_uint(P[P_A]);
+                b = __float_as_uint(P[P_B]);
+                dyn_a = flags & F_DYN_A;
+                dyn_b = flags & F_DYN_B;
+                if (dyn_a) ta = t * deg[a] + rank[2 * c];
+                if (dyn_b) tb = t * deg[b] + rank[2 * c + 1];
+            }
+        }
+        uint64_t last = wall_clock64();
+        while (__any(pending)) {
+            bool progressed = false;
+            if (pending) {
+                bool ready = (!dyn_a || ld_ticket(ticket + a) == ta) && (!dyn_b || ld_ticket(ticket + b) == tb);
+                if (ready) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    pgs_item(P, c, vel, omega, rot, inv_i, lambda, a, b, dyn_a, dyn_b);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    if (dyn_a) __hip_atomic_fetch_add(ticket + a, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (dyn_b) __hip_atomic_fetch_add(ticket + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    pending = false;
+                    progressed = true;
+                }
+            }
+            if (__any(progressed)) {
+                last = wall_clock64();
+            } else {
+                if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+                if (wall_clock64() - last > STALL_TICKS) {
+                    __hip_atomic_store(ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (lane == 0) atomicOr(counters + 2, (uint32_t)SV_STATUS_SOLVER_TIMEOUT);
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// Steps 5 (physics.comp:500-583) for the contacts counters[1] names.
+// Reads lambda_n as the starting impulses (step 4 clears them).
+void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
+    uint32_t *counters = B<uint32_t>(c, SV_ATOMIC_COUNTERS);
+    uint32_t mc = c->caps.max_contacts;
+    if (mc == 0 || n_bodies == 0) return;
+    hipLaunchKernelGGL(k_solve_count, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, mc);
+    launch_rot(c, n_bodies);
+    fill_u32(c, c->s.deg, 0u, n_bodies);
+    uint32_t grid = div_up(mc, BLOCK);
+    if (grid > 8192) grid = 8192;
+    hipLaunchKernelGGL(k_prep, dim3(grid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, dt, c->params.baumgarte,
+                       c->params.slop, B<uint32_t>(c, SV_CONTACT_GEOM_A), B<uint32_t>(c, SV_CONTACT_GEOM_B),
+                       B<uint32_t>(c, SV_GEOM_BODY_IDX), B<float>(c, SV_CONTACT_POS), B<float>(c, SV_CONTACT_NORMAL),
+                       B<float>(c, SV_CONTACT_PENETRATION), B<float>(c, SV_BODY_POS), B<float>(c, SV_BODY_VEL),
+                       B<float>(c, SV_BODY_OMEGA), B<float>(c, SV_BODY_INV_MASS), B<float>(c, SV_BODY_INV_INERTIA),
+                       c->s.rot, B<float>(c, SV_GEOM_FRICTION), B<float>(c, SV_GEOM_RESTITUTION), c->s.cprep,
+                       c->s.inc_key[0], c->s.inc_val[0], c->s.deg);
+    hipLaunchKernelGGL(k_set_ninc, dim3(1), dim3(1), 0, c->stream, c->s.misc);
+    int key_bits = 32 - __builtin_clz(n_bodies);  // keys are in [0, n_bodies]
+    int where = radix_sort(c, c->s.inc_key[0], c->s.inc_val[0], c->s.inc_key[1], c->s.inc_val[1],
+                           c->s.misc + MISC_NINC, 2u * mc, key_bits);
+    scan_exclusive(c, c->s.deg, c->s.deg_off, nullptr, n_bodies, nullptr);
+    uint32_t rgrid = div_up(2u * (uint64_t)mc, BLOCK);
+    if (rgrid > 8192) rgrid = 8192;
+    hipLaunchKernelGGL(k_rank, dim3(rgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.inc_key[where],
+                       c->s.inc_val[where], c->s.deg_off, c->s.rank);
+    fill_u32(c, c->s.ticket, 0u, n_bodies);
+    fill_u32(c, c->s.solve_ctl, 0u, 4);
+    hipLaunchKernelGGL(k_solve, dim3(1), dim3(SOLVE_THREADS), 0, c->stream, c->s.misc, c->params.pgs_iterations,
+                       c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, B<float>(c, SV_BODY_VEL),
+                       B<float>(c, SV_BODY_OMEGA), c->s.rot, B<float>(c, SV_BODY_INV_INERTIA),
+                       B<float>(c, SV_CONTACT_LAMBDA_N), c->s.solve_ctl, counters);
+}
+
+}  // namespace sv

@@ -1,0 +1,428 @@
+// sv_scene.cpp - host-side scene construction with the layouts and float
+// arithmetic of seeVR's data.zig (addMaterial / addBox / addSphere /
+// buildSupportLookup / upload), plus the seeded benchmark scenes.
+// Compiled without floating-point contraction: Zig's default strict float
+// mode evaluates each expression left to right with one rounding per
+// operation, and so does this file.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/seevr.h"
+
+#pragma STDC FP_CONTRACT OFF
+
+namespace {
+
+constexpr uint32_t kLookupThreshold = 32;  // data.zig:22
+constexpr uint32_t kBins = 64;             // data.zig:21
+constexpr uint32_t kEntry = 3 * kBins;     // data.zig:23
+
+}  // namespace
+
+struct sv_scene {
+    sv_caps caps{};
+    uint32_t num_vertices = 0, num_triangles = 0, num_bodies = 0, num_geoms = 0;
+    uint32_t num_hull_verts = 0, num_materials = 0, num_support_entries = 0;
+    std::vector<float> vertices, orig_vertices;
+    std::vector<uint32_t> indices;
+    std::vector<float> body_pos, body_quat, body_vel, body_omega, body_inv_mass, body_inertia, body_inv_inertia;
+    std::vector<uint32_t> body_vert_start, body_vert_count;
+    std::vector<uint32_t> geom_body_idx;
+    std::vector<float> geom_data, geom_friction, geom_restitution;
+    std::vector<uint32_t> geom_vert_start, geom_vert_count, geom_hull_range;
+    std::vector<float> hull_verts;
+    std::vector<uint32_t> support_offset, support_lookup;
+    std::vector<float> material_density;
+};
+
+namespace {
+
+int bad(const char *) { return SV_E_INVALID; }
+
+// data.zig:299-353
+void build_support_lookup(sv_scene *s, uint32_t gi, uint32_t hs, uint32_t hc) {
+    if (hc <= kLookupThreshold) {
+        s->support_offset[gi] = 0;
+        return;
+    }
+    const uint32_t start = s->num_support_entries == 0 ? 1u : s->num_support_entries;
+    s->support_offset[gi] = start;
+    if (s->support_lookup.size() < (size_t)start + kEntry) s->support_lookup.resize((size_t)start + kEntry, 0);
+    static const float perps[3][2][3] = {{{0, 1, 0}, {0, 0, 1}}, {{1, 0, 0}, {0, 0, 1}}, {{1, 0, 0}, {0, 1, 0}}};
+    const float pi = 3.14159265358979323846f;  // std.math.pi as f32
+    const float *hv = s->hull_verts.data() + (size_t)hs * 3;
+    for (uint32_t ax = 0; ax < 3; ax++) {
+        const float *p1 = perps[ax][0], *p2 = perps[ax][1];
+        for (uint32_t bin = 0; bin < kBins; bin++) {
+            float angle = (float)bin / (float)kBins * 2.0f * pi;
+            float ca = cosf(angle), sa = sinf(angle);
+            float dx = ca * p1[0] + sa * p2[0];
+            float dy = ca * p1[1] + sa * p2[1];
+            float dz = ca * p1[2] + sa * p2[2];
+            float best = -1e10f;
+            uint32_t best_i = 0;
+            for (uint32_t v = 0; v < hc; v++) {
+                const float *h = hv + (size_t)v * 3;
+                float d = dx * h[0] + dy * h[1] + dz * h[2];
+                if (d > best) {
+                    best = d;
+                    best_i = v;
+                }
+            }
+            s->support_lookup[start + ax * kBins + bin] = best_i;
+        }
+    }
+    s->num_support_entries = start + kEntry;
+}
+
+bool room(const sv_scene *s, uint32_t verts, uint32_t tris, uint32_t hull) {
+    const sv_caps &k = s->caps;
+    return s->num_bodies < k.max_bodies && (uint64_t)s->num_vertices + verts <= k.max_vertices &&
+           (uint64_t)s->num_triangles + tris <= k.max_triangles &&
+           (uint64_t)s->num_hull_verts + hull <= k.max_hull_verts &&
+           (hull <= kLookupThreshold ||
+            (uint64_t)(s->num_support_entries ? s->num_support_entries : 1u) + kEntry <= k.max_support_entries);
+}
+
+void push_body(sv_scene *s, const float c[3], float inv_mass, const float inertia[3], const float inv_i[3]) {
+    for (int k = 0; k < 3; k++) {
+        s->body_pos.push_back(c[k]);
+        s->body_vel.push_back(0.0f);
+        s->body_omega.push_back(0.0f);
+        s->body_inertia.push_back(inertia[k]);
+        s->body_inv_inertia.push_back(inv_i[k]);
+    }
+    s->body_quat.insert(s->body_quat.end(), {0.0f, 0.0f, 0.0f, 1.0f});
+    s->body_inv_mass.push_back(inv_mass);
+    s->body_vert_start.push_back(0);
+    s->body_vert_count.push_back(0);
+    s->num_bodies++;
+}
+
+void push_geom(sv_scene *s, uint32_t bi, uint32_t hv, uint32_t hc, float fr, float re, uint32_t vs, uint32_t vc) {
+    s->geom_body_idx.push_back(bi);
+    s->geom_data.insert(s->geom_data.end(), {(float)hv, (float)hc, 0, 0, 0, 0, 0, 0});
+    s->geom_friction.push_back(fr);
+    s->geom_restitution.push_back(re);
+    s->geom_vert_start.push_back(vs);
+    s->geom_vert_count.push_back(vc);
+    s->geom_hull_range.push_back(hv);
+    s->geom_hull_range.push_back(hc);
+    s->support_offset.push_back(0);
+}
+
+// ---- seeded workloads -----------------------------------------------------
+struct Rng {
+    uint64_t x;
+    uint64_t next() {  // splitmix64
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    float uni() { return (float)(next() >> 40) * (1.0f / 16777216.0f); }  // [0, 1)
+    float uni(float lo, float hi) { return lo + (hi - lo) * uni(); }
+};
+
+// Uniform random rotation (Shoemake, Graphics Gems III), as (x, y, z, w).
+void random_quat(Rng &r, float q[4]) {
+    double u1 = r.uni(), u2 = r.uni(), u3 = r.uni();
+    double a = std::sqrt(1.0 - u1), b = std::sqrt(u1), t1 = 2.0 * M_PI * u2, t2 = 2.0 * M_PI * u3;
+    q[0] = (float)(a * std::sin(t1));
+    q[1] = (float)(a * std::cos(t1));
+    q[2] = (float)(b * std::sin(t2));
+    q[3] = (float)(b * std::cos(t2));
+}
+
+int add_random_body(sv_scene *s, Rng &r, float p_sphere, float hlo, float hhi, float rlo, float rhi,
+                    uint32_t segs, const float pos[3], uint32_t mat) {
+    static const float col[3] = {0.8f, 0.8f, 0.8f};
+    uint32_t bi;
+    int rc;
+    if (r.uni() < p_sphere) {
+        float rad = r.uni(rlo, rhi);
+        rc = sv_scene_add_sphere(s, pos, rad, col, segs, 0.5f, 0.1f, mat, &bi);
+    } else {
+        float h[3] = {r.uni(hlo, hhi), r.uni(hlo, hhi), r.uni(hlo, hhi)};
+        rc = sv_scene_add_box(s, pos, h, col, 0.5f, 0.1f, mat, &bi);
+    }
+    if (rc) return rc;
+    float q[4];
+    random_quat(r, q);
+    static const float zero[3] = {0, 0, 0};
+    return sv_scene_set_body_state(s, bi, q, zero, zero);
+}
+
+int static_box(sv_scene *s, float cx, float cy, float cz, float hx, float hy, float hz, uint32_t mat) {
+    static const float col[3] = {0.35f, 0.35f, 0.30f};
+    float c[3] = {cx, cy, cz}, h[3] = {hx, hy, hz};
+    uint32_t bi;
+    return sv_scene_add_box(s, c, h, col, 0.5f, 0.1f, mat, &bi);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sv_scene_create(const sv_caps *caps, sv_scene **out) {
+    if (!caps || !out) return bad("null");
+    sv_scene *s = new (std::nothrow) sv_scene();
+    if (!s) return SV_E_NOMEM;
+    s->caps = *caps;
+    *out = s;
+    return SV_OK;
+}
+
+int sv_scene_destroy(sv_scene *s) {
+    delete s;
+    return SV_OK;
+}
+
+// data.zig:356-370 (albedo, roughness, metallic, emission and ior feed only
+// the path tracer; density is the one physics input)
+int sv_scene_add_material(sv_scene *s, const float *, float, float, const float *, float, float density,
+                          uint32_t *out) {
+    if (!s) return bad("null");
+    if (s->num_materials >= 256) return SV_E_CAPACITY;  // data.zig:18
+    s->material_density.push_back(density);
+    if (out) *out = s->num_materials;
+    s->num_materials++;
+    return SV_OK;
+}
+
+// data.zig:392-485
+int sv_scene_add_box(sv_scene *s, const float c[3], const float h[3], const float *, float friction,
+                     float restitution, uint32_t material, uint32_t *out) {
+    if (!s || !c || !h || material >= s->num_materials) return bad("arg");
+    if (!room(s, 8, 12, 8)) return SV_E_CAPACITY;
+    const uint32_t vs = s->num_vertices, ts = s->num_triangles, bi = s->num_bodies;
+    const float bv[8][3] = {{-h[0], -h[1], -h[2]}, {h[0], -h[1], -h[2]}, {h[0], h[1], -h[2]}, {-h[0], h[1], -h[2]},
+                            {-h[0], -h[1], h[2]},  {h[0], -h[1], h[2]},  {h[0], h[1], h[2]},  {-h[0], h[1], h[2]}};
+    static const uint32_t bf[12][3] = {{0, 2, 1}, {0, 3, 2}, {5, 7, 4}, {5, 6, 7}, {4, 3, 0}, {4, 7, 3},
+                                       {1, 6, 5}, {1, 2, 6}, {3, 6, 2}, {3, 7, 6}, {4, 1, 5}, {4, 0, 1}};
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 3; j++) {
+            s->orig_vertices.push_back(bv[i][j]);
+            s->vertices.push_back(c[j] + bv[i][j]);
+        }
+    for (int i = 0; i < 12; i++)
+        for (int j = 0; j < 3; j++) s->indices.push_back(vs + bf[i][j]);
+    s->num_vertices += 8;
+    s->num_triangles += 12;
+    const float density = s->material_density[material];
+    const float volume = h[0] * 2 * h[1] * 2 * h[2] * 2;
+    const float mass = density * volume;
+    float inertia[3] = {0, 0, 0}, inv_i[3] = {0, 0, 0};
+    if (mass > 0) {
+        const float wx = h[0] * 2, wy = h[1] * 2, wz = h[2] * 2;
+        inertia[0] = mass / 12.0f * (wy * wy + wz * wz);
+        inertia[1] = mass / 12.0f * (wx * wx + wz * wz);
+        inertia[2] = mass / 12.0f * (wx * wx + wy * wy);
+        for (int k = 0; k < 3; k++) inv_i[k] = 1.0f / inertia[k];
+    }
+    push_body(s, c, mass > 0 ? 1.0f / mass : 0.0f, inertia, inv_i);
+    const uint32_t hv = s->num_hull_verts;
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 3; j++) s->hull_verts.push_back(bv[i][j]);
+    s->num_hull_verts += 8;
+    const uint32_t gi = s->num_geoms;
+    push_geom(s, bi, hv, 8, friction, restitution, vs, 8);
+    build_support_lookup(s, gi, hv, 8);
+    s->num_geoms++;
+    if (out) *out = bi;
+    return SV_OK;
+}
+
+// data.zig:487-597
+int sv_scene_add_sphere(sv_scene *s, const float c[3], float radius, const float *, uint32_t segments,
+                        float friction, float restitution, uint32_t material, uint32_t *out) {
+    if (!s || !c || segments == 0 || material >= s->num_materials) return bad("arg");
+    const uint32_t nv = (segments + 1) * (segments + 1), nt = segments * segments * 2;
+    if (!room(s, nv, nt, nv)) return SV_E_CAPACITY;
+    const uint32_t vs = s->num_vertices, ts = s->num_triangles, bi = s->num_bodies;
+    const float pi = 3.14159265f;
+    const float segs = (float)segments;
+    for (uint32_t i = 0; i <= segments; i++) {
+        const float fi = (float)i;
+        const float lat = pi * fi / segs;
+        const float sl = sinf(lat), cl = cosf(lat);
+        for (uint32_t j = 0; j <= segments; j++) {
+            const float fj = (float)j;
+            const float lon = 2.0f * pi * fj / segs;
+            const float lx = radius * cosf(lon) * sl;
+            const float ly = radius * cl;
+            const float lz = radius * sinf(lon) * sl;
+            s->orig_vertices.insert(s->orig_vertices.end(), {lx, ly, lz});
+            s->vertices.insert(s->vertices.end(), {c[0] + lx, c[1] + ly, c[2] + lz});
+        }
+    }
+    s->num_vertices += nv;
+    for (uint32_t i = 0; i < segments; i++)
+        for (uint32_t j = 0; j < segments; j++) {
+            const uint32_t cur = vs + i * (segments + 1) + j, nxt = cur + segments + 1;
+            s->indices.insert(s->indices.end(), {cur, nxt, cur + 1, cur + 1, nxt, nxt + 1});
+        }
+    (void)ts;
+    s->num_triangles += nt;
+    const float density = s->material_density[material];
+    const float volume = (4.0f / 3.0f) * pi * radius * radius * radius;
+    const float mass = density * volume;
+    float inertia[3] = {0, 0, 0}, inv_i[3] = {0, 0, 0};
+    if (mass > 0) {
+        const float I = 0.4f * mass * radius * radius;
+        for (int k = 0; k < 3; k++) {
+            inertia[k] = I;
+            inv_i[k] = 1.0f / I;
+        }
+    }
+    push_body(s, c, mass > 0 ? 1.0f / mass : 0.0f, inertia, inv_i);
+    const uint32_t hv = s->num_hull_verts;
+    s->hull_verts.insert(s->hull_verts.end(), s->orig_vertices.begin() + (size_t)vs * 3,
+                         s->orig_vertices.begin() + (size_t)(vs + nv) * 3);
+    s->num_hull_verts += nv;
+    const uint32_t gi = s->num_geoms;
+    push_geom(s, bi, hv, nv, friction, restitution, vs, nv);
+    build_support_lookup(s, gi, hv, nv);
+    s->num_geoms++;
+    if (out) *out = bi;
+    return SV_OK;
+}
+
+int sv_scene_set_body_state(sv_scene *s, uint32_t b, const float q[4], const float v[3], const float w[3]) {
+    if (!s || b >= s->num_bodies) return bad("arg");
+    if (q) std::memcpy(&s->body_quat[(size_t)b * 4], q, 4 * sizeof(float));
+    if (v) std::memcpy(&s->body_vel[(size_t)b * 3], v, 3 * sizeof(float));
+    if (w) std::memcpy(&s->body_omega[(size_t)b * 3], w, 3 * sizeof(float));
+    return SV_OK;
+}
+
+int sv_scene_get_counts(const sv_scene *s, sv_scene_counts *o) {
+    if (!s || !o) return bad("null");
+    o->num_vertices = s->num_vertices;
+    o->num_triangles = s->num_triangles;
+    o->num_bodies = s->num_bodies;
+    o->num_geoms = s->num_geoms;
+    o->num_hull_verts = s->num_hull_verts;
+    o->num_materials = s->num_materials;
+    o->num_support_entries = s->num_support_entries;
+    return SV_OK;
+}
+
+static void derive(sv_scene *s) {
+    // data.zig:633-640: body vert range from the body's geom
+    for (uint32_t gi = 0; gi < s->num_geoms; gi++) {
+        uint32_t bi = s->geom_body_idx[gi];
+        s->body_vert_start[bi] = s->geom_vert_start[gi];
+        s->body_vert_count[bi] = s->geom_vert_count[gi];
+    }
+}
+
+int sv_scene_array(const sv_scene *cs, sv_buf id, const void **host, size_t *bytes) {
+    if (!cs || !host || !bytes) return bad("null");
+    sv_scene *s = const_cast<sv_scene *>(cs);
+    derive(s);
+    auto F = [&](const std::vector<float> &v) { *host = v.data(); *bytes = v.size() * sizeof(float); return SV_OK; };
+    auto U = [&](const std::vector<uint32_t> &v) { *host = v.data(); *bytes = v.size() * sizeof(uint32_t); return SV_OK; };
+    switch (id) {
+    case SV_BODY_POS: return F(s->body_pos);
+    case SV_BODY_QUAT: return F(s->body_quat);
+    case SV_BODY_VEL: return F(s->body_vel);
+    case SV_BODY_OMEGA: return F(s->body_omega);
+    case SV_BODY_INV_MASS: return F(s->body_inv_mass);
+    case SV_BODY_INV_INERTIA: return F(s->body_inv_inertia);
+    case SV_BODY_VERT_START: return U(s->body_vert_start);
+    case SV_BODY_VERT_COUNT: return U(s->body_vert_count);
+    case SV_GEOM_BODY_IDX: return U(s->geom_body_idx);
+    case SV_GEOM_DATA: return F(s->geom_data);
+    case SV_GEOM_FRICTION: return F(s->geom_friction);
+    case SV_GEOM_RESTITUTION: return F(s->geom_restitution);
+    case SV_HULL_VERTS: return F(s->hull_verts);
+    case SV_VERTICES: return F(s->vertices);
+    case SV_ORIGINAL_VERTICES: return F(s->orig_vertices);
+    case SV_SUPPORT_LOOKUP_OFFSET: return U(s->support_offset);
+    case SV_SUPPORT_LOOKUP: return U(s->support_lookup);
+    case SV_INDICES: return U(s->indices);
+    case SV_GEOM_HULL_RANGE: return U(s->geom_hull_range);
+    default: return bad("buffer id has no host staging array");
+    }
+}
+
+// data.zig:599-655 (the physics and BVH inputs)
+int sv_scene_upload(sv_scene *s, sv_ctx *ctx) {
+    if (!s || !ctx) return bad("null");
+    derive(s);
+    static const sv_buf ids[] = {SV_VERTICES, SV_INDICES, SV_ORIGINAL_VERTICES, SV_BODY_POS, SV_BODY_QUAT,
+                                 SV_BODY_VEL, SV_BODY_OMEGA, SV_BODY_INV_MASS, SV_BODY_INV_INERTIA,
+                                 SV_GEOM_BODY_IDX, SV_GEOM_DATA, SV_GEOM_FRICTION, SV_GEOM_RESTITUTION,
+                                 SV_HULL_VERTS, SV_SUPPORT_LOOKUP_OFFSET, SV_SUPPORT_LOOKUP, SV_BODY_VERT_START,
+                                 SV_BODY_VERT_COUNT, SV_GEOM_HULL_RANGE};
+    for (sv_buf id : ids) {
+        const void *p;
+        size_t n;
+        int rc = sv_scene_array(s, id, &p, &n);
+        if (rc) return rc;
+        rc = sv_upload(ctx, id, p, 0, n);
+        if (rc) return rc;
+    }
+    return SV_OK;
+}
+
+int sv_scene_build_workload(sv_scene *s, int workload, uint32_t count, uint32_t segs, uint64_t seed) {
+    if (!s || s->num_bodies) return bad("scene must be empty");
+    Rng r{seed * 0x9E3779B97F4A7C15ull + (uint64_t)workload};
+    const float zero3[3] = {0, 0, 0};
+    uint32_t m_static, m_dyn;
+    int rc = sv_scene_add_material(s, zero3, 0.95f, 0.0f, zero3, 0.0f, 0.0f, &m_static);
+    if (!rc) rc = sv_scene_add_material(s, zero3, 0.5f, 0.0f, zero3, 0.0f, 1000.0f, &m_dyn);
+    if (rc) return rc;
+    switch (workload) {
+    case 1: {  // 1,000 bodies in a 10x10 m column over a 50x1x50 m ground
+        uint32_t n = count ? count : 1000, sg = segs ? segs : 16;
+        if ((rc = static_box(s, 0, -0.5f, 0, 25, 0.5f, 25, m_static))) return rc;
+        for (uint32_t i = 0; i < n && !rc; i++) {
+            float p[3] = {r.uni(-5, 5), r.uni(1, 20), r.uni(-5, 5)};
+            rc = add_random_body(s, r, 0.5f, 0.2f, 0.6f, 0.2f, 0.5f, sg, p, m_dyn);
+        }
+        return rc;
+    }
+    case 2: {  // 32,768 bodies in a 30x30 m container, ground + 4 walls
+        uint32_t n = count ? count : 32768, sg = segs ? segs : 16;
+        if ((rc = static_box(s, 0, -0.5f, 0, 16, 0.5f, 16, m_static))) return rc;
+        if ((rc = static_box(s, 15.5f, 30, 0, 0.5f, 30, 16, m_static))) return rc;
+        if ((rc = static_box(s, -15.5f, 30, 0, 0.5f, 30, 16, m_static))) return rc;
+        if ((rc = static_box(s, 0, 30, 15.5f, 16, 30, 0.5f, m_static))) return rc;
+        if ((rc = static_box(s, 0, 30, -15.5f, 16, 30, 0.5f, m_static))) return rc;
+        for (uint32_t i = 0; i < n && !rc; i++) {
+            float p[3] = {r.uni(-14.4f, 14.4f), r.uni(1, 60), r.uni(-14.4f, 14.4f)};
+            rc = add_random_body(s, r, 0.5f, 0.2f, 0.6f, 0.2f, 0.5f, sg, p, m_dyn);
+        }
+        return rc;
+    }
+    case 4: {  // small scene over its own ground (differentiable workload)
+        uint32_t n = count ? count : 64, sg = segs ? segs : 16;
+        if ((rc = static_box(s, 0, -0.5f, 0, 10, 0.5f, 10, m_static))) return rc;
+        for (uint32_t i = 0; i < n && !rc; i++) {
+            float p[3] = {r.uni(-3, 3), r.uni(0.7f, 6), r.uni(-3, 3)};
+            rc = add_random_body(s, r, 0.5f, 0.2f, 0.6f, 0.2f, 0.5f, sg, p, m_dyn);
+        }
+        return rc;
+    }
+    case 5: {  // 1,048,576 bodies, 70% spheres, 200x200x50 m over 220x1x220 m
+        uint32_t n = count ? count : 1048576, sg = segs ? segs : 8;
+        if ((rc = static_box(s, 0, -0.5f, 0, 110, 0.5f, 110, m_static))) return rc;
+        for (uint32_t i = 0; i < n && !rc; i++) {
+            float p[3] = {r.uni(-100, 100), r.uni(1, 51), r.uni(-100, 100)};
+            rc = add_random_body(s, r, 0.7f, 0.1f, 0.4f, 0.1f, 0.3f, sg, p, m_dyn);
+        }
+        return rc;
+    }
+    default:
+        return bad("unknown workload");
+    }
+}
+
+}  // extern "C"
