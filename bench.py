@@ -83,6 +83,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-sample-substeps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run the 256 MiB copy kernel once (byte-counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -103,6 +105,8 @@ def main() -> int:
     scene.upload(ctx)
     name, arch = ctx.device_info()
     log(f"device: {name} {arch}")
+    copy_gbs = ctx.copy_bandwidth(256 << 20, 1 if args.calib else 20)
+    log(f"plain copy kernel (256 MiB): {copy_gbs:.0f} GB/s read+write")
 
     for i in range(args.warmup):
         ctx.step(n, nv, nh, SUBSTEPS, DT, GRAVITY)
@@ -139,7 +143,7 @@ def main() -> int:
     achieved = alg / (us * 1e-6) / 1e9
     roof = {"bound": "hbm", "kernel": "k_aabb (phys/aabb)", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "avg_us": round(us, 2), "alg_bytes": int(alg)}
+            "avg_us": round(us, 2), "alg_bytes": int(alg), "copy_kernel_gbs": round(copy_gbs, 1)}
     total_us = sum(v[0] * v[1] for v in stages.values())
     breakdown = {k: round(100.0 * v[0] * v[1] / total_us, 1) for k, v in stages.items()}
 

@@ -206,6 +206,11 @@ int sv_bvh_dispatch(sv_ctx *ctx, uint32_t step, uint32_t num_tris);
  * check of the function the support lookup bins with). */
 int sv_debug_atan2(sv_ctx *ctx, const float *y, const float *x, float *out, uint32_t n);
 
+/* Copy `nbytes` device to device `iters` times with a plain copy kernel
+ * (the calibration kernel of the HBM byte counters) and report the average
+ * read + write bandwidth in GB/s. */
+int sv_debug_copy_bandwidth(sv_ctx *ctx, size_t nbytes, uint32_t iters, double *gbps);
+
 /* Thread-local message of the last failure ("" if none). */
 const char *sv_last_error(void);
 /* Device name and gfx arch of the context's device. */

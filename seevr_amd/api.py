@@ -141,6 +141,12 @@ class Context:
                                       y.size))
         return out
 
+    def copy_bandwidth(self, nbytes: int = 1 << 30, iters: int = 10) -> float:
+        """GB/s (read + write) of a plain device copy of nbytes."""
+        out = C.c_double()
+        check(self.lib.sv_debug_copy_bandwidth(self.h, nbytes, iters, C.byref(out)))
+        return out.value
+
     def enable_stage_timing(self, on: bool = True) -> None:
         check(self.lib.sv_enable_stage_timing(self.h, int(on)))
 

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -211,6 +212,10 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
     if (!c) return fail(SV_E_NOMEM, "sv_create: out of host memory");
     c->caps = *caps;
     if (params) c->params = *params; else sv_default_params(&c->params);
+    if (const char *wgs = getenv("SEEVR_SOLVER_WGS")) {
+        int v = atoi(wgs);
+        c->solver_workgroups = v > 1 ? (uint32_t)v : 1u;
+    }
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -473,6 +478,35 @@ int sv_debug_atan2(sv_ctx *c, const float *y, const float *x, float *out, uint32
     if (e == hipSuccess) e = hipMemcpy(out, d + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(SV_E_DEVICE, "sv_debug_atan2: %s", hipGetErrorString(e));
+    return SV_OK;
+}
+
+// Copy `nbytes` device-to-device `iters` times with k_copy_calib and report
+// the average bandwidth (read + write bytes per second).
+int sv_debug_copy_bandwidth(sv_ctx *c, size_t nbytes, uint32_t iters, double *gbps) {
+    if (!c || !gbps || nbytes < 16 || iters == 0) return fail(SV_E_INVALID, "sv_debug_copy_bandwidth: bad argument");
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, nbytes);
+    if (e == hipSuccess) e = hipMalloc(&b, nbytes);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, nbytes, c->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        sv::launch_copy_calib(c, a, b, nbytes);  // warm up
+        (void)hipEventRecord(e0, c->stream);
+        for (uint32_t i = 0; i < iters; i++) sv::launch_copy_calib(c, a, b, nbytes);
+        (void)hipEventRecord(e1, c->stream);
+        e = hipStreamSynchronize(c->stream);
+    }
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return fail(SV_E_DEVICE, "sv_debug_copy_bandwidth: %s", hipGetErrorString(e));
+    *gbps = 2.0 * (double)(nbytes / 16 * 16) * iters / (ms * 1e-3) / 1e9;
     return SV_OK;
 }
 

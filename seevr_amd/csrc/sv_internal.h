@@ -48,6 +48,9 @@ struct sv_ctx {
     void *buf[SV_BUF_COUNT] = {};
     size_t bytes[SV_BUF_COUNT] = {};
     sv::Scratch s;
+    // workgroups of the ordered PGS kernel: 1 = single-CU form (k_solve),
+    // > 1 = multi-CU form (k_solve_mc)
+    uint32_t solver_workgroups = 1;
     // stage timing
     bool timing = false;
     std::vector<std::string> stage_names;
@@ -99,6 +102,7 @@ void launch_update_verts(sv_ctx *c, uint32_t nv, uint32_t nb);
 void launch_clear_counters(sv_ctx *c);
 void launch_debug_atan2(sv_ctx *c, const float *y, const float *x, float *out, uint32_t n);
 void launch_rot(sv_ctx *c, uint32_t n);
+void launch_copy_calib(sv_ctx *c, const void *src, void *dst, size_t nbytes);
 void launch_pack_boxes(sv_ctx *c, uint32_t n);
 
 // ---- sv_pgs.hip ----

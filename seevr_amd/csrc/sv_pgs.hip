@@ -229,6 +229,78 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(
     }
 }
 
+// Multi-CU form of k_solve: waves of any workgroup take 64-item chunks in
+// order from a device counter; hand-offs between items use device-scope
+// release/acquire, so an item may run on any CU.  The earliest unfinished
+// item is always held by a running wave (a wave takes a chunk only while
+// running and keeps it until done), so the schedule cannot deadlock.
+constexpr int MC_THREADS = 256;
+
+__global__ void __launch_bounds__(MC_THREADS) k_solve_mc(
+    const uint32_t *__restrict__ misc, uint32_t iters, const float *__restrict__ prep,
+    const uint32_t *__restrict__ rank, const uint32_t *__restrict__ deg, uint32_t *ticket, float *vel, float *omega,
+    const float *__restrict__ rot, const float *__restrict__ inv_i, float *lambda, uint32_t *ctl,
+    uint32_t *counters) {
+    const uint32_t nc = misc[MISC_NC];
+    const uint64_t total = (uint64_t)iters * nc;
+    const int lane = threadIdx.x & 63;
+    uint32_t *next = ctl + 1;
+    for (;;) {
+        uint32_t chunk = 0;
+        if (lane == 0) chunk = __hip_atomic_fetch_add(next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        chunk = __shfl(chunk, 0, 64);
+        if ((uint64_t)chunk * 64 >= total) return;
+        uint64_t q = (uint64_t)chunk * 64 + lane;
+        bool pending = q < total;
+        uint32_t c = 0, a = 0, b = 0, ta = 0, tb = 0;
+        bool dyn_a = false, dyn_b = false;
+        const float *P = prep;
+        if (pending) {
+            uint32_t t = (uint32_t)(q / nc);
+            c = (uint32_t)(q - (uint64_t)t * nc);
+            P = prep + (size_t)c * CPREP_WORDS;
+            uint32_t flags = __float_as_uint(P[P_FLAGS]);
+            if (flags & F_SKIP) {
+                pending = false;
+            } else {
+                a = __float_as_uint(P[P_A]);
+                b = __float_as_uint(P[P_B]);
+                dyn_a = flags & F_DYN_A;
+                dyn_b = flags & F_DYN_B;
+                if (dyn_a) ta = t * deg[a] + rank[2 * c];
+                if (dyn_b) tb = t * deg[b] + rank[2 * c + 1];
+            }
+        }
+        uint64_t last = wall_clock64();
+        while (__any(pending)) {
+            bool progressed = false;
+            if (pending) {
+                bool ready = (!dyn_a || ld_ticket(ticket + a) == ta) && (!dyn_b || ld_ticket(ticket + b) == tb);
+                if (ready) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    pgs_item(P, c, vel, omega, rot, inv_i, lambda, a, b, dyn_a, dyn_b);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                    if (dyn_a) __hip_atomic_fetch_add(ticket + a, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (dyn_b) __hip_atomic_fetch_add(ticket + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    pending = false;
+                    progressed = true;
+                }
+            }
+            if (__any(progressed)) {
+                last = wall_clock64();
+            } else {
+                if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+                if (wall_clock64() - last > STALL_TICKS) {
+                    __hip_atomic_store(ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (lane == 0) atomicOr(counters + 2, (uint32_t)SV_STATUS_SOLVER_TIMEOUT);
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+    }
+}
+
 }  // namespace
 
 // Steps 5 (physics.comp:500-583) for the contacts counters[1] names.
@@ -260,6 +332,14 @@ void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
                        c->s.inc_val[where], c->s.deg_off, c->s.rank);
     fill_u32(c, c->s.ticket, 0u, n_bodies);
     fill_u32(c, c->s.solve_ctl, 0u, 4);
+    if (c->solver_workgroups > 1) {
+        hipLaunchKernelGGL(k_solve_mc, dim3(c->solver_workgroups), dim3(MC_THREADS), 0, c->stream, c->s.misc,
+                           c->params.pgs_iterations, c->s.cprep, c->s.rank, c->s.deg, c->s.ticket,
+                           B<float>(c, SV_BODY_VEL), B<float>(c, SV_BODY_OMEGA), c->s.rot,
+                           B<float>(c, SV_BODY_INV_INERTIA), B<float>(c, SV_CONTACT_LAMBDA_N), c->s.solve_ctl,
+                           counters);
+        return;
+    }
     hipLaunchKernelGGL(k_solve, dim3(1), dim3(SOLVE_THREADS), 0, c->stream, c->s.misc, c->params.pgs_iterations,
                        c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, B<float>(c, SV_BODY_VEL),
                        B<float>(c, SV_BODY_OMEGA), c->s.rot, B<float>(c, SV_BODY_INV_INERTIA),

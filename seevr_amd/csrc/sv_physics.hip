@@ -541,6 +541,15 @@ __global__ void k_pack_boxes(uint32_t n, const float *__restrict__ amin, const f
     bpbox[2 * i + 1] = make_float4(amax[3 * i], amax[3 * i + 1], amax[3 * i + 2], 0.0f);
 }
 
+// Plain copy of known size: the calibration kernel for HBM byte counters
+// and the achievable-bandwidth reference of the roofline.
+__global__ void __launch_bounds__(BLOCK) k_copy_calib(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                      size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
 __global__ void k_atan2(const float *y, const float *x, float *out, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = sv_atan2f(y[i], x[i]);
@@ -642,6 +651,14 @@ void launch_pack_boxes(sv_ctx *c, uint32_t n) {
     hipLaunchKernelGGL(k_pack_boxes, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, c->stream, n,
                        B<float>(c, SV_BODY_AABB_MIN), B<float>(c, SV_BODY_AABB_MAX), B<float>(c, SV_BODY_INV_MASS),
                        c->s.bpbox);
+}
+
+void launch_copy_calib(sv_ctx *c, const void *src, void *dst, size_t nbytes) {
+    size_t n = nbytes / sizeof(float4);
+    if (!n) return;
+    uint32_t grid = div_up(n, BLOCK);
+    if (grid > 16384) grid = 16384;
+    hipLaunchKernelGGL(k_copy_calib, dim3(grid), dim3(BLOCK), 0, c->stream, (const float4 *)src, (float4 *)dst, n);
 }
 
 void launch_debug_atan2(sv_ctx *c, const float *y, const float *x, float *out, uint32_t n) {
