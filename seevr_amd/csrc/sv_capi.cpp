@@ -117,6 +117,8 @@ void free_ctx(sv_ctx *c) {
 int sync_and_finish(sv_ctx *c) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(SV_E_DEVICE, "device work failed: %s", hipGetErrorString(e));
+    e = hipGetLastError();  // a kernel launch that was rejected (resources, bad configuration)
+    if (e != hipSuccess) return fail(SV_E_DEVICE, "kernel launch failed: %s", hipGetErrorString(e));
     if (c->timing) {
         for (auto &p : c->pending) {
             float ms = 0.0f;
@@ -216,12 +218,18 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
         int v = atoi(wgs);
         c->solver_workgroups = v > 1 ? (uint32_t)v : 1u;
     }
+    if ((uint64_t)c->params.pgs_iterations * caps->max_contacts >= 0xFFFFFFFFull) {
+        delete c;
+        return fail(SV_E_INVALID, "sv_create: pgs_iterations * max_contacts must stay below 2^32");
+    }
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         free_ctx(c);
         return fail(SV_E_DEVICE, "sv_create: no usable HIP device: %s", hipGetErrorString(e));
     }
+    c->solver_lds_bodies = sv::solver_lds_bodies(c);
+    if (const char *e = getenv("SEEVR_SOLVER_LDS_BODIES")) c->solver_lds_bodies = (uint32_t)atoi(e);
     buffer_sizes(c->caps, c->bytes);
     for (int i = 0; i < SV_BUF_COUNT && e == hipSuccess; i++) {
         if (c->bytes[i] == 0) continue;
@@ -279,6 +287,8 @@ int sv_get_params(sv_ctx *c, sv_params *out) {
 
 int sv_set_params(sv_ctx *c, const sv_params *p) {
     if (!c || !p) return fail(SV_E_INVALID, "sv_set_params: null argument");
+    if ((uint64_t)p->pgs_iterations * c->caps.max_contacts >= 0xFFFFFFFFull)
+        return fail(SV_E_INVALID, "sv_set_params: pgs_iterations * max_contacts must stay below 2^32");
     c->params = *p;
     return SV_OK;
 }

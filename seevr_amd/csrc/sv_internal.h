@@ -51,6 +51,8 @@ struct sv_ctx {
     // workgroups of the ordered PGS kernel: 1 = single-CU form (k_solve),
     // > 1 = multi-CU form (k_solve_mc)
     uint32_t solver_workgroups = 1;
+    // largest body count whose solver tickets fit in one workgroup's LDS
+    uint32_t solver_lds_bodies = 0;
     // stage timing
     bool timing = false;
     std::vector<std::string> stage_names;
@@ -107,6 +109,7 @@ void launch_pack_boxes(sv_ctx *c, uint32_t n);
 
 // ---- sv_pgs.hip ----
 void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt);
+uint32_t solver_lds_bodies(sv_ctx *c);
 
 // ---- sv_bvh.hip ----
 void bvh_stage(sv_ctx *c, uint32_t step, uint32_t n, bool bodies);

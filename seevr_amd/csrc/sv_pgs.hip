@@ -13,12 +13,21 @@
 // rank_a(c) is c's position among them.  rank/deg come from a stable radix
 // sort of the (body, contact) incidences.
 //
-// One workgroup of 1024 lanes executes the items in order, 64 per wave
-// round, so every data hand-off stays inside one CU (workgroup-scope
-// release/acquire).  The earliest unfinished item always has all its
-// predecessors done, so the schedule cannot deadlock; a wave that makes no
-// progress for 2 s (a broken invariant) raises an abort flag that ends the
-// kernel and the host reports SV_E_TIMEOUT.
+// Scheduling: every lane holds one item at a time and, as soon as it has
+// applied it, takes the next unassigned item in sweep order from a shared
+// counter (one atomic per wave per refill).  An item's constants (contact
+// frame, effective mass, bias, both bodies' rotations and inverse inertia)
+// are loaded when it is taken, so the dependent part of an item is the
+// ticket check, the two bodies' velocities, the update and the release.
+// The earliest unfinished item is always held by a running lane and has all
+// its predecessors done, so the schedule cannot deadlock.  A wave that makes
+// no progress for 2 s (a broken invariant) raises an abort flag that ends
+// the kernel, and the host reports SV_E_TIMEOUT.
+//
+// Forms: one workgroup of 1024 lanes with tickets in LDS (the hand-off stays
+// inside one CU: workgroup-scope release/acquire), the same with tickets in
+// global memory when they do not fit in LDS, and a multi-workgroup form
+// whose hand-offs use device-scope release/acquire.
 #include "sv_internal.h"
 #include "sv_math.h"
 
@@ -28,7 +37,7 @@ namespace {
 
 constexpr int BLOCK = 256;
 constexpr int SOLVE_THREADS = 1024;
-constexpr int SOLVE_WAVES = SOLVE_THREADS / 64;
+constexpr int MC_THREADS = 256;
 constexpr uint64_t STALL_TICKS = 200000000ull;  // 2 s of the 100 MHz wall clock
 
 template <typename T>
@@ -117,193 +126,202 @@ __global__ void __launch_bounds__(BLOCK) k_rank(const uint32_t *__restrict__ mis
     }
 }
 
-SVD uint32_t ld_ticket(const uint32_t *t) {
-    return __hip_atomic_load(t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// One item of the sweep with its constants in registers.
+struct Item {
+    f3 n, ra, rb;
+    float kn, bias, mu, ima, imb;
+    m3 Ra, Rb;
+    f3 iia, iib;
+    uint32_t c, a, b, ta, tb;
+    bool dyn_a, dyn_b;
+};
+
+// Loads item q's constants; returns false for a skipped contact
+// (both inverse masses zero: physics.comp:531).
+SVD bool load_item(Item &it, uint32_t q, uint32_t nc, const float *__restrict__ prep,
+                   const uint32_t *__restrict__ rank, const uint32_t *__restrict__ deg,
+                   const float *__restrict__ rot, const float *__restrict__ inv_i) {
+    uint32_t t = q / nc;
+    uint32_t c = q - t * nc;
+    const float *P = prep + (size_t)c * CPREP_WORDS;
+    uint32_t flags = __float_as_uint(P[P_FLAGS]);
+    if (flags & F_SKIP) return false;
+    it.c = c;
+    it.a = __float_as_uint(P[P_A]);
+    it.b = __float_as_uint(P[P_B]);
+    it.dyn_a = flags & F_DYN_A;
+    it.dyn_b = flags & F_DYN_B;
+    it.ta = it.dyn_a ? t * deg[it.a] + rank[2 * c] : 0u;
+    it.tb = it.dyn_b ? t * deg[it.b] + rank[2 * c + 1] : 0u;
+    it.n = mk(P[P_N], P[P_N + 1], P[P_N + 2]);
+    it.ra = mk(P[P_RA], P[P_RA + 1], P[P_RA + 2]);
+    it.rb = mk(P[P_RB], P[P_RB + 1], P[P_RB + 2]);
+    it.kn = P[P_KN];
+    it.bias = P[P_BIAS];
+    it.mu = P[P_MU];
+    it.ima = P[P_IMA];
+    it.imb = P[P_IMB];
+    it.Ra = ld_rot(rot, it.a);
+    it.Rb = ld_rot(rot, it.b);
+    it.iia = ld3(inv_i, it.a);
+    it.iib = ld3(inv_i, it.b);
+    return true;
 }
 
-// One item of the sweep: physics.comp:525-580 for contact c.
-SVD void pgs_item(const float *__restrict__ P, uint32_t c, float *__restrict__ vel, float *__restrict__ omega,
-                  const float *__restrict__ rot, const float *__restrict__ inv_i, float *__restrict__ lambda,
-                  uint32_t a, uint32_t b, bool dyn_a, bool dyn_b) {
-    f3 n = mk(P[P_N], P[P_N + 1], P[P_N + 2]);
-    f3 ra = mk(P[P_RA], P[P_RA + 1], P[P_RA + 2]);
-    f3 rb = mk(P[P_RB], P[P_RB + 1], P[P_RB + 2]);
-    float kn = P[P_KN], bias = P[P_BIAS], mu = P[P_MU], ima = P[P_IMA], imb = P[P_IMB];
-    m3 Ra = ld_rot(rot, a), Rb = ld_rot(rot, b);
-    f3 iia = ld3(inv_i, a), iib = ld3(inv_i, b);
-    f3 la = ld3(vel, a), wa = ld3(omega, a);
-    f3 lb = ld3(vel, b), wb = ld3(omega, b);
-
-    f3 va = la + cross3(wa, ra);
-    f3 vb = lb + cross3(wb, rb);
-    float vn = dot3(vb - va, n);
-    float dln = (-vn + bias) / kn;
-    float old = lambda[c];
+// physics.comp:525-580 for one contact: normal impulse with projection,
+// then Coulomb friction with the geometric-mean coefficient.
+SVD void apply_item(const Item &it, float *__restrict__ vel, float *__restrict__ omega,
+                    float *__restrict__ lambda) {
+    f3 la = ld3(vel, it.a), wa = ld3(omega, it.a);
+    f3 lb = ld3(vel, it.b), wb = ld3(omega, it.b);
+    float old = lambda[it.c];
+    f3 va = la + cross3(wa, it.ra);
+    f3 vb = lb + cross3(wb, it.rb);
+    float vn = dot3(vb - va, it.n);
+    float dln = (-vn + it.bias) / it.kn;
     float nl = gmax(old + dln, 0.0f);
     dln = nl - old;
-    lambda[c] = nl;
-    f3 imp = dln * n;
-    la = la - ima * imp;
-    lb = lb + imb * imp;
-    wa = wa - apply_inv_I(Ra, iia, cross3(ra, imp));
-    wb = wb + apply_inv_I(Rb, iib, cross3(rb, imp));
+    lambda[it.c] = nl;
+    f3 imp = dln * it.n;
+    la = la - it.ima * imp;
+    lb = lb + it.imb * imp;
+    wa = wa - apply_inv_I(it.Ra, it.iia, cross3(it.ra, imp));
+    wb = wb + apply_inv_I(it.Rb, it.iib, cross3(it.rb, imp));
 
-    va = la + cross3(wa, ra);
-    vb = lb + cross3(wb, rb);
+    va = la + cross3(wa, it.ra);
+    vb = lb + cross3(wb, it.rb);
     f3 dv = vb - va;
-    f3 vt = dv - dot3(dv, n) * n;
+    f3 vt = dv - dot3(dv, it.n) * it.n;
     float vtl = len3(vt);
     if (vtl > 1e-6f) {
         f3 td = mk(vt.x / vtl, vt.y / vtl, vt.z / vtl);
-        f3 rat = cross3(ra, td), rbt = cross3(rb, td);
-        float kt = ima + imb + dot3(rat, apply_inv_I(Ra, iia, rat)) + dot3(rbt, apply_inv_I(Rb, iib, rbt));
-        float dlt = gmin(vtl / kt, mu * nl);
+        f3 rat = cross3(it.ra, td), rbt = cross3(it.rb, td);
+        float kt = it.ima + it.imb + dot3(rat, apply_inv_I(it.Ra, it.iia, rat)) +
+                   dot3(rbt, apply_inv_I(it.Rb, it.iib, rbt));
+        float dlt = gmin(vtl / kt, it.mu * nl);
         f3 impt = dlt * td;
-        la = la + ima * impt;
-        lb = lb - imb * impt;
-        wa = wa + apply_inv_I(Ra, iia, cross3(ra, impt));
-        wb = wb - apply_inv_I(Rb, iib, cross3(rb, impt));
+        la = la + it.ima * impt;
+        lb = lb - it.imb * impt;
+        wa = wa + apply_inv_I(it.Ra, it.iia, cross3(it.ra, impt));
+        wb = wb - apply_inv_I(it.Rb, it.iib, cross3(it.rb, impt));
     }
-    if (dyn_a) { st3(vel, a, la); st3(omega, a, wa); }
-    if (dyn_b) { st3(vel, b, lb); st3(omega, b, wb); }
+    if (it.dyn_a) { st3(vel, it.a, la); st3(omega, it.a, wa); }
+    if (it.dyn_b) { st3(vel, it.b, lb); st3(omega, it.b, wb); }
 }
 
-__global__ void __launch_bounds__(SOLVE_THREADS) k_solve(
-    const uint32_t *__restrict__ misc, uint32_t iters, const float *__restrict__ prep,
-    const uint32_t *__restrict__ rank, const uint32_t *__restrict__ deg, uint32_t *ticket, float *vel, float *omega,
-    const float *__restrict__ rot, const float *__restrict__ inv_i, float *lambda, uint32_t *ctl,
-    uint32_t *counters) {
-    const uint32_t nc = misc[MISC_NC];
-    const uint64_t total = (uint64_t)iters * nc;
-    const int lane = threadIdx.x & 63;
-    const uint32_t w = threadIdx.x >> 6;
-    for (uint64_t chunk = w; chunk * 64 < total; chunk += SOLVE_WAVES) {
-        uint64_t q = chunk * 64 + lane;
-        bool pending = q < total;
-        uint32_t c = 0, a = 0, b = 0, ta = 0, tb = 0;
-        bool dyn_a = false, dyn_b = false;
-        const float *P = prep;
-        if (pending) {
-            uint32_t t = (uint32_t)(q / nc);
-            c = (uint32_t)(q - (uint64_t)t * nc);
-            P = prep + (size_t)c * CPREP_WORDS;
-            uint32_t flags = __float_as_uint(P[P_FLAGS]);
-            if (flags & F_SKIP) {
-                pending = false;
-            } else {
-                a = __float_as_uint(P[P_A]);
-                b = __float_as_uint(P[P_B]);
-                dyn_a = flags & F_DYN_A;
-                dyn_b = flags & F_DYN_B;
-                if (dyn_a) ta = t * deg[a] + rank[2 * c];
-                if (dyn_b) tb = t * deg[b] + rank[2 * c + 1];
-            }
-        }
-        uint64_t last = wall_clock64();
-        while (__any(pending)) {
-            bool progressed = false;
-            if (pending) {
-                bool ready = (!dyn_a || ld_ticket(ticket + a) == ta) && (!dyn_b || ld_ticket(ticket + b) == tb);
-                if (ready) {
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                    pgs_item(P, c, vel, omega, rot, inv_i, lambda, a, b, dyn_a, dyn_b);
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                    if (dyn_a) __hip_atomic_fetch_add(ticket + a, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (dyn_b) __hip_atomic_fetch_add(ticket + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    pending = false;
-                    progressed = true;
-                }
-            }
-            if (__any(progressed)) {
-                last = wall_clock64();
-            } else {
-                if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
-                if (wall_clock64() - last > STALL_TICKS) {
-                    __hip_atomic_store(ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (lane == 0) atomicOr(counters + 2, (uint32_t)SV_STATUS_SOLVER_TIMEOUT);
-                    return;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-    }
+// Ticket storage and hand-off scope of each solver form.
+enum : int { FORM_LDS = 0, FORM_GLOBAL = 1, FORM_MULTI = 2 };
+
+template <int FORM>
+SVD uint32_t ticket_load(const uint32_t *t) {
+    if (FORM == FORM_LDS) return __hip_atomic_load(t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __hip_atomic_load(t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int FORM>
+SVD void ticket_bump(uint32_t *t) {
+    if (FORM == FORM_LDS) __hip_atomic_fetch_add(t, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else __hip_atomic_fetch_add(t, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int FORM>
+SVD void fence_acquire() {
+    if (FORM == FORM_MULTI) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+template <int FORM>
+SVD void fence_release() {
+    if (FORM == FORM_MULTI) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 }
 
-// Multi-CU form of k_solve: waves of any workgroup take 64-item chunks in
-// order from a device counter; hand-offs between items use device-scope
-// release/acquire, so an item may run on any CU.  The earliest unfinished
-// item is always held by a running wave (a wave takes a chunk only while
-// running and keeps it until done), so the schedule cannot deadlock.
-constexpr int MC_THREADS = 256;
-
-__global__ void __launch_bounds__(MC_THREADS) k_solve_mc(
-    const uint32_t *__restrict__ misc, uint32_t iters, const float *__restrict__ prep,
-    const uint32_t *__restrict__ rank, const uint32_t *__restrict__ deg, uint32_t *ticket, float *vel, float *omega,
-    const float *__restrict__ rot, const float *__restrict__ inv_i, float *lambda, uint32_t *ctl,
+template <int FORM, int THREADS>
+__global__ void __launch_bounds__(THREADS) k_solve(
+    const uint32_t *__restrict__ misc, uint32_t iters, uint32_t n_bodies, const float *__restrict__ prep,
+    const uint32_t *__restrict__ rank, const uint32_t *__restrict__ deg, uint32_t *gticket, float *vel,
+    float *omega, const float *__restrict__ rot, const float *__restrict__ inv_i, float *lambda, uint32_t *ctl,
     uint32_t *counters) {
+    extern __shared__ uint32_t s_ticket[];
+    __shared__ uint32_t s_next;
     const uint32_t nc = misc[MISC_NC];
-    const uint64_t total = (uint64_t)iters * nc;
+    const uint32_t total = iters * nc;
     const int lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t *ticket = gticket;
     uint32_t *next = ctl + 1;
+    if (FORM != FORM_MULTI) {
+        if (FORM == FORM_LDS) {
+            for (uint32_t i = threadIdx.x; i < n_bodies; i += THREADS) s_ticket[i] = 0;
+            ticket = s_ticket;
+        }
+        if (threadIdx.x == 0) s_next = 0;
+        next = &s_next;
+        __syncthreads();
+    }
+    if (total == 0) return;
+
+    Item it;
+    bool has = false, fin = false;
+    uint64_t last = wall_clock64();
     for (;;) {
-        uint32_t chunk = 0;
-        if (lane == 0) chunk = __hip_atomic_fetch_add(next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        chunk = __shfl(chunk, 0, 64);
-        if ((uint64_t)chunk * 64 >= total) return;
-        uint64_t q = (uint64_t)chunk * 64 + lane;
-        bool pending = q < total;
-        uint32_t c = 0, a = 0, b = 0, ta = 0, tb = 0;
-        bool dyn_a = false, dyn_b = false;
-        const float *P = prep;
-        if (pending) {
-            uint32_t t = (uint32_t)(q / nc);
-            c = (uint32_t)(q - (uint64_t)t * nc);
-            P = prep + (size_t)c * CPREP_WORDS;
-            uint32_t flags = __float_as_uint(P[P_FLAGS]);
-            if (flags & F_SKIP) {
-                pending = false;
-            } else {
-                a = __float_as_uint(P[P_A]);
-                b = __float_as_uint(P[P_B]);
-                dyn_a = flags & F_DYN_A;
-                dyn_b = flags & F_DYN_B;
-                if (dyn_a) ta = t * deg[a] + rank[2 * c];
-                if (dyn_b) tb = t * deg[b] + rank[2 * c + 1];
+        // refill every lane that has no item, in sweep order
+        uint64_t need = __ballot(!has && !fin);
+        while (need) {
+            uint32_t cnt = (uint32_t)__popcll(need);
+            int leader = __builtin_ctzll(need);
+            uint32_t base = 0;
+            if (lane == leader) {
+                if (FORM == FORM_MULTI)
+                    base = __hip_atomic_fetch_add(next, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                else
+                    base = __hip_atomic_fetch_add(next, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            base = __shfl(base, leader, 64);
+            if (!has && !fin) {
+                uint32_t q = base + (uint32_t)__popcll(need & lt);
+                if (q >= total) fin = true;
+                else has = load_item(it, q, nc, prep, rank, deg, rot, inv_i);
+            }
+            need = __ballot(!has && !fin);  // lanes whose item was a skipped contact
+        }
+        if (!__any(has)) break;
+        bool progressed = false;
+        if (has) {
+            bool ready = (!it.dyn_a || ticket_load<FORM>(ticket + it.a) == it.ta) &&
+                         (!it.dyn_b || ticket_load<FORM>(ticket + it.b) == it.tb);
+            if (ready) {
+                fence_acquire<FORM>();
+                apply_item(it, vel, omega, lambda);
+                fence_release<FORM>();
+                if (it.dyn_a) ticket_bump<FORM>(ticket + it.a);
+                if (it.dyn_b) ticket_bump<FORM>(ticket + it.b);
+                has = false;
+                progressed = true;
             }
         }
-        uint64_t last = wall_clock64();
-        while (__any(pending)) {
-            bool progressed = false;
-            if (pending) {
-                bool ready = (!dyn_a || ld_ticket(ticket + a) == ta) && (!dyn_b || ld_ticket(ticket + b) == tb);
-                if (ready) {
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                    pgs_item(P, c, vel, omega, rot, inv_i, lambda, a, b, dyn_a, dyn_b);
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                    if (dyn_a) __hip_atomic_fetch_add(ticket + a, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (dyn_b) __hip_atomic_fetch_add(ticket + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    pending = false;
-                    progressed = true;
-                }
+        if (__any(progressed)) {
+            last = wall_clock64();
+        } else {
+            if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+            if (wall_clock64() - last > STALL_TICKS) {
+                __hip_atomic_store(ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0) atomicOr(counters + 2, (uint32_t)SV_STATUS_SOLVER_TIMEOUT);
+                return;
             }
-            if (__any(progressed)) {
-                last = wall_clock64();
-            } else {
-                if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
-                if (wall_clock64() - last > STALL_TICKS) {
-                    __hip_atomic_store(ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (lane == 0) atomicOr(counters + 2, (uint32_t)SV_STATUS_SOLVER_TIMEOUT);
-                    return;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
+            __builtin_amdgcn_s_sleep(1);
         }
     }
 }
 
 }  // namespace
 
-// Steps 5 (physics.comp:500-583) for the contacts counters[1] names.
+// Largest body count whose tickets fit in the single-workgroup kernel's LDS.
+uint32_t solver_lds_bodies(sv_ctx *c) {
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess) lds = 65536;
+    int avail = lds - 1024;  // static LDS of the kernel and slack
+    return avail > 0 ? (uint32_t)(avail / 4) : 0u;
+}
+
+// Step 5 (physics.comp:500-583) for the contacts counters[1] names.
 // Reads lambda_n as the starting impulses (step 4 clears them).
 void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
     uint32_t *counters = B<uint32_t>(c, SV_ATOMIC_COUNTERS);
@@ -330,20 +348,25 @@ void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
     if (rgrid > 8192) rgrid = 8192;
     hipLaunchKernelGGL(k_rank, dim3(rgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.inc_key[where],
                        c->s.inc_val[where], c->s.deg_off, c->s.rank);
-    fill_u32(c, c->s.ticket, 0u, n_bodies);
     fill_u32(c, c->s.solve_ctl, 0u, 4);
+    float *vel = B<float>(c, SV_BODY_VEL), *omega = B<float>(c, SV_BODY_OMEGA);
+    float *inv_i = B<float>(c, SV_BODY_INV_INERTIA), *lam = B<float>(c, SV_CONTACT_LAMBDA_N);
+    uint32_t iters = c->params.pgs_iterations;
     if (c->solver_workgroups > 1) {
-        hipLaunchKernelGGL(k_solve_mc, dim3(c->solver_workgroups), dim3(MC_THREADS), 0, c->stream, c->s.misc,
-                           c->params.pgs_iterations, c->s.cprep, c->s.rank, c->s.deg, c->s.ticket,
-                           B<float>(c, SV_BODY_VEL), B<float>(c, SV_BODY_OMEGA), c->s.rot,
-                           B<float>(c, SV_BODY_INV_INERTIA), B<float>(c, SV_CONTACT_LAMBDA_N), c->s.solve_ctl,
-                           counters);
-        return;
+        fill_u32(c, c->s.ticket, 0u, n_bodies);
+        hipLaunchKernelGGL((k_solve<FORM_MULTI, MC_THREADS>), dim3(c->solver_workgroups), dim3(MC_THREADS), 0,
+                           c->stream, c->s.misc, iters, n_bodies, c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, vel,
+                           omega, c->s.rot, inv_i, lam, c->s.solve_ctl, counters);
+    } else if (n_bodies <= c->solver_lds_bodies) {
+        hipLaunchKernelGGL((k_solve<FORM_LDS, SOLVE_THREADS>), dim3(1), dim3(SOLVE_THREADS), n_bodies * 4u,
+                           c->stream, c->s.misc, iters, n_bodies, c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, vel,
+                           omega, c->s.rot, inv_i, lam, c->s.solve_ctl, counters);
+    } else {
+        fill_u32(c, c->s.ticket, 0u, n_bodies);
+        hipLaunchKernelGGL((k_solve<FORM_GLOBAL, SOLVE_THREADS>), dim3(1), dim3(SOLVE_THREADS), 0, c->stream,
+                           c->s.misc, iters, n_bodies, c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, vel, omega,
+                           c->s.rot, inv_i, lam, c->s.solve_ctl, counters);
     }
-    hipLaunchKernelGGL(k_solve, dim3(1), dim3(SOLVE_THREADS), 0, c->stream, c->s.misc, c->params.pgs_iterations,
-                       c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, B<float>(c, SV_BODY_VEL),
-                       B<float>(c, SV_BODY_OMEGA), c->s.rot, B<float>(c, SV_BODY_INV_INERTIA),
-                       B<float>(c, SV_CONTACT_LAMBDA_N), c->s.solve_ctl, counters);
 }
 
 }  // namespace sv
