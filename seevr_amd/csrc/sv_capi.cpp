@@ -104,7 +104,7 @@ void free_ctx(sv_ctx *c) {
     for (int i = 0; i < SV_BUF_COUNT; i++)
         if (c->buf[i]) (void)hipFree(c->buf[i]);
     sv::Scratch &s = c->s;
-    void *ptrs[] = {s.hrange, s.rot, s.bpbox, s.row_count, s.row_off, s.np_flag, s.np_idx, s.np_res,
+    void *ptrs[] = {s.hrange, s.rot, s.bpbox, s.row_count, s.row_off, s.bp_stage, s.bp_ovf, s.np_flag, s.np_idx, s.np_res,
                     s.cprep, s.inc_key[0], s.inc_key[1], s.inc_val[0], s.inc_val[1], s.deg, s.deg_off,
                     s.rank, s.ticket, s.solve_ctl, s.scan_part, s.sort_hist, s.misc};
     for (void *p : ptrs)
@@ -216,7 +216,7 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
     if (params) c->params = *params; else sv_default_params(&c->params);
     if (const char *wgs = getenv("SEEVR_SOLVER_WGS")) {
         int v = atoi(wgs);
-        c->solver_workgroups = v > 1 ? (uint32_t)v : 1u;
+        c->solver_workgroups = v > 0 ? (uint32_t)v : 0u;
     }
     if ((uint64_t)c->params.pgs_iterations * caps->max_contacts >= 0xFFFFFFFFull) {
         delete c;
@@ -244,6 +244,8 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
     if (e == hipSuccess) e = dalloc(s.bpbox, 2 * N);
     if (e == hipSuccess) e = dalloc(s.row_count, N + 1);
     if (e == hipSuccess) e = dalloc(s.row_off, N + 1);
+    if (e == hipSuccess) e = dalloc(s.bp_stage, N * sv::BP_STAGE);
+    if (e == hipSuccess) e = dalloc(s.bp_ovf, N / 32 + 2);
     if (e == hipSuccess) e = dalloc(s.np_flag, P + 1);
     if (e == hipSuccess) e = dalloc(s.np_idx, P + 1);
     if (e == hipSuccess) e = dalloc(s.np_res, 8 * P + 8);

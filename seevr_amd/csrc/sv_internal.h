@@ -17,6 +17,8 @@ struct Scratch {
     float4 *bpbox = nullptr;      // float4[2N] broadphase boxes: (min, static), (max, 0)
     uint32_t *row_count = nullptr;  // uint[N + 1]
     uint32_t *row_off = nullptr;    // uint[N + 1]
+    uint32_t *bp_stage = nullptr;   // uint[N * BP_STAGE] first partners of each row
+    uint32_t *bp_ovf = nullptr;     // uint[N / 32 + 1] blocks with a row above BP_STAGE
     uint32_t *np_flag = nullptr;    // uint[P] narrowphase hit flags
     uint32_t *np_idx = nullptr;     // uint[P] contact slot of each hit
     float *np_res = nullptr;        // float[8P] pos, normal, depth per pair
@@ -37,6 +39,8 @@ struct Scratch {
 
 // Words per contact in Scratch::cprep (see sv_pgs.hip).
 constexpr int CPREP_WORDS = 20;
+// Broadphase partners staged per row by the test pass (sv_physics.hip).
+constexpr uint32_t BP_STAGE = 32;
 
 }  // namespace sv
 
@@ -48,9 +52,9 @@ struct sv_ctx {
     void *buf[SV_BUF_COUNT] = {};
     size_t bytes[SV_BUF_COUNT] = {};
     sv::Scratch s;
-    // workgroups of the ordered PGS kernel: 1 = single-CU form (k_solve),
-    // > 1 = multi-CU form (k_solve_mc)
-    uint32_t solver_workgroups = 1;
+    // workgroups of the ordered PGS kernel: 0 = chosen by body count,
+    // 1 = single-CU form, > 1 = multi-CU form with that many workgroups
+    uint32_t solver_workgroups = 0;
     // largest body count whose solver tickets fit in one workgroup's LDS
     uint32_t solver_lds_bodies = 0;
     // stage timing

@@ -352,9 +352,13 @@ void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
     float *vel = B<float>(c, SV_BODY_VEL), *omega = B<float>(c, SV_BODY_OMEGA);
     float *inv_i = B<float>(c, SV_BODY_INV_INERTIA), *lam = B<float>(c, SV_CONTACT_LAMBDA_N);
     uint32_t iters = c->params.pgs_iterations;
-    if (c->solver_workgroups > 1) {
+    // Form choice: small scenes keep every hand-off inside one CU; large ones
+    // need the wider in-order window of the multi-CU form (32 workgroups was
+    // fastest on a 32k-body pile, 2M sweep items per substep).
+    uint32_t wgs = c->solver_workgroups ? c->solver_workgroups : (n_bodies >= 2048 ? 32u : 1u);
+    if (wgs > 1) {
         fill_u32(c, c->s.ticket, 0u, n_bodies);
-        hipLaunchKernelGGL((k_solve<FORM_MULTI, MC_THREADS>), dim3(c->solver_workgroups), dim3(MC_THREADS), 0,
+        hipLaunchKernelGGL((k_solve<FORM_MULTI, MC_THREADS>), dim3(wgs), dim3(MC_THREADS), 0,
                            c->stream, c->s.misc, iters, n_bodies, c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, vel,
                            omega, c->s.rot, inv_i, lam, c->s.solve_ctl, counters);
     } else if (n_bodies <= c->solver_lds_bodies) {

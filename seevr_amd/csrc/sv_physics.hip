@@ -86,37 +86,63 @@ __global__ void __launch_bounds__(BLOCK) k_aabb(uint32_t n, const float *__restr
 // ---------------------------------------------------------------------------
 // Broadphase (physics.comp:451-472): every (i, j > i) not both static whose
 // boxes overlap (touching counts).  A block owns 32 consecutive rows, 8 per
-// wave held in scalar registers; it streams j through LDS tiles of 256
-// boxes, tests 64 j per lane-round and turns each row's hits into one
-// 64-bit ballot.  Pass 0 counts hits per row; pass 1 (after an exclusive
-// scan of the counts) writes each row's pairs at its offset in j order,
-// which is the lexicographic (i, j) order.
+// wave; it streams j through LDS tiles of 256 boxes and tests 64 j per
+// lane-round.  Each test is branch-free: for finite IEEE floats
+// max(lo_j - hi_i, lo_i - hi_j) > 0 exactly when hi_i < lo_j or hi_j < lo_i
+// (a difference of two floats is positive iff the first is larger, with
+// gradual underflow), so a row overlaps lane j iff the largest of the three
+// axis separations is not positive.  Lanes past n or at j <= i carry an
+// empty box.  A row's hits in one 64-j round form a 64-bit ballot; hits are
+// numbered in j order, which is the lexicographic (i, j) order.
+//
+// The test pass counts each row's hits and stages its first BP_STAGE
+// partners; after an exclusive scan of the counts, k_bp_place copies staged
+// partners to their final slots, and only blocks with a row of more than
+// BP_STAGE hits run the test again to place the rest (EMIT form).
 constexpr int BP_ROWS_PER_WAVE = 8;
 constexpr int BP_ROWS = BP_ROWS_PER_WAVE * (BLOCK / 64);  // 32
 constexpr int BP_TILE = BLOCK;                              // 256 boxes per LDS tile
+
+SVD bool boxes_apart(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+    float sx = fmaxf(blo.x - ahi.x, alo.x - bhi.x);
+    float sy = fmaxf(blo.y - ahi.y, alo.y - bhi.y);
+    float sz = fmaxf(blo.z - ahi.z, alo.z - bhi.z);
+    return fmaxf(sx, fmaxf(sy, sz)) > 0.0f;
+}
 
 template <bool EMIT>
 __global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, const float4 *__restrict__ box,
                                                       uint32_t *__restrict__ row_count,
                                                       const uint32_t *__restrict__ row_off,
+                                                      uint32_t *__restrict__ stage, uint32_t *__restrict__ blk_ovf,
                                                       uint32_t *__restrict__ pairs, uint32_t max_pairs) {
     __shared__ float4 s_lo[BP_TILE];
     __shared__ float4 s_hi[BP_TILE];
+    if (EMIT && blk_ovf[blockIdx.x] == 0) return;  // every row of this block was staged whole
     const int lane = threadIdx.x & 63;
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t blk_row0 = blockIdx.x * BP_ROWS;
     const uint32_t row0 = blk_row0 + w * BP_ROWS_PER_WAVE;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const float4 empty_lo = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
+    const float4 empty_hi = make_float4(-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), 0.0f);
 
     float4 rlo[BP_ROWS_PER_WAVE], rhi[BP_ROWS_PER_WAVE];
     uint32_t cnt[BP_ROWS_PER_WAVE];
     uint32_t base[BP_ROWS_PER_WAVE];
+    bool any_static = false;
 #pragma unroll
     for (int r = 0; r < BP_ROWS_PER_WAVE; r++) {
         uint32_t i = row0 + r;
         uint32_t ic = i < n ? i : 0u;
         rlo[r] = box[2 * ic];
         rhi[r] = box[2 * ic + 1];
+        if (i >= n) { rlo[r] = empty_lo; rhi[r] = empty_hi; }
+        any_static |= rlo[r].w != 0.0f;
+        // keep the row boxes in vector registers (64 scalar registers would
+        // spill); the empty asm only pins the register class
+        asm volatile("" : "+v"(rlo[r].x), "+v"(rlo[r].y), "+v"(rlo[r].z), "+v"(rhi[r].x), "+v"(rhi[r].y),
+                     "+v"(rhi[r].z));
         cnt[r] = 0;
         base[r] = (EMIT && i < n) ? row_off[i] : 0u;
     }
@@ -127,40 +153,66 @@ __global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, const float4 *
         if (jl < n) {
             s_lo[threadIdx.x] = box[2 * jl];
             s_hi[threadIdx.x] = box[2 * jl + 1];
+        } else {
+            s_lo[threadIdx.x] = empty_lo;
+            s_hi[threadIdx.x] = empty_hi;
         }
         __syncthreads();
+        const bool diag = t0 < row0 + BP_ROWS_PER_WAVE;  // some j <= i in this tile
 #pragma unroll
         for (int ch = 0; ch < BP_TILE / 64; ch++) {
-            uint32_t j = t0 + ch * 64 + lane;
-            bool jin = j < n;
-            float4 lo = s_lo[ch * 64 + lane];
-            float4 hi = s_hi[ch * 64 + lane];
+            const uint32_t j = t0 + ch * 64 + lane;
+            const float4 lo = s_lo[ch * 64 + lane];
+            const float4 hi = s_hi[ch * 64 + lane];
 #pragma unroll
             for (int r = 0; r < BP_ROWS_PER_WAVE; r++) {
-                uint32_t i = row0 + r;
-                bool hit = jin && j > i && i < n && !(rlo[r].w != 0.0f && lo.w != 0.0f) &&
-                           !(rhi[r].x < lo.x || hi.x < rlo[r].x || rhi[r].y < lo.y || hi.y < rlo[r].y ||
-                             rhi[r].z < lo.z || hi.z < rlo[r].z);
-                uint64_t m = __ballot(hit);
-                if (EMIT) {
-                    if (hit) {
-                        uint32_t pos = base[r] + cnt[r] + (uint32_t)__popcll(m & lt);
-                        if (pos < max_pairs) {
-                            pairs[2 * pos] = i;
-                            pairs[2 * pos + 1] = j;
+                const uint32_t i = row0 + r;
+                bool hit = !boxes_apart(rlo[r], rhi[r], lo, hi);
+                if (diag) hit = hit && j > i;
+                if (any_static) hit = hit && !(rlo[r].w != 0.0f && lo.w != 0.0f);
+                const uint64_t m = __ballot(hit);
+                if (m) {
+                    const uint32_t k = cnt[r] + (uint32_t)__popcll(m & lt);
+                    if (EMIT) {
+                        if (hit && k >= BP_STAGE) {
+                            uint32_t pos = base[r] + k;
+                            if (pos < max_pairs) {
+                                pairs[2 * pos] = i;
+                                pairs[2 * pos + 1] = j;
+                            }
                         }
+                    } else if (hit && k < BP_STAGE) {
+                        stage[(size_t)i * BP_STAGE + k] = j;
                     }
+                    cnt[r] += (uint32_t)__popcll(m);
                 }
-                cnt[r] += (uint32_t)__popcll(m);
             }
         }
     }
     if (!EMIT && lane == 0) {
+        bool ovf = false;
 #pragma unroll
         for (int r = 0; r < BP_ROWS_PER_WAVE; r++) {
             uint32_t i = row0 + r;
             if (i < n) row_count[i] = cnt[r];
+            ovf |= cnt[r] > BP_STAGE;
         }
+        if (ovf) atomicOr(&blk_ovf[blockIdx.x], 1u);
+    }
+}
+
+// Staged partners of row i to slots row_off[i] + k (k < min(count, BP_STAGE)).
+__global__ void __launch_bounds__(BLOCK) k_bp_place(uint32_t n, const uint32_t *__restrict__ row_count,
+                                                    const uint32_t *__restrict__ row_off,
+                                                    const uint32_t *__restrict__ stage, uint32_t *__restrict__ pairs,
+                                                    uint32_t max_pairs) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = t / BP_STAGE, k = t % BP_STAGE;
+    if (i >= n || k >= row_count[i]) return;
+    uint32_t pos = row_off[i] + k;
+    if (pos < max_pairs) {
+        pairs[2 * pos] = i;
+        pairs[2 * pos + 1] = stage[(size_t)i * BP_STAGE + k];
     }
 }
 
@@ -586,12 +638,16 @@ void launch_broadphase(sv_ctx *c, uint32_t n) {
         return;
     }
     uint32_t grid = div_up(n, BP_ROWS);
+    uint32_t *pairs = B<uint32_t>(c, SV_COLLISION_PAIRS);
+    fill_u32(c, c->s.bp_ovf, 0u, grid);
     hipLaunchKernelGGL(k_broadphase<false>, dim3(grid), dim3(BLOCK), 0, c->stream, n, c->s.bpbox, c->s.row_count,
-                       (const uint32_t *)nullptr, (uint32_t *)nullptr, c->caps.max_pairs);
+                       (const uint32_t *)nullptr, c->s.bp_stage, c->s.bp_ovf, (uint32_t *)nullptr, c->caps.max_pairs);
     scan_exclusive(c, c->s.row_count, c->s.row_off, nullptr, n, counters);
     hipLaunchKernelGGL(k_post_pairs, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, c->caps.max_pairs);
+    hipLaunchKernelGGL(k_bp_place, dim3(div_up((uint64_t)n * BP_STAGE, BLOCK)), dim3(BLOCK), 0, c->stream, n,
+                       c->s.row_count, c->s.row_off, c->s.bp_stage, pairs, c->caps.max_pairs);
     hipLaunchKernelGGL(k_broadphase<true>, dim3(grid), dim3(BLOCK), 0, c->stream, n, c->s.bpbox,
-                       (uint32_t *)nullptr, c->s.row_off, B<uint32_t>(c, SV_COLLISION_PAIRS), c->caps.max_pairs);
+                       (uint32_t *)nullptr, c->s.row_off, c->s.bp_stage, c->s.bp_ovf, pairs, c->caps.max_pairs);
 }
 
 void launch_narrowphase(sv_ctx *c) {
