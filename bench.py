@@ -46,15 +46,64 @@ def build_workload(count: int, seed: int):
     return sc
 
 
-def algorithmic_bytes(stage: str, n: int, n_hull: int, n_dyn: int) -> float:
-    """Bytes each stage must move at minimum (per launch)."""
+PROFILE = ROOT / "profiles" / "c2_32k_container_kernels.md"
+STAGE_KERNEL = {"phys/pgs_sweep": "k_solve<2, 256>", "phys/aabb": "k_aabb", "phys/gravity": "k_gravity",
+                "phys/integrate": "k_integrate", "phys/broad_phase": "k_broadphase<false>"}
+
+
+def algorithmic_bytes(stage: str, n: int, n_hull: int, n_dyn: int, pairs: int, nc: int) -> float:
+    """Bytes each stage must move at minimum per launch.
+
+    aabb: pos 12 + quat 16 + hull range 8 per body read, 12 per hull vertex
+    read, 24 per body written.  broadphase: 28 per body read (box + static
+    flag) + 8 per pair written.  gravity 52, integrate 84 per dynamic body.
+    Ordered PGS sweep: per item (iteration x contact) the two bodies'
+    velocities and angular velocities read and written (96 B), the
+    accumulated impulse read and written (8 B) and the contact's 20-word
+    constant record read (80 B): 184 B.
+    """
     if stage == "phys/aabb":
         return 36.0 * n + 12.0 * n_hull + 24.0 * n
     if stage == "phys/gravity":
         return 52.0 * n_dyn
     if stage == "phys/integrate":
         return 84.0 * n_dyn
+    if stage == "phys/broad_phase":
+        return 28.0 * n + 8.0 * pairs
+    if stage == "phys/pgs_sweep":
+        return 184.0 * PGS_ITERS * nc
     return 0.0
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary, scaled by the copy-kernel calibration in the same file."""
+    if not PROFILE.exists():
+        return None
+    rows = {}
+    for line in PROFILE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0].startswith(("k_", "__")):
+            try:
+                rows[cells[0]] = (float(cells[2]), float(cells[3]))
+            except ValueError:
+                pass
+    if kernel not in rows or "k_copy_calib" not in rows:
+        return None
+    cal_f, cal_w = rows["k_copy_calib"]
+    true_kib = (256 << 20) / 1024.0
+    f, w = rows[kernel]
+    return 1024.0 * (f * true_kib / cal_f + w * true_kib / cal_w)
+
+
+def roofline_entry(stage, stages, n, nh, n_dyn, pairs, nc):
+    us = stages.get(stage, (0.0, 0))[0]
+    alg = algorithmic_bytes(stage, n, nh, n_dyn, pairs, nc)
+    achieved = alg / (us * 1e-6) / 1e9 if us > 0 else 0.0
+    kern = STAGE_KERNEL.get(stage, stage)
+    return {"bound": "hbm", "kernel": f"{kern} ({stage})", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": profiled_traffic(kern),
+            "avg_us": round(us, 2), "alg_bytes": int(alg)}
 
 
 def cpu_baseline(ctx, scene, n, substeps: int) -> dict:
@@ -120,32 +169,29 @@ def main() -> int:
     log("stage breakdown (us/launch, launches): " +
         ", ".join(f"{k}={v[0]:.1f}x{v[1]}" for k, v in sorted(stages.items(), key=lambda kv: -kv[1][0] * kv[1][1])))
 
-    # timed region: K frames, device synchronized on both sides (each call is synchronous)
+    # Timed region: K frames, device synchronized on both sides (every call is
+    # synchronous).  HIP events on the library stream bracket each stage in
+    # the same frames; they feed the roofline below.
+    ctx.enable_stage_timing(True)
+    ctx.stage_times(reset=True)
     t0 = time.perf_counter()
     for i in range(args.steps):
         ctx.step(n, nv, nh, SUBSTEPS, DT, GRAVITY)
     el = time.perf_counter() - t0
+    stages = ctx.stage_times(reset=True)
+    ctx.enable_stage_timing(False)
     pairs, contacts = ctx.read_counters()
     ms = 1000.0 * el / args.steps
     value = n * SUBSTEPS * args.steps / el
     log(f"timed {args.steps} frames in {el:.2f} s: {ms:.2f} ms/frame, pairs {pairs}, contacts {contacts}")
 
-    # roofline of the dominant HBM-bound stage, measured with HIP events on the library stream
-    ctx.enable_stage_timing(True)
-    ctx.stage_times(reset=True)
-    for i in range(3):
-        ctx.step(n, nv, nh, SUBSTEPS, DT, GRAVITY)
-    stages = ctx.stage_times(reset=True)
-    ctx.enable_stage_timing(False)
-    dom = "phys/aabb"
-    us = stages[dom][0]
-    alg = algorithmic_bytes(dom, n, nh, n_dyn)
-    achieved = alg / (us * 1e-6) / 1e9
-    roof = {"bound": "hbm", "kernel": "k_aabb (phys/aabb)", "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "avg_us": round(us, 2), "alg_bytes": int(alg), "copy_kernel_gbs": round(copy_gbs, 1)}
-    total_us = sum(v[0] * v[1] for v in stages.values())
-    breakdown = {k: round(100.0 * v[0] * v[1] / total_us, 1) for k, v in stages.items()}
+    nc = min(contacts, caps.max_contacts)
+    roof = roofline_entry("phys/pgs_sweep", stages, n, nh, n_dyn, pairs, nc)
+    hbm_stages = {k: roofline_entry(k, stages, n, nh, n_dyn, pairs, nc)
+                  for k in ("phys/aabb", "phys/broad_phase", "phys/integrate", "phys/gravity")}
+    roof["copy_kernel_gbs"] = round(copy_gbs, 1)
+    total_us = sum(v[0] * v[1] for k, v in stages.items() if k != "phys/pgs_sweep")
+    breakdown = {k: round(100.0 * v[0] * v[1] / total_us, 1) for k, v in stages.items() if k != "phys/pgs_sweep"}
 
     cpu = None
     if not args.no_cpu:
@@ -163,7 +209,7 @@ def main() -> int:
                    "bodies": n, "hull_verts": nh, "substeps": SUBSTEPS, "pgs_iterations": PGS_ITERS,
                    "frames_before_timing": args.warmup, "pairs_last_substep": pairs,
                    "contacts_last_substep": contacts},
-        "roofline": roof, "cpu_baseline": cpu, "stage_share_pct": breakdown,
+        "roofline": roof, "roofline_hbm_stages": hbm_stages, "cpu_baseline": cpu, "stage_share_pct": breakdown,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -356,6 +356,7 @@ void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
     // need the wider in-order window of the multi-CU form (32 workgroups was
     // fastest on a 32k-body pile, 2M sweep items per substep).
     uint32_t wgs = c->solver_workgroups ? c->solver_workgroups : (n_bodies >= 2048 ? 32u : 1u);
+    StageTimer sweep_timer(c, "phys/pgs_sweep");
     if (wgs > 1) {
         fill_u32(c, c->s.ticket, 0u, n_bodies);
         hipLaunchKernelGGL((k_solve<FORM_MULTI, MC_THREADS>), dim3(wgs), dim3(MC_THREADS), 0,
