@@ -242,10 +242,11 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
     if (e == hipSuccess) e = dalloc(s.hrange, 2 * N);
     if (e == hipSuccess) e = dalloc(s.rot, 9 * N);
     if (e == hipSuccess) e = dalloc(s.bpbox, 2 * N);
-    if (e == hipSuccess) e = dalloc(s.row_count, N + 1);
-    if (e == hipSuccess) e = dalloc(s.row_off, N + 1);
-    if (e == hipSuccess) e = dalloc(s.bp_stage, N * sv::BP_STAGE);
-    if (e == hipSuccess) e = dalloc(s.bp_ovf, N / 32 + 2);
+    const size_t NSEG = sv::bp_nseg((uint32_t)N);
+    if (e == hipSuccess) e = dalloc(s.row_count, N * NSEG + 1);
+    if (e == hipSuccess) e = dalloc(s.row_off, N * NSEG + 1);
+    if (e == hipSuccess) e = dalloc(s.bp_stage, N * NSEG * sv::BP_STAGE);
+    if (e == hipSuccess) e = dalloc(s.bp_ovf, (N / sv::BP_ROWS + 1) * NSEG);
     if (e == hipSuccess) e = dalloc(s.np_flag, P + 1);
     if (e == hipSuccess) e = dalloc(s.np_idx, P + 1);
     if (e == hipSuccess) e = dalloc(s.np_res, 8 * P + 8);
@@ -267,7 +268,7 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
         return fail(SV_E_NOMEM, "sv_create: device allocation failed: %s", hipGetErrorString(e));
     }
     size_t NP = std::max<size_t>(k.max_triangles, N);
-    sv::prims_reserve(c, (uint32_t)std::max<size_t>(N + 1, P), (uint32_t)std::max<size_t>(2 * C, NP));
+    sv::prims_reserve(c, (uint32_t)std::max<size_t>(N * NSEG + 1, P), (uint32_t)std::max<size_t>(2 * C, NP));
     if (!s.scan_part || !s.sort_hist) {
         free_ctx(c);
         return fail(SV_E_NOMEM, "sv_create: scratch allocation failed");

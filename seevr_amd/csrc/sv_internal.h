@@ -15,10 +15,10 @@ struct Scratch {
     uint32_t *hrange = nullptr;   // uint[2N] hull (start, count) read by the kernels
     float *rot = nullptr;         // float[9N] per-body rotation columns of this substep
     float4 *bpbox = nullptr;      // float4[2N] broadphase boxes: (min, static), (max, 0)
-    uint32_t *row_count = nullptr;  // uint[N + 1]
-    uint32_t *row_off = nullptr;    // uint[N + 1]
-    uint32_t *bp_stage = nullptr;   // uint[N * BP_STAGE] first partners of each row
-    uint32_t *bp_ovf = nullptr;     // uint[N / 32 + 1] blocks with a row above BP_STAGE
+    uint32_t *row_count = nullptr;  // uint[N * nseg + 1] broadphase hits per (row, segment)
+    uint32_t *row_off = nullptr;    // uint[N * nseg + 1] their exclusive scan
+    uint32_t *bp_stage = nullptr;   // uint[N * nseg * BP_STAGE] first partners per (row, segment)
+    uint32_t *bp_ovf = nullptr;     // uint[ceil(N / 32) * nseg] tiles with a (row, segment) above BP_STAGE
     uint32_t *np_flag = nullptr;    // uint[P] narrowphase hit flags
     uint32_t *np_idx = nullptr;     // uint[P] contact slot of each hit
     float *np_res = nullptr;        // float[8P] pos, normal, depth per pair
@@ -39,8 +39,17 @@ struct Scratch {
 
 // Words per contact in Scratch::cprep (see sv_pgs.hip).
 constexpr int CPREP_WORDS = 20;
-// Broadphase partners staged per row by the test pass (sv_physics.hip).
-constexpr uint32_t BP_STAGE = 32;
+// Broadphase tiling (sv_physics.hip): rows per tile, partners staged per
+// (row, column segment), and the column segment length for n bodies (at
+// least 2048, and at most 64 segments).
+constexpr uint32_t BP_ROWS = 32;
+constexpr uint32_t BP_STAGE = 4;
+inline uint32_t bp_seg_len(uint32_t n) {
+    uint32_t s = (n + 63) / 64;
+    s = (s + 255) / 256 * 256;
+    return s < 2048u ? 2048u : s;
+}
+inline uint32_t bp_nseg(uint32_t n) { return n ? (n + bp_seg_len(n) - 1) / bp_seg_len(n) : 1u; }
 
 }  // namespace sv
 

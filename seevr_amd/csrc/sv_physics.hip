@@ -85,23 +85,27 @@ __global__ void __launch_bounds__(BLOCK) k_aabb(uint32_t n, const float *__restr
 
 // ---------------------------------------------------------------------------
 // Broadphase (physics.comp:451-472): every (i, j > i) not both static whose
-// boxes overlap (touching counts).  A block owns 32 consecutive rows, 8 per
-// wave; it streams j through LDS tiles of 256 boxes and tests 64 j per
+// boxes overlap (touching counts).
+//
+// Work is tiled in two dimensions so the triangle balances: a block takes
+// 32 consecutive rows (8 per wave) against one column segment of
+// bp_seg_len(n) bodies, streamed through LDS 256 boxes at a time, 64 j per
 // lane-round.  Each test is branch-free: for finite IEEE floats
 // max(lo_j - hi_i, lo_i - hi_j) > 0 exactly when hi_i < lo_j or hi_j < lo_i
 // (a difference of two floats is positive iff the first is larger, with
-// gradual underflow), so a row overlaps lane j iff the largest of the three
-// axis separations is not positive.  Lanes past n or at j <= i carry an
-// empty box.  A row's hits in one 64-j round form a 64-bit ballot; hits are
-// numbered in j order, which is the lexicographic (i, j) order.
+// gradual underflow), so row i overlaps lane j iff the largest of the three
+// axis separations is not positive.  Lanes past the segment or n carry an
+// empty box.  A row's hits in one round form a 64-bit ballot numbered in j
+// order.
 //
-// The test pass counts each row's hits and stages its first BP_STAGE
-// partners; after an exclusive scan of the counts, k_bp_place copies staged
-// partners to their final slots, and only blocks with a row of more than
-// BP_STAGE hits run the test again to place the rest (EMIT form).
+// The test pass writes, per (row, segment), the hit count and the first
+// BP_STAGE partners.  An exclusive scan over the counts in (row, segment)
+// order gives every (row, segment) its slot range in lexicographic (i, j)
+// order; k_bp_place copies staged partners there, and only tiles holding a
+// (row, segment) with more than BP_STAGE hits are tested again to place the
+// rest (EMIT form).
 constexpr int BP_ROWS_PER_WAVE = 8;
-constexpr int BP_ROWS = BP_ROWS_PER_WAVE * (BLOCK / 64);  // 32
-constexpr int BP_TILE = BLOCK;                              // 256 boxes per LDS tile
+constexpr int BP_TILE = BLOCK;  // 256 boxes per LDS tile
 
 SVD bool boxes_apart(float4 alo, float4 ahi, float4 blo, float4 bhi) {
     float sx = fmaxf(blo.x - ahi.x, alo.x - bhi.x);
@@ -111,17 +115,27 @@ SVD bool boxes_apart(float4 alo, float4 ahi, float4 blo, float4 bhi) {
 }
 
 template <bool EMIT>
-__global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, const float4 *__restrict__ box,
-                                                      uint32_t *__restrict__ row_count,
-                                                      const uint32_t *__restrict__ row_off,
-                                                      uint32_t *__restrict__ stage, uint32_t *__restrict__ blk_ovf,
+__global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, uint32_t nseg, uint32_t seg_len,
+                                                      const float4 *__restrict__ box, uint32_t *__restrict__ cnt_out,
+                                                      const uint32_t *__restrict__ off,
+                                                      uint32_t *__restrict__ stage, uint32_t *__restrict__ tile_ovf,
                                                       uint32_t *__restrict__ pairs, uint32_t max_pairs) {
     __shared__ float4 s_lo[BP_TILE];
     __shared__ float4 s_hi[BP_TILE];
-    if (EMIT && blk_ovf[blockIdx.x] == 0) return;  // every row of this block was staged whole
+    const uint32_t seg = blockIdx.y;
+    const uint32_t blk_row0 = blockIdx.x * BP_ROWS;
+    const uint32_t j_lo = max(seg * seg_len, blk_row0 + 1);
+    const uint32_t j_hi = min(seg * seg_len + seg_len, n);
+    const uint32_t tile = blockIdx.x * nseg + seg;
+    if (EMIT) {
+        if (j_lo >= j_hi || tile_ovf[tile] == 0) return;
+    } else if (j_lo >= j_hi) {  // no j > i in this segment: zero its counts
+        uint32_t i = blk_row0 + threadIdx.x;
+        if (threadIdx.x < BP_ROWS && i < n) cnt_out[(size_t)i * nseg + seg] = 0;
+        return;
+    }
     const int lane = threadIdx.x & 63;
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t blk_row0 = blockIdx.x * BP_ROWS;
     const uint32_t row0 = blk_row0 + w * BP_ROWS_PER_WAVE;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const float4 empty_lo = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
@@ -144,13 +158,13 @@ __global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, const float4 *
         asm volatile("" : "+v"(rlo[r].x), "+v"(rlo[r].y), "+v"(rlo[r].z), "+v"(rhi[r].x), "+v"(rhi[r].y),
                      "+v"(rhi[r].z));
         cnt[r] = 0;
-        base[r] = (EMIT && i < n) ? row_off[i] : 0u;
+        base[r] = (EMIT && i < n) ? off[(size_t)i * nseg + seg] : 0u;
     }
 
-    for (uint32_t t0 = blk_row0 + 1; t0 < n; t0 += BP_TILE) {
+    for (uint32_t t0 = j_lo; t0 < j_hi; t0 += BP_TILE) {
         __syncthreads();
         uint32_t jl = t0 + threadIdx.x;
-        if (jl < n) {
+        if (jl < j_hi) {
             s_lo[threadIdx.x] = box[2 * jl];
             s_hi[threadIdx.x] = box[2 * jl + 1];
         } else {
@@ -182,7 +196,7 @@ __global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, const float4 *
                             }
                         }
                     } else if (hit && k < BP_STAGE) {
-                        stage[(size_t)i * BP_STAGE + k] = j;
+                        stage[((size_t)i * nseg + seg) * BP_STAGE + k] = j;
                     }
                     cnt[r] += (uint32_t)__popcll(m);
                 }
@@ -194,25 +208,26 @@ __global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, const float4 *
 #pragma unroll
         for (int r = 0; r < BP_ROWS_PER_WAVE; r++) {
             uint32_t i = row0 + r;
-            if (i < n) row_count[i] = cnt[r];
+            if (i < n) cnt_out[(size_t)i * nseg + seg] = cnt[r];
             ovf |= cnt[r] > BP_STAGE;
         }
-        if (ovf) atomicOr(&blk_ovf[blockIdx.x], 1u);
+        if (ovf) tile_ovf[tile] = 1u;
     }
 }
 
-// Staged partners of row i to slots row_off[i] + k (k < min(count, BP_STAGE)).
-__global__ void __launch_bounds__(BLOCK) k_bp_place(uint32_t n, const uint32_t *__restrict__ row_count,
-                                                    const uint32_t *__restrict__ row_off,
+// Staged partners of (row i, segment s) to slots off[i*nseg+s] + k.
+__global__ void __launch_bounds__(BLOCK) k_bp_place(uint32_t n, uint32_t nseg, const uint32_t *__restrict__ cnt,
+                                                    const uint32_t *__restrict__ off,
                                                     const uint32_t *__restrict__ stage, uint32_t *__restrict__ pairs,
                                                     uint32_t max_pairs) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t i = t / BP_STAGE, k = t % BP_STAGE;
-    if (i >= n || k >= row_count[i]) return;
-    uint32_t pos = row_off[i] + k;
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t rs = t / BP_STAGE;
+    uint32_t k = (uint32_t)(t % BP_STAGE);
+    if (rs >= (size_t)n * nseg || k >= cnt[rs]) return;
+    uint32_t pos = off[rs] + k;
     if (pos < max_pairs) {
-        pairs[2 * pos] = i;
-        pairs[2 * pos + 1] = stage[(size_t)i * BP_STAGE + k];
+        pairs[2 * pos] = (uint32_t)(rs / nseg);
+        pairs[2 * pos + 1] = stage[rs * BP_STAGE + k];
     }
 }
 
@@ -637,16 +652,19 @@ void launch_broadphase(sv_ctx *c, uint32_t n) {
         hipLaunchKernelGGL(k_post_pairs, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, c->caps.max_pairs);
         return;
     }
-    uint32_t grid = div_up(n, BP_ROWS);
+    const uint32_t seg_len = bp_seg_len(n), nseg = bp_nseg(n);
+    const uint32_t nrb = div_up(n, BP_ROWS);
     uint32_t *pairs = B<uint32_t>(c, SV_COLLISION_PAIRS);
-    fill_u32(c, c->s.bp_ovf, 0u, grid);
-    hipLaunchKernelGGL(k_broadphase<false>, dim3(grid), dim3(BLOCK), 0, c->stream, n, c->s.bpbox, c->s.row_count,
-                       (const uint32_t *)nullptr, c->s.bp_stage, c->s.bp_ovf, (uint32_t *)nullptr, c->caps.max_pairs);
-    scan_exclusive(c, c->s.row_count, c->s.row_off, nullptr, n, counters);
+    fill_u32(c, c->s.bp_ovf, 0u, (size_t)nrb * nseg);
+    hipLaunchKernelGGL(k_broadphase<false>, dim3(nrb, nseg), dim3(BLOCK), 0, c->stream, n, nseg, seg_len, c->s.bpbox,
+                       c->s.row_count, (const uint32_t *)nullptr, c->s.bp_stage, c->s.bp_ovf, (uint32_t *)nullptr,
+                       c->caps.max_pairs);
+    const uint32_t ncnt = n * nseg;
+    scan_exclusive(c, c->s.row_count, c->s.row_off, nullptr, ncnt, counters);
     hipLaunchKernelGGL(k_post_pairs, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, c->caps.max_pairs);
-    hipLaunchKernelGGL(k_bp_place, dim3(div_up((uint64_t)n * BP_STAGE, BLOCK)), dim3(BLOCK), 0, c->stream, n,
-                       c->s.row_count, c->s.row_off, c->s.bp_stage, pairs, c->caps.max_pairs);
-    hipLaunchKernelGGL(k_broadphase<true>, dim3(grid), dim3(BLOCK), 0, c->stream, n, c->s.bpbox,
+    hipLaunchKernelGGL(k_bp_place, dim3(div_up((uint64_t)ncnt * BP_STAGE, BLOCK)), dim3(BLOCK), 0, c->stream, n,
+                       nseg, c->s.row_count, c->s.row_off, c->s.bp_stage, pairs, c->caps.max_pairs);
+    hipLaunchKernelGGL(k_broadphase<true>, dim3(nrb, nseg), dim3(BLOCK), 0, c->stream, n, nseg, seg_len, c->s.bpbox,
                        (uint32_t *)nullptr, c->s.row_off, c->s.bp_stage, c->s.bp_ovf, pairs, c->caps.max_pairs);
 }
 
