@@ -132,6 +132,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-sample-substeps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--solver", default="auto", choices=["auto", "ordered", "flow"],
+                    help="schedule of the contact sweep (all give the serial result bit for bit)")
+    ap.add_argument("--solver-wgs", type=int, default=0, help="workgroups of the sweep kernel (0 = default)")
     ap.add_argument("--calib", action="store_true",
                     help="also run the 256 MiB copy kernel once (byte-counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -152,6 +155,7 @@ def main() -> int:
     caps.max_contacts = 6 * n + 1024
     ctx = Context(caps, default_params(pgs_iterations=PGS_ITERS))
     scene.upload(ctx)
+    ctx.set_solver(args.solver, args.solver_wgs)
     name, arch = ctx.device_info()
     log(f"device: {name} {arch}")
     copy_gbs = ctx.copy_bandwidth(256 << 20, 1 if args.calib else 20)

@@ -222,6 +222,17 @@ int sv_stage_times(sv_ctx *ctx, int reset, const char **names, double *avg_us,
                    uint64_t *calls, int cap);
 int sv_enable_stage_timing(sv_ctx *ctx, int enable);
 
+/* Schedule of the contact sweep (step 5).  Every schedule gives the result
+ * of the reference's serial loop (physics.comp:523-581) bit for bit; they
+ * differ only in speed.  No counterpart in the reference, which runs the
+ * sweep on one invocation.  SV_SOLVER_AUTO: ordered, one workgroup below
+ * 2048 bodies, 32 workgroups from there.  SV_SOLVER_ORDERED: tickets per
+ * body, items taken in sweep order by `workgroups` workgroups (1 = one CU).
+ * SV_SOLVER_FLOW: per-item countdowns, items run as their predecessors
+ * finish, `workgroups` workgroups (0 = 64). */
+enum { SV_SOLVER_AUTO = 0, SV_SOLVER_ORDERED = 1, SV_SOLVER_FLOW = 2 };
+int sv_set_solver(sv_ctx *ctx, int schedule, uint32_t workgroups);
+
 /* ---- host scene construction (src/data.zig) ------------------------- */
 typedef struct sv_scene sv_scene;
 

@@ -150,6 +150,12 @@ class Context:
     def enable_stage_timing(self, on: bool = True) -> None:
         check(self.lib.sv_enable_stage_timing(self.h, int(on)))
 
+    SOLVERS = {"auto": 0, "ordered": 1, "flow": 2}
+
+    def set_solver(self, schedule: str = "auto", workgroups: int = 0) -> None:
+        """Schedule of the contact sweep: "auto", "ordered" or "flow" (all bit-identical)."""
+        check(self.lib.sv_set_solver(self.h, self.SOLVERS[schedule], workgroups))
+
     def stage_times(self, reset: bool = False) -> dict:
         cap = 64
         names = (C.c_char_p * cap)()

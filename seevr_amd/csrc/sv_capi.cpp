@@ -106,7 +106,7 @@ void free_ctx(sv_ctx *c) {
     sv::Scratch &s = c->s;
     void *ptrs[] = {s.hrange, s.rot, s.bpbox, s.row_count, s.row_off, s.bp_stage, s.bp_ovf, s.np_flag, s.np_idx, s.np_res,
                     s.cprep, s.inc_key[0], s.inc_key[1], s.inc_val[0], s.inc_val[1], s.deg, s.deg_off,
-                    s.rank, s.ticket, s.solve_ctl, s.scan_part, s.sort_hist, s.misc};
+                    s.rank, s.ticket, s.solve_ctl, s.flow_count, s.flow_queue, s.scan_part, s.sort_hist, s.misc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -218,6 +218,7 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
         int v = atoi(wgs);
         c->solver_workgroups = v > 0 ? (uint32_t)v : 0u;
     }
+    if (const char *f = getenv("SEEVR_SOLVER")) c->solver_flow = strcmp(f, "flow") == 0;
     if ((uint64_t)c->params.pgs_iterations * caps->max_contacts >= 0xFFFFFFFFull) {
         delete c;
         return fail(SV_E_INVALID, "sv_create: pgs_iterations * max_contacts must stay below 2^32");
@@ -458,6 +459,16 @@ int sv_device_info(sv_ctx *c, char *name, size_t name_len, char *arch, size_t ar
 int sv_enable_stage_timing(sv_ctx *c, int enable) {
     if (!c) return fail(SV_E_INVALID, "sv_enable_stage_timing: null context");
     c->timing = enable != 0;
+    return SV_OK;
+}
+
+int sv_set_solver(sv_ctx *c, int schedule, uint32_t workgroups) {
+    if (!c) return fail(SV_E_INVALID, "sv_set_solver: null context");
+    if (schedule < SV_SOLVER_AUTO || schedule > SV_SOLVER_FLOW)
+        return fail(SV_E_INVALID, "sv_set_solver: unknown schedule %d", schedule);
+    if (workgroups > 4096) return fail(SV_E_INVALID, "sv_set_solver: at most 4096 workgroups");
+    c->solver_flow = schedule == SV_SOLVER_FLOW;
+    c->solver_workgroups = schedule == SV_SOLVER_AUTO ? 0u : (workgroups ? workgroups : (c->solver_flow ? 0u : 1u));
     return SV_OK;
 }
 

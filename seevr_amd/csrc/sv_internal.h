@@ -29,7 +29,11 @@ struct Scratch {
     uint32_t *deg_off = nullptr;    // uint[N + 1]
     uint32_t *rank = nullptr;       // uint[2C] rank of each incidence in its body
     uint32_t *ticket = nullptr;     // uint[N] completed updates per body (solver)
-    uint32_t *solve_ctl = nullptr;  // uint[4]: [0] abort flag, [1] scratch
+    uint32_t *solve_ctl = nullptr;  // uint[4]: [0] abort flag, [1..3] solver counters
+    uint32_t *flow_count = nullptr; // uint[iters * C] flow-form countdowns (allocated on first use)
+    uint32_t flow_count_cap = 0;
+    uint32_t *flow_queue = nullptr; // uint[iters * C] flow-form ready queue
+    uint32_t flow_queue_cap = 0;
     uint32_t *scan_part = nullptr;  // scan partials
     uint32_t scan_part_cap = 0;
     uint32_t *sort_hist = nullptr;  // radix histograms
@@ -66,6 +70,9 @@ struct sv_ctx {
     uint32_t solver_workgroups = 0;
     // largest body count whose solver tickets fit in one workgroup's LDS
     uint32_t solver_lds_bodies = 0;
+    // sweep schedule: false = ordered (tickets, items taken in sweep order),
+    // true = flow (countdowns, items run as their predecessors finish)
+    bool solver_flow = false;
     // stage timing
     bool timing = false;
     std::vector<std::string> stage_names;
@@ -87,6 +94,7 @@ enum : int {
     MISC_TMP = 3,
     MISC_SCAN_TOTAL = 4,
     MISC_DEG_TOTAL = 5,
+    MISC_NACT = 6,     // contacts the sweep updates (not both inverse masses zero)
 };
 
 // ---- sv_prims.hip ----

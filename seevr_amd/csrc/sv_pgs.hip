@@ -46,8 +46,15 @@ inline T *B(sv_ctx *c, int id) { return reinterpret_cast<T *>(c->buf[id]); }
 // cprep word offsets
 enum : int {
     P_N = 0, P_RA = 3, P_RB = 6, P_KN = 9, P_BIAS = 10, P_MU = 11, P_IMA = 12, P_IMB = 13,
-    P_A = 14, P_B = 15, P_FLAGS = 16
+    P_A = 14, P_B = 15, P_FLAGS = 16, P_SUCC_A = 17, P_SUCC_B = 18
 };
+// Successor words (flow form): the contact that next updates the same body,
+// in the same iteration or (WRAP) in the next one; HASPRED marks a contact
+// with an earlier update of that body in iteration 0.
+constexpr uint32_t SUCC_NONE = 0xFFFFFFFFu;
+constexpr uint32_t SUCC_WRAP = 0x80000000u;
+constexpr uint32_t SUCC_HASPRED = 0x40000000u;
+constexpr uint32_t SUCC_ID = 0x3FFFFFFFu;
 enum : uint32_t { F_SKIP = 1u, F_DYN_A = 2u, F_DYN_B = 4u };
 
 SVD f3 apply_inv_I(const m3 &R, f3 inv_i, f3 v) { return mvec(R, hmul(inv_i, mtvec(R, v))); }
@@ -65,14 +72,14 @@ __global__ void k_solve_count(const uint32_t *counters, uint32_t *misc, uint32_t
 // orientations, which the sweep does not change, so it equals the value the
 // serial loop recomputes every iteration (physics.comp:544-546).
 __global__ void __launch_bounds__(BLOCK) k_prep(
-    const uint32_t *__restrict__ misc, uint32_t n_bodies, float dt, float baumgarte, float slop,
+    uint32_t *__restrict__ misc_w, uint32_t n_bodies, float dt, float baumgarte, float slop,
     const uint32_t *__restrict__ cga, const uint32_t *__restrict__ cgb, const uint32_t *__restrict__ gbody,
     const float *__restrict__ cpos, const float *__restrict__ cnrm, const float *__restrict__ cpen,
     const float *__restrict__ pos, const float *__restrict__ vel, const float *__restrict__ omega,
     const float *__restrict__ inv_mass, const float *__restrict__ inv_i, const float *__restrict__ rot,
     const float *__restrict__ fric, const float *__restrict__ rest, float *__restrict__ prep,
     uint32_t *__restrict__ key, uint32_t *__restrict__ val, uint32_t *__restrict__ deg) {
-    uint32_t nc = misc[MISC_NC];
+    uint32_t nc = misc_w[MISC_NC];
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         uint32_t ga = cga[c], gb = cgb[c];
         uint32_t a = gbody[ga], b = gbody[gb];
@@ -105,6 +112,9 @@ __global__ void __launch_bounds__(BLOCK) k_prep(
         P[P_A] = __uint_as_float(a);
         P[P_B] = __uint_as_float(b);
         P[P_FLAGS] = __uint_as_float(flags);
+        P[P_SUCC_A] = __uint_as_float(SUCC_NONE);
+        P[P_SUCC_B] = __uint_as_float(SUCC_NONE);
+        if (!skip) atomicAdd(&misc_w[MISC_NACT], 1u);
         bool ia = !skip && dyn_a, ib = !skip && dyn_b;
         key[2 * c] = ia ? a : n_bodies;
         val[2 * c] = 2 * c;
@@ -311,6 +321,195 @@ __global__ void __launch_bounds__(THREADS) k_solve(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Flow form.  Item (t, c) has one predecessor per dynamic body (the previous
+// update of that body in sweep order: none for the body's first contact in
+// iteration 0) and one successor per dynamic body.  A countdown per item
+// starts at its predecessor count; a finishing item decrements both
+// successors' countdowns, and a successor whose countdown reaches zero is
+// ready: the finishing lane runs it next, a second ready successor goes to a
+// queue, and lanes without work take items from the queue in push order.
+// Ready items share no dynamic body, so any interleaving of them gives the
+// serial result.  Every hand-off is a device-scope release (before the
+// decrement or the queue store) and acquire (after the decrement that reached
+// zero or the load that found the queue slot filled).
+
+// Successor words from the sorted incidences: position p of body x's
+// segment is followed by p + 1, the segment's last position wraps to its
+// first in the next iteration.
+__global__ void __launch_bounds__(BLOCK) k_succ(const uint32_t *__restrict__ misc, uint32_t n_bodies,
+                                                const uint32_t *__restrict__ skey, const uint32_t *__restrict__ sval,
+                                                const uint32_t *__restrict__ deg_off, const uint32_t *__restrict__ deg,
+                                                float *__restrict__ prep) {
+    uint32_t ninc = misc[MISC_NINC];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < ninc; p += gridDim.x * blockDim.x) {
+        uint32_t x = skey[p];
+        if (x >= n_bodies) continue;
+        uint32_t v = sval[p], start = deg_off[x], end = start + deg[x];
+        bool last = p + 1 >= end;
+        uint32_t np = last ? start : p + 1;
+        uint32_t succ = (sval[np] >> 1) | (last ? SUCC_WRAP : 0u) | (p > start ? SUCC_HASPRED : 0u);
+        prep[(size_t)(v >> 1) * CPREP_WORDS + P_SUCC_A + (v & 1u)] = __uint_as_float(succ);
+    }
+}
+
+SVD uint32_t flow_npred(const float *__restrict__ P, uint32_t t) {
+    uint32_t flags = __float_as_uint(P[P_FLAGS]);
+    bool da = flags & F_DYN_A, db = flags & F_DYN_B;
+    if (t > 0) return (uint32_t)da + (uint32_t)db;
+    uint32_t sa = __float_as_uint(P[P_SUCC_A]), sb = __float_as_uint(P[P_SUCC_B]);
+    return (uint32_t)(da && (sa & SUCC_HASPRED)) + (uint32_t)(db && (sb & SUCC_HASPRED));
+}
+
+// countdowns = predecessor counts; queue slots empty (0)
+__global__ void __launch_bounds__(BLOCK) k_flow_init(const uint32_t *__restrict__ misc, uint32_t iters,
+                                                     const float *__restrict__ prep, uint32_t *__restrict__ count,
+                                                     uint32_t *__restrict__ queue) {
+    const uint32_t nc = misc[MISC_NC];
+    const uint32_t span = iters * nc;
+    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < span; id += gridDim.x * blockDim.x) {
+        uint32_t t = id / nc, c = id - t * nc;
+        count[id] = flow_npred(prep + (size_t)c * CPREP_WORDS, t);
+        queue[id] = 0u;
+    }
+}
+
+// items without predecessors (iteration 0, first contact of each dynamic
+// body) start the queue; a slot holds item id + 1
+__global__ void __launch_bounds__(BLOCK) k_flow_roots(const uint32_t *__restrict__ misc,
+                                                      const float *__restrict__ prep, uint32_t *__restrict__ queue,
+                                                      uint32_t *ctl) {
+    const uint32_t nc = misc[MISC_NC];
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const float *P = prep + (size_t)c * CPREP_WORDS;
+        if (__float_as_uint(P[P_FLAGS]) & F_SKIP) continue;
+        if (flow_npred(P, 0) != 0) continue;
+        uint32_t pos = atomicAdd(ctl + 2, 1u);
+        queue[pos] = c + 1u;
+    }
+}
+
+// ctl: [0] abort flag, [1] queue head (slots claimed), [2] queue tail
+// (slots pushed), [3] items done.  The queue never holds more than the
+// iters * nact items of the sweep, and span = iters * nc bounds every slot.
+__global__ void __launch_bounds__(MC_THREADS) k_flow(
+    const uint32_t *__restrict__ misc, uint32_t iters, const float *__restrict__ prep, float *vel, float *omega,
+    const float *__restrict__ rot, const float *__restrict__ inv_i, float *lambda, uint32_t *count, uint32_t *queue,
+    uint32_t *ctl, uint32_t *counters) {
+    const uint32_t nc = misc[MISC_NC];
+    const uint32_t span = iters * nc;
+    const uint32_t total = iters * misc[MISC_NACT];
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (total == 0) return;
+    Item it;
+    bool has = false, fin = false;
+    uint32_t id = 0, qpos = SUCC_NONE;
+    uint64_t last = wall_clock64();
+    for (;;) {
+        // lanes with no item and no claimed slot claim the next slots (one atomic per wave)
+        const bool want = !has && !fin && qpos == SUCC_NONE;
+        uint64_t need = __ballot(want);
+        if (need) {
+            int leader = __builtin_ctzll(need);
+            uint32_t base = 0;
+            if (lane == leader)
+                base = __hip_atomic_fetch_add(ctl + 1, (uint32_t)__popcll(need), __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+            base = __shfl(base, leader, 64);
+            if (want) {
+                qpos = base + (uint32_t)__popcll(need & lt);
+                if (qpos >= span) fin = true;  // beyond the last slot that can be filled
+            }
+        }
+        bool got = false;
+        if (!has && !fin) {
+            uint32_t v = __hip_atomic_load(queue + qpos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (v) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                id = v - 1u;
+                has = true;
+                got = true;
+                qpos = SUCC_NONE;
+            }
+        }
+        bool ran = false;
+        if (has) {
+            uint32_t t = id / nc, c = id - t * nc;
+            const float *P = prep + (size_t)c * CPREP_WORDS;
+            uint32_t flags = __float_as_uint(P[P_FLAGS]);
+            it.c = c;
+            it.a = __float_as_uint(P[P_A]);
+            it.b = __float_as_uint(P[P_B]);
+            it.dyn_a = flags & F_DYN_A;
+            it.dyn_b = flags & F_DYN_B;
+            it.n = mk(P[P_N], P[P_N + 1], P[P_N + 2]);
+            it.ra = mk(P[P_RA], P[P_RA + 1], P[P_RA + 2]);
+            it.rb = mk(P[P_RB], P[P_RB + 1], P[P_RB + 2]);
+            it.kn = P[P_KN];
+            it.bias = P[P_BIAS];
+            it.mu = P[P_MU];
+            it.ima = P[P_IMA];
+            it.imb = P[P_IMB];
+            it.Ra = ld_rot(rot, it.a);
+            it.Rb = ld_rot(rot, it.b);
+            it.iia = ld3(inv_i, it.a);
+            it.iib = ld3(inv_i, it.b);
+            uint32_t sa = __float_as_uint(P[P_SUCC_A]), sb = __float_as_uint(P[P_SUCC_B]);
+            apply_item(it, vel, omega, lambda);
+            ran = true;
+            uint32_t ida = SUCC_NONE, idb = SUCC_NONE;
+            if (it.dyn_a) {
+                uint32_t ts = t + ((sa & SUCC_WRAP) ? 1u : 0u);
+                if (ts < iters) ida = ts * nc + (sa & SUCC_ID);
+            }
+            if (it.dyn_b) {
+                uint32_t ts = t + ((sb & SUCC_WRAP) ? 1u : 0u);
+                if (ts < iters) idb = ts * nc + (sb & SUCC_ID);
+            }
+            // publish this update, then count it down in both successors
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            bool ra = false, rb = false;
+            if (ida != SUCC_NONE)
+                ra = __hip_atomic_fetch_sub(count + ida, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u;
+            if (idb != SUCC_NONE)
+                rb = __hip_atomic_fetch_sub(count + idb, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u;
+            if (ra || rb) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            has = ra || rb;
+            id = ra ? ida : idb;
+            const bool push = ra && rb;  // both ready: run a, queue b
+            uint64_t pm = __ballot(push);
+            if (pm) {
+                int leader = __builtin_ctzll(pm);
+                uint32_t base = 0;
+                if (lane == leader)
+                    base = __hip_atomic_fetch_add(ctl + 2, (uint32_t)__popcll(pm), __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+                base = __shfl(base, leader, 64);
+                if (push)
+                    __hip_atomic_store(queue + base + (uint32_t)__popcll(pm & lt), idb + 1u, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        uint64_t rm = __ballot(ran);
+        if (rm && lane == __builtin_ctzll(rm))
+            __hip_atomic_fetch_add(ctl + 3, (uint32_t)__popcll(rm), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (rm || __ballot(got)) {
+            last = wall_clock64();
+        } else {
+            if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+            if (__hip_atomic_load(ctl + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= total) return;
+            if (wall_clock64() - last > STALL_TICKS) {
+                __hip_atomic_store(ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0) atomicOr(counters + 2, (uint32_t)SV_STATUS_SOLVER_TIMEOUT);
+                return;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (!__ballot(!fin)) return;
+    }
+}
+
 }  // namespace
 
 // Largest body count whose tickets fit in the single-workgroup kernel's LDS.
@@ -330,6 +529,16 @@ void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
     hipLaunchKernelGGL(k_solve_count, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, mc);
     launch_rot(c, n_bodies);
     fill_u32(c, c->s.deg, 0u, n_bodies);
+    fill_u32(c, c->s.misc + MISC_NACT, 0u, 1);
+    uint32_t iters = c->params.pgs_iterations;
+    bool flow = c->solver_flow;
+    if (flow) {
+        size_t span = (size_t)iters * mc;
+        ensure_u32(c->s.flow_count, c->s.flow_count_cap, (uint32_t)span);
+        ensure_u32(c->s.flow_queue, c->s.flow_queue_cap, (uint32_t)span);
+        // without its buffers the sweep runs in the ordered form (same result)
+        if (c->s.flow_count_cap < span || c->s.flow_queue_cap < span) flow = false;
+    }
     uint32_t grid = div_up(mc, BLOCK);
     if (grid > 8192) grid = 8192;
     hipLaunchKernelGGL(k_prep, dim3(grid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, dt, c->params.baumgarte,
@@ -346,12 +555,26 @@ void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
     scan_exclusive(c, c->s.deg, c->s.deg_off, nullptr, n_bodies, nullptr);
     uint32_t rgrid = div_up(2u * (uint64_t)mc, BLOCK);
     if (rgrid > 8192) rgrid = 8192;
-    hipLaunchKernelGGL(k_rank, dim3(rgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.inc_key[where],
-                       c->s.inc_val[where], c->s.deg_off, c->s.rank);
     fill_u32(c, c->s.solve_ctl, 0u, 4);
     float *vel = B<float>(c, SV_BODY_VEL), *omega = B<float>(c, SV_BODY_OMEGA);
     float *inv_i = B<float>(c, SV_BODY_INV_INERTIA), *lam = B<float>(c, SV_CONTACT_LAMBDA_N);
-    uint32_t iters = c->params.pgs_iterations;
+    if (flow) {
+        hipLaunchKernelGGL(k_succ, dim3(rgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies,
+                           c->s.inc_key[where], c->s.inc_val[where], c->s.deg_off, c->s.deg, c->s.cprep);
+        uint32_t igrid = div_up((uint64_t)iters * mc, BLOCK);
+        if (igrid > 16384) igrid = 16384;
+        hipLaunchKernelGGL(k_flow_init, dim3(igrid), dim3(BLOCK), 0, c->stream, c->s.misc, iters, c->s.cprep,
+                           c->s.flow_count, c->s.flow_queue);
+        hipLaunchKernelGGL(k_flow_roots, dim3(grid), dim3(BLOCK), 0, c->stream, c->s.misc, c->s.cprep,
+                           c->s.flow_queue, c->s.solve_ctl);
+        uint32_t wgs = c->solver_workgroups ? c->solver_workgroups : 64u;
+        StageTimer sweep_timer(c, "phys/pgs_sweep");
+        hipLaunchKernelGGL(k_flow, dim3(wgs), dim3(MC_THREADS), 0, c->stream, c->s.misc, iters, c->s.cprep, vel,
+                           omega, c->s.rot, inv_i, lam, c->s.flow_count, c->s.flow_queue, c->s.solve_ctl, counters);
+        return;
+    }
+    hipLaunchKernelGGL(k_rank, dim3(rgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.inc_key[where],
+                       c->s.inc_val[where], c->s.deg_off, c->s.rank);
     // Form choice: small scenes keep every hand-off inside one CU; large ones
     // need the wider in-order window of the multi-CU form (32 workgroups was
     // fastest on a 32k-body pile, 2M sweep items per substep).

@@ -95,11 +95,17 @@ def test_broadphase_narrowphase_bitwise(settled):
     ctx.close()
 
 
-def test_solve_bitwise(settled):
+SOLVER_FORMS = [("auto", 0), ("ordered", 1), ("ordered", 8), ("flow", 1), ("flow", 64)]
+
+
+@pytest.mark.parametrize("schedule,wgs", SOLVER_FORMS)
+def test_solve_bitwise(settled, schedule, wgs):
+    """Every sweep schedule reproduces the serial Gauss-Seidel loop bit for bit."""
     sc, w = _stage_world(settled)
     for st in (1, 8, 2, 3):
         w.oracle_stage(st)
     ctx = new_ctx(sc, w.max_pairs, w.max_contacts)
+    ctx.set_solver(schedule, wgs)
     w.upload(ctx)  # includes the oracle's contact list and counters
     dt = float(np.float32(1 / 60) / np.float32(10))
     w.oracle_stage(4)
@@ -133,10 +139,11 @@ def test_integrate_verts_bitwise(settled):
     ctx.close()
 
 
-@pytest.mark.parametrize("frames", [1, 8])
-def test_full_step_matches_oracle(settled, frames):
+@pytest.mark.parametrize("frames,schedule", [(1, "auto"), (8, "auto"), (8, "flow")])
+def test_full_step_matches_oracle(settled, frames, schedule):
     sc, w = _stage_world(settled)
     ctx = new_ctx(sc, w.max_pairs, w.max_contacts)
+    ctx.set_solver(schedule)
     w.upload(ctx)
     for f in range(frames):
         ctx.step(w.n, w.nv, w.nh, 10, 1 / 60, (0, -9.81, 0))
