@@ -135,6 +135,8 @@ def main() -> int:
     ap.add_argument("--solver", default="auto", choices=["auto", "ordered", "flow"],
                     help="schedule of the contact sweep (all give the serial result bit for bit)")
     ap.add_argument("--solver-wgs", type=int, default=0, help="workgroups of the sweep kernel (0 = default)")
+    ap.add_argument("--broadphase", default="auto", choices=["auto", "all_pairs", "lbvh"],
+                    help="broadphase method (both give the all-pairs list bit for bit)")
     ap.add_argument("--calib", action="store_true",
                     help="also run the 256 MiB copy kernel once (byte-counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -156,6 +158,7 @@ def main() -> int:
     ctx = Context(caps, default_params(pgs_iterations=PGS_ITERS))
     scene.upload(ctx)
     ctx.set_solver(args.solver, args.solver_wgs)
+    ctx.set_broadphase(args.broadphase)
     name, arch = ctx.device_info()
     log(f"device: {name} {arch}")
     copy_gbs = ctx.copy_bandwidth(256 << 20, 1 if args.calib else 20)

@@ -233,6 +233,15 @@ int sv_enable_stage_timing(sv_ctx *ctx, int enable);
 enum { SV_SOLVER_AUTO = 0, SV_SOLVER_ORDERED = 1, SV_SOLVER_FLOW = 2 };
 int sv_set_solver(sv_ctx *ctx, int schedule, uint32_t workgroups);
 
+/* Broadphase method (step 2).  Both give the pair list of the reference's
+ * all-pairs loop (physics.comp:451-472): the same set, in lexicographic
+ * (i, j) order, truncated to max_pairs with counters[0] = the total.  No
+ * counterpart in the reference.  SV_BROADPHASE_ALL_PAIRS: the LDS-tiled
+ * N^2 test.  SV_BROADPHASE_LBVH: each body walks an LBVH built over the
+ * body boxes of this substep.  SV_BROADPHASE_AUTO picks by body count. */
+enum { SV_BROADPHASE_AUTO = 0, SV_BROADPHASE_ALL_PAIRS = 1, SV_BROADPHASE_LBVH = 2 };
+int sv_set_broadphase(sv_ctx *ctx, int method);
+
 /* ---- host scene construction (src/data.zig) ------------------------- */
 typedef struct sv_scene sv_scene;
 

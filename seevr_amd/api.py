@@ -156,6 +156,12 @@ class Context:
         """Schedule of the contact sweep: "auto", "ordered" or "flow" (all bit-identical)."""
         check(self.lib.sv_set_solver(self.h, self.SOLVERS[schedule], workgroups))
 
+    BROADPHASES = {"auto": 0, "all_pairs": 1, "lbvh": 2}
+
+    def set_broadphase(self, method: str = "auto") -> None:
+        """Broadphase method: "auto", "all_pairs" or "lbvh" (identical pair lists)."""
+        check(self.lib.sv_set_broadphase(self.h, self.BROADPHASES[method]))
+
     def stage_times(self, reset: bool = False) -> dict:
         cap = 64
         names = (C.c_char_p * cap)()

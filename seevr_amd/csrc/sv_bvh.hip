@@ -204,9 +204,22 @@ __global__ void k_clear_flags(uint32_t n, uint32_t *flags) {
     if (i + 1 < n) flags[i] = 0;
 }
 
-// bvh.comp:379-397.  The flag update is an acquire-release RMW at device
-// scope: the first arrival's writes of its child box are published before
-// its flag, and the second arrival reads both children only after seeing it.
+// Arrival at a parent during the refit: returns the flag's previous value
+// (0 for the first arrival).  The release fence publishes this thread's
+// child box before the flag update and the acquire fence makes the other
+// child's box visible to the second arrival.  The release is written out
+// (rather than an acq_rel RMW) so that it carries the wait for the box
+// stores to complete.
+SVD uint32_t arrive_second(uint32_t *flag) {
+    release_agent();
+    uint32_t old = __hip_atomic_fetch_or(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return old;
+}
+
+// bvh.comp:379-397, with the arrival above: the first arrival's writes of
+// its child box are published before its flag, and the second arrival reads
+// both children only after seeing it.
 __global__ void k_propagate(uint32_t n, const uint32_t *__restrict__ parent_of, const uint32_t *__restrict__ left,
                             const uint32_t *__restrict__ right, uint32_t *flags, float *nmin, float *nmax) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -216,8 +229,7 @@ __global__ void k_propagate(uint32_t n, const uint32_t *__restrict__ parent_of, 
     // [0, 64], so a climb takes at most 65 steps; the bound only guards a
     // corrupt parent array.
     for (int depth = 0; p != 0xFFFFFFFFu && depth < 80; depth++) {
-        uint32_t old = __hip_atomic_fetch_or(flags + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0u) return;
+        if (arrive_second(flags + p) == 0u) return;
         uint32_t li = left[p], ri = right[p];
         st3(nmin, p, vmin3(ld3(nmin, li), ld3(nmin, ri)));
         st3(nmax, p, vmax3(ld3(nmax, li), ld3(nmax, ri)));
@@ -230,7 +242,253 @@ __global__ void k_copy_u32(uint32_t n, const uint32_t *src, uint32_t *dst) {
     if (i < n) dst[i] = src[i];
 }
 
+// ---------------------------------------------------------------------------
+// Broadphase by LBVH traversal (extension beside physics.comp:451-472).
+//
+// The tree is the body LBVH above (box centres, 30-bit Morton codes, Karras
+// hierarchy) with float4 nodes: lo.w carries the smallest body index below
+// the node and hi.w whether every body below is static.  Each body j walks
+// the tree with its own box and reports every leaf body i < j whose box is
+// not apart from its own and that is not static together with j.  A node is
+// skipped when j's box is apart from the node's box (the node box is the
+// exact union of the leaves, and float subtraction is monotonic, so no leaf
+// below can overlap), when its smallest index is >= j, or when j and the
+// whole subtree are static.  The reported set is exactly the all-pairs set.
+// Pairs are counted per row i, the row counts scanned into offsets (total =
+// counters[0]), rows that start below max_pairs filled, and the filled part
+// sorted by (i, j): the first max_pairs entries are the all-pairs kernel's.
+
+__global__ void k_bb_centroids(uint32_t n, const float4 *__restrict__ box, float *__restrict__ cent,
+                               uint32_t *__restrict__ ids) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 lo = box[2 * i], hi = box[2 * i + 1];
+    cent[3 * i + 0] = (lo.x + hi.x) * 0.5f;
+    cent[3 * i + 1] = (lo.y + hi.y) * 0.5f;
+    cent[3 * i + 2] = (lo.z + hi.z) * 0.5f;
+    ids[i] = i;
+}
+
+__global__ void k_bb_leaves(uint32_t n, const uint32_t *__restrict__ ids, const float4 *__restrict__ box,
+                            float4 *__restrict__ nlo, float4 *__restrict__ nhi) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t b = ids[k], leaf = n - 1 + k;
+    float4 lo = box[2 * b], hi = box[2 * b + 1];
+    nlo[leaf] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(b));
+    nhi[leaf] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(lo.w != 0.0f ? 1u : 0u));
+}
+
+// Bottom-up union as in k_propagate, carrying the index minimum and the
+// all-static flag.
+__global__ void k_bb_propagate(uint32_t n, const uint32_t *__restrict__ parent_of, const uint32_t *__restrict__ left,
+                               const uint32_t *__restrict__ right, uint32_t *flags, float4 *nlo, float4 *nhi) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t p = parent_of[n - 1 + k];
+    for (int depth = 0; p != 0xFFFFFFFFu && depth < 80; depth++) {
+        if (arrive_second(flags + p) == 0u) return;
+        uint32_t li = left[p], ri = right[p];
+        float4 al = nlo[li], ah = nhi[li], bl = nlo[ri], bh = nhi[ri];
+        uint32_t mi = min(__float_as_uint(al.w), __float_as_uint(bl.w));
+        uint32_t st = __float_as_uint(ah.w) & __float_as_uint(bh.w);
+        nlo[p] = make_float4(gmin(al.x, bl.x), gmin(al.y, bl.y), gmin(al.z, bl.z), __uint_as_float(mi));
+        nhi[p] = make_float4(gmax(ah.x, bh.x), gmax(ah.y, bh.y), gmax(ah.z, bh.z), __uint_as_float(st));
+        p = parent_of[p];
+    }
+}
+
+constexpr int BB_STACK = 64;  // > tree depth: node prefixes strictly grow downward and lie in [2, 63]
+
+// One lane per body j, in Morton order so neighbouring lanes walk similar
+// paths.  COUNT: bump the row count of every reported i.  EMIT: place the
+// pair in row i's slot range if the row starts below max_pairs.
+template <bool EMIT>
+__global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *__restrict__ ids,
+                                                    const float4 *__restrict__ box, const float4 *__restrict__ nlo,
+                                                    const float4 *__restrict__ nhi, const uint32_t *__restrict__ left,
+                                                    const uint32_t *__restrict__ right, uint32_t *cnt,
+                                                    const uint32_t *__restrict__ off, uint32_t *fill,
+                                                    uint32_t *__restrict__ pk, uint32_t *__restrict__ pv,
+                                                    uint32_t max_pairs, int bits, bool combined) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t j = ids[t];
+    const float4 jl = box[2 * j], jh = box[2 * j + 1];
+    const bool js = jl.w != 0.0f;
+    uint32_t stack[BB_STACK];
+    int sp = 0;
+    uint32_t node = 0;
+    for (;;) {
+        const uint32_t ch[2] = {left[node], right[node]};
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const uint32_t c = ch[q];
+            const float4 lo = nlo[c], hi = nhi[c];
+            const uint32_t mi = __float_as_uint(lo.w);
+            if (mi >= j || (js && __float_as_uint(hi.w) != 0u) || boxes_apart(jl, jh, lo, hi)) continue;
+            if (c >= n - 1) {
+                if (!EMIT) {
+                    atomicAdd(cnt + mi, 1u);
+                } else {
+                    const uint32_t o = off[mi];
+                    if (o < max_pairs) {
+                        const uint32_t s = o + atomicAdd(fill + mi, 1u);
+                        if (combined) {
+                            pk[s] = (mi << bits) | j;
+                        } else {
+                            pk[s] = j;
+                            pv[s] = mi;
+                        }
+                    }
+                }
+            } else if (sp < BB_STACK) {
+                stack[sp++] = c;
+            }
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+}
+
+// Per row: clear the fill counter; rows that start below max_pairs are
+// filled, and the last of them ends the part that is sorted.
+__global__ void k_bb_rows(uint32_t n, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
+                          uint32_t *__restrict__ fill, uint32_t *misc, uint32_t max_pairs) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fill[i] = 0u;
+    if (cnt[i] != 0u && off[i] < max_pairs) atomicMax(misc + MISC_BPE, off[i] + cnt[i]);
+}
+
+__global__ void k_bb_write(const uint32_t *__restrict__ misc, const uint32_t *__restrict__ ki,
+                           const uint32_t *__restrict__ kj, int bits, bool combined, uint32_t *__restrict__ pairs) {
+    const uint32_t np = misc[MISC_NP];
+    const uint32_t mask = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        uint32_t i, j;
+        if (combined) {
+            i = ki[p] >> bits;
+            j = ki[p] & mask;
+        } else {
+            i = ki[p];
+            j = kj[p];
+        }
+        pairs[2 * p] = i;
+        pairs[2 * p + 1] = j;
+    }
+}
+
+__global__ void k_bb_post(uint32_t *counters, uint32_t *misc, uint32_t max_pairs) {
+    uint32_t t = counters[0];
+    misc[MISC_NP] = t < max_pairs ? t : max_pairs;
+    if (t > max_pairs) counters[2] |= SV_STATUS_PAIR_OVERFLOW;
+}
+
 }  // namespace
+
+namespace {
+
+template <typename T>
+hipError_t lazy_alloc(sv_ctx *c, T *&p, size_t count) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, (count ? count : 1) * sizeof(T));
+    if (e == hipSuccess) {
+        c->s.lazy.push_back(q);
+        p = static_cast<T *>(q);
+    }
+    return e;
+}
+
+bool bb_reserve(sv_ctx *c) {
+    if (c->s.bb_ready) return true;
+    Scratch &s = c->s;
+    size_t N = c->caps.max_bodies, P = c->caps.max_pairs;
+    size_t nodes = 2 * N + 1, parts = div_up(N, BLOCK * BOUNDS_ITEMS) + 1;
+    hipError_t e = lazy_alloc(c, s.bb_lo, nodes);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_hi, nodes);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_left, nodes);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_right, nodes);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_parent, nodes);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_count, nodes);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_flags, N + 1);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_cent, 3 * N + 3);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_bounds, 6 + 6 * parts);
+    for (int q = 0; q < 2 && e == hipSuccess; q++) {
+        e = lazy_alloc(c, s.bb_mk[q], N + 1);
+        if (e == hipSuccess) e = lazy_alloc(c, s.bb_mv[q], N + 1);
+        if (e == hipSuccess) e = lazy_alloc(c, s.bb_pk[q], P + N + 1);
+        if (e == hipSuccess) e = lazy_alloc(c, s.bb_pv[q], P + N + 1);
+    }
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_cnt, N + 1);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_off, N + 1);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_fill, N + 1);
+    s.bb_ready = e == hipSuccess;
+    return s.bb_ready;
+}
+
+}  // namespace
+
+bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
+    uint32_t *counters = B<uint32_t>(c, SV_ATOMIC_COUNTERS);
+    uint32_t *pairs = B<uint32_t>(c, SV_COLLISION_PAIRS);
+    const uint32_t mp = c->caps.max_pairs;
+    hipStream_t st = c->stream;
+    if (!bb_reserve(c)) return false;
+    if (n < 2) {
+        fill_u32(c, counters, 0u, 1);
+        hipLaunchKernelGGL(k_bb_post, dim3(1), dim3(1), 0, st, counters, c->s.misc, mp);
+        return true;
+    }
+    Scratch &s = c->s;
+    const uint32_t g = div_up(n, BLOCK);
+    // tree over the broadphase boxes
+    hipLaunchKernelGGL(k_bb_centroids, dim3(g), dim3(BLOCK), 0, st, n, s.bpbox, s.bb_cent, s.bb_mv[0]);
+    uint32_t parts = div_up(n, BLOCK * BOUNDS_ITEMS);
+    hipLaunchKernelGGL(k_bounds_partial, dim3(parts), dim3(BLOCK), 0, st, n, s.bb_cent, s.bb_bounds + 6);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(BLOCK), 0, st, parts, s.bb_bounds + 6, s.bb_bounds);
+    hipLaunchKernelGGL(k_morton, dim3(g), dim3(BLOCK), 0, st, n, s.bb_bounds, s.bb_cent, s.bb_mk[0]);
+    int w = radix_sort(c, s.bb_mk[0], s.bb_mv[0], s.bb_mk[1], s.bb_mv[1], nullptr, n, 30);
+    const uint32_t *ids = s.bb_mv[w];
+    hipLaunchKernelGGL(k_hierarchy, dim3(g), dim3(BLOCK), 0, st, n, s.bb_mk[w], s.bb_left, s.bb_right, s.bb_count,
+                       s.bb_parent);
+    hipLaunchKernelGGL(k_bb_leaves, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_lo, s.bb_hi);
+    hipLaunchKernelGGL(k_clear_flags, dim3(g), dim3(BLOCK), 0, st, n, s.bb_flags);
+    hipLaunchKernelGGL(k_bb_propagate, dim3(g), dim3(BLOCK), 0, st, n, s.bb_parent, s.bb_left, s.bb_right,
+                       s.bb_flags, s.bb_lo, s.bb_hi);
+    // count, scan, fill, sort
+    const int bits = 32 - __builtin_clz(n - 1);  // body indices < 2^bits
+    const bool combined = 2 * bits <= 32;
+    fill_u32(c, s.bb_cnt, 0u, n);
+    hipLaunchKernelGGL(k_bb_pairs<false>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_lo, s.bb_hi, s.bb_left,
+                       s.bb_right, s.bb_cnt, (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                       (uint32_t *)nullptr, mp, bits, combined);
+    scan_exclusive(c, s.bb_cnt, s.bb_off, nullptr, n, counters);
+    hipLaunchKernelGGL(k_bb_post, dim3(1), dim3(1), 0, st, counters, s.misc, mp);
+    fill_u32(c, s.misc + MISC_BPE, 0u, 1);
+    hipLaunchKernelGGL(k_bb_rows, dim3(g), dim3(BLOCK), 0, st, n, s.bb_cnt, s.bb_off, s.bb_fill, s.misc, mp);
+    hipLaunchKernelGGL(k_bb_pairs<true>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_lo, s.bb_hi, s.bb_left,
+                       s.bb_right, (uint32_t *)nullptr, s.bb_off, s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits,
+                       combined);
+    const uint32_t cap = mp + n;
+    const uint32_t *ki, *kj;
+    if (combined) {
+        int r = radix_sort(c, s.bb_pk[0], s.bb_pv[0], s.bb_pk[1], s.bb_pv[1], s.misc + MISC_BPE, cap, 2 * bits);
+        ki = s.bb_pk[r];
+        kj = nullptr;
+    } else {
+        // stable by j, then stable by i: lexicographic (i, j)
+        int r = radix_sort(c, s.bb_pk[0], s.bb_pv[0], s.bb_pk[1], s.bb_pv[1], s.misc + MISC_BPE, cap, bits);
+        int r2 = radix_sort(c, s.bb_pv[r], s.bb_pk[r], s.bb_pk[1 - r], s.bb_pv[1 - r], s.misc + MISC_BPE, cap, bits);
+        ki = r2 == 0 ? s.bb_pv[r] : s.bb_pk[1 - r];
+        kj = r2 == 0 ? s.bb_pk[r] : s.bb_pv[1 - r];
+    }
+    uint32_t wg = div_up(mp, BLOCK);
+    if (wg > 4096) wg = 4096;
+    if (wg == 0) wg = 1;
+    hipLaunchKernelGGL(k_bb_write, dim3(wg), dim3(BLOCK), 0, st, s.misc, ki, kj, bits, combined, pairs);
+    return true;
+}
 
 void bvh_stage(sv_ctx *c, uint32_t step, uint32_t n, bool bodies) {
     if (n == 0) return;

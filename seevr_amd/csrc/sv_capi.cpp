@@ -109,6 +109,8 @@ void free_ctx(sv_ctx *c) {
                     s.rank, s.ticket, s.solve_ctl, s.flow_count, s.flow_queue, s.scan_part, s.sort_hist, s.misc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (void *p : s.lazy)
+        if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -469,6 +471,14 @@ int sv_set_solver(sv_ctx *c, int schedule, uint32_t workgroups) {
     if (workgroups > 4096) return fail(SV_E_INVALID, "sv_set_solver: at most 4096 workgroups");
     c->solver_flow = schedule == SV_SOLVER_FLOW;
     c->solver_workgroups = schedule == SV_SOLVER_AUTO ? 0u : (workgroups ? workgroups : (c->solver_flow ? 0u : 1u));
+    return SV_OK;
+}
+
+int sv_set_broadphase(sv_ctx *c, int method) {
+    if (!c) return fail(SV_E_INVALID, "sv_set_broadphase: null context");
+    if (method < SV_BROADPHASE_AUTO || method > SV_BROADPHASE_LBVH)
+        return fail(SV_E_INVALID, "sv_set_broadphase: unknown method %d", method);
+    c->broadphase_mode = (uint32_t)method;
     return SV_OK;
 }
 

@@ -39,6 +39,19 @@ struct Scratch {
     uint32_t *sort_hist = nullptr;  // radix histograms
     uint32_t sort_hist_cap = 0;
     uint32_t *misc = nullptr;       // uint[64] small device scalars
+    // LBVH broadphase (allocated on first use, sized from the capacities)
+    float4 *bb_lo = nullptr;        // float4[2N-1] node min; w = smallest body index below (bits)
+    float4 *bb_hi = nullptr;        // float4[2N-1] node max; w = 1 if every body below is static (bits)
+    uint32_t *bb_left = nullptr, *bb_right = nullptr, *bb_parent = nullptr, *bb_count = nullptr;  // [2N-1]
+    uint32_t *bb_flags = nullptr;   // uint[N] refit arrival flags
+    float *bb_cent = nullptr;       // float[3N] box centres
+    float *bb_bounds = nullptr;     // float[6 + 6 * parts] centre bounds and partials
+    uint32_t *bb_mk[2] = {nullptr, nullptr}, *bb_mv[2] = {nullptr, nullptr};  // [N] Morton sort
+    uint32_t *bb_cnt = nullptr, *bb_off = nullptr, *bb_fill = nullptr;         // [N + 1] per-row pairs
+    uint32_t *bb_pk[2] = {nullptr, nullptr}, *bb_pv[2] = {nullptr, nullptr};  // [P + N] pair sort
+    bool bb_ready = false;
+    // every lazily allocated block, freed with the context
+    std::vector<void *> lazy;
 };
 
 // Words per contact in Scratch::cprep (see sv_pgs.hip).
@@ -73,6 +86,8 @@ struct sv_ctx {
     // sweep schedule: false = ordered (tickets, items taken in sweep order),
     // true = flow (countdowns, items run as their predecessors finish)
     bool solver_flow = false;
+    // broadphase: 0 = chosen by body count, 1 = all pairs (tiled N^2), 2 = LBVH traversal
+    uint32_t broadphase_mode = 0;
     // stage timing
     bool timing = false;
     std::vector<std::string> stage_names;
@@ -95,6 +110,7 @@ enum : int {
     MISC_SCAN_TOTAL = 4,
     MISC_DEG_TOTAL = 5,
     MISC_NACT = 6,     // contacts the sweep updates (not both inverse masses zero)
+    MISC_BPE = 7,      // pairs the LBVH broadphase emits before sorting (rows that start below the cap)
 };
 
 // ---- sv_prims.hip ----
@@ -134,6 +150,12 @@ uint32_t solver_lds_bodies(sv_ctx *c);
 
 // ---- sv_bvh.hip ----
 void bvh_stage(sv_ctx *c, uint32_t step, uint32_t n, bool bodies);
+// Broadphase by LBVH traversal over Scratch::bpbox: the same pair list,
+// order, truncation and counters as the all-pairs kernel.  Returns false
+// (and launches nothing) when its scratch cannot be allocated.
+bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n);
+// Body count from which the automatic broadphase choice is the LBVH.
+constexpr uint32_t BP_LBVH_MIN_BODIES = 1u << 17;
 
 // ---- stage timing (sv_capi.cpp) ----
 int timing_begin(sv_ctx *c, const char *name);

@@ -127,3 +127,26 @@ SVD float sv_atan2f(float y, float x) {
     if (__builtin_signbit(x)) r = SV_PI - r;
     return __builtin_copysignf(r, y);
 }
+
+// Broadphase separation (physics.comp:461-466): the boxes are disjoint iff on
+// some axis max_a < min_b or max_b < min_a, i.e. iff the larger of the two
+// gaps is positive on some axis (exact for finite bounds; touching boxes
+// overlap).  Only x, y, z are read.
+SVD bool boxes_apart(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+    float sx = fmaxf(blo.x - ahi.x, alo.x - bhi.x);
+    float sy = fmaxf(blo.y - ahi.y, alo.y - bhi.y);
+    float sz = fmaxf(blo.z - ahi.z, alo.z - bhi.z);
+    return fmaxf(sx, fmaxf(sy, sz)) > 0.0f;
+}
+
+// Device-scope release that is complete on return: the release fence writes
+// back this XCD's L2, and the explicit wait holds the wave until that
+// write-back and every earlier load and store have completed.  hipcc (ROCm
+// 7.2) was seen to emit the fence's buffer_wbl2 with no following
+// s_waitcnt vmcnt(0) when the stores came from an earlier loop iteration,
+// which lets the flag update that follows overtake the data it publishes.
+// s_waitcnt 0 = vmcnt(0) expcnt(0) lgkmcnt(0).
+SVD void release_agent() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);
+}

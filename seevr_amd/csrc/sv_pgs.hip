@@ -239,7 +239,7 @@ SVD void fence_acquire() {
 }
 template <int FORM>
 SVD void fence_release() {
-    if (FORM == FORM_MULTI) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (FORM == FORM_MULTI) release_agent();
     else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 }
 
@@ -468,7 +468,7 @@ __global__ void __launch_bounds__(MC_THREADS) k_flow(
                 if (ts < iters) idb = ts * nc + (sb & SUCC_ID);
             }
             // publish this update, then count it down in both successors
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            release_agent();
             bool ra = false, rb = false;
             if (ida != SUCC_NONE)
                 ra = __hip_atomic_fetch_sub(count + ida, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u;
@@ -486,9 +486,11 @@ __global__ void __launch_bounds__(MC_THREADS) k_flow(
                     base = __hip_atomic_fetch_add(ctl + 2, (uint32_t)__popcll(pm), __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT);
                 base = __shfl(base, leader, 64);
-                if (push)
-                    __hip_atomic_store(queue + base + (uint32_t)__popcll(pm & lt), idb + 1u, __ATOMIC_RELEASE,
+                if (push) {
+                    release_agent();
+                    __hip_atomic_store(queue + base + (uint32_t)__popcll(pm & lt), idb + 1u, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
         }
         uint64_t rm = __ballot(ran);

@@ -107,13 +107,6 @@ __global__ void __launch_bounds__(BLOCK) k_aabb(uint32_t n, const float *__restr
 constexpr int BP_ROWS_PER_WAVE = 8;
 constexpr int BP_TILE = BLOCK;  // 256 boxes per LDS tile
 
-SVD bool boxes_apart(float4 alo, float4 ahi, float4 blo, float4 bhi) {
-    float sx = fmaxf(blo.x - ahi.x, alo.x - bhi.x);
-    float sy = fmaxf(blo.y - ahi.y, alo.y - bhi.y);
-    float sz = fmaxf(blo.z - ahi.z, alo.z - bhi.z);
-    return fmaxf(sx, fmaxf(sy, sz)) > 0.0f;
-}
-
 template <bool EMIT>
 __global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, uint32_t nseg, uint32_t seg_len,
                                                       const float4 *__restrict__ box, uint32_t *__restrict__ cnt_out,
@@ -652,6 +645,9 @@ void launch_broadphase(sv_ctx *c, uint32_t n) {
         hipLaunchKernelGGL(k_post_pairs, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, c->caps.max_pairs);
         return;
     }
+    const bool lbvh = c->broadphase_mode == 2 || (c->broadphase_mode == 0 && n >= BP_LBVH_MIN_BODIES);
+    // same pair list either way; without LBVH scratch the all-pairs kernel runs
+    if (lbvh && launch_broadphase_lbvh(c, n)) return;
     const uint32_t seg_len = bp_seg_len(n), nseg = bp_nseg(n);
     const uint32_t nrb = div_up(n, BP_ROWS);
     uint32_t *pairs = B<uint32_t>(c, SV_COLLISION_PAIRS);

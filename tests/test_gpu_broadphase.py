@@ -1,0 +1,101 @@
+"""LBVH-traversal broadphase against the all-pairs loop of physics.comp:451-472.
+
+Both device methods must give the oracle's pair list bit for bit: the same
+set, in lexicographic (i, j) order, the first max_pairs of them kept and
+counters[0] holding the total.  Boxes are placed on an integer grid so that
+many of them touch exactly (touching counts as overlap), some are
+duplicates and about a quarter are static (static-static pairs are skipped).
+"""
+import numpy as np
+import pytest
+
+from harness import World, assert_bitwise, device_pairs, make_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def box_world(n, seed, max_pairs, static_frac=0.25, spread=None):
+    """A World of n bodies whose AABBs are overwritten with random grid boxes."""
+    sc = make_scene(1, max(n - 1, 1), seed=seed, segments=4)
+    w = World.from_scene(sc, max_pairs=max_pairs, max_contacts=16)
+    assert w.n == n
+    rng = np.random.default_rng(seed)
+    spread = 1.6 * n ** (1.0 / 3.0) if spread is None else spread
+    lo = np.floor(rng.uniform(0, spread, (n, 3))).astype(np.float32)
+    ext = rng.integers(1, 3, (n, 3)).astype(np.float32)
+    dup = rng.random(n) < 0.05
+    src = rng.integers(0, n, n)
+    lo[dup] = lo[src[dup]]
+    ext[dup] = ext[src[dup]]
+    w.a["BODY_AABB_MIN"][:] = lo.ravel()
+    w.a["BODY_AABB_MAX"][:] = (lo + ext).ravel()
+    inv_mass = w.a["BODY_INV_MASS"]
+    inv_mass[:] = np.where(rng.random(n) < static_frac, 0.0, 1.0).astype(np.float32)
+    return sc, w
+
+
+def device_broadphase(sc, w, method):
+    from seevr_amd import Context, caps_for, default_params
+    ctx = Context(caps_for(sc, w.max_pairs, w.max_contacts), default_params())
+    sc.upload(ctx)
+    ctx.set_broadphase(method)
+    w.upload(ctx)
+    ctx.dispatch(8, 1)
+    ctx.dispatch(2, w.n)
+    cnt = ctx.download("ATOMIC_COUNTERS", 3)
+    pairs = device_pairs(ctx, min(int(cnt[0]), w.max_pairs))
+    ctx.close()
+    return cnt, pairs
+
+
+def oracle_broadphase(w):
+    w.a["ATOMIC_COUNTERS"][:] = 0
+    w.oracle_stage(8)
+    w.oracle_stage(2)
+    return w.a["ATOMIC_COUNTERS"].copy(), w.pairs()
+
+
+@pytest.mark.parametrize("method", ["all_pairs", "lbvh"])
+@pytest.mark.parametrize("n,seed", [(2, 1), (3, 2), (64, 3), (700, 4), (3000, 5)])
+def test_pairs_match_oracle(n, seed, method):
+    sc, w = box_world(n, seed, max_pairs=400000)
+    want_cnt, want = oracle_broadphase(w)
+    got_cnt, got = device_broadphase(sc, w, method)
+    assert int(got_cnt[0]) == int(want_cnt[0])
+    if n >= 64:
+        assert int(want_cnt[0]) > n // 2, "scene should be dense"
+    assert_bitwise(got, want, f"{method} pairs n={n}")
+
+
+@pytest.mark.parametrize("method", ["all_pairs", "lbvh"])
+@pytest.mark.parametrize("cap", [1, 37, 500])
+def test_truncation_keeps_lexicographic_prefix(cap, method):
+    """A row straddles the cap: only its smallest partners are kept."""
+    sc, w = box_world(300, 11, max_pairs=cap)
+    want_cnt, want = oracle_broadphase(w)
+    assert int(want_cnt[0]) > cap
+    got_cnt, got = device_broadphase(sc, w, method)
+    assert int(got_cnt[0]) == int(want_cnt[0])
+    assert int(got_cnt[2]) & 1, "pair overflow flag"
+    assert_bitwise(got, want, f"{method} truncated pairs cap={cap}")
+
+
+def test_all_static_and_two_bodies():
+    sc, w = box_world(200, 12, max_pairs=1000, static_frac=1.0)
+    got_cnt, got = device_broadphase(sc, w, "lbvh")
+    assert int(got_cnt[0]) == 0 and got.size == 0
+    sc1 = make_scene(1, 1, seed=3, segments=4)
+    w1 = World.from_scene(sc1, max_pairs=100, max_contacts=16)
+    want_cnt, want = oracle_broadphase(w1.copy())
+    got_cnt, got = device_broadphase(sc1, w1, "lbvh")
+    assert int(got_cnt[0]) == int(want_cnt[0])
+    assert_bitwise(got, want, "two-body scene")
+
+
+def test_lbvh_equals_all_pairs_at_scale():
+    """40k bodies: the two device methods agree (the oracle pins both above)."""
+    sc, w = box_world(40000, 21, max_pairs=2_000_000)
+    a_cnt, a = device_broadphase(sc, w, "all_pairs")
+    b_cnt, b = device_broadphase(sc, w, "lbvh")
+    assert int(a_cnt[0]) == int(b_cnt[0]) > 40000
+    assert_bitwise(b, a, "lbvh vs all-pairs at 40k")

@@ -139,11 +139,13 @@ def test_integrate_verts_bitwise(settled):
     ctx.close()
 
 
-@pytest.mark.parametrize("frames,schedule", [(1, "auto"), (8, "auto"), (8, "flow")])
-def test_full_step_matches_oracle(settled, frames, schedule):
+@pytest.mark.parametrize("frames,schedule,broadphase",
+                         [(1, "auto", "auto"), (8, "auto", "auto"), (8, "flow", "auto"), (8, "auto", "lbvh")])
+def test_full_step_matches_oracle(settled, frames, schedule, broadphase):
     sc, w = _stage_world(settled)
     ctx = new_ctx(sc, w.max_pairs, w.max_contacts)
     ctx.set_solver(schedule)
+    ctx.set_broadphase(broadphase)
     w.upload(ctx)
     for f in range(frames):
         ctx.step(w.n, w.nv, w.nh, 10, 1 / 60, (0, -9.81, 0))
