@@ -123,6 +123,83 @@ def cpu_baseline(ctx, scene, n, substeps: int) -> dict:
                       f"frames (N={n}), scalar C oracle, 1 core, {el:.1f} s"}
 
 
+def batched_section(frames: int, warmup: int, scenes: int = 4096, plain_sample: int = 32) -> dict:
+    """The batched form on the 4,096-scene workload: env-steps/s over `frames`
+    timed frames (one frame = 10 substeps of every scene, replayed as one
+    hipGraph), against the plain path stepping a sample of the same scenes
+    one by one (the per-scene frame time is averaged over the sample)."""
+    from seevr_amd import Batch, Context, Scene, caps_for, default_caps, default_params
+    from seevr_amd._lib import BatchCaps
+    t0 = time.perf_counter()
+    big = default_caps(max_bodies=scenes * 257 + 16, max_hull_verts=scenes * 257 * 290,
+                       max_vertices=scenes * 257 * 290, max_triangles=scenes * 257 * 520,
+                       max_support_entries=scenes * 257 * 193 + 1)
+    sc = Scene(big)
+    start, gs = sc.build_batch(0, scenes, 8, 256, 16, 1)
+    cnt = sc.counts()
+    log(f"batched: {scenes} scenes, {cnt['num_bodies']} bodies built in {time.perf_counter() - t0:.1f} s")
+    bc = BatchCaps(max_scenes=scenes, max_bodies=cnt["num_bodies"], max_hull_verts=cnt["num_hull_verts"],
+                   max_vertices=cnt["num_vertices"], max_triangles=max(cnt["num_triangles"], 1),
+                   max_support_entries=cnt["num_support_entries"], max_scene_bodies=512, max_scene_pairs=4096,
+                   max_scene_contacts=2048)
+    b = Batch(bc, default_params(pgs_iterations=PGS_ITERS))
+    sc.upload(b.ctx)
+    grav = np.zeros((scenes, 3), np.float32)
+    grav[:, 1] = np.float32(GRAVITY[1]) * gs
+    b.set_scenes(start, grav)
+    sc.close()
+    nv = cnt["num_vertices"]
+    for i in range(warmup):
+        b.step(nv, SUBSTEPS, DT)
+    threads, wgs, us = b.autotune(SUBSTEPS, DT)
+    log(f"batched autotune: {threads} threads x {wgs} workgroups/CU, {us:.0f} us/substep")
+    b.step(nv, SUBSTEPS, DT)
+    t0 = time.perf_counter()
+    for i in range(frames):
+        b.step(nv, SUBSTEPS, DT)
+    el = time.perf_counter() - t0
+    c = b.counters()
+    env_steps = scenes * frames / el
+    # plain path, one scene at a time, on an evenly spaced sample, each
+    # started from that scene's current batched state
+    ks = list(range(0, scenes, max(scenes // plain_sample, 1)))[:plain_sample]
+    tot, bodies = 0.0, 0
+    for k in ks:
+        s1 = Scene(default_caps(max_bodies=300, max_hull_verts=300 * 290, max_vertices=300 * 290,
+                                max_triangles=300 * 520, max_support_entries=300 * 193 + 1))
+        s1.build_batch(k, 1, 8, 256, 16, 1)
+        c1 = s1.counts()
+        ctx = Context(caps_for(s1, 4096, 2048), default_params(pgs_iterations=PGS_ITERS))
+        s1.upload(ctx)
+        n1, v1, h1 = c1["num_bodies"], c1["num_vertices"], c1["num_hull_verts"]
+        for key, w in (("BODY_POS", 3), ("BODY_QUAT", 4), ("BODY_VEL", 3), ("BODY_OMEGA", 3)):
+            ctx.upload(key, b.ctx.download(key, w * n1, w * int(start[k])))
+        g = tuple(float(x) for x in grav[k])
+        ctx.step(n1, v1, h1, SUBSTEPS, DT, g)
+        t1 = time.perf_counter()
+        for _ in range(5):
+            ctx.step(n1, v1, h1, SUBSTEPS, DT, g)
+        tot += (time.perf_counter() - t1) / 5
+        bodies += n1
+        ctx.close()
+        s1.close()
+    b.close()
+    plain_env = len(ks) / tot
+    log(f"batched: {1000 * el / frames:.2f} ms/frame, {env_steps:.0f} env-steps/s; plain one by one "
+        f"{1000 * tot / len(ks):.2f} ms per scene-frame -> {plain_env:.0f} env-steps/s")
+    return {"metric": "env-steps/s @4096 uneven scenes", "value": round(env_steps, 1), "unit": "env-steps/s",
+            "frames": frames, "frames_before_timing": warmup + 1,
+            "data": "synthetic: 4096 seeded scenes, floor(exp(U[ln 8, ln 256])) bodies each plus a ground, "
+                    "gravity x U[0.8, 1.2]", "ms_per_frame": round(1000 * el / frames, 3),
+            "scenes": scenes, "bodies": cnt["num_bodies"],
+            "body_substeps_per_s": round(cnt["num_bodies"] * SUBSTEPS * frames / el, 1),
+            "contacts_last_substep": {"mean": round(float(c[:, 1].mean()), 2), "max": int(c[:, 1].max())},
+            "launch_shape": {"threads": threads, "workgroups_per_cu": wgs, "autotune_us_per_substep": round(us, 1)},
+            "plain_one_by_one": {"env_steps_per_s": round(plain_env, 1), "sample_scenes": len(ks),
+                                 "sample_mean_bodies": round(bodies / len(ks), 1)},
+            "speedup_vs_plain": round(env_steps / plain_env, 1)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +209,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-sample-substeps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-engine section")
+    ap.add_argument("--batched-frames", type=int, default=60)
+    ap.add_argument("--batched-warmup", type=int, default=240)
     ap.add_argument("--solver", default="auto", choices=["auto", "ordered", "flow"],
                     help="schedule of the contact sweep (all give the serial result bit for bit)")
     ap.add_argument("--solver-wgs", type=int, default=0, help="workgroups of the sweep kernel (0 = default)")
@@ -205,6 +285,12 @@ def main() -> int:
         log("timing the CPU oracle baseline ...")
         cpu = cpu_baseline(ctx, scene, n, args.cpu_sample_substeps)
         log(f"cpu baseline: {cpu['value']:.0f} body-substeps/s")
+    ctx.close()
+    scene.close()
+
+    batched = None
+    if not args.no_batched:
+        batched = batched_section(args.batched_frames, args.batched_warmup)
 
     out = {
         "metric": "body-substeps/s @32k bodies", "value": round(value, 1), "unit": "body-substeps/s",
@@ -217,9 +303,9 @@ def main() -> int:
                    "frames_before_timing": args.warmup, "pairs_last_substep": pairs,
                    "contacts_last_substep": contacts},
         "roofline": roof, "roofline_hbm_stages": hbm_stages, "cpu_baseline": cpu, "stage_share_pct": breakdown,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
-    ctx.close()
     return 0
 
 

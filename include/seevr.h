@@ -286,6 +286,62 @@ int sv_scene_upload(sv_scene *scene, sv_ctx *ctx);
 int sv_scene_build_workload(sv_scene *scene, int workload, uint32_t count,
                             uint32_t sphere_segments, uint64_t seed);
 
+/* Independent scenes for the batched engine (library extension): scenes
+ * [first, first + count) of a seeded family, each with its own ground at the
+ * origin and floor(exp(U[ln min_bodies, ln max_bodies])) bodies.  Scene k
+ * draws from its own stream, so any sub-range builds the same bodies.
+ * Writes body_start[0..count] and gravity_scale[0..count). */
+int sv_scene_build_batch(sv_scene *scene, uint32_t first, uint32_t count, uint32_t min_bodies,
+                         uint32_t max_bodies, uint32_t sphere_segments, uint64_t seed,
+                         uint32_t *body_start, float *gravity_scale);
+
+/* ---- batched engine (library extension, no reference counterpart) -----
+ * Steps many independent scenes of uneven body counts.  The bodies of all
+ * scenes are concatenated in the buffers of one context (sv_batch_context;
+ * fill them with sv_upload / sv_scene_upload); scene s owns bodies
+ * [body_start[s], body_start[s+1]) and collides only with itself.  Each
+ * scene gets max_scene_pairs pair slots and max_scene_contacts contact slots
+ * at offset s times those capacities in SV_COLLISION_PAIRS and the contact
+ * buffers.  A substep of every scene is one persistent launch (workgroups
+ * take scenes, largest first, from a queue); a frame of `substeps` launches
+ * plus the render-vertex update is replayed as one captured hipGraph.  Every
+ * scene's results equal sv_physics_step on that scene alone, bit for bit,
+ * including its pair and contact order and overflow truncation. */
+#define SV_BATCH_MAX_SCENE_BODIES 512
+#define SV_BATCH_MAX_SCENE_CONTACTS 2048
+typedef struct sv_batch sv_batch;
+typedef struct sv_batch_caps {
+    uint32_t max_scenes;
+    uint32_t max_bodies;          /* all scenes together                     */
+    uint32_t max_hull_verts;
+    uint32_t max_vertices;
+    uint32_t max_triangles;       /* render triangles (BVH input)            */
+    uint32_t max_support_entries;
+    uint32_t max_scene_bodies;    /* <= SV_BATCH_MAX_SCENE_BODIES            */
+    uint32_t max_scene_pairs;     /* pair list capacity of each scene        */
+    uint32_t max_scene_contacts;  /* <= SV_BATCH_MAX_SCENE_CONTACTS          */
+} sv_batch_caps;
+int sv_batch_create(const sv_batch_caps *caps, const sv_params *params, sv_batch **out);
+int sv_batch_destroy(sv_batch *batch);
+/* The context that holds the scenes' buffers (owned by the batch). */
+sv_ctx *sv_batch_context(sv_batch *batch);
+/* Scene table: body_start[0..num_scenes] and per-scene gravity[3 * s]. */
+int sv_batch_set_scenes(sv_batch *batch, uint32_t num_scenes, const uint32_t *body_start,
+                        const float *gravity);
+/* One frame of every scene: `substeps` substeps of dt / substeps, then the
+ * render vertices [0, num_vertices). */
+int sv_batch_step(sv_batch *batch, uint32_t num_vertices, uint32_t substeps, float dt);
+/* Per scene: total pairs found, total contacts found (last substep) and the
+ * SV_STATUS_* bits of the last frame, 3 words per scene. */
+int sv_batch_read_counters(sv_batch *batch, uint32_t *out);
+/* Launch shape: threads per workgroup (64, 128, 256 or 512), workgroups per
+ * CU of the persistent grid, and whether frames replay a hipGraph. */
+int sv_batch_set_config(sv_batch *batch, uint32_t threads, uint32_t workgroups_per_cu, int use_graph);
+/* Times every launch shape on the current scenes (state is restored) and
+ * keeps the fastest; returns it and its time per substep. */
+int sv_batch_autotune(sv_batch *batch, uint32_t substeps, float dt, uint32_t *threads,
+                      uint32_t *workgroups_per_cu, double *us_per_substep);
+
 #ifdef __cplusplus
 }
 #endif

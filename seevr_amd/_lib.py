@@ -54,6 +54,12 @@ class Params(C.Structure):
                 ("hull_range_u32", C.c_uint32)]
 
 
+class BatchCaps(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("max_scenes", "max_bodies", "max_hull_verts", "max_vertices",
+                                          "max_triangles", "max_support_entries", "max_scene_bodies", "max_scene_pairs",
+                                          "max_scene_contacts")]
+
+
 class SceneCounts(C.Structure):
     _fields_ = [("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("num_bodies", C.c_uint32),
                 ("num_geoms", C.c_uint32), ("num_hull_verts", C.c_uint32), ("num_materials", C.c_uint32),
@@ -67,7 +73,8 @@ EXPORTS = [
     "sv_debug_copy_bandwidth", "sv_last_error", "sv_device_info", "sv_stage_times", "sv_enable_stage_timing", "sv_set_solver", "sv_set_broadphase", "sv_scene_create",
     "sv_scene_destroy", "sv_scene_add_material", "sv_scene_add_box", "sv_scene_add_sphere",
     "sv_scene_set_body_state", "sv_scene_get_counts", "sv_scene_array", "sv_scene_upload",
-    "sv_scene_build_workload",
+    "sv_scene_build_workload", "sv_scene_build_batch", "sv_batch_create", "sv_batch_destroy", "sv_batch_context",
+    "sv_batch_set_scenes", "sv_batch_step", "sv_batch_read_counters", "sv_batch_set_config", "sv_batch_autotune",
 ]
 
 _lib = None
@@ -125,6 +132,15 @@ def load(path: Path | str | None = None) -> C.CDLL:
         "sv_scene_array": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(sz)]),
         "sv_scene_upload": (C.c_int, [vp, vp]),
         "sv_scene_build_workload": (C.c_int, [vp, C.c_int, u32, u32, C.c_uint64]),
+        "sv_scene_build_batch": (C.c_int, [vp, u32, u32, u32, u32, u32, C.c_uint64, C.POINTER(u32), fp]),
+        "sv_batch_create": (C.c_int, [C.POINTER(BatchCaps), C.POINTER(Params), C.POINTER(vp)]),
+        "sv_batch_destroy": (C.c_int, [vp]),
+        "sv_batch_context": (vp, [vp]),
+        "sv_batch_set_scenes": (C.c_int, [vp, u32, C.POINTER(u32), fp]),
+        "sv_batch_step": (C.c_int, [vp, u32, u32, f32]),
+        "sv_batch_read_counters": (C.c_int, [vp, C.POINTER(u32)]),
+        "sv_batch_set_config": (C.c_int, [vp, u32, u32, C.c_int]),
+        "sv_batch_autotune": (C.c_int, [vp, u32, f32, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -13,7 +13,7 @@ from typing import Iterable, Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import BUF, BUF_DTYPE, Caps, Params, SceneCounts, check
+from ._lib import BUF, BUF_DTYPE, BatchCaps, Caps, Params, SceneCounts, check
 
 
 def _fvec(vals: Iterable[float], n: int):
@@ -53,11 +53,23 @@ class Context:
         h = C.c_void_p()
         check(self.lib.sv_create(C.byref(self.caps), C.byref(self.params), C.byref(h)))
         self.h = h
+        self.owned = True
+
+    @classmethod
+    def _view(cls, handle, caps: Caps, params: Params) -> "Context":
+        """A non-owning view of a context that another object (a Batch) owns."""
+        ctx = cls.__new__(cls)
+        ctx.lib = _lib.load()
+        ctx.caps, ctx.params = caps, params
+        ctx.h = C.c_void_p(handle)
+        ctx.owned = False
+        return ctx
 
     # -- lifetime -----------------------------------------------------------
     def close(self) -> None:
         if getattr(self, "h", None):
-            self.lib.sv_destroy(self.h)
+            if getattr(self, "owned", True):
+                self.lib.sv_destroy(self.h)
             self.h = None
 
     def __enter__(self):
@@ -229,6 +241,17 @@ class Scene:
     def build_workload(self, workload: int, count: int = 0, segments: int = 0, seed: int = 1) -> None:
         check(self.lib.sv_scene_build_workload(self.h, workload, count, segments, seed))
 
+    def build_batch(self, first: int, count: int, min_bodies: int = 8, max_bodies: int = 256, segments: int = 0,
+                    seed: int = 1) -> tuple[np.ndarray, np.ndarray]:
+        """Scenes [first, first + count) of the batched family; returns
+        (body_start[count + 1], gravity_scale[count])."""
+        start = np.zeros(count + 1, np.uint32)
+        gs = np.zeros(count, np.float32)
+        check(self.lib.sv_scene_build_batch(self.h, first, count, min_bodies, max_bodies, segments, seed,
+                                            start.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                            gs.ctypes.data_as(C.POINTER(C.c_float))))
+        return start, gs
+
     def counts(self) -> dict:
         c = SceneCounts()
         check(self.lib.sv_scene_get_counts(self.h, C.byref(c)))
@@ -257,3 +280,65 @@ def caps_for(scene: Scene, pairs: int, contacts: int, triangles: int | None = No
                         max_triangles=max(c["num_triangles"] if triangles is None else triangles, 1),
                         max_pairs=pairs, max_contacts=contacts,
                         max_support_entries=max(c["num_support_entries"], 1))
+
+
+class Batch:
+    """Batched engine (sv_batch_*): independent scenes stepped together."""
+
+    def __init__(self, caps: BatchCaps, params: Params | None = None):
+        self.lib = _lib.load()
+        self.caps = caps
+        self.params = params if params is not None else default_params()
+        h = C.c_void_p()
+        check(self.lib.sv_batch_create(C.byref(caps), C.byref(self.params), C.byref(h)))
+        self.h = h
+        cc = default_caps(max_bodies=caps.max_bodies, max_hull_verts=caps.max_hull_verts,
+                          max_vertices=caps.max_vertices, max_triangles=max(caps.max_triangles, 1),
+                          max_pairs=caps.max_scenes * caps.max_scene_pairs,
+                          max_contacts=caps.max_scenes * caps.max_scene_contacts,
+                          max_support_entries=caps.max_support_entries)
+        self.ctx = Context._view(self.lib.sv_batch_context(h), cc, self.params)
+        self.num_scenes = 0
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.ctx.close()
+            self.lib.sv_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_scenes(self, body_start, gravity) -> None:
+        bs = np.ascontiguousarray(body_start, np.uint32)
+        g = np.ascontiguousarray(gravity, np.float32).reshape(-1)
+        n = bs.size - 1
+        assert g.size == 3 * n
+        check(self.lib.sv_batch_set_scenes(self.h, n, bs.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           g.ctypes.data_as(C.POINTER(C.c_float))))
+        self.num_scenes = n
+
+    def step(self, num_vertices: int, substeps: int = 10, dt: float = 1.0 / 60.0) -> None:
+        check(self.lib.sv_batch_step(self.h, num_vertices, substeps, C.c_float(dt)))
+
+    def counters(self) -> np.ndarray:
+        out = np.zeros(3 * max(self.num_scenes, 1), np.uint32)
+        check(self.lib.sv_batch_read_counters(self.h, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out[: 3 * self.num_scenes].reshape(-1, 3)
+
+    def set_config(self, threads: int = 256, workgroups_per_cu: int = 2, use_graph: bool = True) -> None:
+        check(self.lib.sv_batch_set_config(self.h, threads, workgroups_per_cu, int(use_graph)))
+
+    def autotune(self, substeps: int = 10, dt: float = 1.0 / 60.0) -> tuple[int, int, float]:
+        t, w, us = C.c_uint32(), C.c_uint32(), C.c_double()
+        check(self.lib.sv_batch_autotune(self.h, substeps, C.c_float(dt), C.byref(t), C.byref(w), C.byref(us)))
+        return t.value, w.value, us.value

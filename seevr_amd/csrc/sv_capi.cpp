@@ -150,6 +150,8 @@ bool valid_buf(int id) { return id >= 0 && id < SV_BUF_COUNT; }
 
 namespace sv {
 
+int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
+
 int timing_begin(sv_ctx *c, const char *name) {
     if (!c->timing) return -1;
     int stage = -1;

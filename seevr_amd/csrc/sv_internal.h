@@ -157,6 +157,9 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n);
 // Body count from which the automatic broadphase choice is the LBVH.
 constexpr uint32_t BP_LBVH_MIN_BODIES = 1u << 17;
 
+// ---- errors (sv_capi.cpp): sets sv_last_error() and returns code ----
+int set_error(int code, const char *msg);
+
 // ---- stage timing (sv_capi.cpp) ----
 int timing_begin(sv_ctx *c, const char *name);
 void timing_end(sv_ctx *c, int token);

@@ -30,6 +30,7 @@
 // whose hand-offs use device-scope release/acquire.
 #include "sv_internal.h"
 #include "sv_math.h"
+#include "sv_dev.h"
 
 namespace sv {
 
@@ -43,22 +44,6 @@ constexpr uint64_t STALL_TICKS = 200000000ull;  // 2 s of the 100 MHz wall clock
 template <typename T>
 inline T *B(sv_ctx *c, int id) { return reinterpret_cast<T *>(c->buf[id]); }
 
-// cprep word offsets
-enum : int {
-    P_N = 0, P_RA = 3, P_RB = 6, P_KN = 9, P_BIAS = 10, P_MU = 11, P_IMA = 12, P_IMB = 13,
-    P_A = 14, P_B = 15, P_FLAGS = 16, P_SUCC_A = 17, P_SUCC_B = 18
-};
-// Successor words (flow form): the contact that next updates the same body,
-// in the same iteration or (WRAP) in the next one; HASPRED marks a contact
-// with an earlier update of that body in iteration 0.
-constexpr uint32_t SUCC_NONE = 0xFFFFFFFFu;
-constexpr uint32_t SUCC_WRAP = 0x80000000u;
-constexpr uint32_t SUCC_HASPRED = 0x40000000u;
-constexpr uint32_t SUCC_ID = 0x3FFFFFFFu;
-enum : uint32_t { F_SKIP = 1u, F_DYN_A = 2u, F_DYN_B = 4u };
-
-SVD f3 apply_inv_I(const m3 &R, f3 inv_i, f3 v) { return mvec(R, hmul(inv_i, mtvec(R, v))); }
-
 __global__ void k_set_ninc(uint32_t *misc) { misc[MISC_NINC] = 2u * misc[MISC_NC]; }
 
 // contacts to solve: min(counters[1], max_contacts) (physics.comp:502)
@@ -67,10 +52,8 @@ __global__ void k_solve_count(const uint32_t *counters, uint32_t *misc, uint32_t
     misc[MISC_NC] = t < max_contacts ? t : max_contacts;
 }
 
-// Per-contact constants of the sweep.  The bias uses the velocities before
-// the sweep (physics.comp:504-521); k_n depends only on positions and
-// orientations, which the sweep does not change, so it equals the value the
-// serial loop recomputes every iteration (physics.comp:544-546).
+// Per-contact constants of the sweep (contact_prep), incidence keys and
+// per-body degrees.
 __global__ void __launch_bounds__(BLOCK) k_prep(
     uint32_t *__restrict__ misc_w, uint32_t n_bodies, float dt, float baumgarte, float slop,
     const uint32_t *__restrict__ cga, const uint32_t *__restrict__ cgb, const uint32_t *__restrict__ gbody,
@@ -83,37 +66,9 @@ __global__ void __launch_bounds__(BLOCK) k_prep(
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         uint32_t ga = cga[c], gb = cgb[c];
         uint32_t a = gbody[ga], b = gbody[gb];
-        f3 n = ld3(cnrm, c), cp = ld3(cpos, c);
-        f3 ra = cp - ld3(pos, a), rb = cp - ld3(pos, b);
-        f3 va = ld3(vel, a) + cross3(ld3(omega, a), ra);
-        f3 vb = ld3(vel, b) + cross3(ld3(omega, b), rb);
-        float vn = dot3(vb - va, n);
-        float e = gmax(rest[ga], rest[gb]);
-        float bias = baumgarte * gmax(cpen[c] - slop, 0.0f) / dt + e * gmax(-vn, 0.0f);
-        float ima = inv_mass[a], imb = inv_mass[b];
-        f3 iia = ld3(inv_i, a), iib = ld3(inv_i, b);
-        m3 Ra = ld_rot(rot, a), Rb = ld_rot(rot, b);
-        f3 ran = cross3(ra, n), rbn = cross3(rb, n);
-        float kn = ima + imb + dot3(ran, apply_inv_I(Ra, iia, ran)) + dot3(rbn, apply_inv_I(Rb, iib, rbn));
-        float mu = __builtin_sqrtf(fric[ga] * fric[gb]);
-        bool skip = (ima == 0.0f && imb == 0.0f) || a == b;
-        bool dyn_a = !(ima == 0.0f && iia.x == 0.0f && iia.y == 0.0f && iia.z == 0.0f);
-        bool dyn_b = !(imb == 0.0f && iib.x == 0.0f && iib.y == 0.0f && iib.z == 0.0f);
-        uint32_t flags = (skip ? F_SKIP : 0u) | (dyn_a ? F_DYN_A : 0u) | (dyn_b ? F_DYN_B : 0u);
-        float *P = prep + (size_t)c * CPREP_WORDS;
-        P[P_N + 0] = n.x; P[P_N + 1] = n.y; P[P_N + 2] = n.z;
-        P[P_RA + 0] = ra.x; P[P_RA + 1] = ra.y; P[P_RA + 2] = ra.z;
-        P[P_RB + 0] = rb.x; P[P_RB + 1] = rb.y; P[P_RB + 2] = rb.z;
-        P[P_KN] = kn;
-        P[P_BIAS] = bias;
-        P[P_MU] = mu;
-        P[P_IMA] = ima;
-        P[P_IMB] = imb;
-        P[P_A] = __uint_as_float(a);
-        P[P_B] = __uint_as_float(b);
-        P[P_FLAGS] = __uint_as_float(flags);
-        P[P_SUCC_A] = __uint_as_float(SUCC_NONE);
-        P[P_SUCC_B] = __uint_as_float(SUCC_NONE);
+        uint32_t flags = contact_prep(a, b, ga, gb, ld3(cnrm, c), ld3(cpos, c), cpen[c], dt, baumgarte, slop, pos,
+                                      vel, omega, inv_mass, inv_i, rot, fric, rest, prep + (size_t)c * CPREP_WORDS);
+        bool skip = flags & F_SKIP, dyn_a = flags & F_DYN_A, dyn_b = flags & F_DYN_B;
         if (!skip) atomicAdd(&misc_w[MISC_NACT], 1u);
         bool ia = !skip && dyn_a, ib = !skip && dyn_b;
         key[2 * c] = ia ? a : n_bodies;
@@ -135,16 +90,6 @@ __global__ void __launch_bounds__(BLOCK) k_rank(const uint32_t *__restrict__ mis
         if (k < n_bodies) rank[val[p]] = p - deg_off[k];
     }
 }
-
-// One item of the sweep with its constants in registers.
-struct Item {
-    f3 n, ra, rb;
-    float kn, bias, mu, ima, imb;
-    m3 Ra, Rb;
-    f3 iia, iib;
-    uint32_t c, a, b, ta, tb;
-    bool dyn_a, dyn_b;
-};
 
 // Loads item q's constants; returns false for a skipped contact
 // (both inverse masses zero: physics.comp:531).
@@ -176,47 +121,6 @@ SVD bool load_item(Item &it, uint32_t q, uint32_t nc, const float *__restrict__ 
     it.iia = ld3(inv_i, it.a);
     it.iib = ld3(inv_i, it.b);
     return true;
-}
-
-// physics.comp:525-580 for one contact: normal impulse with projection,
-// then Coulomb friction with the geometric-mean coefficient.
-SVD void apply_item(const Item &it, float *__restrict__ vel, float *__restrict__ omega,
-                    float *__restrict__ lambda) {
-    f3 la = ld3(vel, it.a), wa = ld3(omega, it.a);
-    f3 lb = ld3(vel, it.b), wb = ld3(omega, it.b);
-    float old = lambda[it.c];
-    f3 va = la + cross3(wa, it.ra);
-    f3 vb = lb + cross3(wb, it.rb);
-    float vn = dot3(vb - va, it.n);
-    float dln = (-vn + it.bias) / it.kn;
-    float nl = gmax(old + dln, 0.0f);
-    dln = nl - old;
-    lambda[it.c] = nl;
-    f3 imp = dln * it.n;
-    la = la - it.ima * imp;
-    lb = lb + it.imb * imp;
-    wa = wa - apply_inv_I(it.Ra, it.iia, cross3(it.ra, imp));
-    wb = wb + apply_inv_I(it.Rb, it.iib, cross3(it.rb, imp));
-
-    va = la + cross3(wa, it.ra);
-    vb = lb + cross3(wb, it.rb);
-    f3 dv = vb - va;
-    f3 vt = dv - dot3(dv, it.n) * it.n;
-    float vtl = len3(vt);
-    if (vtl > 1e-6f) {
-        f3 td = mk(vt.x / vtl, vt.y / vtl, vt.z / vtl);
-        f3 rat = cross3(it.ra, td), rbt = cross3(it.rb, td);
-        float kt = it.ima + it.imb + dot3(rat, apply_inv_I(it.Ra, it.iia, rat)) +
-                   dot3(rbt, apply_inv_I(it.Rb, it.iib, rbt));
-        float dlt = gmin(vtl / kt, it.mu * nl);
-        f3 impt = dlt * td;
-        la = la + it.ima * impt;
-        lb = lb - it.imb * impt;
-        wa = wa + apply_inv_I(it.Ra, it.iia, cross3(it.ra, impt));
-        wb = wb - apply_inv_I(it.Rb, it.iib, cross3(it.rb, impt));
-    }
-    if (it.dyn_a) { st3(vel, it.a, la); st3(omega, it.a, wa); }
-    if (it.dyn_b) { st3(vel, it.b, lb); st3(omega, it.b, wb); }
 }
 
 // Ticket storage and hand-off scope of each solver form.

@@ -3,6 +3,7 @@
 // integration, render vertices.  The ordered PGS solver is in sv_pgs.hip.
 #include "sv_internal.h"
 #include "sv_math.h"
+#include "sv_dev.h"
 
 namespace sv {
 
@@ -35,13 +36,7 @@ __global__ void k_gravity(uint32_t n, float dt, float gx, float gy, float gz, fl
                           const float *__restrict__ inv_mass, float *__restrict__ vel, float *__restrict__ omega) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (inv_mass[i] <= 0.0f) return;
-    vel[i * 3 + 0] = vel[i * 3 + 0] * ld + gx * dt;
-    vel[i * 3 + 1] = vel[i * 3 + 1] * ld + gy * dt;
-    vel[i * 3 + 2] = vel[i * 3 + 2] * ld + gz * dt;
-    omega[i * 3 + 0] *= ad;
-    omega[i * 3 + 1] *= ad;
-    omega[i * 3 + 2] *= ad;
+    gravity_body(i, dt, gx, gy, gz, ld, ad, inv_mass, vel, omega);
 }
 
 // Per-body AABB, the semantics of physics.comp:431-448: min/max over
@@ -57,22 +52,9 @@ __global__ void __launch_bounds__(BLOCK) k_aabb(uint32_t n, const float *__restr
     uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     int lane = threadIdx.x & 63;
     if (b >= n) return;
-    m3 R = quat_mat_at(quat, b);
-    f3 p = ld3(pos, b);
-    uint32_t vs = hrange[2 * b], vc = hrange[2 * b + 1];
-    f3 mn = mk(1e10f, 1e10f, 1e10f), mx = mk(-1e10f, -1e10f, -1e10f);
-    for (uint32_t v = lane; v < vc; v += 64) {
-        f3 wv = p + mvec(R, ld3(hull, vs + v));
-        mn = vmin3(mn, wv);
-        mx = vmax3(mx, wv);
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        f3 omn = mk(__shfl_xor(mn.x, off, 64), __shfl_xor(mn.y, off, 64), __shfl_xor(mn.z, off, 64));
-        f3 omx = mk(__shfl_xor(mx.x, off, 64), __shfl_xor(mx.y, off, 64), __shfl_xor(mx.z, off, 64));
-        mn = vmin3(mn, omn);
-        mx = vmax3(mx, omx);
-    }
+    m3 R;
+    f3 mn, mx;
+    body_aabb_wave(b, lane, pos, quat, hrange, hull, R, mn, mx);
     if (lane == 0) {
         st3(amin_out, b, mn);
         st3(amax_out, b, mx);
@@ -231,240 +213,6 @@ __global__ void k_post_pairs(uint32_t *counters, uint32_t *misc, uint32_t max_pa
     if (t > max_pairs) counters[2] |= SV_STATUS_PAIR_OVERFLOW;
 }
 
-// ---------------------------------------------------------------------------
-// MPR narrowphase (physics.comp:107-399), one lane per pair.
-struct Body {
-    f3 p;
-    m3 R;
-    uint32_t vs, vc, lk;
-};
-
-struct Geo {
-    const float *hull;
-    const uint32_t *lookup;
-};
-
-SVD Body load_body(uint32_t b, const float *pos, const float *rot, const uint32_t *hrange,
-                   const uint32_t *sl_off) {
-    Body B;
-    B.p = ld3(pos, b);
-    B.R = ld_rot(rot, b);
-    B.vs = hrange[2 * b];
-    B.vc = hrange[2 * b + 1];
-    B.lk = sl_off[b];
-    return B;
-}
-
-// physics.comp:107-162: farthest hull vertex along world direction d.
-SVD f3 support(const Body &B, f3 d, const Geo &g) {
-    f3 ld = mtvec(B.R, d);
-    float best = -1e10f;
-    f3 bv = mk(0.0f, 0.0f, 0.0f);
-    if (B.lk == 0) {
-        for (uint32_t i = 0; i < B.vc; i++) {
-            f3 v = ld3(g.hull, B.vs + i);
-            float dd = dot3(v, ld);
-            if (dd > best) {
-                best = dd;
-                bv = v;
-            }
-        }
-        return B.p + mvec(B.R, bv);
-    }
-    float ang[3];
-    ang[0] = sv_atan2f(ld.z, ld.y);
-    ang[1] = sv_atan2f(ld.z, ld.x);
-    ang[2] = sv_atan2f(ld.y, ld.x);
-#pragma unroll
-    for (uint32_t ax = 0; ax < 3; ax++) {
-        float a = ang[ax];
-        if (a < 0.0f) a += 6.2831853f;
-        float bin_f = a / 6.2831853f * 64.0f;
-        uint32_t b0 = ((uint32_t)bin_f) % 64u;
-        uint32_t b1 = (b0 + 1u) % 64u;
-        uint32_t i0 = g.lookup[B.lk + ax * 64u + b0];
-        f3 v0 = ld3(g.hull, B.vs + i0);
-        float d0 = dot3(v0, ld);
-        if (d0 > best) {
-            best = d0;
-            bv = v0;
-        }
-        uint32_t i1 = g.lookup[B.lk + ax * 64u + b1];
-        f3 v1 = ld3(g.hull, B.vs + i1);
-        float d1 = dot3(v1, ld);
-        if (d1 > best) {
-            best = d1;
-            bv = v1;
-        }
-    }
-    return B.p + mvec(B.R, bv);
-}
-
-constexpr float CCD_EPS = 1e-6f;
-constexpr float CCD_TOL = 1e-6f;
-constexpr int CCD_ITERS = 50;
-
-// Minkowski portal: Minkowski points and the support points on A and B.
-struct Portal {
-    f3 v[4], sa[4], sb[4];
-};
-
-struct Pair {
-    Body A, Bb;
-};
-
-SVD void mink(const Pair &P, const Geo &g, f3 d, f3 &v, f3 &sa, f3 &sb) {
-    sa = support(P.A, d, g);
-    sb = support(P.Bb, -d, g);
-    v = sa - sb;
-}
-
-SVD f3 portal_dir(f3 v1, f3 v2, f3 v3) { return norm3(cross3(v2 - v1, v3 - v1)); }
-
-SVD bool reach_tol(f3 v1, f3 v2, f3 v3, f3 v4, f3 dir) {
-    float dv1 = dot3(v1, dir), dv2 = dot3(v2, dir), dv3 = dot3(v3, dir), dv4 = dot3(v4, dir);
-    float d1 = gmin(gmin(dv4 - dv1, dv4 - dv2), dv4 - dv3);
-    return d1 < CCD_TOL + CCD_EPS * gmax(1.0f, d1);
-}
-
-// physics.comp:200-221
-SVD void expand_portal(Portal &s, f3 v4, f3 s4a, f3 s4b) {
-    f3 v4v0 = cross3(v4, s.v[0]);
-    int slot;
-    if (dot3(s.v[1], v4v0) > 0.0f) {
-        slot = dot3(s.v[2], v4v0) > 0.0f ? 1 : 3;
-    } else {
-        slot = dot3(s.v[3], v4v0) > 0.0f ? 2 : 1;
-    }
-    if (slot == 1) { s.v[1] = v4; s.sa[1] = s4a; s.sb[1] = s4b; }
-    else if (slot == 2) { s.v[2] = v4; s.sa[2] = s4a; s.sb[2] = s4b; }
-    else { s.v[3] = v4; s.sa[3] = s4a; s.sb[3] = s4b; }
-}
-
-// physics.comp:224-247: barycentric contact point, averaged over A and B.
-SVD f3 find_pos(const Portal &s) {
-    f3 v0 = s.v[0], v1 = s.v[1], v2 = s.v[2], v3 = s.v[3];
-    float b0 = dot3(cross3(v1, v2), v3);
-    float b1 = -dot3(cross3(v0, v2), v3);
-    float b2 = dot3(cross3(v0, v1), v3);
-    float b3 = -dot3(cross3(v0, v1), v2);
-    float sum = b0 + b1 + b2 + b3;
-    if (__builtin_fabsf(sum) < CCD_EPS) {
-        f3 dir = portal_dir(v1, v2, v3);
-        b0 = 0.0f;
-        b1 = dot3(cross3(v2, v3), dir);
-        b2 = dot3(cross3(v3, v1), dir);
-        b3 = dot3(cross3(v1, v2), dir);
-        sum = b0 + b1 + b2 + b3;
-    }
-    f3 p1 = ((b0 * s.sa[0] + b1 * s.sa[1]) + b2 * s.sa[2]) + b3 * s.sa[3];
-    f3 p2 = ((b0 * s.sb[0] + b1 * s.sb[1]) + b2 * s.sb[2]) + b3 * s.sb[3];
-    return (0.5f / sum) * (p1 + p2);
-}
-
-// physics.comp:250-320: -1 separated, 0 portal found, 1 touching, 2 collinear
-SVD int discover_portal(const Pair &P, const Geo &g, Portal &s) {
-    s.sa[0] = P.A.p;
-    s.sb[0] = P.Bb.p;
-    s.v[0] = s.sa[0] - s.sb[0];
-    if (__builtin_fabsf(s.v[0].x) < CCD_EPS && __builtin_fabsf(s.v[0].y) < CCD_EPS &&
-        __builtin_fabsf(s.v[0].z) < CCD_EPS)
-        s.v[0].x += 10.0f * CCD_EPS;
-    f3 dir = norm3(-s.v[0]);
-    mink(P, g, dir, s.v[1], s.sa[1], s.sb[1]);
-    if (dot3(s.v[1], dir) < CCD_EPS) return -1;
-    dir = cross3(s.v[0], s.v[1]);
-    if (dot3(dir, dir) < CCD_EPS) {
-        if (__builtin_fabsf(s.v[1].x) < CCD_EPS && __builtin_fabsf(s.v[1].y) < CCD_EPS &&
-            __builtin_fabsf(s.v[1].z) < CCD_EPS)
-            return 1;
-        return 2;
-    }
-    dir = norm3(dir);
-    mink(P, g, dir, s.v[2], s.sa[2], s.sb[2]);
-    if (dot3(s.v[2], dir) < CCD_EPS) return -1;
-    f3 va = s.v[1] - s.v[0];
-    f3 vb = s.v[2] - s.v[0];
-    dir = norm3(cross3(va, vb));
-    if (dot3(dir, s.v[0]) > 0.0f) {
-        f3 t = s.v[1]; s.v[1] = s.v[2]; s.v[2] = t;
-        t = s.sa[1]; s.sa[1] = s.sa[2]; s.sa[2] = t;
-        t = s.sb[1]; s.sb[1] = s.sb[2]; s.sb[2] = t;
-        dir = -dir;
-    }
-    for (int trials = 0; trials < 15;) {
-        f3 v3c, s3a, s3b;
-        mink(P, g, dir, v3c, s3a, s3b);
-        if (dot3(v3c, dir) < CCD_EPS) return -1;
-        if (dot3(cross3(s.v[1], v3c), s.v[0]) < -CCD_EPS) {
-            s.v[2] = v3c; s.sa[2] = s3a; s.sb[2] = s3b;
-            dir = norm3(cross3(s.v[1] - s.v[0], s.v[2] - s.v[0]));
-            trials++;
-            continue;
-        }
-        if (dot3(cross3(v3c, s.v[2]), s.v[0]) < -CCD_EPS) {
-            s.v[1] = v3c; s.sa[1] = s3a; s.sb[1] = s3b;
-            dir = norm3(cross3(s.v[1] - s.v[0], s.v[2] - s.v[0]));
-            trials++;
-            continue;
-        }
-        s.v[3] = v3c; s.sa[3] = s3a; s.sb[3] = s3b;
-        return 0;
-    }
-    return -1;
-}
-
-// physics.comp:323-343
-SVD int refine_portal(const Pair &P, const Geo &g, Portal &s) {
-    for (int it = 0; it < CCD_ITERS; it++) {
-        f3 dir = portal_dir(s.v[1], s.v[2], s.v[3]);
-        if (dot3(s.v[1], dir) > -CCD_EPS) return 0;
-        f3 v4, s4a, s4b;
-        mink(P, g, dir, v4, s4a, s4b);
-        if (!(dot3(v4, dir) > -CCD_EPS) || reach_tol(s.v[1], s.v[2], s.v[3], v4, dir)) return -1;
-        expand_portal(s, v4, s4a, s4b);
-    }
-    return 1;
-}
-
-// physics.comp:346-365 (always terminates with a result by the last iteration)
-SVD void find_penetration(const Pair &P, const Geo &g, Portal &s, f3 &normal, float &depth) {
-    for (int it = 0; it < CCD_ITERS; it++) {
-        f3 dir = portal_dir(s.v[1], s.v[2], s.v[3]);
-        f3 v4, s4a, s4b;
-        mink(P, g, dir, v4, s4a, s4b);
-        if (reach_tol(s.v[1], s.v[2], s.v[3], v4, dir) || it >= CCD_ITERS - 1) {
-            depth = dot3(dir, s.v[1]);
-            normal = dir;
-            return;
-        }
-        expand_portal(s, v4, s4a, s4b);
-    }
-}
-
-// physics.comp:368-399
-SVD bool mpr(const Pair &P, const Geo &g, f3 &normal, float &depth, f3 &cp) {
-    Portal s;
-    int res = discover_portal(P, g, s);
-    if (res == 1) {
-        normal = norm3(s.v[0]);
-        depth = 0.0f;
-        cp = 0.5f * (s.sa[1] + s.sb[1]);
-        return true;
-    }
-    if (res == 2) {
-        depth = len3(s.v[1]);
-        normal = norm3(s.v[1]);
-        cp = 0.5f * (s.sa[1] + s.sb[1]);
-        return true;
-    }
-    if (res != 0) return false;
-    if (refine_portal(P, g, s) < 0) return false;
-    find_penetration(P, g, s, normal, depth);
-    cp = find_pos(s);
-    return true;
-}
-
 // Step 3 (physics.comp:475-491), result per pair slot: hit flag and
 // (pos, normal, depth); contacts are then compacted in pair order.
 __global__ void __launch_bounds__(BLOCK) k_narrow(const uint32_t *__restrict__ misc, const uint32_t *__restrict__ pairs,
@@ -534,31 +282,7 @@ __global__ void k_integrate(uint32_t n, float dt, const float *__restrict__ inv_
                             const float *__restrict__ omega) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (inv_mass[i] <= 0.0f) return;
-    pos[i * 3 + 0] += vel[i * 3 + 0] * dt;
-    pos[i * 3 + 1] += vel[i * 3 + 1] * dt;
-    pos[i * 3 + 2] += vel[i * 3 + 2] * dt;
-    float ox = omega[i * 3], oy = omega[i * 3 + 1], oz = omega[i * 3 + 2];
-    float qx = quat[i * 4], qy = quat[i * 4 + 1], qz = quat[i * 4 + 2], qw = quat[i * 4 + 3];
-    float dx = ox * qw + oy * qz - oz * qy;
-    float dy = -ox * qz + oy * qw + oz * qx;
-    float dz = ox * qy - oy * qx + oz * qw;
-    float dw = -ox * qx - oy * qy - oz * qz;
-    qx += 0.5f * dx * dt;
-    qy += 0.5f * dy * dt;
-    qz += 0.5f * dz * dt;
-    qw += 0.5f * dw * dt;
-    float l = __builtin_sqrtf(qx * qx + qy * qy + qz * qz + qw * qw);
-    if (l > 1e-8f) {
-        qx /= l;
-        qy /= l;
-        qz /= l;
-        qw /= l;
-    }
-    quat[i * 4] = qx;
-    quat[i * 4 + 1] = qy;
-    quat[i * 4 + 2] = qz;
-    quat[i * 4 + 3] = qw;
+    integrate_body(i, dt, inv_mass, pos, quat, vel, omega);
 }
 
 // Step 7 (physics.comp:607-627): owner body by binary search on
