@@ -14,9 +14,7 @@
 //
 // The per-body (body, contact) incidence order the sweep needs comes from a
 // bitonic sort in LDS of keys (local body << 12 | 2 * contact + side).
-#include "sv_internal.h"
-#include "sv_math.h"
-#include "sv_dev.h"
+#include "sv_batch.h"
 
 #include <algorithm>
 #include <new>
@@ -27,62 +25,7 @@ namespace sv {
 
 namespace {
 
-constexpr uint32_t NB_MAX = SV_BATCH_MAX_SCENE_BODIES;
-constexpr uint32_t C_MAX = SV_BATCH_MAX_SCENE_CONTACTS;
-constexpr uint32_t INC_MAX = 2 * C_MAX;  // incidences, a power of two
-constexpr uint32_t MAX_SUBSTEPS = 4096;
-constexpr uint64_t STALL_TICKS = 200000000ull;  // 2 s of the 100 MHz wall clock
-static_assert((INC_MAX & (INC_MAX - 1)) == 0, "incidence sort size must be a power of two");
-static_assert(NB_MAX <= 512 && 2 * C_MAX <= 4096, "incidence key packing: 9-bit body, 12-bit slot");
-
-struct BatchArgs {
-    const uint32_t *sc_body;   // [S + 1] first body of each scene
-    const float *sc_grav;      // [3S]
-    const uint32_t *sc_order;  // [S] processing order
-    uint32_t *sc_cnt;          // [3S] pairs, contacts, status bits
-    uint32_t num_scenes;
-    uint32_t *queue;           // this substep's work counter
-    uint32_t *abort_flag;
-    float *pos, *quat, *vel, *omega;
-    const float *inv_mass, *inv_i, *hull, *fric, *rest;
-    const uint32_t *hrange, *sl_off, *sl, *gbody;
-    float *amin, *amax, *rot;
-    uint32_t *pairs;
-    uint32_t ps;  // pair slots per scene
-    float *cpos, *cnrm, *cpen, *lambda;
-    uint32_t *cga, *cgb;
-    uint32_t cs;  // contact slots per scene
-    float *cprep;
-    float dt, ld, ad, baumgarte, slop;
-    uint32_t iters;
-};
-
-// Exclusive scan of one value per thread over the workgroup; *total gets
-// the sum.  Every thread of the block must call it.
-template <int T>
-SVD uint32_t block_scan(uint32_t v, uint32_t *s_w, uint32_t *total) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) s_w[w] = x;
-    __syncthreads();
-    uint32_t base = 0, tot = 0;
-#pragma unroll
-    for (int q = 0; q < T / 64; q++) {
-        uint32_t t = s_w[q];
-        base += q < w ? t : 0u;
-        tot += t;
-    }
-    __syncthreads();
-    *total = tot;
-    return base + x - v;
-}
-
-template <int T>
+template <int T, bool RECORD>
 __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
     constexpr int W = T / 64;
     __shared__ float4 s_lo[NB_MAX], s_hi[NB_MAX];
@@ -218,6 +161,7 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
             if (nct > A.cs) cnt[2] |= SV_STATUS_CONTACT_OVERFLOW;
         }
         const uint32_t nc = min(nct, A.cs);
+        if (RECORD && tid == 0) A.t_nc[sc] = nc;
         __syncthreads();
 
         // sweep constants and incidence keys; impulses start at zero (step 4)
@@ -247,39 +191,7 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
             }
         }
         __syncthreads();
-        // bitonic sort of the incidence keys
-        for (uint32_t kk = 2; kk <= L; kk <<= 1) {
-            for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
-                for (uint32_t i = tid; i < L; i += T) {
-                    const uint32_t ixj = i ^ jj;
-                    if (ixj > i) {
-                        const uint32_t x = s_key[i], y = s_key[ixj];
-                        const bool up = (i & kk) == 0;
-                        if ((x > y) == up) {
-                            s_key[i] = y;
-                            s_key[ixj] = x;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        // rank of each incidence among its body's, and each body's degree
-        for (uint32_t p = tid; p < L; p += T) {
-            const uint32_t key = s_key[p];
-            if (key == 0xFFFFFFFFu) continue;
-            const uint32_t x = key >> 12;
-            if (p == 0 || (s_key[p - 1] >> 12) != x) s_start[x] = (uint16_t)p;
-        }
-        __syncthreads();
-        for (uint32_t p = tid; p < L; p += T) {
-            const uint32_t key = s_key[p];
-            if (key == 0xFFFFFFFFu) continue;
-            const uint32_t x = key >> 12;
-            const uint32_t r = p - s_start[x];
-            s_rank[key & 0xFFFu] = (uint16_t)r;
-            if (p + 1 == L || (s_key[p + 1] >> 12) != x) s_deg[x] = (uint16_t)(r + 1);
-        }
+        rank_incidences<T>(s_key, s_rank, s_start, s_deg, L);
         if (tid == 0) s_next = 0u;
         __syncthreads();
 
@@ -309,6 +221,7 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
                             if (!(flags & F_SKIP)) {
                                 const uint32_t ga = __float_as_uint(P[P_A]), gb = __float_as_uint(P[P_B]);
                                 it.c = c;
+                                it.q = q2;
                                 it.a = ga - bs;
                                 it.b = gb - bs;
                                 it.dyn_a = flags & F_DYN_A;
@@ -343,6 +256,14 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
                                                         __HIP_MEMORY_SCOPE_WORKGROUP) == it.tb);
                     if (ready) {
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                        if (RECORD) {  // the item's inputs, for the reverse sweep
+                            float *tp = A.t_items + ((size_t)sc * A.iters * A.cs + it.q) * TAPE_ITEM;
+                            st3(tp, 0, ld3(s_vel, it.a));
+                            st3(tp, 1, ld3(s_omega, it.a));
+                            st3(tp, 2, ld3(s_vel, it.b));
+                            st3(tp, 3, ld3(s_omega, it.b));
+                            tp[12] = s_lambda[it.c];
+                        }
                         apply_item(it, s_vel, s_omega, s_lambda);
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                         if (it.dyn_a)
@@ -373,6 +294,7 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
             for (int k = 0; k < 3; k++) {
                 A.vel[3 * b + k] = s_vel[3 * i + k];
                 A.omega[3 * b + k] = s_omega[3 * i + k];
+                if (RECORD) A.t_w2[3 * b + k] = s_omega[3 * i + k];
             }
         }
         for (uint32_t c = tid; c < A.cs; c += T) A.lambda[cseg + c] = c < nc ? s_lambda[c] : 0.0f;
@@ -384,62 +306,39 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
     }
 }
 
+}  // namespace
+
 typedef void (*BatchKernel)(BatchArgs);
 
-BatchKernel batch_kernel(uint32_t threads) {
+BatchKernel batch_kernel(uint32_t threads, bool record) {
     switch (threads) {
-    case 64: return k_batch_substep<64>;
-    case 128: return k_batch_substep<128>;
-    case 256: return k_batch_substep<256>;
-    case 512: return k_batch_substep<512>;
+    case 64: return record ? k_batch_substep<64, true> : k_batch_substep<64, false>;
+    case 128: return record ? k_batch_substep<128, true> : k_batch_substep<128, false>;
+    case 256: return record ? k_batch_substep<256, true> : k_batch_substep<256, false>;
+    case 512: return record ? k_batch_substep<512, true> : k_batch_substep<512, false>;
     default: return nullptr;
     }
 }
 
-}  // namespace
+void batch_launch_substep(const BatchArgs &A, uint32_t threads, uint32_t grid, bool record, hipStream_t stream) {
+    hipLaunchKernelGGL(batch_kernel(threads, record), dim3(grid), dim3(threads), 0, stream, A);
+}
+
 }  // namespace sv
 
-struct sv_batch {
-    sv_ctx *ctx = nullptr;
-    sv_batch_caps caps{};
-    uint32_t num_scenes = 0, num_bodies = 0;
-    uint32_t *sc_body = nullptr, *sc_order = nullptr, *sc_cnt = nullptr, *queue = nullptr, *abort_flag = nullptr;
-    float *sc_grav = nullptr;
-    float *snap = nullptr;  // pos, quat, vel, omega, counters (autotune)
-    int cus = 0;
-    uint32_t threads = 256, wgs_per_cu = 2;
-    bool use_graph = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    bool graph_valid = false;
-    uint32_t g_nv = 0, g_substeps = 0;
-    float g_dt = 0.0f;
-    sv_params g_params{};
-};
+namespace sv {
 
-namespace {
-
-using sv::set_error;
-
-int dev_fail(hipError_t e, const char *what) {
+int batch_dev_fail(hipError_t e, const char *what) {
     char buf[256];
     snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(e));
     return set_error(SV_E_DEVICE, buf);
 }
 
-void drop_graph(sv_batch *b) {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
-    b->graph_valid = false;
-}
-
-sv::BatchArgs batch_args(sv_batch *b, float dt, uint32_t *queue) {
+BatchArgs batch_args(sv_batch *b, float dt, uint32_t *queue) {
     sv_ctx *c = b->ctx;
     auto F = [&](int id) { return reinterpret_cast<float *>(c->buf[id]); };
     auto U = [&](int id) { return reinterpret_cast<uint32_t *>(c->buf[id]); };
-    sv::BatchArgs A;
+    BatchArgs A;
     A.sc_body = b->sc_body;
     A.sc_grav = b->sc_grav;
     A.sc_order = b->sc_order;
@@ -479,42 +378,60 @@ sv::BatchArgs batch_args(sv_batch *b, float dt, uint32_t *queue) {
     A.baumgarte = c->params.baumgarte;
     A.slop = c->params.slop;
     A.iters = c->params.pgs_iterations;
+    A.t_items = nullptr;
+    A.t_w2 = nullptr;
+    A.t_nc = nullptr;
     return A;
 }
 
 // Substeps of every scene on the context stream (no render vertices).
 void enqueue_substeps(sv_batch *b, uint32_t substeps, float sub_dt, uint32_t threads, uint32_t wgs_per_cu) {
     sv_ctx *c = b->ctx;
-    sv::launch_hull_ranges(c, b->num_bodies);
-    sv::fill_u32(c, b->queue, 0u, substeps);
-    sv::BatchKernel k = sv::batch_kernel(threads);
+    launch_hull_ranges(c, b->num_bodies);
+    fill_u32(c, b->queue, 0u, substeps);
     uint32_t grid = (uint32_t)b->cus * wgs_per_cu;
     if (grid > b->num_scenes) grid = b->num_scenes;
     if (grid == 0) grid = 1;
     for (uint32_t s = 0; s < substeps; s++)
-        hipLaunchKernelGGL(k, dim3(grid), dim3(threads), 0, c->stream, batch_args(b, sub_dt, b->queue + s));
+        batch_launch_substep(batch_args(b, sub_dt, b->queue + s), threads, grid, false, c->stream);
 }
 
-void enqueue_frame(sv_batch *b, uint32_t nv, uint32_t substeps, float dt) {
-    sv_ctx *c = b->ctx;
-    sv::fill_u32(c, b->sc_cnt, 0u, 3 * (size_t)b->num_scenes);
-    enqueue_substeps(b, substeps, dt / (float)substeps, b->threads, b->wgs_per_cu);
-    sv::launch_update_verts(c, nv, b->num_bodies);
-}
-
-int finish(sv_batch *b) {
+int batch_finish(sv_batch *b) {
     hipError_t e = hipStreamSynchronize(b->ctx->stream);
-    if (e != hipSuccess) return dev_fail(e, "batched step");
+    if (e != hipSuccess) return batch_dev_fail(e, "batched step");
     e = hipGetLastError();
-    if (e != hipSuccess) return dev_fail(e, "batched step launch");
+    if (e != hipSuccess) return batch_dev_fail(e, "batched step launch");
     uint32_t ab = 0;
     e = hipMemcpy(&ab, b->abort_flag, sizeof(ab), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return dev_fail(e, "batched step status");
+    if (e != hipSuccess) return batch_dev_fail(e, "batched step status");
     if (ab) {
         (void)hipMemset(b->abort_flag, 0, sizeof(uint32_t));
         return set_error(SV_E_TIMEOUT, "batched step: a scene's contact sweep made no progress for 2 s");
     }
     return SV_OK;
+}
+
+}  // namespace sv
+
+namespace {
+
+using sv::set_error;
+
+int dev_fail(hipError_t e, const char *what) { return sv::batch_dev_fail(e, what); }
+
+void drop_graph(sv_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_valid = false;
+}
+
+void enqueue_frame(sv_batch *b, uint32_t nv, uint32_t substeps, float dt) {
+    sv_ctx *c = b->ctx;
+    sv::fill_u32(c, b->sc_cnt, 0u, 3 * (size_t)b->num_scenes);
+    sv::enqueue_substeps(b, substeps, dt / (float)substeps, b->threads, b->wgs_per_cu);
+    sv::launch_update_verts(c, nv, b->num_bodies);
 }
 
 bool same_params(const sv_params &a, const sv_params &b) {
@@ -626,7 +543,7 @@ int sv_batch_set_scenes(sv_batch *b, uint32_t num_scenes, const uint32_t *body_s
 
 int sv_batch_set_config(sv_batch *b, uint32_t threads, uint32_t wgs_per_cu, int use_graph) {
     if (!b) return set_error(SV_E_INVALID, "sv_batch_set_config: null batch");
-    if (!sv::batch_kernel(threads)) return set_error(SV_E_INVALID, "sv_batch_set_config: threads must be 64-512");
+    if (!sv::batch_kernel(threads, false)) return set_error(SV_E_INVALID, "sv_batch_set_config: threads must be 64-512");
     if (wgs_per_cu < 1 || wgs_per_cu > 16) return set_error(SV_E_INVALID, "sv_batch_set_config: 1-16 per CU");
     b->threads = threads;
     b->wgs_per_cu = wgs_per_cu;
@@ -643,7 +560,7 @@ int sv_batch_step(sv_batch *b, uint32_t nv, uint32_t substeps, float dt) {
     sv_ctx *c = b->ctx;
     if (!b->use_graph) {
         enqueue_frame(b, nv, substeps, dt);
-        return finish(b);
+        return sv::batch_finish(b);
     }
     if (!b->graph_valid || b->g_nv != nv || b->g_substeps != substeps || b->g_dt != dt ||
         !same_params(b->g_params, c->params)) {
@@ -666,7 +583,7 @@ int sv_batch_step(sv_batch *b, uint32_t nv, uint32_t substeps, float dt) {
     }
     hipError_t e = hipGraphLaunch(b->exec, c->stream);
     if (e != hipSuccess) return dev_fail(e, "sv_batch_step: graph launch");
-    return finish(b);
+    return sv::batch_finish(b);
 }
 
 int sv_batch_read_counters(sv_batch *b, uint32_t *out) {
@@ -713,13 +630,13 @@ int sv_batch_autotune(sv_batch *b, uint32_t substeps, float dt, uint32_t *thread
     int rc = SV_OK;
     for (uint32_t t : cand_t) {
         int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(sv::batch_kernel(t)),
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(sv::batch_kernel(t, false)),
                                                           (int)t, 0) != hipSuccess || occ < 1)
             continue;
         for (uint32_t w = 1; w <= (uint32_t)occ && w <= 8; w++) {
-            enqueue_substeps(b, 1, sub_dt, t, w);  // warm the shape once
+            sv::enqueue_substeps(b, 1, sub_dt, t, w);  // warm the shape once
             (void)hipEventRecord(e0, c->stream);
-            enqueue_substeps(b, substeps, sub_dt, t, w);
+            sv::enqueue_substeps(b, substeps, sub_dt, t, w);
             (void)hipEventRecord(e1, c->stream);
             e = copy_all(false);
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -741,7 +658,7 @@ int sv_batch_autotune(sv_batch *b, uint32_t substeps, float dt, uint32_t *thread
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    rc = finish(b);
+    rc = sv::batch_finish(b);
     if (rc) return rc;
     b->threads = bt;
     b->wgs_per_cu = bw;

@@ -391,7 +391,7 @@ struct Item {
     float kn, bias, mu, ima, imb;
     m3 Ra, Rb;
     f3 iia, iib;
-    uint32_t c, a, b, ta, tb;
+    uint32_t c, a, b, ta, tb, q;
     bool dyn_a, dyn_b;
 };
 
