@@ -288,11 +288,12 @@ int sv_scene_build_workload(sv_scene *scene, int workload, uint32_t count,
 
 /* Independent scenes for the batched engine (library extension): scenes
  * [first, first + count) of a seeded family, each with its own ground at the
- * origin and floor(exp(U[ln min_bodies, ln max_bodies])) bodies.  Scene k
- * draws from its own stream, so any sub-range builds the same bodies.
- * Writes body_start[0..count] and gravity_scale[0..count). */
+ * origin and floor(exp(U[ln min_bodies, ln max_bodies])) bodies spawned at
+ * heights 1 m to `top` m (0 = 20).  Scene k draws from its own stream, so
+ * any sub-range builds the same bodies.  Writes body_start[0..count] and
+ * gravity_scale[0..count). */
 int sv_scene_build_batch(sv_scene *scene, uint32_t first, uint32_t count, uint32_t min_bodies,
-                         uint32_t max_bodies, uint32_t sphere_segments, uint64_t seed,
+                         uint32_t max_bodies, uint32_t sphere_segments, float top, uint64_t seed,
                          uint32_t *body_start, float *gravity_scale);
 
 /* ---- batched engine (library extension, no reference counterpart) -----
@@ -341,6 +342,24 @@ int sv_batch_set_config(sv_batch *batch, uint32_t threads, uint32_t workgroups_p
  * keeps the fastest; returns it and its time per substep. */
 int sv_batch_autotune(sv_batch *batch, uint32_t substeps, float dt, uint32_t *threads,
                       uint32_t *workgroups_per_cu, double *us_per_substep);
+
+/* ---- differentiable batched rollout (library extension) ---------------
+ * Gradients of `frames` frames of every scene with respect to the initial
+ * state and the per-geom friction and restitution, with collision detection
+ * held fixed per substep (the contacts of the forward run are constants).
+ * State vectors are packed as pos[3N] | quat[4N] | vel[3N] | omega[3N]
+ * (13N floats, N = all bodies of the batch); pointers may be host or device
+ * memory (sv_upload / sv_download also take either).  sv_batch_diff_forward steps the frames like sv_batch_step
+ * (without render vertices) and checkpoints every frame start;
+ * sv_batch_diff_backward takes the adjoint of the final state (NULL = 0),
+ * recomputes each frame from its checkpoint and writes the adjoint of the
+ * initial state and of each geom's friction and restitution.  The batch
+ * state is left at the end of the forward run. */
+int sv_batch_diff_reserve(sv_batch *batch, uint32_t max_frames, uint32_t substeps);
+int sv_batch_diff_forward(sv_batch *batch, uint32_t frames, uint32_t substeps, float dt);
+int sv_batch_diff_backward(sv_batch *batch, uint32_t frames, uint32_t substeps, float dt,
+                           const float *grad_state, float *out_state, float *out_friction,
+                           float *out_restitution);
 
 #ifdef __cplusplus
 }

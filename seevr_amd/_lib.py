@@ -75,6 +75,7 @@ EXPORTS = [
     "sv_scene_set_body_state", "sv_scene_get_counts", "sv_scene_array", "sv_scene_upload",
     "sv_scene_build_workload", "sv_scene_build_batch", "sv_batch_create", "sv_batch_destroy", "sv_batch_context",
     "sv_batch_set_scenes", "sv_batch_step", "sv_batch_read_counters", "sv_batch_set_config", "sv_batch_autotune",
+    "sv_batch_diff_reserve", "sv_batch_diff_forward", "sv_batch_diff_backward",
 ]
 
 _lib = None
@@ -132,7 +133,7 @@ def load(path: Path | str | None = None) -> C.CDLL:
         "sv_scene_array": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(sz)]),
         "sv_scene_upload": (C.c_int, [vp, vp]),
         "sv_scene_build_workload": (C.c_int, [vp, C.c_int, u32, u32, C.c_uint64]),
-        "sv_scene_build_batch": (C.c_int, [vp, u32, u32, u32, u32, u32, C.c_uint64, C.POINTER(u32), fp]),
+        "sv_scene_build_batch": (C.c_int, [vp, u32, u32, u32, u32, u32, f32, C.c_uint64, C.POINTER(u32), fp]),
         "sv_batch_create": (C.c_int, [C.POINTER(BatchCaps), C.POINTER(Params), C.POINTER(vp)]),
         "sv_batch_destroy": (C.c_int, [vp]),
         "sv_batch_context": (vp, [vp]),
@@ -141,6 +142,9 @@ def load(path: Path | str | None = None) -> C.CDLL:
         "sv_batch_read_counters": (C.c_int, [vp, C.POINTER(u32)]),
         "sv_batch_set_config": (C.c_int, [vp, u32, u32, C.c_int]),
         "sv_batch_autotune": (C.c_int, [vp, u32, f32, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_double)]),
+        "sv_batch_diff_reserve": (C.c_int, [vp, u32, u32]),
+        "sv_batch_diff_forward": (C.c_int, [vp, u32, u32, f32]),
+        "sv_batch_diff_backward": (C.c_int, [vp, u32, u32, f32, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

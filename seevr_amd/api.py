@@ -242,12 +242,12 @@ class Scene:
         check(self.lib.sv_scene_build_workload(self.h, workload, count, segments, seed))
 
     def build_batch(self, first: int, count: int, min_bodies: int = 8, max_bodies: int = 256, segments: int = 0,
-                    seed: int = 1) -> tuple[np.ndarray, np.ndarray]:
+                    seed: int = 1, top: float = 0.0) -> tuple[np.ndarray, np.ndarray]:
         """Scenes [first, first + count) of the batched family; returns
         (body_start[count + 1], gravity_scale[count])."""
         start = np.zeros(count + 1, np.uint32)
         gs = np.zeros(count, np.float32)
-        check(self.lib.sv_scene_build_batch(self.h, first, count, min_bodies, max_bodies, segments, seed,
+        check(self.lib.sv_scene_build_batch(self.h, first, count, min_bodies, max_bodies, segments, C.c_float(top), seed,
                                             start.ctypes.data_as(C.POINTER(C.c_uint32)),
                                             gs.ctypes.data_as(C.POINTER(C.c_float))))
         return start, gs
@@ -299,6 +299,7 @@ class Batch:
                           max_support_entries=caps.max_support_entries)
         self.ctx = Context._view(self.lib.sv_batch_context(h), cc, self.params)
         self.num_scenes = 0
+        self.num_bodies = 0
 
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -326,6 +327,7 @@ class Batch:
         check(self.lib.sv_batch_set_scenes(self.h, n, bs.ctypes.data_as(C.POINTER(C.c_uint32)),
                                            g.ctypes.data_as(C.POINTER(C.c_float))))
         self.num_scenes = n
+        self.num_bodies = int(bs[-1])
 
     def step(self, num_vertices: int, substeps: int = 10, dt: float = 1.0 / 60.0) -> None:
         check(self.lib.sv_batch_step(self.h, num_vertices, substeps, C.c_float(dt)))
@@ -342,3 +344,49 @@ class Batch:
         t, w, us = C.c_uint32(), C.c_uint32(), C.c_double()
         check(self.lib.sv_batch_autotune(self.h, substeps, C.c_float(dt), C.byref(t), C.byref(w), C.byref(us)))
         return t.value, w.value, us.value
+
+    # -- differentiable rollout ---------------------------------------------
+    def diff_reserve(self, max_frames: int, substeps: int = 10) -> None:
+        check(self.lib.sv_batch_diff_reserve(self.h, max_frames, substeps))
+
+    def diff_forward(self, frames: int, substeps: int = 10, dt: float = 1.0 / 60.0) -> None:
+        check(self.lib.sv_batch_diff_forward(self.h, frames, substeps, C.c_float(dt)))
+
+    def diff_backward(self, frames: int, grad_state, substeps: int = 10, dt: float = 1.0 / 60.0,
+                      out=None):
+        """grad_state: adjoint of the final packed state (13N floats; numpy
+        array, device address as int, or None).  Returns (adjoint of the
+        initial packed state, d/dfriction, d/drestitution) as numpy arrays, or
+        writes into out = (state_addr, fric_addr, rest_addr) device addresses."""
+        n = self.ctx.caps.max_bodies
+        gs = None
+        if isinstance(grad_state, int):
+            gs = C.c_void_p(grad_state)
+        elif grad_state is not None:
+            ga = np.ascontiguousarray(grad_state, np.float32)
+            gs = ga.ctypes.data_as(C.c_void_p)
+        if out is not None:
+            check(self.lib.sv_batch_diff_backward(self.h, frames, substeps, C.c_float(dt), gs,
+                                                  *(C.c_void_p(a) for a in out)))
+            return None
+        nb = self.num_bodies
+        so = np.zeros(13 * nb, np.float32)
+        fo = np.zeros(nb, np.float32)
+        ro = np.zeros(nb, np.float32)
+        check(self.lib.sv_batch_diff_backward(self.h, frames, substeps, C.c_float(dt), gs,
+                                              so.ctypes.data_as(C.c_void_p), fo.ctypes.data_as(C.c_void_p),
+                                              ro.ctypes.data_as(C.c_void_p)))
+        return so, fo, ro
+
+    def packed_state(self) -> np.ndarray:
+        nb = self.num_bodies
+        return np.concatenate([self.ctx.download(k, w * nb) for k, w in
+                               (("BODY_POS", 3), ("BODY_QUAT", 4), ("BODY_VEL", 3), ("BODY_OMEGA", 3))])
+
+    def set_packed_state(self, x) -> None:
+        nb = self.num_bodies
+        x = np.ascontiguousarray(x, np.float32)
+        off = 0
+        for k, w in (("BODY_POS", 3), ("BODY_QUAT", 4), ("BODY_VEL", 3), ("BODY_OMEGA", 3)):
+            self.ctx.upload(k, x[off:off + w * nb])
+            off += w * nb

@@ -135,6 +135,17 @@ struct sv_batch {
     uint32_t g_nv = 0, g_substeps = 0;
     float g_dt = 0.0f;
     sv_params g_params{};
+    // differentiable form (sv_diff.hip), sized by sv_batch_diff_reserve
+    uint32_t d_frames = 0, d_substeps = 0, d_iters = 0;
+    float *d_ckpt = nullptr;     // [(frames + 1)][13N] state at each frame start (pos, quat, vel, omega)
+    float *d_state = nullptr;    // [substeps][13N] state at each substep start of the frame being reversed
+    float *d_w2 = nullptr;       // [substeps][3N] angular velocity after each sweep
+    float *d_prep = nullptr;     // [substeps][S * cs][CPREP_WORDS] sweep constants
+    float *d_items = nullptr;    // [substeps][S][iters * cs][TAPE_ITEM] sweep item inputs
+    uint32_t *d_nc = nullptr;    // [substeps][S] contacts solved
+    float *d_adj = nullptr;      // [13N] adjoint of the state
+    float *d_acc = nullptr;      // [S * cs][9] per-contact constant adjoints
+    float *d_contrib = nullptr;  // [S * 2cs][18] per-incidence adjoint contributions
 };
 
 namespace sv {

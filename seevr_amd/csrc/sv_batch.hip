@@ -173,6 +173,8 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
                 contact_prep(a, b, ga, gb, ld3(A.cnrm, k), ld3(A.cpos, k), A.cpen[k], A.dt, A.baumgarte, A.slop,
                              A.pos, A.vel, A.omega, A.inv_mass, A.inv_i, A.rot, A.fric, A.rest,
                              A.cprep + k * CPREP_WORDS);
+            A.cprep[k * CPREP_WORDS + P_GEOM_A] = __uint_as_float(ga);
+            A.cprep[k * CPREP_WORDS + P_GEOM_B] = __uint_as_float(gb);
             const bool live = !(flags & F_SKIP);
             s_key[2 * c] = (live && (flags & F_DYN_A)) ? (((a - bs) << 12) | (2 * c)) : 0xFFFFFFFFu;
             s_key[2 * c + 1] = (live && (flags & F_DYN_B)) ? (((b - bs) << 12) | (2 * c + 1)) : 0xFFFFFFFFu;
@@ -444,7 +446,9 @@ void free_batch(sv_batch *b) {
     if (!b) return;
     if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
     drop_graph(b);
-    void *ptrs[] = {b->sc_body, b->sc_order, b->sc_cnt, b->queue, b->abort_flag, b->sc_grav, b->snap};
+    void *ptrs[] = {b->sc_body, b->sc_order, b->sc_cnt, b->queue, b->abort_flag, b->sc_grav, b->snap,
+                    b->d_ckpt, b->d_state, b->d_w2, b->d_prep, b->d_items, b->d_nc, b->d_adj, b->d_acc,
+                    b->d_contrib};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->ctx) sv_destroy(b->ctx);

@@ -314,7 +314,7 @@ int sv_upload(sv_ctx *c, sv_buf id, const void *src, size_t off, size_t n) {
         return fail(SV_E_RANGE, "sv_upload: [%zu, %zu) outside buffer %d of %zu bytes", off, off + n, (int)id,
                     c->bytes[id]);
     if (!n) return SV_OK;
-    HIP_OK(hipMemcpyAsync(static_cast<char *>(c->buf[id]) + off, src, n, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(static_cast<char *>(c->buf[id]) + off, src, n, hipMemcpyDefault, c->stream));
     return sync_and_finish(c);
 }
 
@@ -324,7 +324,7 @@ int sv_download(sv_ctx *c, sv_buf id, void *dst, size_t off, size_t n) {
         return fail(SV_E_RANGE, "sv_download: [%zu, %zu) outside buffer %d of %zu bytes", off, off + n, (int)id,
                     c->bytes[id]);
     if (!n) return SV_OK;
-    HIP_OK(hipMemcpyAsync(dst, static_cast<const char *>(c->buf[id]) + off, n, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(dst, static_cast<const char *>(c->buf[id]) + off, n, hipMemcpyDefault, c->stream));
     return sync_and_finish(c);
 }
 

@@ -329,7 +329,10 @@ SVD bool mpr(const Pair &P, const Geo &g, f3 &normal, float &depth, f3 &cp) {
 // cprep word offsets
 enum : int {
     P_N = 0, P_RA = 3, P_RB = 6, P_KN = 9, P_BIAS = 10, P_MU = 11, P_IMA = 12, P_IMB = 13,
-    P_A = 14, P_B = 15, P_FLAGS = 16, P_SUCC_A = 17, P_SUCC_B = 18
+    P_A = 14, P_B = 15, P_FLAGS = 16, P_SUCC_A = 17, P_SUCC_B = 18,
+    // the batched path keeps the contact's geoms in the words the flow
+    // schedule uses for successors
+    P_GEOM_A = 17, P_GEOM_B = 18
 };
 // Successor words (flow form): the contact that next updates the same body,
 // in the same iteration or (WRAP) in the next one; HASPRED marks a contact

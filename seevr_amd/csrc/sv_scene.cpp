@@ -428,13 +428,14 @@ int sv_scene_build_workload(sv_scene *s, int workload, uint32_t count, uint32_t 
 // Independent scenes for the batched engine, each with its own ground at the
 // origin: scene k has n_k = floor(exp(U[ln min_n, ln max_n])) bodies (the
 // box/sphere mix of the 1,000-body column) in a 10x10 m column at heights
-// 1-20 m over a 50x1x50 m ground, and a gravity scale U[0.8, 1.2].  Scene k
+// 1 m to `top` m (20 m when 0) over a 50x1x50 m ground, and a gravity scale U[0.8, 1.2].  Scene k
 // draws from its own stream (seed, k), so building scenes [first,
 // first+count) alone gives the same bodies as building them in a larger
 // batch.  Writes body_start[0..count] (first body of each scene, then the
 // end) and gravity_scale[0..count).
 int sv_scene_build_batch(sv_scene *s, uint32_t first, uint32_t count, uint32_t min_n, uint32_t max_n, uint32_t segs,
-                         uint64_t seed, uint32_t *body_start, float *gravity_scale) {
+                         float top, uint64_t seed, uint32_t *body_start, float *gravity_scale) {
+    const float hi = top > 1.0f ? top : 20.0f;
     if (!s || !body_start || !gravity_scale) return bad("null");
     if (s->num_bodies) return bad("scene must be empty");
     if (min_n < 1 || max_n < min_n) return bad("body range");
@@ -453,7 +454,7 @@ int sv_scene_build_batch(sv_scene *s, uint32_t first, uint32_t count, uint32_t m
         body_start[k - first] = s->num_bodies;
         rc = static_box(s, 0, -0.5f, 0, 25, 0.5f, 25, m_static);
         for (uint32_t i = 0; i < n && !rc; i++) {
-            float p[3] = {r.uni(-5, 5), r.uni(1, 20), r.uni(-5, 5)};
+            float p[3] = {r.uni(-5, 5), r.uni(1, hi), r.uni(-5, 5)};
             rc = add_random_body(s, r, 0.5f, 0.2f, 0.6f, 0.2f, 0.5f, sg, p, m_dyn);
         }
     }
