@@ -200,6 +200,80 @@ def batched_section(frames: int, warmup: int, scenes: int = 4096, plain_sample: 
             "speedup_vs_plain": round(env_steps / plain_env, 1)}
 
 
+def differentiable_section(frames: int = 60, scenes: int = 256, bodies: int = 64, reps: int = 3) -> dict:
+    """Cost of one backward pass relative to the forward rollout on the
+    256-scene x 64-body workload: initial v, w ~ N(0, 1), per-scene,
+    per-material friction U[0.2, 0.8] and restitution U[0, 0.5], 60 frames.
+    Forward = sv_batch_diff_forward (rollout + frame checkpoints); backward =
+    sv_batch_diff_backward (recompute each frame from its checkpoint with the
+    tape, then the adjoint kernels).  The loss is the squared distance of the
+    final positions to targets (final positions + U[-0.5, 0.5])."""
+    from seevr_amd import Batch, Scene, default_caps, default_params
+    from seevr_amd._lib import BatchCaps
+    rng = np.random.default_rng(4)
+    big = default_caps(max_bodies=scenes * (bodies + 1), max_hull_verts=scenes * (bodies + 1) * 290,
+                       max_vertices=scenes * (bodies + 1) * 290, max_triangles=scenes * (bodies + 1) * 520,
+                       max_support_entries=scenes * (bodies + 1) * 193 + 1)
+    sc = Scene(big)
+    start, gs = sc.build_batch(0, scenes, bodies, bodies, 16, 4, top=6.0)
+    cnt = sc.counts()
+    bc = BatchCaps(max_scenes=scenes, max_bodies=cnt["num_bodies"], max_hull_verts=cnt["num_hull_verts"],
+                   max_vertices=cnt["num_vertices"], max_triangles=max(cnt["num_triangles"], 1),
+                   max_support_entries=cnt["num_support_entries"], max_scene_bodies=512, max_scene_pairs=2048,
+                   max_scene_contacts=512)
+    b = Batch(bc, default_params(pgs_iterations=PGS_ITERS))
+    sc.upload(b.ctx)
+    sc.close()
+    grav = np.zeros((scenes, 3), np.float32)
+    grav[:, 1] = np.float32(GRAVITY[1]) * gs
+    b.set_scenes(start, grav)
+    n = b.num_bodies
+    x0 = b.packed_state()
+    inv_mass = b.ctx.download("BODY_INV_MASS", n)
+    dyn = inv_mass > 0
+    vel = x0[7 * n:10 * n].reshape(n, 3)
+    omg = x0[10 * n:13 * n].reshape(n, 3)
+    vel[dyn] = rng.normal(size=(dyn.sum(), 3))
+    omg[dyn] = rng.normal(size=(dyn.sum(), 3))
+    mu_m = rng.uniform(0.2, 0.8, (scenes, 2)).astype(np.float32)
+    e_m = rng.uniform(0.0, 0.5, (scenes, 2)).astype(np.float32)
+    scene_of = np.repeat(np.arange(scenes), np.diff(start))
+    mat = np.where(dyn, 1, 0)
+    fric, rest = mu_m[scene_of, mat], e_m[scene_of, mat]
+    b.ctx.upload("GEOM_FRICTION", fric)
+    b.ctx.upload("GEOM_RESTITUTION", rest)
+    b.diff_reserve(frames, SUBSTEPS)
+    b.set_packed_state(x0)
+    b.diff_forward(frames, SUBSTEPS, DT)
+    target = b.packed_state()[: 3 * n] + rng.uniform(-0.5, 0.5, 3 * n).astype(np.float32)
+    fwd, bwd = [], []
+    for _ in range(reps):
+        b.set_packed_state(x0)
+        t0 = time.perf_counter()
+        b.diff_forward(frames, SUBSTEPS, DT)
+        fwd.append(time.perf_counter() - t0)
+        xf = b.packed_state()
+        g = np.zeros(13 * n, np.float32)
+        g[: 3 * n] = 2.0 * (xf[: 3 * n] - target)
+        t0 = time.perf_counter()
+        gstate, gfric, grest = b.diff_backward(frames, g, SUBSTEPS, DT)
+        bwd.append(time.perf_counter() - t0)
+    c = b.counters()
+    b.close()
+    tf, tb = min(fwd), min(bwd)
+    gmu = np.zeros((scenes, 2))
+    np.add.at(gmu, (scene_of, mat), gfric)
+    log(f"differentiable: forward {1000 * tf:.1f} ms, backward {1000 * tb:.1f} ms, ratio {(tf + tb) / tf:.2f}")
+    return {"metric": "fwd+bwd vs fwd cost", "value": round((tf + tb) / tf, 3), "unit": "x",
+            "higher_is_better": False, "scenes": scenes, "bodies_per_scene": bodies + 1, "frames": frames,
+            "forward_ms": round(1000 * tf, 2), "backward_ms": round(1000 * tb, 2),
+            "contacts_last_substep": {"mean": round(float(c[:, 1].mean()), 2), "max": int(c[:, 1].max())},
+            "grad_norms": {"state": float(np.linalg.norm(gstate)), "friction_per_material": float(np.linalg.norm(gmu))},
+            "data": "synthetic: 256 seeded scenes of 64 bodies over a ground, v and w ~ N(0, 1), per-scene "
+                    "per-material friction U[0.2, 0.8] and restitution U[0, 0.5]; targets = final positions "
+                    "+ U[-0.5, 0.5]"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +286,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-engine section")
     ap.add_argument("--batched-frames", type=int, default=60)
     ap.add_argument("--batched-warmup", type=int, default=240)
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-rollout section")
     ap.add_argument("--solver", default="auto", choices=["auto", "ordered", "flow"],
                     help="schedule of the contact sweep (all give the serial result bit for bit)")
     ap.add_argument("--solver-wgs", type=int, default=0, help="workgroups of the sweep kernel (0 = default)")
@@ -291,6 +366,9 @@ def main() -> int:
     batched = None
     if not args.no_batched:
         batched = batched_section(args.batched_frames, args.batched_warmup)
+    diff = None
+    if not args.no_diff:
+        diff = differentiable_section()
 
     out = {
         "metric": "body-substeps/s @32k bodies", "value": round(value, 1), "unit": "body-substeps/s",
@@ -303,7 +381,7 @@ def main() -> int:
                    "frames_before_timing": args.warmup, "pairs_last_substep": pairs,
                    "contacts_last_substep": contacts},
         "roofline": roof, "roofline_hbm_stages": hbm_stages, "cpu_baseline": cpu, "stage_share_pct": breakdown,
-        "batched": batched,
+        "batched": batched, "differentiable": diff,
     }
     print(json.dumps(out), flush=True)
     return 0

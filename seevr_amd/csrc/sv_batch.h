@@ -42,6 +42,8 @@ struct BatchArgs {
     float *t_items;   // [S][iters * cs][TAPE_ITEM] inputs of every sweep item
     float *t_w2;      // [3N] angular velocities after the sweep
     uint32_t *t_nc;   // [S] contacts solved
+    float *t_state;   // [13N] state at the substep start: pos | quat | vel | omega
+    uint32_t n_all;   // N, the bodies of all scenes (offsets into t_state)
 };
 
 // Exclusive scan of one value per thread over the workgroup; *total gets
@@ -138,14 +140,15 @@ struct sv_batch {
     // differentiable form (sv_diff.hip), sized by sv_batch_diff_reserve
     uint32_t d_frames = 0, d_substeps = 0, d_iters = 0;
     float *d_ckpt = nullptr;     // [(frames + 1)][13N] state at each frame start (pos, quat, vel, omega)
-    float *d_state = nullptr;    // [substeps][13N] state at each substep start of the frame being reversed
+    float *d_state = nullptr;    // [2][substeps][13N] state at each substep start (two tape sets)
     float *d_w2 = nullptr;       // [substeps][3N] angular velocity after each sweep
     float *d_prep = nullptr;     // [substeps][S * cs][CPREP_WORDS] sweep constants
     float *d_items = nullptr;    // [substeps][S][iters * cs][TAPE_ITEM] sweep item inputs
     uint32_t *d_nc = nullptr;    // [substeps][S] contacts solved
     float *d_adj = nullptr;      // [13N] adjoint of the state
-    float *d_acc = nullptr;      // [S * cs][9] per-contact constant adjoints
     float *d_contrib = nullptr;  // [S * 2cs][18] per-incidence adjoint contributions
+    hipStream_t d_stream = nullptr;  // adjoint kernels (default priority), beside the context stream
+    hipEvent_t d_ev[5] = {};
 };
 
 namespace sv {

@@ -56,9 +56,22 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
         const float gx = A.sc_grav[3 * sc], gy = A.sc_grav[3 * sc + 1], gz = A.sc_grav[3 * sc + 2];
         uint32_t *cnt = A.sc_cnt + 3 * sc;
 
-        // step 0: gravity
-        for (uint32_t i = tid; i < ns; i += T)
+        // step 0: gravity (the recording form first keeps the substep's
+        // starting state; each lane copies the bodies it then updates)
+        for (uint32_t i = tid; i < ns; i += T) {
+            if (RECORD) {
+                const size_t b = bs + i, N = A.n_all;
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    A.t_state[3 * b + k] = A.pos[3 * b + k];
+                    A.t_state[7 * N + 3 * b + k] = A.vel[3 * b + k];
+                    A.t_state[10 * N + 3 * b + k] = A.omega[3 * b + k];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) A.t_state[3 * N + 4 * b + k] = A.quat[4 * b + k];
+            }
             gravity_body(bs + i, A.dt, gx, gy, gz, A.ld, A.ad, A.inv_mass, A.vel, A.omega);
+        }
 
         // AABB, rotation cache and broadphase boxes: one wave per body
         for (uint32_t i = wv; i < ns; i += W) {
@@ -383,6 +396,8 @@ BatchArgs batch_args(sv_batch *b, float dt, uint32_t *queue) {
     A.t_items = nullptr;
     A.t_w2 = nullptr;
     A.t_nc = nullptr;
+    A.t_state = nullptr;
+    A.n_all = b->num_bodies;
     return A;
 }
 
@@ -447,10 +462,16 @@ void free_batch(sv_batch *b) {
     if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
     drop_graph(b);
     void *ptrs[] = {b->sc_body, b->sc_order, b->sc_cnt, b->queue, b->abort_flag, b->sc_grav, b->snap,
-                    b->d_ckpt, b->d_state, b->d_w2, b->d_prep, b->d_items, b->d_nc, b->d_adj, b->d_acc,
+                    b->d_ckpt, b->d_state, b->d_w2, b->d_prep, b->d_items, b->d_nc, b->d_adj,
                     b->d_contrib};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : b->d_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->d_stream) {
+        (void)hipStreamSynchronize(b->d_stream);
+        (void)hipStreamDestroy(b->d_stream);
+    }
     if (b->ctx) sv_destroy(b->ctx);
     delete b;
 }

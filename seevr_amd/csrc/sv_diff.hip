@@ -36,6 +36,8 @@ namespace sv {
 namespace {
 
 constexpr uint32_t ACC = 9;       // per-contact constant adjoints: ra 3, rb 3, kn, bias, mu
+constexpr uint32_t DC_MAX = 512;  // contacts per scene in the differentiable form (LDS tile)
+constexpr uint32_t DINC_MAX = 2 * DC_MAX;
 constexpr uint32_t CONTRIB = 18;  // per incidence: p 3, v 3, w 3, R 9
 
 struct AdjArgs {
@@ -53,7 +55,7 @@ struct AdjArgs {
     const float *inv_mass, *inv_i, *fric, *rest;
     float *adj;          // [13N] in: after the substep, out: before it
     float *afric, *arest;
-    float *acc, *contrib;
+    float *contrib;
     uint32_t n, cs, iters;
     float dt, ld, ad;
 };
@@ -214,11 +216,12 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
     __shared__ float s_pb[3 * NB_MAX], s_qb[4 * NB_MAX], s_vb[3 * NB_MAX], s_wb[3 * NB_MAX];
     __shared__ float s_Rb[9 * NB_MAX];
     __shared__ float s_v1[3 * NB_MAX], s_w1[3 * NB_MAX];
-    __shared__ uint32_t s_key[INC_MAX];
-    __shared__ uint16_t s_rank[INC_MAX];
+    __shared__ uint32_t s_key[DINC_MAX];
+    __shared__ uint16_t s_rank[DINC_MAX];
     __shared__ uint16_t s_start[NB_MAX], s_deg[NB_MAX];
     __shared__ uint32_t s_rt[NB_MAX];
-    __shared__ float s_lb[C_MAX];
+    __shared__ float s_lb[DC_MAX];
+    __shared__ float s_acc[DC_MAX * ACC];
     __shared__ uint32_t s_q, s_next;
     const int tid = threadIdx.x, lane = tid & 63;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -303,9 +306,8 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
             s_key[2 * c] = (live && (flags & F_DYN_A)) ? (((a - bs) << 12) | (2 * c)) : 0xFFFFFFFFu;
             s_key[2 * c + 1] = (live && (flags & F_DYN_B)) ? (((b - bs) << 12) | (2 * c + 1)) : 0xFFFFFFFFu;
             s_lb[c] = 0.0f;
-            float *acc = A.acc + (cseg + c) * ACC;
 #pragma unroll
-            for (int k = 0; k < (int)ACC; k++) acc[k] = 0.0f;
+            for (int k = 0; k < (int)ACC; k++) s_acc[c * ACC + k] = 0.0f;
         }
         uint32_t L = 2;
         while (L < 2 * nc) L <<= 1;
@@ -320,6 +322,8 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
             Item it;
             bool has = false, fin = total == 0;
             uint32_t ra_pos = 0, rb_pos = 0;
+            f3 t_la, t_wa, t_lb, t_wb;
+            float t_old = 0.0f;
             uint64_t last = wall_clock64();
             for (;;) {
                 uint64_t need = __ballot(!has && !fin);
@@ -363,6 +367,12 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
                                 it.Rb = quat_mat_at(quat0, gb);
                                 it.iia = ld3(A.inv_i, ga);
                                 it.iib = ld3(A.inv_i, gb);
+                                const float *tp = A.items + ((size_t)sc * A.iters * A.cs + q2) * TAPE_ITEM;
+                                t_la = ld3(tp, 0);
+                                t_wa = ld3(tp, 1);
+                                t_lb = ld3(tp, 2);
+                                t_wb = ld3(tp, 3);
+                                t_old = tp[12];
                                 has = true;
                             }
                         }
@@ -378,10 +388,9 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
                                                                        __HIP_MEMORY_SCOPE_WORKGROUP) == rb_pos);
                     if (ready) {
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                        const float *tp = A.items + ((size_t)sc * A.iters * A.cs + it.q) * TAPE_ITEM;
                         const f3 z = mk(0.0f, 0.0f, 0.0f);
                         ItemAdj o;
-                        item_vjp(it, ld3(tp, 0), ld3(tp, 1), ld3(tp, 2), ld3(tp, 3), tp[12],
+                        item_vjp(it, t_la, t_wa, t_lb, t_wb, t_old,
                                  it.dyn_a ? ld3(s_vb, it.a) : z, it.dyn_a ? ld3(s_wb, it.a) : z,
                                  it.dyn_b ? ld3(s_vb, it.b) : z, it.dyn_b ? ld3(s_wb, it.b) : z, s_lb[it.c], o);
                         if (it.dyn_a) {
@@ -395,7 +404,7 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
                             acc_rot(s_Rb + 9 * it.b, o.Rb);
                         }
                         s_lb[it.c] = o.old;
-                        float *acc = A.acc + (cseg + it.c) * ACC;
+                        float *acc = s_acc + it.c * ACC;
                         acc[0] += o.ra.x; acc[1] += o.ra.y; acc[2] += o.ra.z;
                         acc[3] += o.rb.x; acc[4] += o.rb.y; acc[5] += o.rb.z;
                         acc[6] += o.kn; acc[7] += o.bias; acc[8] += o.mu;
@@ -431,7 +440,7 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
             const uint32_t ga_b = __float_as_uint(P[P_A]), gb_b = __float_as_uint(P[P_B]);
             const uint32_t gga = __float_as_uint(P[P_GEOM_A]), ggb = __float_as_uint(P[P_GEOM_B]);
             const uint32_t a = ga_b - bs, b = gb_b - bs;
-            const float *acc = A.acc + (cseg + c) * ACC;
+            const float *acc = s_acc + c * ACC;
             f3 gra = mk(acc[0], acc[1], acc[2]), grb = mk(acc[3], acc[4], acc[5]);
             const float gkn = acc[6], gbias = acc[7], gmu = acc[8];
             const f3 n = mk(P[P_N], P[P_N + 1], P[P_N + 2]);
@@ -563,16 +572,21 @@ int sv_batch_diff_reserve(sv_batch *b, uint32_t max_frames, uint32_t substeps) {
     if (!b) return set_error(SV_E_INVALID, "sv_batch_diff_reserve: null batch");
     if (!max_frames || !substeps || substeps > 64)
         return set_error(SV_E_INVALID, "sv_batch_diff_reserve: frames > 0 and 1-64 substeps");
+    if (b->caps.max_scene_contacts > sv::DC_MAX)
+        return set_error(SV_E_CAPACITY, "sv_batch_diff_reserve: max_scene_contacts above 512 (the adjoint's LDS tile)");
     const size_t N = b->caps.max_bodies, S = b->caps.max_scenes, cs = b->caps.max_scene_contacts;
     const size_t iters = b->ctx->params.pgs_iterations;
     hipError_t e = dmalloc(b->d_ckpt, (max_frames + 1) * 13 * N);
-    if (e == hipSuccess) e = dmalloc(b->d_state, substeps * 13 * N);
-    if (e == hipSuccess) e = dmalloc(b->d_w2, substeps * 3 * N);
-    if (e == hipSuccess) e = dmalloc(b->d_prep, substeps * S * cs * sv::CPREP_WORDS);
-    if (e == hipSuccess) e = dmalloc(b->d_items, substeps * S * iters * cs * sv::TAPE_ITEM);
-    if (e == hipSuccess) e = dmalloc(b->d_nc, substeps * S);
+    // two tape sets: one frame is recomputed while the previous one is reversed
+    if (e == hipSuccess) e = dmalloc(b->d_state, 2 * substeps * 13 * N);
+    if (e == hipSuccess) e = dmalloc(b->d_w2, 2 * substeps * 3 * N);
+    if (e == hipSuccess) e = dmalloc(b->d_prep, 2 * substeps * S * cs * sv::CPREP_WORDS);
+    if (e == hipSuccess) e = dmalloc(b->d_items, 2 * substeps * S * iters * cs * sv::TAPE_ITEM);
+    if (e == hipSuccess) e = dmalloc(b->d_nc, 2 * substeps * S);
+    if (e == hipSuccess && !b->d_stream) e = hipStreamCreateWithFlags(&b->d_stream, hipStreamNonBlocking);
+    for (int k = 0; k < 5 && e == hipSuccess; k++)
+        if (!b->d_ev[k]) e = hipEventCreateWithFlags(&b->d_ev[k], hipEventDisableTiming);
     if (e == hipSuccess) e = dmalloc(b->d_adj, 13 * N);
-    if (e == hipSuccess) e = dmalloc(b->d_acc, S * cs * sv::ACC);
     if (e == hipSuccess) e = dmalloc(b->d_contrib, 2 * S * cs * sv::CONTRIB);
     if (e != hipSuccess) {
         b->d_frames = 0;
@@ -626,36 +640,52 @@ int sv_batch_diff_backward(sv_batch *b, uint32_t frames, uint32_t substeps, floa
     }
     sv::fill_u32(c, reinterpret_cast<uint32_t *>(afric), 0u, N);
     sv::fill_u32(c, reinterpret_cast<uint32_t *>(arest), 0u, N);
+    sv::launch_hull_ranges(c, b->num_bodies);
     uint32_t grid = (uint32_t)b->cus * b->wgs_per_cu;
     if (grid > S) grid = (uint32_t)S;
     if (grid == 0) grid = 1;
+    // Frame f is recomputed (tape set f & 1) on the context stream while the
+    // adjoint of frame f + 1 runs on the second stream; a tape set is
+    // rewritten only after the adjoint that read it has finished.
+    hipStream_t ast = b->d_stream;
+    hipEvent_t *ev_rec = b->d_ev, *ev_adj = b->d_ev + 2, ev_misc = b->d_ev[4];
+    if (e == hipSuccess) e = hipEventRecord(ev_misc, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ast, ev_misc, 0);
+    const size_t T_STATE = substeps * N13, T_W2 = substeps * 3 * N, T_PREP = substeps * S * cs * sv::CPREP_WORDS,
+                 T_ITEMS = substeps * S * iters * cs * sv::TAPE_ITEM, T_NC = substeps * S;
     for (int64_t f = (int64_t)frames - 1; f >= 0 && e == hipSuccess; f--) {
-        e = copy_state(b, b->d_ckpt + (size_t)f * N13, false);
+        const int k = (int)(f & 1);
+        float *t_state = b->d_state + k * T_STATE, *t_w2 = b->d_w2 + k * T_W2, *t_prep = b->d_prep + k * T_PREP;
+        float *t_items = b->d_items + k * T_ITEMS;
+        uint32_t *t_nc = b->d_nc + k * T_NC, *queue = b->queue + (size_t)k * 2 * substeps;
+        if (f + 2 <= (int64_t)frames - 1) e = hipStreamWaitEvent(st, ev_adj[k], 0);
+        if (e == hipSuccess) e = copy_state(b, b->d_ckpt + (size_t)f * N13, false);
         if (e != hipSuccess) break;
-        sv::launch_hull_ranges(c, b->num_bodies);
-        sv::fill_u32(c, b->queue, 0u, 2 * (size_t)substeps);
-        for (uint32_t s = 0; s < substeps && e == hipSuccess; s++) {
-            e = copy_state(b, b->d_state + s * N13, true);
-            sv::BatchArgs A = sv::batch_args(b, sub_dt, b->queue + s);
-            A.cprep = b->d_prep + s * S * cs * sv::CPREP_WORDS;
-            A.t_items = b->d_items + s * S * iters * cs * sv::TAPE_ITEM;
-            A.t_w2 = b->d_w2 + s * 3 * N;
-            A.t_nc = b->d_nc + s * S;
+        sv::fill_u32(c, queue, 0u, 2 * (size_t)substeps);
+        for (uint32_t s = 0; s < substeps; s++) {
+            sv::BatchArgs A = sv::batch_args(b, sub_dt, queue + s);
+            A.t_state = t_state + s * N13;
+            A.cprep = t_prep + s * S * cs * sv::CPREP_WORDS;
+            A.t_items = t_items + s * S * iters * cs * sv::TAPE_ITEM;
+            A.t_w2 = t_w2 + s * 3 * N;
+            A.t_nc = t_nc + s * S;
             sv::batch_launch_substep(A, b->threads, grid, true, st);
         }
+        e = hipEventRecord(ev_rec[k], st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ast, ev_rec[k], 0);
         for (int64_t s = (int64_t)substeps - 1; s >= 0 && e == hipSuccess; s--) {
             sv::AdjArgs J;
             J.sc_body = b->sc_body;
             J.sc_grav = b->sc_grav;
             J.num_scenes = (uint32_t)S;
-            J.queue = b->queue + substeps + s;
+            J.queue = queue + substeps + s;
             J.abort_flag = b->abort_flag;
             J.sc_cnt = b->sc_cnt;
-            J.state = b->d_state + s * N13;
-            J.w2 = b->d_w2 + s * 3 * N;
-            J.prep = b->d_prep + s * S * cs * sv::CPREP_WORDS;
-            J.items = b->d_items + s * S * iters * cs * sv::TAPE_ITEM;
-            J.nc = b->d_nc + s * S;
+            J.state = t_state + s * N13;
+            J.w2 = t_w2 + s * 3 * N;
+            J.prep = t_prep + s * S * cs * sv::CPREP_WORDS;
+            J.items = t_items + s * S * iters * cs * sv::TAPE_ITEM;
+            J.nc = t_nc + s * S;
             J.inv_mass = reinterpret_cast<const float *>(c->buf[SV_BODY_INV_MASS]);
             J.inv_i = reinterpret_cast<const float *>(c->buf[SV_BODY_INV_INERTIA]);
             J.fric = reinterpret_cast<const float *>(c->buf[SV_GEOM_FRICTION]);
@@ -663,7 +693,6 @@ int sv_batch_diff_backward(sv_batch *b, uint32_t frames, uint32_t substeps, floa
             J.adj = b->d_adj;
             J.afric = afric;
             J.arest = arest;
-            J.acc = b->d_acc;
             J.contrib = b->d_contrib;
             J.n = (uint32_t)N;
             J.cs = (uint32_t)cs;
@@ -671,14 +700,18 @@ int sv_batch_diff_backward(sv_batch *b, uint32_t frames, uint32_t substeps, floa
             J.dt = sub_dt;
             J.ld = c->params.linear_damping;
             J.ad = c->params.angular_damping;
-            hipLaunchKernelGGL(sv::adj_kernel(b->threads), dim3(grid), dim3(b->threads), 0, st, J);
+            hipLaunchKernelGGL(sv::adj_kernel(b->threads), dim3(grid), dim3(b->threads), 0, ast, J);
         }
+        if (e == hipSuccess) e = hipEventRecord(ev_adj[k], ast);
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(out_state, b->d_adj, N13 * sizeof(float), hipMemcpyDefault, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(out_friction, afric, N * sizeof(float), hipMemcpyDefault, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(out_restitution, arest, N * sizeof(float), hipMemcpyDefault, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out_state, b->d_adj, N13 * sizeof(float), hipMemcpyDefault, ast);
+    if (e == hipSuccess) e = hipMemcpyAsync(out_friction, afric, N * sizeof(float), hipMemcpyDefault, ast);
+    if (e == hipSuccess) e = hipMemcpyAsync(out_restitution, arest, N * sizeof(float), hipMemcpyDefault, ast);
+    if (e == hipSuccess) e = hipEventRecord(ev_misc, ast);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, ev_misc, 0);
     if (e == hipSuccess) e = copy_state(b, b->d_ckpt + (size_t)frames * N13, false);
     int rc = e == hipSuccess ? sv::batch_finish(b) : sv::batch_dev_fail(e, "sv_batch_diff_backward");
+    (void)hipStreamSynchronize(ast);
     (void)hipFree(afric);
     (void)hipFree(arest);
     return rc;
