@@ -287,7 +287,7 @@ def main() -> int:
     ap.add_argument("--batched-frames", type=int, default=60)
     ap.add_argument("--batched-warmup", type=int, default=240)
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-rollout section")
-    ap.add_argument("--solver", default="auto", choices=["auto", "ordered", "flow"],
+    ap.add_argument("--solver", default="auto", choices=["auto", "ordered", "flow", "colored"],
                     help="schedule of the contact sweep (all give the serial result bit for bit)")
     ap.add_argument("--solver-wgs", type=int, default=0, help="workgroups of the sweep kernel (0 = default)")
     ap.add_argument("--broadphase", default="auto", choices=["auto", "all_pairs", "lbvh"],
@@ -346,6 +346,8 @@ def main() -> int:
     ms = 1000.0 * el / args.steps
     value = n * SUBSTEPS * args.steps / el
     log(f"timed {args.steps} frames in {el:.2f} s: {ms:.2f} ms/frame, pairs {pairs}, contacts {contacts}")
+    if args.solver == "colored":
+        log(f"contacts per colour (last class = one-lane remainder): {ctx.color_counts()}")
 
     nc = min(contacts, caps.max_contacts)
     roof = roofline_entry("phys/pgs_sweep", stages, n, nh, n_dyn, pairs, nc)

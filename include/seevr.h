@@ -222,16 +222,30 @@ int sv_stage_times(sv_ctx *ctx, int reset, const char **names, double *avg_us,
                    uint64_t *calls, int cap);
 int sv_enable_stage_timing(sv_ctx *ctx, int enable);
 
-/* Schedule of the contact sweep (step 5).  Every schedule gives the result
- * of the reference's serial loop (physics.comp:523-581) bit for bit; they
- * differ only in speed.  No counterpart in the reference, which runs the
- * sweep on one invocation.  SV_SOLVER_AUTO: ordered, one workgroup below
- * 2048 bodies, 32 workgroups from there.  SV_SOLVER_ORDERED: tickets per
- * body, items taken in sweep order by `workgroups` workgroups (1 = one CU).
+/* Schedule of the contact sweep (step 5).  No counterpart in the
+ * reference, which runs the sweep on one invocation (physics.comp:523-581).
+ * The first three give the reference's serial loop bit for bit and differ
+ * only in speed.  SV_SOLVER_AUTO: ordered, one workgroup below 2048 bodies,
+ * 32 workgroups from there.  SV_SOLVER_ORDERED: tickets per body, items
+ * taken in sweep order by `workgroups` workgroups (1 = one CU).
  * SV_SOLVER_FLOW: per-item countdowns, items run as their predecessors
- * finish, `workgroups` workgroups (0 = 64). */
-enum { SV_SOLVER_AUTO = 0, SV_SOLVER_ORDERED = 1, SV_SOLVER_FLOW = 2 };
+ * finish, `workgroups` workgroups (0 = 64).  SV_SOLVER_COLORED: the same
+ * Gauss-Seidel item over the contacts sorted by a graph colouring (no two
+ * contacts of a colour share a dynamic body), all contacts of a colour at
+ * once; a different order, so a different (deterministic) result, judged by
+ * sv_solver_residual.  Needs cooperative launch and < 2^24 contacts. */
+enum { SV_SOLVER_AUTO = 0, SV_SOLVER_ORDERED = 1, SV_SOLVER_FLOW = 2, SV_SOLVER_COLORED = 3 };
 int sv_set_solver(sv_ctx *ctx, int schedule, uint32_t workgroups);
+
+/* Residual of the normal constraints for the contact list and velocities in
+ * the context, to call after step 5 (sv_physics_dispatch) and before step 6:
+ * per live contact, |v_n - bias| if its impulse is positive, else
+ * max(bias - v_n, 0) (m/s).  Writes the RMS, the maximum and the count.  No
+ * counterpart in the reference. */
+int sv_solver_residual(sv_ctx *ctx, double *rms, double *max_abs, uint32_t *count);
+/* Contacts per colour of the last coloured sweep, then the remainder class
+ * swept by one lane; returns the number of entries written (<= cap). */
+int sv_debug_color_counts(sv_ctx *ctx, uint32_t *out, uint32_t cap);
 
 /* Broadphase method (step 2).  Both give the pair list of the reference's
  * all-pairs loop (physics.comp:451-472): the same set, in lexicographic

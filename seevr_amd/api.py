@@ -162,11 +162,24 @@ class Context:
     def enable_stage_timing(self, on: bool = True) -> None:
         check(self.lib.sv_enable_stage_timing(self.h, int(on)))
 
-    SOLVERS = {"auto": 0, "ordered": 1, "flow": 2}
+    SOLVERS = {"auto": 0, "ordered": 1, "flow": 2, "colored": 3}
 
     def set_solver(self, schedule: str = "auto", workgroups: int = 0) -> None:
         """Schedule of the contact sweep: "auto", "ordered" or "flow" (all bit-identical)."""
         check(self.lib.sv_set_solver(self.h, self.SOLVERS[schedule], workgroups))
+
+    def residual(self) -> tuple[float, float, int]:
+        """(RMS, max, count) of the normal-constraint residual after step 5."""
+        r, m, n = C.c_double(), C.c_double(), C.c_uint32()
+        check(self.lib.sv_solver_residual(self.h, C.byref(r), C.byref(m), C.byref(n)))
+        return r.value, m.value, n.value
+
+    def color_counts(self) -> list[int]:
+        out = (C.c_uint32 * 64)()
+        n = self.lib.sv_debug_color_counts(self.h, out, 64)
+        if n < 0:
+            check(n)
+        return [int(out[i]) for i in range(n)]
 
     BROADPHASES = {"auto": 0, "all_pairs": 1, "lbvh": 2}
 

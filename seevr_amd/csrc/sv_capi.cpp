@@ -120,6 +120,8 @@ int sync_and_finish(sv_ctx *c) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(SV_E_DEVICE, "device work failed: %s", hipGetErrorString(e));
     e = hipGetLastError();  // a kernel launch that was rejected (resources, bad configuration)
+    if (e == hipSuccess && c->launch_err != hipSuccess) e = c->launch_err;
+    c->launch_err = hipSuccess;
     if (e != hipSuccess) return fail(SV_E_DEVICE, "kernel launch failed: %s", hipGetErrorString(e));
     if (c->timing) {
         for (auto &p : c->pending) {
@@ -468,12 +470,34 @@ int sv_enable_stage_timing(sv_ctx *c, int enable) {
 
 int sv_set_solver(sv_ctx *c, int schedule, uint32_t workgroups) {
     if (!c) return fail(SV_E_INVALID, "sv_set_solver: null context");
-    if (schedule < SV_SOLVER_AUTO || schedule > SV_SOLVER_FLOW)
+    if (schedule < SV_SOLVER_AUTO || schedule > SV_SOLVER_COLORED)
         return fail(SV_E_INVALID, "sv_set_solver: unknown schedule %d", schedule);
     if (workgroups > 4096) return fail(SV_E_INVALID, "sv_set_solver: at most 4096 workgroups");
+    if (schedule == SV_SOLVER_COLORED) {
+        int coop = 0;
+        if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device) != hipSuccess || !coop)
+            return fail(SV_E_INVALID, "sv_set_solver: the device has no cooperative launch");
+        if (c->caps.max_contacts >= (1u << 24))
+            return fail(SV_E_CAPACITY, "sv_set_solver: the coloured sweep takes fewer than 2^24 contacts");
+    }
     c->solver_flow = schedule == SV_SOLVER_FLOW;
-    c->solver_workgroups = schedule == SV_SOLVER_AUTO ? 0u : (workgroups ? workgroups : (c->solver_flow ? 0u : 1u));
+    c->solver_colored = schedule == SV_SOLVER_COLORED;
+    c->solver_workgroups = schedule == SV_SOLVER_AUTO
+                               ? 0u
+                               : (workgroups ? workgroups : (c->solver_flow || c->solver_colored ? 0u : 1u));
     return SV_OK;
+}
+
+int sv_solver_residual(sv_ctx *c, double *rms, double *max_abs, uint32_t *count) {
+    if (!c) return fail(SV_E_INVALID, "sv_solver_residual: null context");
+    int rc = sv::solver_residual(c, rms, max_abs, count);
+    return rc ? fail(rc, "sv_solver_residual: device error") : SV_OK;
+}
+
+int sv_debug_color_counts(sv_ctx *c, uint32_t *out, uint32_t cap) {
+    if (!c || !out) return fail(SV_E_INVALID, "sv_debug_color_counts: null argument");
+    int n = sv::color_counts(c, out, cap);
+    return n < 0 ? fail(n, "sv_debug_color_counts: no coloured sweep has run") : n;
 }
 
 int sv_set_broadphase(sv_ctx *c, int method) {

@@ -50,6 +50,10 @@ struct Scratch {
     uint32_t *bb_cnt = nullptr, *bb_off = nullptr, *bb_fill = nullptr;         // [N + 1] per-row pairs
     uint32_t *bb_pk[2] = {nullptr, nullptr}, *bb_pv[2] = {nullptr, nullptr};  // [P + N] pair sort
     bool bb_ready = false;
+    // graph-coloured sweep and residual (allocated on first use)
+    unsigned long long *col_claim = nullptr;  // [N] colour-round claims per body
+    uint32_t *col_color = nullptr;            // [C] colour of each contact
+    uint32_t *col_cnt = nullptr, *col_off = nullptr, *col_bar = nullptr, *res_acc = nullptr;
     // every lazily allocated block, freed with the context
     std::vector<void *> lazy;
 };
@@ -86,6 +90,11 @@ struct sv_ctx {
     // sweep schedule: false = ordered (tickets, items taken in sweep order),
     // true = flow (countdowns, items run as their predecessors finish)
     bool solver_flow = false;
+    // graph-coloured sweep (a different order than the reference's: judged by
+    // the constraint residual, not by parity)
+    bool solver_colored = false;
+    // launch failure to report at the next synchronization
+    hipError_t launch_err = hipSuccess;
     // broadphase: 0 = chosen by body count, 1 = all pairs (tiled N^2), 2 = LBVH traversal
     uint32_t broadphase_mode = 0;
     // stage timing
@@ -147,6 +156,16 @@ void launch_pack_boxes(sv_ctx *c, uint32_t n);
 // ---- sv_pgs.hip ----
 void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt);
 uint32_t solver_lds_bodies(sv_ctx *c);
+
+// ---- sv_color.hip ----
+// Coloured sweep of the contacts prepared by the sweep's prep stage; false
+// when it cannot run here (no cooperative launch, too many contacts).
+bool launch_solve_colored(sv_ctx *c, uint32_t n_bodies);
+// Normal-constraint residual of the current contact list and velocities.
+int solver_residual(sv_ctx *c, double *rms, double *max_abs, uint32_t *count);
+// Contacts per colour of the last coloured sweep (colours, then the
+// remainder class); returns the number written or a negative status.
+int color_counts(sv_ctx *c, uint32_t *out, uint32_t cap);
 
 // ---- sv_bvh.hip ----
 void bvh_stage(sv_ctx *c, uint32_t step, uint32_t n, bool bodies);

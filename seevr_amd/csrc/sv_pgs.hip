@@ -454,6 +454,10 @@ void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
                        B<float>(c, SV_BODY_OMEGA), B<float>(c, SV_BODY_INV_MASS), B<float>(c, SV_BODY_INV_INERTIA),
                        c->s.rot, B<float>(c, SV_GEOM_FRICTION), B<float>(c, SV_GEOM_RESTITUTION), c->s.cprep,
                        c->s.inc_key[0], c->s.inc_val[0], c->s.deg);
+    if (c->solver_colored) {
+        if (!launch_solve_colored(c, n_bodies)) c->launch_err = hipErrorLaunchFailure;
+        return;
+    }
     hipLaunchKernelGGL(k_set_ninc, dim3(1), dim3(1), 0, c->stream, c->s.misc);
     int key_bits = 32 - __builtin_clz(n_bodies);  // keys are in [0, n_bodies]
     int where = radix_sort(c, c->s.inc_key[0], c->s.inc_val[0], c->s.inc_key[1], c->s.inc_val[1],
