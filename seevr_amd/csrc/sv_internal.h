@@ -50,6 +50,9 @@ struct Scratch {
     uint32_t *bb_cnt = nullptr, *bb_off = nullptr, *bb_fill = nullptr;         // [N + 1] per-row pairs
     uint32_t *bb_pk[2] = {nullptr, nullptr}, *bb_pv[2] = {nullptr, nullptr};  // [P + N] pair sort
     bool bb_ready = false;
+    // level-ordered BVH refit (allocated on first use)
+    uint32_t *lv_depth = nullptr, *lv_order = nullptr;  // [max(T, N)] node depth, nodes by depth
+    uint32_t *lv_small = nullptr;                       // histogram, offsets, fill, deepest level
     // graph-coloured sweep and residual (allocated on first use)
     unsigned long long *col_claim = nullptr;  // [N] colour-round claims per body
     uint32_t *col_color = nullptr;            // [C] colour of each contact
