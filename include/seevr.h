@@ -290,6 +290,20 @@ int sv_scene_array(const sv_scene *scene, sv_buf id, const void **host, size_t *
  * body_vert_count and the exact SV_GEOM_HULL_RANGE extension. */
 int sv_scene_upload(sv_scene *scene, sv_ctx *ctx);
 
+/* Scene files (library extension; the reference lists scene save/load as
+ * a to-do, README.md:99).  The file holds the scene's source arrays (the
+ * s_* staging slices of data.zig:123-170) with a section table and an
+ * FNV-1a checksum per section; loading checks every count, index and range
+ * against the scene's capacities and replaces the scene's contents only when
+ * all checks pass.  serialize with dst = NULL returns the size only. */
+int sv_scene_serialize(const sv_scene *scene, void *dst, size_t capacity, size_t *bytes_out);
+int sv_scene_deserialize(sv_scene *scene, const void *src, size_t bytes);
+int sv_scene_save(const sv_scene *scene, const char *path);
+int sv_scene_load(sv_scene *scene, const char *path);
+/* Copy a context's body state (pos, quat, vel, omega) and render vertices
+ * into the scene, so that a saved scene resumes the simulation there. */
+int sv_scene_read_state(sv_scene *scene, sv_ctx *ctx);
+
 /* Seeded benchmark scenes (library extension, no reference counterpart):
  *   1: 1,000 bodies in a 10x10 m column over a 50x1x50 m ground
  *   2: 32,768 bodies in a 30x30 m walled container (ground + 4 walls)

@@ -73,7 +73,8 @@ EXPORTS = [
     "sv_debug_copy_bandwidth", "sv_last_error", "sv_device_info", "sv_stage_times", "sv_enable_stage_timing", "sv_set_solver", "sv_solver_residual", "sv_debug_color_counts", "sv_set_broadphase", "sv_scene_create",
     "sv_scene_destroy", "sv_scene_add_material", "sv_scene_add_box", "sv_scene_add_sphere",
     "sv_scene_set_body_state", "sv_scene_get_counts", "sv_scene_array", "sv_scene_upload",
-    "sv_scene_build_workload", "sv_scene_build_batch", "sv_batch_create", "sv_batch_destroy", "sv_batch_context",
+    "sv_scene_build_workload", "sv_scene_build_batch", "sv_scene_serialize", "sv_scene_deserialize",
+    "sv_scene_save", "sv_scene_load", "sv_scene_read_state", "sv_batch_create", "sv_batch_destroy", "sv_batch_context",
     "sv_batch_set_scenes", "sv_batch_step", "sv_batch_read_counters", "sv_batch_set_config", "sv_batch_autotune",
     "sv_batch_diff_reserve", "sv_batch_diff_forward", "sv_batch_diff_backward",
 ]
@@ -136,6 +137,11 @@ def load(path: Path | str | None = None) -> C.CDLL:
         "sv_scene_upload": (C.c_int, [vp, vp]),
         "sv_scene_build_workload": (C.c_int, [vp, C.c_int, u32, u32, C.c_uint64]),
         "sv_scene_build_batch": (C.c_int, [vp, u32, u32, u32, u32, u32, f32, C.c_uint64, C.POINTER(u32), fp]),
+        "sv_scene_serialize": (C.c_int, [vp, vp, sz, C.POINTER(sz)]),
+        "sv_scene_deserialize": (C.c_int, [vp, vp, sz]),
+        "sv_scene_save": (C.c_int, [vp, C.c_char_p]),
+        "sv_scene_load": (C.c_int, [vp, C.c_char_p]),
+        "sv_scene_read_state": (C.c_int, [vp, vp]),
         "sv_batch_create": (C.c_int, [C.POINTER(BatchCaps), C.POINTER(Params), C.POINTER(vp)]),
         "sv_batch_destroy": (C.c_int, [vp]),
         "sv_batch_context": (vp, [vp]),

@@ -254,6 +254,28 @@ class Scene:
     def build_workload(self, workload: int, count: int = 0, segments: int = 0, seed: int = 1) -> None:
         check(self.lib.sv_scene_build_workload(self.h, workload, count, segments, seed))
 
+    # -- scene files ----------------------------------------------------------
+    def to_bytes(self) -> bytes:
+        n = C.c_size_t()
+        check(self.lib.sv_scene_serialize(self.h, None, 0, C.byref(n)))
+        buf = (C.c_uint8 * n.value)()
+        check(self.lib.sv_scene_serialize(self.h, buf, n.value, C.byref(n)))
+        return bytes(buf)
+
+    def from_bytes(self, data: bytes) -> None:
+        buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
+        check(self.lib.sv_scene_deserialize(self.h, buf, len(data)))
+
+    def save(self, path) -> None:
+        check(self.lib.sv_scene_save(self.h, str(path).encode()))
+
+    def load(self, path) -> None:
+        check(self.lib.sv_scene_load(self.h, str(path).encode()))
+
+    def read_state(self, ctx: "Context") -> None:
+        """Body state and render vertices of a context back into the scene."""
+        check(self.lib.sv_scene_read_state(self.h, ctx.h))
+
     def build_batch(self, first: int, count: int, min_bodies: int = 8, max_bodies: int = 256, segments: int = 0,
                     seed: int = 1, top: float = 0.0) -> tuple[np.ndarray, np.ndarray]:
         """Scenes [first, first + count) of the batched family; returns

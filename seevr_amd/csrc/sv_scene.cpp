@@ -6,6 +6,7 @@
 // operation, and so does this file.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
@@ -460,6 +461,244 @@ int sv_scene_build_batch(sv_scene *s, uint32_t first, uint32_t count, uint32_t m
     }
     body_start[count] = s->num_bodies;
     return rc;
+}
+
+}  // extern "C"
+
+// ---- scene files (library extension; README.md:99 lists save/load) -------
+//
+// Little-endian, one file per scene:
+//   0  char[8]  magic "SVSCENE\0"
+//   8  u32      version (1)
+//  12  u32      section count S
+//  16  u32[7]   counts: vertices, triangles, bodies, geoms, hull verts,
+//               materials, support entries (sv_scene_counts order)
+//  44  u32      0
+//  48  S x 32-byte entries {char tag[4]; u32 element bytes (4);
+//                            u64 payload offset; u64 payload bytes;
+//                            u64 FNV-1a 64 of the payload}
+//  payloads at 64-byte aligned offsets, in table order.
+// Sections are the scene's source arrays; body_vert_start/count and the
+// hull ranges are derived again on load.  Every count, index and range is
+// checked before anything is replaced.
+namespace sv {
+int set_error(int code, const char *msg);
+}
+
+namespace {
+
+constexpr char kMagic[8] = {'S', 'V', 'S', 'C', 'E', 'N', 'E', '\0'};
+constexpr uint32_t kVersion = 1;
+constexpr size_t kHeader = 48, kEntryBytes = 32, kAlign = 64;
+
+struct Section {
+    char tag[5];
+    bool is_float;
+    std::vector<float> *f;
+    std::vector<uint32_t> *u;
+    uint64_t elems;  // expected element count from the counts
+};
+
+std::vector<Section> sections(sv_scene *s, const uint32_t n[7]) {
+    const uint64_t nv = n[0], nt = n[1], nb = n[2], ng = n[3], nh = n[4], nm = n[5], ns = n[6];
+    return {
+        {"VERT", true, &s->vertices, nullptr, 3 * nv},         {"OVRT", true, &s->orig_vertices, nullptr, 3 * nv},
+        {"INDX", false, nullptr, &s->indices, 3 * nt},          {"BPOS", true, &s->body_pos, nullptr, 3 * nb},
+        {"BQUT", true, &s->body_quat, nullptr, 4 * nb},         {"BVEL", true, &s->body_vel, nullptr, 3 * nb},
+        {"BOMG", true, &s->body_omega, nullptr, 3 * nb},        {"BIMS", true, &s->body_inv_mass, nullptr, nb},
+        {"BINE", true, &s->body_inertia, nullptr, 3 * nb},      {"BINI", true, &s->body_inv_inertia, nullptr, 3 * nb},
+        {"GBDY", false, nullptr, &s->geom_body_idx, ng},        {"GDAT", true, &s->geom_data, nullptr, 8 * ng},
+        {"GFRC", true, &s->geom_friction, nullptr, ng},         {"GRST", true, &s->geom_restitution, nullptr, ng},
+        {"GVST", false, nullptr, &s->geom_vert_start, ng},      {"GVCT", false, nullptr, &s->geom_vert_count, ng},
+        {"HULL", true, &s->hull_verts, nullptr, 3 * nh},        {"SLOF", false, nullptr, &s->support_offset, ng},
+        {"SLUT", false, nullptr, &s->support_lookup, ns},       {"MDEN", true, &s->material_density, nullptr, nm},
+    };
+}
+
+uint64_t fnv1a(const uint8_t *p, size_t n) {
+    uint64_t h = 0xCBF29CE484222325ull;
+    for (size_t i = 0; i < n; i++) {
+        h ^= p[i];
+        h *= 0x100000001B3ull;
+    }
+    return h;
+}
+
+void put32(uint8_t *p, uint32_t v) { std::memcpy(p, &v, 4); }
+void put64(uint8_t *p, uint64_t v) { std::memcpy(p, &v, 8); }
+uint32_t get32(const uint8_t *p) { uint32_t v; std::memcpy(&v, p, 4); return v; }
+uint64_t get64(const uint8_t *p) { uint64_t v; std::memcpy(&v, p, 8); return v; }
+size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+int file_error(const char *msg) { return sv::set_error(SV_E_INVALID, msg); }
+
+// Structural checks of a loaded scene: every index the physics and BVH
+// steps follow stays inside its array.
+const char *check_scene(const sv_scene &s) {
+    const uint32_t nb = s.num_bodies, nv = s.num_vertices, nh = s.num_hull_verts;
+    if (s.num_geoms != nb) return "scene file: geoms and bodies must be 1:1";
+    for (uint32_t i : s.indices)
+        if (i >= nv) return "scene file: triangle index out of range";
+    for (uint32_t g = 0; g < s.num_geoms; g++) {
+        if (s.geom_body_idx[g] >= nb) return "scene file: geom body index out of range";
+        const float hs = s.geom_data[8 * g], hc = s.geom_data[8 * g + 1];
+        if (!(hs >= 0 && hc >= 1 && hs == (float)(uint32_t)hs && hc == (float)(uint32_t)hc &&
+              (uint64_t)hs + (uint64_t)hc <= nh))
+            return "scene file: hull range out of range";
+        if ((uint64_t)s.geom_vert_start[g] + s.geom_vert_count[g] > nv) return "scene file: vertex range out of range";
+        const uint32_t so = s.support_offset[g];
+        if (so != 0 && (uint64_t)so + kEntry > s.num_support_entries) return "scene file: support lookup out of range";
+        if (so != 0)
+            for (uint32_t k = 0; k < kEntry; k++)
+                if (s.support_lookup[so + k] >= (uint32_t)hc) return "scene file: support lookup entry out of range";
+    }
+    if (s.num_materials > 256) return "scene file: more than 256 materials";
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sv_scene_serialize(const sv_scene *cs, void *dst, size_t cap, size_t *bytes_out) {
+    if (!cs || !bytes_out) return bad("null");
+    sv_scene *s = const_cast<sv_scene *>(cs);
+    const uint32_t n[7] = {s->num_vertices, s->num_triangles, s->num_bodies, s->num_geoms,
+                           s->num_hull_verts, s->num_materials, s->num_support_entries};
+    auto secs = sections(s, n);
+    size_t off = align_up(kHeader + kEntryBytes * secs.size());
+    std::vector<size_t> offs;
+    for (auto &x : secs) {
+        offs.push_back(off);
+        off = align_up(off + x.elems * 4);
+    }
+    *bytes_out = off;
+    if (!dst) return SV_OK;
+    if (cap < off) return sv::set_error(SV_E_RANGE, "scene serialize: destination smaller than the scene file");
+    uint8_t *p = static_cast<uint8_t *>(dst);
+    std::memset(p, 0, off);
+    std::memcpy(p, kMagic, 8);
+    put32(p + 8, kVersion);
+    put32(p + 12, (uint32_t)secs.size());
+    for (int i = 0; i < 7; i++) put32(p + 16 + 4 * i, n[i]);
+    for (size_t i = 0; i < secs.size(); i++) {
+        const Section &x = secs[i];
+        const void *src = x.is_float ? (const void *)x.f->data() : (const void *)x.u->data();
+        const size_t sz = (x.is_float ? x.f->size() : x.u->size());
+        if (sz != x.elems) return sv::set_error(SV_E_INVALID, "scene serialize: array size disagrees with counts");
+        uint8_t *e = p + kHeader + kEntryBytes * i;
+        std::memcpy(e, x.tag, 4);
+        put32(e + 4, 4);
+        put64(e + 8, offs[i]);
+        put64(e + 16, x.elems * 4);
+        if (sz) std::memcpy(p + offs[i], src, x.elems * 4);
+        put64(e + 24, fnv1a(p + offs[i], x.elems * 4));
+    }
+    return SV_OK;
+}
+
+int sv_scene_deserialize(sv_scene *s, const void *src, size_t bytes) {
+    if (!s || (!src && bytes)) return bad("null");
+    const uint8_t *p = static_cast<const uint8_t *>(src);
+    if (bytes < kHeader || std::memcmp(p, kMagic, 8) != 0) return file_error("scene file: bad magic");
+    if (get32(p + 8) != kVersion) return file_error("scene file: unsupported version");
+    const uint32_t ns = get32(p + 12);
+    uint32_t n[7];
+    for (int i = 0; i < 7; i++) n[i] = get32(p + 16 + 4 * i);
+    if (kHeader + (uint64_t)kEntryBytes * ns > bytes) return file_error("scene file: truncated section table");
+    const sv_caps &k = s->caps;
+    if (n[2] > k.max_bodies || n[0] > k.max_vertices || n[1] > k.max_triangles || n[4] > k.max_hull_verts ||
+        n[6] > k.max_support_entries)
+        return sv::set_error(SV_E_CAPACITY, "scene file: counts exceed the scene's capacities");
+    sv_scene t;
+    t.caps = s->caps;
+    auto secs = sections(&t, n);
+    std::vector<bool> seen(secs.size(), false);
+    for (uint32_t i = 0; i < ns; i++) {
+        const uint8_t *e = p + kHeader + kEntryBytes * i;
+        size_t j = 0;
+        while (j < secs.size() && std::memcmp(e, secs[j].tag, 4) != 0) j++;
+        if (j == secs.size()) continue;  // unknown sections are skipped (forward compatible)
+        if (seen[j]) return file_error("scene file: duplicate section");
+        const uint64_t off = get64(e + 8), len = get64(e + 16);
+        if (get32(e + 4) != 4) return file_error("scene file: element size must be 4");
+        if (len != secs[j].elems * 4) return file_error("scene file: section length disagrees with counts");
+        if (off > bytes || len > bytes - off) return file_error("scene file: section outside the file");
+        if (fnv1a(p + off, len) != get64(e + 24)) return file_error("scene file: section checksum mismatch");
+        if (secs[j].is_float) {
+            secs[j].f->resize(secs[j].elems);
+            if (len) std::memcpy(secs[j].f->data(), p + off, len);
+        } else {
+            secs[j].u->resize(secs[j].elems);
+            if (len) std::memcpy(secs[j].u->data(), p + off, len);
+        }
+        seen[j] = true;
+    }
+    for (size_t j = 0; j < secs.size(); j++)
+        if (!seen[j]) return file_error("scene file: missing section");
+    t.num_vertices = n[0];
+    t.num_triangles = n[1];
+    t.num_bodies = n[2];
+    t.num_geoms = n[3];
+    t.num_hull_verts = n[4];
+    t.num_materials = n[5];
+    t.num_support_entries = n[6];
+    if (const char *why = check_scene(t)) return file_error(why);
+    t.body_vert_start.assign(t.num_bodies, 0);
+    t.body_vert_count.assign(t.num_bodies, 0);
+    t.geom_hull_range.resize(2 * (size_t)t.num_geoms);
+    for (uint32_t g = 0; g < t.num_geoms; g++) {
+        t.geom_hull_range[2 * g] = (uint32_t)t.geom_data[8 * g];
+        t.geom_hull_range[2 * g + 1] = (uint32_t)t.geom_data[8 * g + 1];
+    }
+    derive(&t);
+    *s = std::move(t);
+    return SV_OK;
+}
+
+int sv_scene_save(const sv_scene *s, const char *path) {
+    if (!s || !path) return bad("null");
+    size_t n = 0;
+    int rc = sv_scene_serialize(s, nullptr, 0, &n);
+    if (rc) return rc;
+    std::vector<uint8_t> buf(n);
+    rc = sv_scene_serialize(s, buf.data(), n, &n);
+    if (rc) return rc;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return sv::set_error(SV_E_INVALID, "scene save: cannot open the file for writing");
+    const bool ok = std::fwrite(buf.data(), 1, n, f) == n;
+    const bool closed = std::fclose(f) == 0;
+    return ok && closed ? SV_OK : sv::set_error(SV_E_INVALID, "scene save: write failed");
+}
+
+int sv_scene_load(sv_scene *s, const char *path) {
+    if (!s || !path) return bad("null");
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return sv::set_error(SV_E_INVALID, "scene load: cannot open the file");
+    std::vector<uint8_t> buf;
+    uint8_t chunk[1 << 16];
+    size_t got;
+    while ((got = std::fread(chunk, 1, sizeof chunk, f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+    const bool err = std::ferror(f) != 0;
+    std::fclose(f);
+    if (err) return sv::set_error(SV_E_INVALID, "scene load: read failed");
+    return sv_scene_deserialize(s, buf.data(), buf.size());
+}
+
+// Copies the body state and render vertices of a context back into the
+// scene (positions, orientations, velocities after simulated frames), so a
+// saved scene resumes the simulation where the context stands.
+int sv_scene_read_state(sv_scene *s, sv_ctx *ctx) {
+    if (!s || !ctx) return bad("null");
+    struct { sv_buf id; std::vector<float> *v; } io[] = {
+        {SV_BODY_POS, &s->body_pos}, {SV_BODY_QUAT, &s->body_quat}, {SV_BODY_VEL, &s->body_vel},
+        {SV_BODY_OMEGA, &s->body_omega}, {SV_VERTICES, &s->vertices}};
+    for (auto &x : io) {
+        if (x.v->empty()) continue;
+        int rc = sv_download(ctx, x.id, x.v->data(), 0, x.v->size() * sizeof(float));
+        if (rc) return rc;
+    }
+    return SV_OK;
 }
 
 }  // extern "C"
