@@ -274,7 +274,52 @@ def differentiable_section(frames: int = 60, scenes: int = 256, bodies: int = 64
                     "+ U[-0.5, 0.5]"}
 
 
-def large_scene_section(frames: int = 120, timed: int = 20) -> dict:
+def warm_start_section(ctx, n, nv, nh, factor: float = 1.0, iterations=(4, 10, 30)) -> dict:
+    """Warm start beside the reference path, on the pile where the timed
+    frames left it: one frame with warm start from that state fills the
+    contact cache, then the next substep's collision stages and a sweep of
+    k iterations run from the same state cold (impulses cleared, the
+    reference) and warm (carried over); the normal-constraint residual and
+    the sweep time are reported for each.  The frame is rerun before every
+    probe so each starts from the same state and cache."""
+    state = {k: ctx.download(k, w * n) for k, w in (("BODY_POS", 3), ("BODY_QUAT", 4), ("BODY_VEL", 3),
+                                                      ("BODY_OMEGA", 3))}
+    dt_sub = float(np.float32(DT) / np.float32(SUBSTEPS))
+    out = {}
+    for k in iterations:
+        row = {}
+        for mode in ("cold", "warm"):
+            for key, v in state.items():
+                ctx.upload(key, v)
+            ctx.set_params(pgs_iterations=PGS_ITERS)
+            ctx.set_warm_start(factor)
+            ctx.step(n, nv, nh, SUBSTEPS, DT, GRAVITY)
+            if mode == "cold":
+                ctx.set_warm_start(0.0)
+            ctx.set_params(pgs_iterations=k)
+            ctx.dispatch(8, 1)
+            ctx.dispatch(0, n, dt_sub, GRAVITY)
+            ctx.dispatch(9, n)
+            ctx.dispatch(2, n)
+            ctx.dispatch(3, 0, num_bodies=n)
+            ctx.dispatch(4, 0)
+            ctx.enable_stage_timing(True)
+            ctx.stage_times(reset=True)
+            ctx.dispatch(5, n, dt_sub)
+            st = ctx.stage_times(reset=True)
+            ctx.enable_stage_timing(False)
+            rms, mx, cnt = ctx.residual()
+            row[mode] = {"residual_rms": rms, "residual_max": mx, "contacts": cnt,
+                         "solve_us": round(st.get("phys/pgs_solve", (0.0, 0))[0], 1)}
+        out[f"{k}_iterations"] = row
+    ctx.set_warm_start(0.0)
+    ctx.set_params(pgs_iterations=PGS_ITERS)
+    log(f"warm start (factor {factor}) vs cold from the same state: " +
+        ", ".join(f"{k}: rms {v['cold']['residual_rms']:.4g} -> {v['warm']['residual_rms']:.4g}" for k, v in out.items()))
+    return {"factor": factor, "from_same_state": out}
+
+
+def large_scene_section(frames: int = 120, timed: int = 20, compare: bool = True) -> dict:
     """The 1,048,576-body scene (70% spheres of 81 hull verts, 200x200x50 m
     over a 220x1x220 m ground; 62M hull verts, so hull ranges are read as
     uint32): `frames` frames with the LBVH broadphase, the last `timed` of
@@ -311,7 +356,7 @@ def large_scene_section(frames: int = 120, timed: int = 20) -> dict:
                                                        ("BODY_OMEGA", 3))}
     dt_sub = float(np.float32(DT) / np.float32(SUBSTEPS))
     cmp = {}
-    for solver in ("auto", "colored"):
+    for solver in (("auto", "colored") if compare else ()):
         for k, v in state.items():
             ctx.upload(k, v)
         ctx.set_solver(solver)
@@ -364,6 +409,11 @@ def main() -> int:
     ap.add_argument("--batched-warmup", type=int, default=240)
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-rollout section")
     ap.add_argument("--no-large", action="store_true", help="skip the 1M-body section")
+    ap.add_argument("--large-frames", type=int, default=120)
+    ap.add_argument("--large-timed", type=int, default=20)
+    ap.add_argument("--no-large-compare", action="store_true",
+                    help="skip the ordered-vs-coloured comparison of the 1M-body section")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm-start comparison")
     ap.add_argument("--solver", default="auto", choices=["auto", "ordered", "flow", "colored"],
                     help="schedule of the contact sweep (all give the serial result bit for bit)")
     ap.add_argument("--solver-wgs", type=int, default=0, help="workgroups of the sweep kernel (0 = default)")
@@ -439,6 +489,9 @@ def main() -> int:
         log("timing the CPU oracle baseline ...")
         cpu = cpu_baseline(ctx, scene, n, args.cpu_sample_substeps)
         log(f"cpu baseline: {cpu['value']:.0f} body-substeps/s")
+    warm = None
+    if not args.no_warm:
+        warm = warm_start_section(ctx, n, nv, nh)
     ctx.close()
     scene.close()
 
@@ -450,7 +503,7 @@ def main() -> int:
         diff = differentiable_section()
     large = None
     if not args.no_large:
-        large = large_scene_section()
+        large = large_scene_section(args.large_frames, args.large_timed, not args.no_large_compare)
 
     out = {
         "metric": "body-substeps/s @32k bodies", "value": round(value, 1), "unit": "body-substeps/s",
@@ -463,7 +516,7 @@ def main() -> int:
                    "frames_before_timing": args.warmup, "pairs_last_substep": pairs,
                    "contacts_last_substep": contacts},
         "roofline": roof, "roofline_hbm_stages": hbm_stages, "cpu_baseline": cpu, "stage_share_pct": breakdown,
-        "batched": batched, "differentiable": diff, "large_scene": large,
+        "warm_start": warm, "batched": batched, "differentiable": diff, "large_scene": large,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -256,6 +256,17 @@ int sv_debug_color_counts(sv_ctx *ctx, uint32_t *out, uint32_t cap);
 enum { SV_BROADPHASE_AUTO = 0, SV_BROADPHASE_ALL_PAIRS = 1, SV_BROADPHASE_LBVH = 2 };
 int sv_set_broadphase(sv_ctx *ctx, int method);
 
+/* Warm start (library extension beside the reference path; the reference
+ * clears every impulse each substep, physics.comp:494-497, and lists warm
+ * start as a to-do, todo warmup.md:1-3).  factor in (0, 1]: after each
+ * sweep the contacts' (geom a, geom b) keys and impulses are kept; at the
+ * next step 4 a contact of the same pair starts from factor times that
+ * impulse, applied to both bodies at its first update in the sweep.
+ * factor 0 (the default) is the reference's path.  Any call empties the
+ * cache.  Applies to sv_physics_step and sv_physics_dispatch (not the
+ * batched engine). */
+int sv_set_warm_start(sv_ctx *ctx, float factor);
+
 /* ---- host scene construction (src/data.zig) ------------------------- */
 typedef struct sv_scene sv_scene;
 

@@ -451,7 +451,7 @@ static inline v3 apply_inv_I(const or_bufs *b, uint32_t bi, v3 v) {
 }
 
 /* physics.comp:500-583: serial projected Gauss-Seidel. */
-void OR(step_solve)(or_bufs *b, R dt) {
+static void solve_impl(or_bufs *b, R dt, int warm, const uint32_t *order) {
     uint32_t nc = b->counters[1] < b->max_contacts ? b->counters[1] : b->max_contacts;
     R *bias = (R *)malloc(sizeof(R) * (nc ? nc : 1));
     for (uint32_t ci = 0; ci < nc; ci++) {
@@ -466,8 +466,23 @@ void OR(step_solve)(or_bufs *b, R dt) {
         R e = gmax(b->geom_restitution[ga], b->geom_restitution[gb]);
         bias[ci] = b->baumgarte * gmax(pen - b->slop, (R)0) / dt + e * gmax(-vn, (R)0);
     }
+    /* warm start (extension, no reference line): every carried-over impulse
+     * applied to both bodies, in contact order, before the sweep */
+    for (uint32_t ci = 0; warm && ci < nc; ci++) {
+        uint32_t a = b->geom_body_idx[b->contact_geom_a[ci]], bb = b->geom_body_idx[b->contact_geom_b[ci]];
+        R ima = b->body_inv_mass[a], imb = b->body_inv_mass[bb];
+        if ((ima == (R)0 && imb == (R)0) || !(b->lambda_n[ci] > (R)0)) continue;
+        v3 n = ld3(b->contact_normal, ci), cp = ld3(b->contact_pos, ci);
+        v3 ra = sub(cp, ld3(b->body_pos, a)), rb = sub(cp, ld3(b->body_pos, bb));
+        v3 imp = scl(b->lambda_n[ci], n);
+        st3(b->body_vel, a, sub(ld3(b->body_vel, a), scl(ima, imp)));
+        st3(b->body_omega, a, sub(ld3(b->body_omega, a), apply_inv_I(b, a, cross(ra, imp))));
+        st3(b->body_vel, bb, add(ld3(b->body_vel, bb), scl(imb, imp)));
+        st3(b->body_omega, bb, add(ld3(b->body_omega, bb), apply_inv_I(b, bb, cross(rb, imp))));
+    }
     for (uint32_t it = 0; it < b->pgs_iterations; it++) {
-        for (uint32_t ci = 0; ci < nc; ci++) {
+        for (uint32_t k = 0; k < nc; k++) {
+            uint32_t ci = order ? order[k] : k;
             uint32_t ga = b->contact_geom_a[ci], gb = b->contact_geom_b[ci];
             uint32_t a = b->geom_body_idx[ga], bb = b->geom_body_idx[gb];
             R ima = b->body_inv_mass[a], imb = b->body_inv_mass[bb];
@@ -509,6 +524,49 @@ void OR(step_solve)(or_bufs *b, R dt) {
         }
     }
     free(bias);
+}
+
+void OR(step_solve)(or_bufs *b, R dt) { solve_impl(b, dt, 0, NULL); }
+
+/* Warm start (library extension beside the reference path): the sweep of
+ * physics.comp:500-583 after every contact with a positive starting impulse
+ * has applied it to both bodies (bias from the velocities before that). */
+void OR(step_solve_warm)(or_bufs *b, R dt) { solve_impl(b, dt, 1, NULL); }
+
+/* The same sweep visiting the contacts in order[0..nc) (a permutation of
+ * the contact indices; biases and the warm pre-pass stay in index order):
+ * the serial form of a sweep in another order, such as colour classes. */
+void OR(step_solve_in_order)(or_bufs *b, R dt, int warm, const uint32_t *order) { solve_impl(b, dt, warm, order); }
+
+/* Step 4 with warm start: contact c < nc starts from factor times the cached
+ * impulse of the first cache entry not below its (geom a, geom b) key, when
+ * that entry has the key (binary lower bound); every other slot 0. */
+void OR(warm_lookup)(or_bufs *b, const uint64_t *keys, const R *lam, uint32_t n_prev, R factor) {
+    uint32_t nc = b->counters[1] < b->max_contacts ? b->counters[1] : b->max_contacts;
+    for (uint32_t c = 0; c < b->max_contacts; c++) {
+        R v = 0;
+        if (c < nc && n_prev) {
+            uint64_t k = ((uint64_t)b->contact_geom_a[c] << 32) | b->contact_geom_b[c];
+            uint32_t lo = 0, hi = n_prev;
+            while (lo < hi) {
+                uint32_t mid = lo + (hi - lo) / 2;
+                if (keys[mid] < k) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < n_prev && keys[lo] == k) v = factor * lam[lo];
+        }
+        b->lambda_n[c] = v;
+    }
+}
+
+/* After the sweep: the cache becomes this sweep's keys and impulses. */
+uint32_t OR(warm_save)(const or_bufs *b, uint64_t *keys, R *lam) {
+    uint32_t nc = b->counters[1] < b->max_contacts ? b->counters[1] : b->max_contacts;
+    for (uint32_t c = 0; c < nc; c++) {
+        keys[c] = ((uint64_t)b->contact_geom_a[c] << 32) | b->contact_geom_b[c];
+        lam[c] = b->lambda_n[c];
+    }
+    return nc;
 }
 
 /* physics.comp:586-604 */
@@ -563,6 +621,27 @@ void OR(physics_step)(or_bufs *b, uint32_t num_bodies, uint32_t num_vertices,
         OR(step_narrow_phase)(b);
         OR(step_clear)(b);
         OR(step_solve)(b, (R)sub_dt);
+        OR(step_integrate)(b, num_bodies, (R)sub_dt);
+    }
+    OR(step_update_verts)(b, num_vertices, num_bodies);
+}
+
+/* physics_step with warm start; keys/lam hold max_contacts entries and
+ * *n_prev the cache size, carried between calls. */
+void OR(physics_step_warm)(or_bufs *b, uint32_t num_bodies, uint32_t num_vertices, uint32_t substeps, float dt,
+                           const float g[3], R factor, uint64_t *keys, R *lam, uint32_t *n_prev) {
+    if (num_bodies == 0 || substeps == 0) return;
+    float sub_dt = dt / (float)substeps;
+    R gr[3] = {(R)g[0], (R)g[1], (R)g[2]};
+    for (uint32_t s = 0; s < substeps; s++) {
+        OR(step_clear_counters)(b);
+        OR(step_gravity)(b, num_bodies, (R)sub_dt, gr);
+        OR(step_compute_aabb)(b, num_bodies);
+        OR(step_broad_phase)(b, num_bodies);
+        OR(step_narrow_phase)(b);
+        OR(warm_lookup)(b, keys, lam, *n_prev, factor);
+        OR(step_solve_warm)(b, (R)sub_dt);
+        *n_prev = OR(warm_save)(b, keys, lam);
         OR(step_integrate)(b, num_bodies, (R)sub_dt);
     }
     OR(step_update_verts)(b, num_vertices, num_bodies);

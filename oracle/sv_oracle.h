@@ -76,6 +76,13 @@ void OR(step_clear_counters)(or_bufs *b);
 void OR(physics_step)(or_bufs *b, uint32_t num_bodies, uint32_t num_vertices,
                       uint32_t num_hull_verts, uint32_t substeps, float dt,
                       const float g[3]);
+/* Warm start (library extension beside the reference path). */
+void OR(step_solve_warm)(or_bufs *b, or_real dt);
+void OR(step_solve_in_order)(or_bufs *b, or_real dt, int warm, const uint32_t *order);
+void OR(warm_lookup)(or_bufs *b, const uint64_t *keys, const or_real *lam, uint32_t n_prev, or_real factor);
+uint32_t OR(warm_save)(const or_bufs *b, uint64_t *keys, or_real *lam);
+void OR(physics_step_warm)(or_bufs *b, uint32_t num_bodies, uint32_t num_vertices, uint32_t substeps, float dt,
+                           const float g[3], or_real factor, uint64_t *keys, or_real *lam, uint32_t *n_prev);
 /* Per-pair pieces, exposed for known-answer tests. */
 void OR(support)(const or_bufs *b, uint32_t bi, const or_real d[3], or_real out[3]);
 int OR(mpr)(const or_bufs *b, uint32_t a, uint32_t bb, or_real normal[3],

@@ -187,6 +187,12 @@ class Context:
         """Broadphase method: "auto", "all_pairs" or "lbvh" (identical pair lists)."""
         check(self.lib.sv_set_broadphase(self.h, self.BROADPHASES[method]))
 
+    def set_warm_start(self, factor: float) -> None:
+        """Warm start beside the reference path: 0 = off (the reference's
+        cleared impulses); in (0, 1], contacts of a pair that was in contact
+        at the previous sweep start from factor times its impulse."""
+        check(self.lib.sv_set_warm_start(self.h, C.c_float(factor)))
+
     def stage_times(self, reset: bool = False) -> dict:
         cap = 64
         names = (C.c_char_p * cap)()

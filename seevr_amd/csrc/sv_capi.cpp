@@ -500,6 +500,14 @@ int sv_debug_color_counts(sv_ctx *c, uint32_t *out, uint32_t cap) {
     return n < 0 ? fail(n, "sv_debug_color_counts: no coloured sweep has run") : n;
 }
 
+int sv_set_warm_start(sv_ctx *c, float factor) {
+    if (!c) return fail(SV_E_INVALID, "sv_set_warm_start: null context");
+    if (!(factor >= 0.0f && factor <= 1.0f)) return fail(SV_E_INVALID, "sv_set_warm_start: factor must be in [0, 1]");
+    if (factor > 0.0f && !sv::warm_reserve(c)) return fail(SV_E_NOMEM, "sv_set_warm_start: cache allocation failed");
+    c->warm_factor = factor;
+    return SV_OK;
+}
+
 int sv_set_broadphase(sv_ctx *c, int method) {
     if (!c) return fail(SV_E_INVALID, "sv_set_broadphase: null context");
     if (method < SV_BROADPHASE_AUTO || method > SV_BROADPHASE_LBVH)

@@ -341,7 +341,9 @@ constexpr uint32_t SUCC_NONE = 0xFFFFFFFFu;
 constexpr uint32_t SUCC_WRAP = 0x80000000u;
 constexpr uint32_t SUCC_HASPRED = 0x40000000u;
 constexpr uint32_t SUCC_ID = 0x3FFFFFFFu;
-enum : uint32_t { F_SKIP = 1u, F_DYN_A = 2u, F_DYN_B = 4u };
+// F_WARM: the contact starts from a carried-over impulse (warm start),
+// applied to both bodies before the sweep.
+enum : uint32_t { F_SKIP = 1u, F_DYN_A = 2u, F_DYN_B = 4u, F_WARM = 8u };
 
 SVD f3 apply_inv_I(const m3 &R, f3 inv_i, f3 v) { return mvec(R, hmul(inv_i, mtvec(R, v))); }
 

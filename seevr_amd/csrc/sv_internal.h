@@ -57,6 +57,9 @@ struct Scratch {
     unsigned long long *col_claim = nullptr;  // [N] colour-round claims per body
     uint32_t *col_color = nullptr;            // [C] colour of each contact
     uint32_t *col_cnt = nullptr, *col_off = nullptr, *col_bar = nullptr, *res_acc = nullptr;
+    // warm start: (geom a, geom b) keys and impulses of the last sweep
+    unsigned long long *warm_key = nullptr;  // [C]
+    float *warm_lam = nullptr;               // [C]
     // every lazily allocated block, freed with the context
     std::vector<void *> lazy;
 };
@@ -100,6 +103,10 @@ struct sv_ctx {
     hipError_t launch_err = hipSuccess;
     // broadphase: 0 = chosen by body count, 1 = all pairs (tiled N^2), 2 = LBVH traversal
     uint32_t broadphase_mode = 0;
+    // warm start (extension beside the reference path): 0 = off; otherwise a
+    // contact whose body pair was in contact at the previous sweep starts
+    // from warm_factor times that sweep's impulse
+    float warm_factor = 0.0f;
     // stage timing
     bool timing = false;
     std::vector<std::string> stage_names;
@@ -123,6 +130,7 @@ enum : int {
     MISC_DEG_TOTAL = 5,
     MISC_NACT = 6,     // contacts the sweep updates (not both inverse masses zero)
     MISC_BPE = 7,      // pairs the LBVH broadphase emits before sorting (rows that start below the cap)
+    MISC_WARM_N = 8,   // contacts in the warm-start cache
 };
 
 // ---- sv_prims.hip ----
@@ -158,6 +166,19 @@ void launch_pack_boxes(sv_ctx *c, uint32_t n);
 
 // ---- sv_pgs.hip ----
 void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt);
+
+// ---- sv_warm.hip ----
+// Allocates the warm-start cache (false on failure) and empties it.
+bool warm_reserve(sv_ctx *c);
+// Step 4 with warm start: impulses of contacts whose (geom a, geom b) key is
+// in the cache start from warm_factor times the cached impulse, the rest 0.
+void launch_warm_lookup(sv_ctx *c);
+// After the sweep: the cache becomes this sweep's keys and impulses.
+void launch_warm_save(sv_ctx *c);
+// Before the sweep: every warm-started contact's impulse applied to its
+// dynamic bodies, body by body in contact order (sval = incidences sorted
+// by body, Scratch::deg / deg_off their segments).
+void launch_warm_apply(sv_ctx *c, uint32_t n_bodies, const uint32_t *sval);
 uint32_t solver_lds_bodies(sv_ctx *c);
 
 // ---- sv_color.hip ----

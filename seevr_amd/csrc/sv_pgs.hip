@@ -61,7 +61,8 @@ __global__ void __launch_bounds__(BLOCK) k_prep(
     const float *__restrict__ pos, const float *__restrict__ vel, const float *__restrict__ omega,
     const float *__restrict__ inv_mass, const float *__restrict__ inv_i, const float *__restrict__ rot,
     const float *__restrict__ fric, const float *__restrict__ rest, float *__restrict__ prep,
-    uint32_t *__restrict__ key, uint32_t *__restrict__ val, uint32_t *__restrict__ deg) {
+    uint32_t *__restrict__ key, uint32_t *__restrict__ val, uint32_t *__restrict__ deg,
+    const float *__restrict__ lambda, uint32_t warm) {
     uint32_t nc = misc_w[MISC_NC];
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         uint32_t ga = cga[c], gb = cgb[c];
@@ -69,6 +70,8 @@ __global__ void __launch_bounds__(BLOCK) k_prep(
         uint32_t flags = contact_prep(a, b, ga, gb, ld3(cnrm, c), ld3(cpos, c), cpen[c], dt, baumgarte, slop, pos,
                                       vel, omega, inv_mass, inv_i, rot, fric, rest, prep + (size_t)c * CPREP_WORDS);
         bool skip = flags & F_SKIP, dyn_a = flags & F_DYN_A, dyn_b = flags & F_DYN_B;
+        if (warm && !skip && lambda[c] > 0.0f)
+            prep[(size_t)c * CPREP_WORDS + P_FLAGS] = __uint_as_float(flags | F_WARM);
         if (!skip) atomicAdd(&misc_w[MISC_NACT], 1u);
         bool ia = !skip && dyn_a, ib = !skip && dyn_b;
         key[2 * c] = ia ? a : n_bodies;
@@ -347,7 +350,7 @@ __global__ void __launch_bounds__(MC_THREADS) k_flow(
             it.b = __float_as_uint(P[P_B]);
             it.dyn_a = flags & F_DYN_A;
             it.dyn_b = flags & F_DYN_B;
-            it.n = mk(P[P_N], P[P_N + 1], P[P_N + 2]);
+                    it.n = mk(P[P_N], P[P_N + 1], P[P_N + 2]);
             it.ra = mk(P[P_RA], P[P_RA + 1], P[P_RA + 2]);
             it.rb = mk(P[P_RB], P[P_RB + 1], P[P_RB + 2]);
             it.kn = P[P_KN];
@@ -427,8 +430,9 @@ uint32_t solver_lds_bodies(sv_ctx *c) {
 }
 
 // Step 5 (physics.comp:500-583) for the contacts counters[1] names.
-// Reads lambda_n as the starting impulses (step 4 clears them).
-void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
+// Reads lambda_n as the starting impulses (step 4 clears them, or, with
+// warm start, carries them over).
+void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
     uint32_t *counters = B<uint32_t>(c, SV_ATOMIC_COUNTERS);
     uint32_t mc = c->caps.max_contacts;
     if (mc == 0 || n_bodies == 0) return;
@@ -453,8 +457,10 @@ void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
                        B<float>(c, SV_CONTACT_PENETRATION), B<float>(c, SV_BODY_POS), B<float>(c, SV_BODY_VEL),
                        B<float>(c, SV_BODY_OMEGA), B<float>(c, SV_BODY_INV_MASS), B<float>(c, SV_BODY_INV_INERTIA),
                        c->s.rot, B<float>(c, SV_GEOM_FRICTION), B<float>(c, SV_GEOM_RESTITUTION), c->s.cprep,
-                       c->s.inc_key[0], c->s.inc_val[0], c->s.deg);
-    if (c->solver_colored) {
+                       c->s.inc_key[0], c->s.inc_val[0], c->s.deg, B<float>(c, SV_CONTACT_LAMBDA_N),
+                       c->warm_factor > 0.0f ? 1u : 0u);
+    const bool warm = c->warm_factor > 0.0f;
+    if (c->solver_colored && !warm) {
         if (!launch_solve_colored(c, n_bodies)) c->launch_err = hipErrorLaunchFailure;
         return;
     }
@@ -463,6 +469,11 @@ void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
     int where = radix_sort(c, c->s.inc_key[0], c->s.inc_val[0], c->s.inc_key[1], c->s.inc_val[1],
                            c->s.misc + MISC_NINC, 2u * mc, key_bits);
     scan_exclusive(c, c->s.deg, c->s.deg_off, nullptr, n_bodies, nullptr);
+    if (warm) launch_warm_apply(c, n_bodies, c->s.inc_val[where]);
+    if (c->solver_colored) {
+        if (!launch_solve_colored(c, n_bodies)) c->launch_err = hipErrorLaunchFailure;
+        return;
+    }
     uint32_t rgrid = div_up(2u * (uint64_t)mc, BLOCK);
     if (rgrid > 8192) rgrid = 8192;
     fill_u32(c, c->s.solve_ctl, 0u, 4);
@@ -505,6 +516,11 @@ void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
                            c->s.misc, iters, n_bodies, c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, vel, omega,
                            c->s.rot, inv_i, lam, c->s.solve_ctl, counters);
     }
+}
+
+void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
+    launch_sweep(c, n_bodies, dt);
+    if (c->warm_factor > 0.0f) launch_warm_save(c);
 }
 
 }  // namespace sv

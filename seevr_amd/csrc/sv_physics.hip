@@ -412,6 +412,10 @@ void launch_narrowphase(sv_ctx *c) {
 void launch_clear_lambda(sv_ctx *c) {
     uint32_t n = c->caps.max_contacts;
     if (!n) return;
+    if (c->warm_factor > 0.0f) {
+        launch_warm_lookup(c);
+        return;
+    }
     hipLaunchKernelGGL(k_clear_lambda, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, c->stream,
                        B<float>(c, SV_CONTACT_LAMBDA_N), n);
 }

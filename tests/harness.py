@@ -85,6 +85,12 @@ def oracle(double: bool = False) -> C.CDLL:
         "step_update_verts": (None, [B, u32, u32]),
         "step_clear_counters": (None, [B]),
         "physics_step": (None, [B, u32, u32, u32, u32, C.c_float, C.POINTER(C.c_float)]),
+        "step_solve_warm": (None, [B, R]),
+        "step_solve_in_order": (None, [B, R, C.c_int, vp]),
+        "warm_lookup": (None, [B, vp, vp, u32, R]),
+        "warm_save": (u32, [B, vp, vp]),
+        "physics_step_warm": (None, [B, u32, u32, u32, C.c_float, C.POINTER(C.c_float), R, vp, vp,
+                                     C.POINTER(u32)]),
         "support": (None, [B, u32, C.POINTER(R), C.POINTER(R)]),
         "mpr": (C.c_int, [B, u32, u32, C.POINTER(R), C.POINTER(R), C.POINTER(R)]),
         "expand_bits": (u32, [u32]),
@@ -174,6 +180,9 @@ class World:
         w = World.__new__(World)
         w.__dict__.update(self.__dict__)
         w.a = {k: v.copy() for k, v in self.a.items()}
+        for k in ("warm_keys", "warm_lam"):
+            if getattr(self, k, None) is not None:
+                setattr(w, k, getattr(self, k).copy())
         w._bufs = None
         return w
 
@@ -208,6 +217,8 @@ class World:
             b.slop = float(np.float32(0.005))
             b.lin_damping = float(np.float32(0.999))
             b.ang_damping = float(np.float32(0.998))
+        for k, v in getattr(self, "param_over", {}).items():
+            setattr(b, k, v)
         self._bufs = b
         return b
 
@@ -218,6 +229,22 @@ class World:
         b = self.bufs()
         gg = (C.c_float * 3)(*g)
         self.lib().physics_step(C.byref(b), self.n, self.nv, self.nh, substeps, C.c_float(dt), gg)
+
+    def oracle_step_warm(self, factor: float, substeps=10, dt=1.0 / 60.0, g=(0.0, -9.81, 0.0)):
+        """physics_step with warm start; the cache lives on the World."""
+        if getattr(self, "warm_keys", None) is None:
+            self.warm_reset()
+        b = self.bufs()
+        R = C.c_double if self.double else C.c_float
+        n = C.c_uint32(self.warm_n)
+        self.lib().physics_step_warm(C.byref(b), self.n, self.nv, substeps, C.c_float(dt), (C.c_float * 3)(*g),
+                                     R(factor), ptr(self.warm_keys), ptr(self.warm_lam), C.byref(n))
+        self.warm_n = n.value
+
+    def warm_reset(self):
+        self.warm_keys = np.zeros(max(self.max_contacts, 1), np.uint64)
+        self.warm_lam = np.zeros(max(self.max_contacts, 1), np.float64 if self.double else np.float32)
+        self.warm_n = 0
 
     def oracle_stage(self, step: int, count: int = None, dt: float = 0.0, g=(0.0, 0.0, 0.0)):
         b = self.bufs()
