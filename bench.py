@@ -310,7 +310,7 @@ def warm_start_section(ctx, n, nv, nh, factor: float = 1.0, iterations=(4, 10, 3
             ctx.enable_stage_timing(False)
             rms, mx, cnt = ctx.residual()
             row[mode] = {"residual_rms": rms, "residual_max": mx, "contacts": cnt,
-                         "solve_us": round(st.get("phys/pgs_solve", (0.0, 0))[0], 1)}
+                         "sweep_us": round(st.get("phys/pgs_sweep", (0.0, 0))[0], 1)}
         out[f"{k}_iterations"] = row
     ctx.set_warm_start(0.0)
     ctx.set_params(pgs_iterations=PGS_ITERS)
