@@ -211,6 +211,10 @@ int sv_debug_atan2(sv_ctx *ctx, const float *y, const float *x, float *out, uint
  * read + write bandwidth in GB/s. */
 int sv_debug_copy_bandwidth(sv_ctx *ctx, size_t nbytes, uint32_t iters, double *gbps);
 
+/* XCC (XCD) hardware register read by the first wave of each of `nblocks`
+ * one-wave blocks (a placement probe: which XCD runs which workgroup). */
+int sv_debug_xcc_ids(sv_ctx *ctx, uint32_t nblocks, uint32_t *out);
+
 /* Thread-local message of the last failure ("" if none). */
 const char *sv_last_error(void);
 /* Device name and gfx arch of the context's device. */

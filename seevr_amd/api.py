@@ -187,6 +187,12 @@ class Context:
         """Broadphase method: "auto", "all_pairs" or "lbvh" (identical pair lists)."""
         check(self.lib.sv_set_broadphase(self.h, self.BROADPHASES[method]))
 
+    def xcc_ids(self, nblocks: int) -> np.ndarray:
+        """XCD each of nblocks one-wave blocks ran on (placement probe)."""
+        out = np.zeros(nblocks, np.uint32)
+        check(self.lib.sv_debug_xcc_ids(self.h, nblocks, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out
+
     def set_warm_start(self, factor: float) -> None:
         """Warm start beside the reference path: 0 = off (the reference's
         cleared impulses); in (0, 1], contacts of a pair that was in contact

@@ -379,6 +379,19 @@ __global__ void __launch_bounds__(BLOCK) k_bb_refit_level(const uint32_t *__rest
 
 constexpr int BB_STACK = 64;  // > tree depth: node prefixes strictly grow downward and lie in [2, 63]
 
+// Workgroups go to the 8 XCDs round robin; block b is given the logical
+// block that makes each XCD's share one contiguous range (so each XCD's L2
+// holds the subtrees of one region of Morton order).  A bijection of
+// [0, gridDim.x).
+SVD uint32_t xcd_block(uint32_t b, uint32_t g) {
+    const uint32_t x = b & 7u, q = b >> 3, per = g >> 3, extra = g & 7u;
+    return x < extra ? x * (per + 1u) + q : extra * (per + 1u) + (x - extra) * per + q;
+}
+
+// The first BB_LDS stack entries of each lane live in LDS ([entry][lane],
+// conflict free); deeper ones in a private array.
+constexpr int BB_LDS = 16;
+
 // One lane per body j, in Morton order so neighbouring lanes walk similar
 // paths.  COUNT: bump the row count of every reported i.  EMIT: place the
 // pair in row i's slot range if the row starts below max_pairs.
@@ -390,12 +403,14 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
                                                     const uint32_t *__restrict__ off, uint32_t *fill,
                                                     uint32_t *__restrict__ pk, uint32_t *__restrict__ pv,
                                                     uint32_t max_pairs, int bits, bool combined) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t s_stack[BB_LDS * BLOCK];
+    uint32_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t j = ids[t];
     const float4 jl = box[2 * j], jh = box[2 * j + 1];
     const bool js = jl.w != 0.0f;
-    uint32_t stack[BB_STACK];
+    uint32_t *lstack = s_stack + threadIdx.x;
+    uint32_t stack[BB_STACK - BB_LDS];
     int sp = 0;
     uint32_t node = 0;
     for (;;) {
@@ -421,12 +436,15 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
                         }
                     }
                 }
+            } else if (sp < BB_LDS) {
+                lstack[sp++ * BLOCK] = c;
             } else if (sp < BB_STACK) {
-                stack[sp++] = c;
+                stack[sp++ - BB_LDS] = c;
             }
         }
         if (sp == 0) break;
-        node = stack[--sp];
+        --sp;
+        node = sp < BB_LDS ? lstack[sp * BLOCK] : stack[sp - BB_LDS];
     }
 }
 

@@ -225,6 +225,7 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
         c->solver_workgroups = v > 0 ? (uint32_t)v : 0u;
     }
     if (const char *f = getenv("SEEVR_SOLVER")) c->solver_flow = strcmp(f, "flow") == 0;
+    if (const char *x = getenv("SEEVR_SOLVER_COHERENT")) c->solver_coherent = atoi(x) != 0;
     if ((uint64_t)c->params.pgs_iterations * caps->max_contacts >= 0xFFFFFFFFull) {
         delete c;
         return fail(SV_E_INVALID, "sv_create: pgs_iterations * max_contacts must stay below 2^32");
@@ -551,6 +552,19 @@ int sv_debug_atan2(sv_ctx *c, const float *y, const float *x, float *out, uint32
 
 // Copy `nbytes` device-to-device `iters` times with k_copy_calib and report
 // the average bandwidth (read + write bytes per second).
+int sv_debug_xcc_ids(sv_ctx *c, uint32_t nblocks, uint32_t *out) {
+    if (!c || !out || nblocks == 0 || nblocks > 65536) return fail(SV_E_INVALID, "sv_debug_xcc_ids: bad argument");
+    uint32_t *d = nullptr;
+    hipError_t e = hipMalloc(&d, nblocks * sizeof(uint32_t));
+    if (e == hipSuccess) {
+        sv::launch_xcc_ids(c, d, nblocks);
+        e = hipMemcpyAsync(out, d, nblocks * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (d) (void)hipFree(d);
+    return e == hipSuccess ? SV_OK : fail(SV_E_DEVICE, "sv_debug_xcc_ids: %s", hipGetErrorString(e));
+}
+
 int sv_debug_copy_bandwidth(sv_ctx *c, size_t nbytes, uint32_t iters, double *gbps) {
     if (!c || !gbps || nbytes < 16 || iters == 0) return fail(SV_E_INVALID, "sv_debug_copy_bandwidth: bad argument");
     void *a = nullptr, *b = nullptr;

@@ -18,6 +18,8 @@
 // one lane sweeps in index order.  A stable radix sort by colour gives the
 // order; one cooperative launch then runs iterations x colours with a grid
 // barrier between colours.
+#include <type_traits>
+
 #include "sv_internal.h"
 #include "sv_math.h"
 #include "sv_dev.h"
@@ -115,17 +117,23 @@ __global__ void k_color_offsets(const uint32_t *__restrict__ cnt, uint32_t *__re
 
 // Grid-wide barrier for a cooperative launch: generation counter bar[1],
 // arrivals bar[0]; device-scope release before arriving, acquire after.
+// COH: the sweep's mutable words are accessed at the point of coherence
+// (CohMem), so the barrier only waits for every lane's stores to complete
+// and needs no L2 write-back or invalidation.
+template <bool COH>
 SVD bool grid_sync(uint32_t *bar, uint32_t *abort_flag, uint32_t *counters) {
+    if (COH) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     __shared__ uint32_t s_ok;
     if (threadIdx.x == 0) {
-        release_agent();
+        if (!COH) release_agent();
         const uint32_t gen = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t arrived = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
         uint32_t ok = 1u;
         if (arrived == gridDim.x) {
             __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            release_agent();
+            if (COH) __builtin_amdgcn_s_waitcnt(0);
+            else release_agent();
             __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const uint64_t t0 = wall_clock64();
@@ -143,7 +151,8 @@ SVD bool grid_sync(uint32_t *bar, uint32_t *abort_flag, uint32_t *counters) {
                 __builtin_amdgcn_s_sleep(1);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (COH) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         s_ok = ok;
     }
     __syncthreads();
@@ -174,6 +183,7 @@ SVD void load_constants(Item &it, const float *__restrict__ P, uint32_t c, const
 
 // iterations x colours; colour k = order[off[k] .. off[k+1]) (ascending
 // contact index); the remainder class is swept by one lane.
+template <bool COH>
 __global__ void __launch_bounds__(BLOCK) k_solve_colored(const uint32_t *__restrict__ order,
                                                          const uint32_t *__restrict__ off, uint32_t iters,
                                                          const float *__restrict__ prep, float *vel, float *omega,
@@ -191,9 +201,9 @@ __global__ void __launch_bounds__(BLOCK) k_solve_colored(const uint32_t *__restr
                 const uint32_t c = order[i];
                 Item it;
                 load_constants(it, prep + (size_t)c * CPREP_WORDS, c, rot, inv_i);
-                apply_item(it, vel, omega, lambda);
+                apply_item<typename std::conditional<COH, CohMem, PlainMem>::type>(it, vel, omega, lambda);
             }
-            if (!grid_sync(bar, abort_flag, counters)) return;
+            if (!grid_sync<COH>(bar, abort_flag, counters)) return;
         }
         const uint32_t r0 = off[COLOR_ROUNDS], r1 = off[COLOR_BINS];
         if (r1 > r0) {
@@ -202,9 +212,9 @@ __global__ void __launch_bounds__(BLOCK) k_solve_colored(const uint32_t *__restr
                     const uint32_t c = order[i];
                     Item it;
                     load_constants(it, prep + (size_t)c * CPREP_WORDS, c, rot, inv_i);
-                    apply_item(it, vel, omega, lambda);
+                    apply_item<typename std::conditional<COH, CohMem, PlainMem>::type>(it, vel, omega, lambda);
                 }
-            if (!grid_sync(bar, abort_flag, counters)) return;
+            if (!grid_sync<COH>(bar, abort_flag, counters)) return;
         }
     }
 }
@@ -276,9 +286,10 @@ bool launch_solve_colored(sv_ctx *c, uint32_t n_bodies) {
         coop = hipDeviceGetAttribute(&v, hipDeviceAttributeCooperativeLaunch, c->device) == hipSuccess && v ? 1 : 0;
     }
     if (!coop) return false;
+    const void *kern = c->solver_coherent ? reinterpret_cast<const void *>(k_solve_colored<true>)
+                                          : reinterpret_cast<const void *>(k_solve_colored<false>);
     int occ = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(k_solve_colored), BLOCK, 0) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BLOCK, 0) != hipSuccess ||
         occ < 1 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
         return false;
     hipStream_t st = c->stream;
@@ -318,8 +329,7 @@ bool launch_solve_colored(sv_ctx *c, uint32_t n_bodies) {
     if (cgrid < 1) cgrid = 1;
     if (cgrid > maxg) cgrid = maxg;
     StageTimer sweep_timer(c, "phys/pgs_sweep");
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_solve_colored), dim3(cgrid), dim3(BLOCK), args,
-                                      0, st) == hipSuccess;
+    return hipLaunchCooperativeKernel(kern, dim3(cgrid), dim3(BLOCK), args, 0, st) == hipSuccess;
 }
 
 int color_counts(sv_ctx *c, uint32_t *out, uint32_t cap) {

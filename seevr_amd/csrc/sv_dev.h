@@ -400,20 +400,39 @@ struct Item {
     bool dyn_a, dyn_b;
 };
 
+// Access to the words a sweep item reads and writes (velocities, angular
+// velocities, impulses).  PlainMem: ordinary loads and stores (LDS or one
+// CU, or after cache maintenance).  CohMem: device-scope relaxed atomics,
+// which read and write at the device's point of coherence, so items on
+// different XCDs hand these words over without L2 write-back or invalidation.
+struct PlainMem {
+    static SVD float ld(const float *p) { return *p; }
+    static SVD void st(float *p, float v) { *p = v; }
+};
+struct CohMem {
+    static SVD float ld(const float *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+    static SVD void st(float *p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+};
+template <class M>
+SVD f3 mld3(const float *p, uint32_t i) { return mk(M::ld(p + i * 3), M::ld(p + i * 3 + 1), M::ld(p + i * 3 + 2)); }
+template <class M>
+SVD void mst3(float *p, uint32_t i, f3 v) { M::st(p + i * 3, v.x); M::st(p + i * 3 + 1, v.y); M::st(p + i * 3 + 2, v.z); }
+
 // physics.comp:525-580 for one contact: normal impulse with projection,
 // then Coulomb friction with the geometric-mean coefficient.
+template <class M = PlainMem>
 SVD void apply_item(const Item &it, float *__restrict__ vel, float *__restrict__ omega,
                     float *__restrict__ lambda) {
-    f3 la = ld3(vel, it.a), wa = ld3(omega, it.a);
-    f3 lb = ld3(vel, it.b), wb = ld3(omega, it.b);
-    float old = lambda[it.c];
+    f3 la = mld3<M>(vel, it.a), wa = mld3<M>(omega, it.a);
+    f3 lb = mld3<M>(vel, it.b), wb = mld3<M>(omega, it.b);
+    float old = M::ld(lambda + it.c);
     f3 va = la + cross3(wa, it.ra);
     f3 vb = lb + cross3(wb, it.rb);
     float vn = dot3(vb - va, it.n);
     float dln = (-vn + it.bias) / it.kn;
     float nl = gmax(old + dln, 0.0f);
     dln = nl - old;
-    lambda[it.c] = nl;
+    M::st(lambda + it.c, nl);
     f3 imp = dln * it.n;
     la = la - it.ima * imp;
     lb = lb + it.imb * imp;
@@ -437,8 +456,8 @@ SVD void apply_item(const Item &it, float *__restrict__ vel, float *__restrict__
         wa = wa + apply_inv_I(it.Ra, it.iia, cross3(it.ra, impt));
         wb = wb - apply_inv_I(it.Rb, it.iib, cross3(it.rb, impt));
     }
-    if (it.dyn_a) { st3(vel, it.a, la); st3(omega, it.a, wa); }
-    if (it.dyn_b) { st3(vel, it.b, lb); st3(omega, it.b, wb); }
+    if (it.dyn_a) { mst3<M>(vel, it.a, la); mst3<M>(omega, it.a, wa); }
+    if (it.dyn_b) { mst3<M>(vel, it.b, lb); mst3<M>(omega, it.b, wb); }
 }
 
 }  // namespace sv

@@ -99,6 +99,10 @@ struct sv_ctx {
     // graph-coloured sweep (a different order than the reference's: judged by
     // the constraint residual, not by parity)
     bool solver_colored = false;
+    // multi-CU ordered and coloured sweeps: hand-offs through device-coherent
+    // accesses (no L2 write-back / invalidation); false = plain accesses with
+    // device-scope release/acquire fences
+    bool solver_coherent = true;
     // launch failure to report at the next synchronization
     hipError_t launch_err = hipSuccess;
     // broadphase: 0 = chosen by body count, 1 = all pairs (tiled N^2), 2 = LBVH traversal
@@ -162,6 +166,7 @@ void launch_clear_counters(sv_ctx *c);
 void launch_debug_atan2(sv_ctx *c, const float *y, const float *x, float *out, uint32_t n);
 void launch_rot(sv_ctx *c, uint32_t n);
 void launch_copy_calib(sv_ctx *c, const void *src, void *dst, size_t nbytes);
+void launch_xcc_ids(sv_ctx *c, uint32_t *out, uint32_t n);
 void launch_pack_boxes(sv_ctx *c, uint32_t n);
 
 // ---- sv_pgs.hip ----

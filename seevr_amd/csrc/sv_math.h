@@ -12,6 +12,9 @@
 
 #define SVD __device__ __forceinline__
 
+// XCC (XCD) the wave runs on: hardware register HW_REG_XCC_ID, bits 0-15.
+#define xcc_id() ((uint32_t)__builtin_amdgcn_s_getreg((15 << 11) | 20))
+
 struct f3 {
     float x, y, z;
 };

@@ -26,8 +26,14 @@
 //
 // Forms: one workgroup of 1024 lanes with tickets in LDS (the hand-off stays
 // inside one CU: workgroup-scope release/acquire), the same with tickets in
-// global memory when they do not fit in LDS, and a multi-workgroup form
-// whose hand-offs use device-scope release/acquire.
+// global memory when they do not fit in LDS, and multi-workgroup forms.  In
+// the default multi-workgroup form (FORM_COH) the velocities, angular
+// velocities and impulses are read and written with device-scope relaxed
+// atomics, which go to the device's point of coherence, and a hand-off is a
+// wait for the item's stores followed by the ticket bump; the other
+// (FORM_MULTI) uses plain accesses with device-scope release/acquire fences,
+// which write back and invalidate the XCD's L2 on every hand-off (1.65x
+// slower on a 32k-body pile).
 #include "sv_internal.h"
 #include "sv_math.h"
 #include "sv_dev.h"
@@ -126,8 +132,14 @@ SVD bool load_item(Item &it, uint32_t q, uint32_t nc, const float *__restrict__ 
     return true;
 }
 
-// Ticket storage and hand-off scope of each solver form.
-enum : int { FORM_LDS = 0, FORM_GLOBAL = 1, FORM_MULTI = 2 };
+// Ticket storage and hand-off scope of each solver form.  FORM_COH is the
+// multi-workgroup form with the items' mutable words accessed at the
+// device's point of coherence (CohMem): a hand-off is then the stores, a
+// wait for their completion and the ticket bump, and the taker's ticket
+// read and loads, with no L2 write-back (release) or L2 invalidation
+// (acquire), which FORM_MULTI pays on every hand-off and which serialise
+// on each XCD's L2.
+enum : int { FORM_LDS = 0, FORM_GLOBAL = 1, FORM_MULTI = 2, FORM_COH = 3 };
 
 template <int FORM>
 SVD uint32_t ticket_load(const uint32_t *t) {
@@ -144,10 +156,17 @@ SVD void fence_acquire() {
     if (FORM == FORM_MULTI) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
+// FORM_COH: the item's loads are issued only after the ticket reads they
+// depend on returned (the branch on them); the workgroup fence keeps the
+// compiler from moving them, and the release waits for every store of the
+// item to complete at the point of coherence before the ticket bump.
 template <int FORM>
 SVD void fence_release() {
     if (FORM == FORM_MULTI) release_agent();
-    else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    else if (FORM == FORM_COH) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+    } else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 }
 
 template <int FORM, int THREADS>
@@ -161,10 +180,11 @@ __global__ void __launch_bounds__(THREADS) k_solve(
     const uint32_t nc = misc[MISC_NC];
     const uint32_t total = iters * nc;
     const int lane = threadIdx.x & 63;
+    constexpr bool MULTI = FORM == FORM_MULTI || FORM == FORM_COH;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t *ticket = gticket;
     uint32_t *next = ctl + 1;
-    if (FORM != FORM_MULTI) {
+    if (!MULTI) {
         if (FORM == FORM_LDS) {
             for (uint32_t i = threadIdx.x; i < n_bodies; i += THREADS) s_ticket[i] = 0;
             ticket = s_ticket;
@@ -186,7 +206,7 @@ __global__ void __launch_bounds__(THREADS) k_solve(
             int leader = __builtin_ctzll(need);
             uint32_t base = 0;
             if (lane == leader) {
-                if (FORM == FORM_MULTI)
+                if (MULTI)
                     base = __hip_atomic_fetch_add(next, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 else
                     base = __hip_atomic_fetch_add(next, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -206,7 +226,8 @@ __global__ void __launch_bounds__(THREADS) k_solve(
                          (!it.dyn_b || ticket_load<FORM>(ticket + it.b) == it.tb);
             if (ready) {
                 fence_acquire<FORM>();
-                apply_item(it, vel, omega, lambda);
+                if (FORM == FORM_COH) apply_item<CohMem>(it, vel, omega, lambda);
+                else apply_item<PlainMem>(it, vel, omega, lambda);
                 fence_release<FORM>();
                 if (it.dyn_a) ticket_bump<FORM>(ticket + it.a);
                 if (it.dyn_b) ticket_bump<FORM>(ticket + it.b);
@@ -497,15 +518,23 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
     hipLaunchKernelGGL(k_rank, dim3(rgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.inc_key[where],
                        c->s.inc_val[where], c->s.deg_off, c->s.rank);
     // Form choice: small scenes keep every hand-off inside one CU; large ones
-    // need the wider in-order window of the multi-CU form (32 workgroups was
-    // fastest on a 32k-body pile, 2M sweep items per substep).
-    uint32_t wgs = c->solver_workgroups ? c->solver_workgroups : (n_bodies >= 2048 ? 32u : 1u);
+    // need the wider in-order window of the multi-CU form (with coherent
+    // hand-offs 64 workgroups was fastest on a 32k-body pile, 2M sweep items
+    // per substep: 2.96 ms against 3.21 ms at 32 and 3.89 ms at 128).
+    uint32_t wgs = c->solver_workgroups ? c->solver_workgroups : (n_bodies >= 2048 ? 64u : 1u);
     StageTimer sweep_timer(c, "phys/pgs_sweep");
     if (wgs > 1) {
         fill_u32(c, c->s.ticket, 0u, n_bodies);
-        hipLaunchKernelGGL((k_solve<FORM_MULTI, MC_THREADS>), dim3(wgs), dim3(MC_THREADS), 0,
-                           c->stream, c->s.misc, iters, n_bodies, c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, vel,
-                           omega, c->s.rot, inv_i, lam, c->s.solve_ctl, counters);
+        // (all working workgroups on one XCD was 4.5x slower with L2
+        // maintenance hand-offs: the XCD's L2 serialises them)
+        if (c->solver_coherent)
+            hipLaunchKernelGGL((k_solve<FORM_COH, MC_THREADS>), dim3(wgs), dim3(MC_THREADS), 0, c->stream, c->s.misc,
+                               iters, n_bodies, c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, vel, omega, c->s.rot,
+                               inv_i, lam, c->s.solve_ctl, counters);
+        else
+            hipLaunchKernelGGL((k_solve<FORM_MULTI, MC_THREADS>), dim3(wgs), dim3(MC_THREADS), 0, c->stream,
+                               c->s.misc, iters, n_bodies, c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, vel, omega,
+                               c->s.rot, inv_i, lam, c->s.solve_ctl, counters);
     } else if (n_bodies <= c->solver_lds_bodies) {
         hipLaunchKernelGGL((k_solve<FORM_LDS, SOLVE_THREADS>), dim3(1), dim3(SOLVE_THREADS), n_bodies * 4u,
                            c->stream, c->s.misc, iters, n_bodies, c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, vel,

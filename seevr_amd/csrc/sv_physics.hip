@@ -334,6 +334,11 @@ __global__ void __launch_bounds__(BLOCK) k_copy_calib(const float4 *__restrict__
     for (; i < n; i += stride) dst[i] = src[i];
 }
 
+// XCC register of the first wave of each block (placement probe)
+__global__ void k_xcc_ids(uint32_t *out, uint32_t n) {
+    if (threadIdx.x == 0 && blockIdx.x < n) out[blockIdx.x] = xcc_id();
+}
+
 __global__ void k_atan2(const float *y, const float *x, float *out, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = sv_atan2f(y[i], x[i]);
@@ -457,6 +462,11 @@ void launch_copy_calib(sv_ctx *c, const void *src, void *dst, size_t nbytes) {
     uint32_t grid = div_up(n, BLOCK);
     if (grid > 16384) grid = 16384;
     hipLaunchKernelGGL(k_copy_calib, dim3(grid), dim3(BLOCK), 0, c->stream, (const float4 *)src, (float4 *)dst, n);
+}
+
+void launch_xcc_ids(sv_ctx *c, uint32_t *out, uint32_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_xcc_ids, dim3(n), dim3(64), 0, c->stream, out, n);
 }
 
 void launch_debug_atan2(sv_ctx *c, const float *y, const float *x, float *out, uint32_t n) {
