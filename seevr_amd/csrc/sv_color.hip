@@ -286,8 +286,13 @@ bool launch_solve_colored(sv_ctx *c, uint32_t n_bodies) {
         coop = hipDeviceGetAttribute(&v, hipDeviceAttributeCooperativeLaunch, c->device) == hipSuccess && v ? 1 : 0;
     }
     if (!coop) return false;
-    const void *kern = c->solver_coherent ? reinterpret_cast<const void *>(k_solve_colored<true>)
-                                          : reinterpret_cast<const void *>(k_solve_colored<false>);
+    // coherent accesses pay off when barriers dominate (1M-body scene, 6.3M
+    // contact slots: 6.05 ms against 7.39 ms per sweep) but not on smaller
+    // lists, where the plain form's L2 hits win (32k-body pile, 197k slots:
+    // 4.87 ms against 6.46 ms)
+    const bool coh = c->solver_coherent && c->caps.max_contacts >= (1u << 20);
+    const void *kern = coh ? reinterpret_cast<const void *>(k_solve_colored<true>)
+                           : reinterpret_cast<const void *>(k_solve_colored<false>);
     int occ = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BLOCK, 0) != hipSuccess ||
         occ < 1 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)

@@ -34,6 +34,8 @@
 // (FORM_MULTI) uses plain accesses with device-scope release/acquire fences,
 // which write back and invalidate the XCD's L2 on every hand-off (1.65x
 // slower on a 32k-body pile).
+#include <type_traits>
+
 #include "sv_internal.h"
 #include "sv_math.h"
 #include "sv_dev.h"
@@ -317,9 +319,27 @@ __global__ void __launch_bounds__(BLOCK) k_flow_roots(const uint32_t *__restrict
     }
 }
 
+// Hand-off fences of the flow form: COH = mutable words at the point of
+// coherence (CohMem), else device-scope release/acquire.
+template <bool COH>
+SVD void handoff_acquire() {
+    if (COH) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+template <bool COH>
+SVD void handoff_release() {
+    if (COH) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+    } else {
+        release_agent();
+    }
+}
+
 // ctl: [0] abort flag, [1] queue head (slots claimed), [2] queue tail
 // (slots pushed), [3] items done.  The queue never holds more than the
 // iters * nact items of the sweep, and span = iters * nc bounds every slot.
+template <bool COH>
 __global__ void __launch_bounds__(MC_THREADS) k_flow(
     const uint32_t *__restrict__ misc, uint32_t iters, const float *__restrict__ prep, float *vel, float *omega,
     const float *__restrict__ rot, const float *__restrict__ inv_i, float *lambda, uint32_t *count, uint32_t *queue,
@@ -354,7 +374,7 @@ __global__ void __launch_bounds__(MC_THREADS) k_flow(
         if (!has && !fin) {
             uint32_t v = __hip_atomic_load(queue + qpos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (v) {
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                handoff_acquire<COH>();
                 id = v - 1u;
                 has = true;
                 got = true;
@@ -384,7 +404,7 @@ __global__ void __launch_bounds__(MC_THREADS) k_flow(
             it.iia = ld3(inv_i, it.a);
             it.iib = ld3(inv_i, it.b);
             uint32_t sa = __float_as_uint(P[P_SUCC_A]), sb = __float_as_uint(P[P_SUCC_B]);
-            apply_item(it, vel, omega, lambda);
+            apply_item<typename std::conditional<COH, CohMem, PlainMem>::type>(it, vel, omega, lambda);
             ran = true;
             uint32_t ida = SUCC_NONE, idb = SUCC_NONE;
             if (it.dyn_a) {
@@ -396,13 +416,13 @@ __global__ void __launch_bounds__(MC_THREADS) k_flow(
                 if (ts < iters) idb = ts * nc + (sb & SUCC_ID);
             }
             // publish this update, then count it down in both successors
-            release_agent();
+            handoff_release<COH>();
             bool ra = false, rb = false;
             if (ida != SUCC_NONE)
                 ra = __hip_atomic_fetch_sub(count + ida, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u;
             if (idb != SUCC_NONE)
                 rb = __hip_atomic_fetch_sub(count + idb, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u;
-            if (ra || rb) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (ra || rb) handoff_acquire<COH>();
             has = ra || rb;
             id = ra ? ida : idb;
             const bool push = ra && rb;  // both ready: run a, queue b
@@ -415,7 +435,7 @@ __global__ void __launch_bounds__(MC_THREADS) k_flow(
                                                   __HIP_MEMORY_SCOPE_AGENT);
                 base = __shfl(base, leader, 64);
                 if (push) {
-                    release_agent();
+                    handoff_release<COH>();
                     __hip_atomic_store(queue + base + (uint32_t)__popcll(pm & lt), idb + 1u, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                 }
@@ -511,7 +531,8 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
                            c->s.flow_queue, c->s.solve_ctl);
         uint32_t wgs = c->solver_workgroups ? c->solver_workgroups : 64u;
         StageTimer sweep_timer(c, "phys/pgs_sweep");
-        hipLaunchKernelGGL(k_flow, dim3(wgs), dim3(MC_THREADS), 0, c->stream, c->s.misc, iters, c->s.cprep, vel,
+        hipLaunchKernelGGL(c->solver_coherent ? k_flow<true> : k_flow<false>, dim3(wgs), dim3(MC_THREADS), 0, c->stream,
+                           c->s.misc, iters, c->s.cprep, vel,
                            omega, c->s.rot, inv_i, lam, c->s.flow_count, c->s.flow_queue, c->s.solve_ctl, counters);
         return;
     }
