@@ -47,7 +47,7 @@ def build_workload(count: int, seed: int):
 
 
 PROFILE = ROOT / "profiles" / "c2_32k_container_kernels.md"
-STAGE_KERNEL = {"phys/pgs_sweep": "k_solve<2, 256>", "phys/aabb": "k_aabb", "phys/gravity": "k_gravity",
+STAGE_KERNEL = {"phys/pgs_sweep": "k_solve_seq<256>", "phys/aabb": "k_aabb", "phys/gravity": "k_gravity",
                 "phys/integrate": "k_integrate", "phys/broad_phase": "k_broadphase<false>"}
 
 
@@ -274,6 +274,42 @@ def differentiable_section(frames: int = 60, scenes: int = 256, bodies: int = 64
                     "+ U[-0.5, 0.5]"}
 
 
+def sweep_variant_section(ctx, n) -> dict:
+    """The multi-workgroup sweep's hand-off variants from the same state (the
+    pile where the timed frames left it): one substep's collision stages,
+    then the sweep timed with HIP events, for each variant; all must leave
+    bit-identical velocities."""
+    state = {k: ctx.download(k, w * n) for k, w in (("BODY_POS", 3), ("BODY_QUAT", 4), ("BODY_VEL", 3),
+                                                      ("BODY_OMEGA", 3))}
+    dt_sub = float(np.float32(DT) / np.float32(SUBSTEPS))
+    out, ref = {}, None
+    for variant in ("seq", "coherent", "fenced"):
+        for key, v in state.items():
+            ctx.upload(key, v)
+        ctx.set_sweep_variant(variant)
+        ctx.dispatch(8, 1)
+        ctx.dispatch(0, n, dt_sub, GRAVITY)
+        ctx.dispatch(9, n)
+        ctx.dispatch(2, n)
+        ctx.dispatch(3, 0, num_bodies=n)
+        ctx.dispatch(4, 0)
+        ctx.enable_stage_timing(True)
+        ctx.stage_times(reset=True)
+        ctx.dispatch(5, n, dt_sub)
+        st = ctx.stage_times(reset=True)
+        ctx.enable_stage_timing(False)
+        vel = np.concatenate([ctx.download("BODY_VEL", 3 * n), ctx.download("BODY_OMEGA", 3 * n)])
+        ref = vel if ref is None else ref
+        out[variant] = {"sweep_us": round(st.get("phys/pgs_sweep", (0.0, 0))[0], 1),
+                        "bit_identical": bool(np.array_equal(vel.view(np.uint32), ref.view(np.uint32)))}
+    ctx.set_sweep_variant("seq")
+    for key, v in state.items():
+        ctx.upload(key, v)
+    log("sweep hand-off variants from the same state: " +
+        ", ".join(f"{k} {v['sweep_us']:.0f} us" for k, v in out.items()))
+    return out
+
+
 def warm_start_section(ctx, n, nv, nh, factor: float = 1.0, iterations=(4, 10, 30)) -> dict:
     """Warm start beside the reference path, on the pile where the timed
     frames left it: one frame with warm start from that state fills the
@@ -484,6 +520,8 @@ def main() -> int:
     total_us = sum(v[0] * v[1] for k, v in stages.items() if k != "phys/pgs_sweep")
     breakdown = {k: round(100.0 * v[0] * v[1] / total_us, 1) for k, v in stages.items() if k != "phys/pgs_sweep"}
 
+    variants = sweep_variant_section(ctx, n)
+
     cpu = None
     if not args.no_cpu:
         log("timing the CPU oracle baseline ...")
@@ -516,7 +554,7 @@ def main() -> int:
                    "frames_before_timing": args.warmup, "pairs_last_substep": pairs,
                    "contacts_last_substep": contacts},
         "roofline": roof, "roofline_hbm_stages": hbm_stages, "cpu_baseline": cpu, "stage_share_pct": breakdown,
-        "warm_start": warm, "batched": batched, "differentiable": diff, "large_scene": large,
+        "sweep_variants": variants, "warm_start": warm, "batched": batched, "differentiable": diff, "large_scene": large,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -215,6 +215,13 @@ int sv_debug_copy_bandwidth(sv_ctx *ctx, size_t nbytes, uint32_t iters, double *
  * one-wave blocks (a placement probe: which XCD runs which workgroup). */
 int sv_debug_xcc_ids(sv_ctx *ctx, uint32_t nblocks, uint32_t *out);
 
+/* Hand-off variant of the multi-workgroup sweeps (all give the same result):
+ * 0 = sequence words (default: each body's velocities and update count in
+ * one 16-byte word, one round trip per hand-off), 1 = tickets with
+ * device-coherent accesses, 2 = tickets with L2 write-back / invalidation
+ * fences.  For tests and measurements. */
+int sv_debug_set_sweep_variant(sv_ctx *ctx, int variant);
+
 /* Thread-local message of the last failure ("" if none). */
 const char *sv_last_error(void);
 /* Device name and gfx arch of the context's device. */

@@ -187,6 +187,12 @@ class Context:
         """Broadphase method: "auto", "all_pairs" or "lbvh" (identical pair lists)."""
         check(self.lib.sv_set_broadphase(self.h, self.BROADPHASES[method]))
 
+    SWEEP_VARIANTS = {"seq": 0, "coherent": 1, "fenced": 2}
+
+    def set_sweep_variant(self, variant: str) -> None:
+        """Hand-off variant of the multi-workgroup sweeps (same results)."""
+        check(self.lib.sv_debug_set_sweep_variant(self.h, self.SWEEP_VARIANTS[variant]))
+
     def xcc_ids(self, nblocks: int) -> np.ndarray:
         """XCD each of nblocks one-wave blocks ran on (placement probe)."""
         out = np.zeros(nblocks, np.uint32)

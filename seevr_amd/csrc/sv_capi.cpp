@@ -225,7 +225,6 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
         c->solver_workgroups = v > 0 ? (uint32_t)v : 0u;
     }
     if (const char *f = getenv("SEEVR_SOLVER")) c->solver_flow = strcmp(f, "flow") == 0;
-    if (const char *x = getenv("SEEVR_SOLVER_COHERENT")) c->solver_coherent = atoi(x) != 0;
     if ((uint64_t)c->params.pgs_iterations * caps->max_contacts >= 0xFFFFFFFFull) {
         delete c;
         return fail(SV_E_INVALID, "sv_create: pgs_iterations * max_contacts must stay below 2^32");
@@ -506,6 +505,14 @@ int sv_set_warm_start(sv_ctx *c, float factor) {
     if (!(factor >= 0.0f && factor <= 1.0f)) return fail(SV_E_INVALID, "sv_set_warm_start: factor must be in [0, 1]");
     if (factor > 0.0f && !sv::warm_reserve(c)) return fail(SV_E_NOMEM, "sv_set_warm_start: cache allocation failed");
     c->warm_factor = factor;
+    return SV_OK;
+}
+
+int sv_debug_set_sweep_variant(sv_ctx *c, int variant) {
+    if (!c) return fail(SV_E_INVALID, "sv_debug_set_sweep_variant: null context");
+    if (variant < sv::SWEEP_SEQ || variant > sv::SWEEP_FENCED)
+        return fail(SV_E_INVALID, "sv_debug_set_sweep_variant: unknown variant %d", variant);
+    c->sweep_variant = variant;
     return SV_OK;
 }
 

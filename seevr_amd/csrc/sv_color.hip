@@ -290,7 +290,7 @@ bool launch_solve_colored(sv_ctx *c, uint32_t n_bodies) {
     // contact slots: 6.05 ms against 7.39 ms per sweep) but not on smaller
     // lists, where the plain form's L2 hits win (32k-body pile, 197k slots:
     // 4.87 ms against 6.46 ms)
-    const bool coh = c->solver_coherent && c->caps.max_contacts >= (1u << 20);
+    const bool coh = c->sweep_variant != SWEEP_FENCED && c->caps.max_contacts >= (1u << 20);
     const void *kern = coh ? reinterpret_cast<const void *>(k_solve_colored<true>)
                            : reinterpret_cast<const void *>(k_solve_colored<false>);
     int occ = 0, cus = 0;

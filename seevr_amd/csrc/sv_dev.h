@@ -418,21 +418,16 @@ SVD f3 mld3(const float *p, uint32_t i) { return mk(M::ld(p + i * 3), M::ld(p + 
 template <class M>
 SVD void mst3(float *p, uint32_t i, f3 v) { M::st(p + i * 3, v.x); M::st(p + i * 3 + 1, v.y); M::st(p + i * 3 + 2, v.z); }
 
-// physics.comp:525-580 for one contact: normal impulse with projection,
-// then Coulomb friction with the geometric-mean coefficient.
-template <class M = PlainMem>
-SVD void apply_item(const Item &it, float *__restrict__ vel, float *__restrict__ omega,
-                    float *__restrict__ lambda) {
-    f3 la = mld3<M>(vel, it.a), wa = mld3<M>(omega, it.a);
-    f3 lb = mld3<M>(vel, it.b), wb = mld3<M>(omega, it.b);
-    float old = M::ld(lambda + it.c);
+// physics.comp:525-580 for one contact on the values of its two bodies and
+// its impulse: normal impulse with projection, then Coulomb friction with
+// the geometric-mean coefficient.  Returns the new impulse.
+SVD float item_update(const Item &it, f3 &la, f3 &wa, f3 &lb, f3 &wb, float old) {
     f3 va = la + cross3(wa, it.ra);
     f3 vb = lb + cross3(wb, it.rb);
     float vn = dot3(vb - va, it.n);
     float dln = (-vn + it.bias) / it.kn;
     float nl = gmax(old + dln, 0.0f);
     dln = nl - old;
-    M::st(lambda + it.c, nl);
     f3 imp = dln * it.n;
     la = la - it.ima * imp;
     lb = lb + it.imb * imp;
@@ -456,6 +451,17 @@ SVD void apply_item(const Item &it, float *__restrict__ vel, float *__restrict__
         wa = wa + apply_inv_I(it.Ra, it.iia, cross3(it.ra, impt));
         wb = wb - apply_inv_I(it.Rb, it.iib, cross3(it.rb, impt));
     }
+    return nl;
+}
+
+// One sweep item through memory: load, update, store.
+template <class M = PlainMem>
+SVD void apply_item(const Item &it, float *__restrict__ vel, float *__restrict__ omega,
+                    float *__restrict__ lambda) {
+    f3 la = mld3<M>(vel, it.a), wa = mld3<M>(omega, it.a);
+    f3 lb = mld3<M>(vel, it.b), wb = mld3<M>(omega, it.b);
+    float nl = item_update(it, la, wa, lb, wb, M::ld(lambda + it.c));
+    M::st(lambda + it.c, nl);
     if (it.dyn_a) { mst3<M>(vel, it.a, la); mst3<M>(omega, it.a, wa); }
     if (it.dyn_b) { mst3<M>(vel, it.b, lb); mst3<M>(omega, it.b, wb); }
 }

@@ -57,12 +57,19 @@ struct Scratch {
     unsigned long long *col_claim = nullptr;  // [N] colour-round claims per body
     uint32_t *col_color = nullptr;            // [C] colour of each contact
     uint32_t *col_cnt = nullptr, *col_off = nullptr, *col_bar = nullptr, *res_acc = nullptr;
+    // sequence-form sweep words: (v, count), (w, count) per body and
+    // (lambda, iteration) per contact
+    float4 *seq_v = nullptr;  // [2N]
+    uint2 *seq_c = nullptr;   // [C]
     // warm start: (geom a, geom b) keys and impulses of the last sweep
     unsigned long long *warm_key = nullptr;  // [C]
     float *warm_lam = nullptr;               // [C]
     // every lazily allocated block, freed with the context
     std::vector<void *> lazy;
 };
+
+// Hand-off variants of the multi-CU sweeps (sv_debug_set_sweep_variant).
+enum { SWEEP_SEQ = 0, SWEEP_COHERENT = 1, SWEEP_FENCED = 2 };
 
 // Words per contact in Scratch::cprep (see sv_pgs.hip).
 constexpr int CPREP_WORDS = 20;
@@ -99,10 +106,10 @@ struct sv_ctx {
     // graph-coloured sweep (a different order than the reference's: judged by
     // the constraint residual, not by parity)
     bool solver_colored = false;
-    // multi-CU ordered and coloured sweeps: hand-offs through device-coherent
-    // accesses (no L2 write-back / invalidation); false = plain accesses with
-    // device-scope release/acquire fences
-    bool solver_coherent = true;
+    // hand-off variant of the multi-CU sweeps (SV_SWEEP_*): sequence words
+    // (state and update count in one word), coherent tickets, or tickets with
+    // L2 write-back / invalidation fences
+    int sweep_variant = sv::SWEEP_SEQ;
     // launch failure to report at the next synchronization
     hipError_t launch_err = hipSuccess;
     // broadphase: 0 = chosen by body count, 1 = all pairs (tiled N^2), 2 = LBVH traversal

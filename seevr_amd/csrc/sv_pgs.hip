@@ -252,6 +252,155 @@ __global__ void __launch_bounds__(THREADS) k_solve(
 }
 
 // ---------------------------------------------------------------------------
+// Sequence form: the multi-workgroup ordered sweep with each body's state
+// and its update count in the same 16-byte words, so a hand-off is a single
+// round trip.  Body x has two words at the device's point of coherence,
+// (v, k) and (w, k), k = updates of x completed; contact c has (lambda, t),
+// t = iterations of c completed.  Item (t, c) polls its bodies' four words
+// and its contact's word with device-coherent loads and runs when every
+// count it depends on matches (k_a = t deg(a) + rank_a(c), the same for b,
+// and t): the values it read are then those the predecessor wrote, because
+// a 16-byte (8-byte) aligned word is written and read whole, and no later
+// update of those words can happen before this item's.  It writes the new
+// words with device-coherent stores and goes on: there is no separate
+// ticket and no wait for store completion.  The sweep is identical to the
+// serial one, item for item, like the ticket forms.
+
+// raw buffer resource over a device array (offsets in bytes, out-of-range
+// loads return 0 and stores are dropped); device-coherent cache policy SC1
+constexpr int CPOL_SC1 = 16;
+SVD __amdgpu_buffer_rsrc_t seq_rsrc(const void *p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, 0x7FFFFFFF, 0x00020000);
+}
+
+// words of the sequence form from the reference buffers
+__global__ void __launch_bounds__(BLOCK) k_seq_pack(const uint32_t *__restrict__ misc, uint32_t n_bodies,
+                                                    const float *__restrict__ vel, const float *__restrict__ omega,
+                                                    const float *__restrict__ lambda, float4 *__restrict__ sv,
+                                                    uint2 *__restrict__ sc) {
+    const uint32_t nc = misc[MISC_NC];
+    const uint32_t m = n_bodies > nc ? n_bodies : nc;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        if (i < n_bodies) {
+            sv[2 * i] = make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0f);
+            sv[2 * i + 1] = make_float4(omega[3 * i], omega[3 * i + 1], omega[3 * i + 2], 0.0f);
+        }
+        if (i < nc) sc[i] = make_uint2(__float_as_uint(lambda[i]), 0u);
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_seq_unpack(const uint32_t *__restrict__ misc, uint32_t n_bodies,
+                                                      const float4 *__restrict__ sv, const uint2 *__restrict__ sc,
+                                                      float *__restrict__ vel, float *__restrict__ omega,
+                                                      float *__restrict__ lambda) {
+    const uint32_t nc = misc[MISC_NC];
+    const uint32_t m = n_bodies > nc ? n_bodies : nc;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        if (i < n_bodies) {
+            const float4 v = sv[2 * i], w = sv[2 * i + 1];
+            vel[3 * i] = v.x;
+            vel[3 * i + 1] = v.y;
+            vel[3 * i + 2] = v.z;
+            omega[3 * i] = w.x;
+            omega[3 * i + 1] = w.y;
+            omega[3 * i + 2] = w.z;
+        }
+        if (i < nc) lambda[i] = __uint_as_float(sc[i].x);
+    }
+}
+
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS) k_solve_seq(
+    const uint32_t *__restrict__ misc, uint32_t iters, const float *__restrict__ prep,
+    const uint32_t *__restrict__ rank, const uint32_t *__restrict__ deg, float4 *sv, uint2 *sc,
+    const float *__restrict__ rot, const float *__restrict__ inv_i, uint32_t *ctl, uint32_t *counters) {
+    const uint32_t nc = misc[MISC_NC];
+    const uint32_t total = iters * nc;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (total == 0) return;
+    const __amdgpu_buffer_rsrc_t rv = seq_rsrc(sv), rc = seq_rsrc(sc);
+
+    Item it;
+    uint32_t t = 0;
+    bool has = false, fin = false;
+    uint64_t last = wall_clock64();
+    for (;;) {
+        uint64_t need = __ballot(!has && !fin);
+        while (need) {
+            uint32_t cnt = (uint32_t)__popcll(need);
+            int leader = __builtin_ctzll(need);
+            uint32_t base = 0;
+            if (lane == leader)
+                base = __hip_atomic_fetch_add(ctl + 1, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            base = __shfl(base, leader, 64);
+            if (!has && !fin) {
+                uint32_t q = base + (uint32_t)__popcll(need & lt);
+                if (q >= total) fin = true;
+                else {
+                    has = load_item(it, q, nc, prep, rank, deg, rot, inv_i);
+                    t = q / nc;
+                }
+            }
+            need = __ballot(!has && !fin);
+        }
+        if (!__any(has)) break;
+        bool progressed = false;
+        if (has) {
+            const int oa = (int)(it.a * 32u), ob = (int)(it.b * 32u), oc = (int)(it.c * 8u);
+            const auto a0 = __builtin_amdgcn_raw_buffer_load_b128(rv, oa, 0, CPOL_SC1);
+            const auto a1 = __builtin_amdgcn_raw_buffer_load_b128(rv, oa + 16, 0, CPOL_SC1);
+            const auto b0 = __builtin_amdgcn_raw_buffer_load_b128(rv, ob, 0, CPOL_SC1);
+            const auto b1 = __builtin_amdgcn_raw_buffer_load_b128(rv, ob + 16, 0, CPOL_SC1);
+            const auto lc = __builtin_amdgcn_raw_buffer_load_b64(rc, oc, 0, CPOL_SC1);
+            const bool ready = (!it.dyn_a || (a0[3] == it.ta && a1[3] == it.ta)) &&
+                               (!it.dyn_b || (b0[3] == it.tb && b1[3] == it.tb)) && lc[1] == t;
+            if (ready) {
+                f3 la = mk(__uint_as_float(a0[0]), __uint_as_float(a0[1]), __uint_as_float(a0[2]));
+                f3 wa = mk(__uint_as_float(a1[0]), __uint_as_float(a1[1]), __uint_as_float(a1[2]));
+                f3 lb = mk(__uint_as_float(b0[0]), __uint_as_float(b0[1]), __uint_as_float(b0[2]));
+                f3 wb = mk(__uint_as_float(b1[0]), __uint_as_float(b1[1]), __uint_as_float(b1[2]));
+                const float nl = item_update(it, la, wa, lb, wb, __uint_as_float(lc[0]));
+                typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+                typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+                if (it.dyn_a) {
+                    const uint32_t k = it.ta + 1u;
+                    __builtin_amdgcn_raw_buffer_store_b128(
+                        u4{__float_as_uint(la.x), __float_as_uint(la.y), __float_as_uint(la.z), k}, rv, oa, 0,
+                        CPOL_SC1);
+                    __builtin_amdgcn_raw_buffer_store_b128(
+                        u4{__float_as_uint(wa.x), __float_as_uint(wa.y), __float_as_uint(wa.z), k}, rv, oa + 16, 0,
+                        CPOL_SC1);
+                }
+                if (it.dyn_b) {
+                    const uint32_t k = it.tb + 1u;
+                    __builtin_amdgcn_raw_buffer_store_b128(
+                        u4{__float_as_uint(lb.x), __float_as_uint(lb.y), __float_as_uint(lb.z), k}, rv, ob, 0,
+                        CPOL_SC1);
+                    __builtin_amdgcn_raw_buffer_store_b128(
+                        u4{__float_as_uint(wb.x), __float_as_uint(wb.y), __float_as_uint(wb.z), k}, rv, ob + 16, 0,
+                        CPOL_SC1);
+                }
+                __builtin_amdgcn_raw_buffer_store_b64(u2{__float_as_uint(nl), t + 1u}, rc, oc, 0, CPOL_SC1);
+                has = false;
+                progressed = true;
+            }
+        }
+        if (__any(progressed)) {
+            last = wall_clock64();
+        } else {
+            if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+            if (wall_clock64() - last > STALL_TICKS) {
+                __hip_atomic_store(ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0) atomicOr(counters + 2, (uint32_t)SV_STATUS_SOLVER_TIMEOUT);
+                return;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Flow form.  Item (t, c) has one predecessor per dynamic body (the previous
 // update of that body in sweep order: none for the body's first contact in
 // iteration 0) and one successor per dynamic body.  A countdown per item
@@ -462,6 +611,24 @@ __global__ void __launch_bounds__(MC_THREADS) k_flow(
 
 }  // namespace
 
+// Sequence-form words (allocated on first use, sized from the capacities).
+bool seq_reserve(sv_ctx *c) {
+    Scratch &s = c->s;
+    if (s.seq_v) return true;
+    void *v = nullptr, *w = nullptr;
+    if (hipMalloc(&v, 2 * (size_t)(c->caps.max_bodies ? c->caps.max_bodies : 1) * sizeof(float4)) != hipSuccess)
+        return false;
+    if (hipMalloc(&w, (size_t)(c->caps.max_contacts ? c->caps.max_contacts : 1) * sizeof(uint2)) != hipSuccess) {
+        (void)hipFree(v);
+        return false;
+    }
+    s.lazy.push_back(v);
+    s.lazy.push_back(w);
+    s.seq_v = static_cast<float4 *>(v);
+    s.seq_c = static_cast<uint2 *>(w);
+    return true;
+}
+
 // Largest body count whose tickets fit in the single-workgroup kernel's LDS.
 uint32_t solver_lds_bodies(sv_ctx *c) {
     int lds = 0;
@@ -531,24 +698,44 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
                            c->s.flow_queue, c->s.solve_ctl);
         uint32_t wgs = c->solver_workgroups ? c->solver_workgroups : 64u;
         StageTimer sweep_timer(c, "phys/pgs_sweep");
-        hipLaunchKernelGGL(c->solver_coherent ? k_flow<true> : k_flow<false>, dim3(wgs), dim3(MC_THREADS), 0, c->stream,
+        hipLaunchKernelGGL(c->sweep_variant != SWEEP_FENCED ? k_flow<true> : k_flow<false>, dim3(wgs), dim3(MC_THREADS), 0, c->stream,
                            c->s.misc, iters, c->s.cprep, vel,
                            omega, c->s.rot, inv_i, lam, c->s.flow_count, c->s.flow_queue, c->s.solve_ctl, counters);
         return;
     }
     hipLaunchKernelGGL(k_rank, dim3(rgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.inc_key[where],
                        c->s.inc_val[where], c->s.deg_off, c->s.rank);
+    const bool seq_ok = c->sweep_variant == SWEEP_SEQ && (uint64_t)n_bodies * 32u < 0x7FFFFFFFull &&
+                        (uint64_t)mc * 8u < 0x7FFFFFFFull && seq_reserve(c);
     // Form choice: small scenes keep every hand-off inside one CU; large ones
-    // need the wider in-order window of the multi-CU form (with coherent
-    // hand-offs 64 workgroups was fastest on a 32k-body pile, 2M sweep items
-    // per substep: 2.96 ms against 3.21 ms at 32 and 3.89 ms at 128).
-    uint32_t wgs = c->solver_workgroups ? c->solver_workgroups : (n_bodies >= 2048 ? 64u : 1u);
+    // need the wider in-order window of the multi-CU form.  On a 32k-body
+    // pile (2M sweep items per substep) the sequence form was fastest at 32
+    // workgroups (2.24 ms; 2.34 at 64, 3.49 at 128), the ticket form with
+    // coherent accesses at 64 (2.96 ms; 3.21 at 32).
+    const bool seq_form = c->sweep_variant == SWEEP_SEQ;
+    uint32_t wgs = c->solver_workgroups ? c->solver_workgroups
+                                        : (n_bodies >= 2048 ? (seq_form ? 32u : 64u) : 1u);
+    if (wgs > 1 && seq_ok) {
+        uint32_t pgrid = div_up(n_bodies > mc ? n_bodies : mc, BLOCK);
+        if (pgrid > 8192) pgrid = 8192;
+        hipLaunchKernelGGL(k_seq_pack, dim3(pgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, vel, omega, lam,
+                           c->s.seq_v, c->s.seq_c);
+        {
+            StageTimer sweep_timer(c, "phys/pgs_sweep");
+            hipLaunchKernelGGL(k_solve_seq<MC_THREADS>, dim3(wgs), dim3(MC_THREADS), 0, c->stream, c->s.misc, iters,
+                               c->s.cprep, c->s.rank, c->s.deg, c->s.seq_v, c->s.seq_c, c->s.rot, inv_i,
+                               c->s.solve_ctl, counters);
+        }
+        hipLaunchKernelGGL(k_seq_unpack, dim3(pgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.seq_v,
+                           c->s.seq_c, vel, omega, lam);
+        return;
+    }
     StageTimer sweep_timer(c, "phys/pgs_sweep");
     if (wgs > 1) {
         fill_u32(c, c->s.ticket, 0u, n_bodies);
         // (all working workgroups on one XCD was 4.5x slower with L2
         // maintenance hand-offs: the XCD's L2 serialises them)
-        if (c->solver_coherent)
+        if (c->sweep_variant != SWEEP_FENCED)
             hipLaunchKernelGGL((k_solve<FORM_COH, MC_THREADS>), dim3(wgs), dim3(MC_THREADS), 0, c->stream, c->s.misc,
                                iters, n_bodies, c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, vel, omega, c->s.rot,
                                inv_i, lam, c->s.solve_ctl, counters);

@@ -95,17 +95,21 @@ def test_broadphase_narrowphase_bitwise(settled):
     ctx.close()
 
 
-SOLVER_FORMS = [("auto", 0), ("ordered", 1), ("ordered", 8), ("flow", 1), ("flow", 64)]
+# (schedule, workgroups, hand-off variant of the multi-workgroup forms)
+SOLVER_FORMS = [("auto", 0, "seq"), ("ordered", 1, "seq"), ("ordered", 8, "seq"), ("ordered", 64, "seq"),
+                ("ordered", 8, "coherent"), ("ordered", 8, "fenced"), ("flow", 1, "seq"), ("flow", 64, "coherent"),
+                ("flow", 64, "fenced")]
 
 
-@pytest.mark.parametrize("schedule,wgs", SOLVER_FORMS)
-def test_solve_bitwise(settled, schedule, wgs):
+@pytest.mark.parametrize("schedule,wgs,variant", SOLVER_FORMS)
+def test_solve_bitwise(settled, schedule, wgs, variant):
     """Every sweep schedule reproduces the serial Gauss-Seidel loop bit for bit."""
     sc, w = _stage_world(settled)
     for st in (1, 8, 2, 3):
         w.oracle_stage(st)
     ctx = new_ctx(sc, w.max_pairs, w.max_contacts)
     ctx.set_solver(schedule, wgs)
+    ctx.set_sweep_variant(variant)
     w.upload(ctx)  # includes the oracle's contact list and counters
     dt = float(np.float32(1 / 60) / np.float32(10))
     w.oracle_stage(4)
@@ -139,12 +143,13 @@ def test_integrate_verts_bitwise(settled):
     ctx.close()
 
 
-@pytest.mark.parametrize("frames,schedule,broadphase",
-                         [(1, "auto", "auto"), (8, "auto", "auto"), (8, "flow", "auto"), (8, "auto", "lbvh")])
-def test_full_step_matches_oracle(settled, frames, schedule, broadphase):
+@pytest.mark.parametrize("frames,schedule,wgs,broadphase",
+                         [(1, "auto", 0, "auto"), (8, "auto", 0, "auto"), (8, "flow", 0, "auto"),
+                          (8, "auto", 0, "lbvh"), (8, "ordered", 64, "auto")])
+def test_full_step_matches_oracle(settled, frames, schedule, wgs, broadphase):
     sc, w = _stage_world(settled)
     ctx = new_ctx(sc, w.max_pairs, w.max_contacts)
-    ctx.set_solver(schedule)
+    ctx.set_solver(schedule, wgs)
     ctx.set_broadphase(broadphase)
     w.upload(ctx)
     for f in range(frames):

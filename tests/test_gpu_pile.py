@@ -9,8 +9,8 @@ from pile_trajectory import digest, load
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("solver", ["auto", "flow"])
-def test_pile_trajectory_matches_fixture(solver):
+@pytest.mark.parametrize("solver,wgs", [("auto", 0), ("flow", 0), ("ordered", 32)])
+def test_pile_trajectory_matches_fixture(solver, wgs):
     from seevr_amd import Context, caps_for, default_params
     fx = load()
     sc = make_scene(1, 1000, seed=1)
@@ -18,7 +18,7 @@ def test_pile_trajectory_matches_fixture(solver):
     ctx = Context(caps_for(sc, fx["max_pairs"], fx["max_contacts"]),
                   default_params(pgs_iterations=fx["pgs_iterations"]))
     sc.upload(ctx)
-    ctx.set_solver(solver)
+    ctx.set_solver(solver, wgs)
     n = c["num_bodies"]
     cps = {cp["frame"]: cp for cp in fx["checkpoints"]}
     for f in range(1, max(cps) + 1):
