@@ -48,6 +48,7 @@ constexpr int BLOCK = 256;
 constexpr int SOLVE_THREADS = 1024;
 constexpr int MC_THREADS = 256;
 constexpr uint64_t STALL_TICKS = 200000000ull;  // 2 s of the 100 MHz wall clock
+constexpr uint64_t ABORT_CHECK_TICKS = 2000ull;  // 20 us
 
 template <typename T>
 inline T *B(sv_ctx *c, int id) { return reinterpret_cast<T *>(c->buf[id]); }
@@ -240,11 +241,17 @@ __global__ void __launch_bounds__(THREADS) k_solve(
         if (__any(progressed)) {
             last = wall_clock64();
         } else {
-            if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
-            if (wall_clock64() - last > STALL_TICKS) {
-                __hip_atomic_store(ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (lane == 0) atomicOr(counters + 2, (uint32_t)SV_STATUS_SOLVER_TIMEOUT);
-                return;
+            // the abort flag (another wave's stall) is read only after 20 us
+            // without progress: each read is a round trip to the point of
+            // coherence that would otherwise lengthen every poll
+            const uint64_t idle = wall_clock64() - last;
+            if (idle > ABORT_CHECK_TICKS) {
+                if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+                if (idle > STALL_TICKS) {
+                    __hip_atomic_store(ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (lane == 0) atomicOr(counters + 2, (uint32_t)SV_STATUS_SOLVER_TIMEOUT);
+                    return;
+                }
             }
             __builtin_amdgcn_s_sleep(1);
         }
@@ -389,11 +396,17 @@ __global__ void __launch_bounds__(THREADS) k_solve_seq(
         if (__any(progressed)) {
             last = wall_clock64();
         } else {
-            if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
-            if (wall_clock64() - last > STALL_TICKS) {
-                __hip_atomic_store(ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (lane == 0) atomicOr(counters + 2, (uint32_t)SV_STATUS_SOLVER_TIMEOUT);
-                return;
+            // the abort flag (another wave's stall) is read only after 20 us
+            // without progress: each read is a round trip to the point of
+            // coherence that would otherwise lengthen every poll
+            const uint64_t idle = wall_clock64() - last;
+            if (idle > ABORT_CHECK_TICKS) {
+                if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+                if (idle > STALL_TICKS) {
+                    __hip_atomic_store(ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (lane == 0) atomicOr(counters + 2, (uint32_t)SV_STATUS_SOLVER_TIMEOUT);
+                    return;
+                }
             }
             __builtin_amdgcn_s_sleep(1);
         }
