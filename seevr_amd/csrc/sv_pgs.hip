@@ -328,8 +328,16 @@ __global__ void __launch_bounds__(THREADS) k_solve_seq(
     if (total == 0) return;
     const __amdgpu_buffer_rsrc_t rv = seq_rsrc(sv), rc = seq_rsrc(sc);
 
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
     Item it;
     uint32_t t = 0;
+    // words of the held item and which of them already show the count it
+    // waits for: a word that does cannot change before this item writes it,
+    // so only the others are polled again
+    u4 a0, a1, b0, b1;
+    u2 lc;
+    uint32_t got = 0;
     bool has = false, fin = false;
     uint64_t last = wall_clock64();
     for (;;) {
@@ -347,6 +355,7 @@ __global__ void __launch_bounds__(THREADS) k_solve_seq(
                 else {
                     has = load_item(it, q, nc, prep, rank, deg, rot, inv_i);
                     t = q / nc;
+                    got = 0;
                 }
             }
             need = __ballot(!has && !fin);
@@ -355,21 +364,22 @@ __global__ void __launch_bounds__(THREADS) k_solve_seq(
         bool progressed = false;
         if (has) {
             const int oa = (int)(it.a * 32u), ob = (int)(it.b * 32u), oc = (int)(it.c * 8u);
-            const auto a0 = __builtin_amdgcn_raw_buffer_load_b128(rv, oa, 0, CPOL_SC1);
-            const auto a1 = __builtin_amdgcn_raw_buffer_load_b128(rv, oa + 16, 0, CPOL_SC1);
-            const auto b0 = __builtin_amdgcn_raw_buffer_load_b128(rv, ob, 0, CPOL_SC1);
-            const auto b1 = __builtin_amdgcn_raw_buffer_load_b128(rv, ob + 16, 0, CPOL_SC1);
-            const auto lc = __builtin_amdgcn_raw_buffer_load_b64(rc, oc, 0, CPOL_SC1);
-            const bool ready = (!it.dyn_a || (a0[3] == it.ta && a1[3] == it.ta)) &&
-                               (!it.dyn_b || (b0[3] == it.tb && b1[3] == it.tb)) && lc[1] == t;
+            if (!(got & 1u)) a0 = __builtin_amdgcn_raw_buffer_load_b128(rv, oa, 0, CPOL_SC1);
+            if (!(got & 2u)) a1 = __builtin_amdgcn_raw_buffer_load_b128(rv, oa + 16, 0, CPOL_SC1);
+            if (!(got & 4u)) b0 = __builtin_amdgcn_raw_buffer_load_b128(rv, ob, 0, CPOL_SC1);
+            if (!(got & 8u)) b1 = __builtin_amdgcn_raw_buffer_load_b128(rv, ob + 16, 0, CPOL_SC1);
+            if (!(got & 16u)) lc = __builtin_amdgcn_raw_buffer_load_b64(rc, oc, 0, CPOL_SC1);
+            // a static body's words never change: any count will do
+            got |= ((!it.dyn_a || a0[3] == it.ta) ? 1u : 0u) | ((!it.dyn_a || a1[3] == it.ta) ? 2u : 0u) |
+                   ((!it.dyn_b || b0[3] == it.tb) ? 4u : 0u) | ((!it.dyn_b || b1[3] == it.tb) ? 8u : 0u) |
+                   (lc[1] == t ? 16u : 0u);
+            const bool ready = got == 31u;
             if (ready) {
                 f3 la = mk(__uint_as_float(a0[0]), __uint_as_float(a0[1]), __uint_as_float(a0[2]));
                 f3 wa = mk(__uint_as_float(a1[0]), __uint_as_float(a1[1]), __uint_as_float(a1[2]));
                 f3 lb = mk(__uint_as_float(b0[0]), __uint_as_float(b0[1]), __uint_as_float(b0[2]));
                 f3 wb = mk(__uint_as_float(b1[0]), __uint_as_float(b1[1]), __uint_as_float(b1[2]));
                 const float nl = item_update(it, la, wa, lb, wb, __uint_as_float(lc[0]));
-                typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-                typedef unsigned int u2 __attribute__((ext_vector_type(2)));
                 if (it.dyn_a) {
                     const uint32_t k = it.ta + 1u;
                     __builtin_amdgcn_raw_buffer_store_b128(
