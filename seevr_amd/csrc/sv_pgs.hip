@@ -47,6 +47,7 @@ namespace {
 constexpr int BLOCK = 256;
 constexpr int SOLVE_THREADS = 1024;
 constexpr int MC_THREADS = 256;
+constexpr int SEQ_THREADS = 64;  // sequence form: one wave per workgroup
 constexpr uint64_t STALL_TICKS = 200000000ull;  // 2 s of the 100 MHz wall clock
 constexpr uint64_t ABORT_CHECK_TICKS = 2000ull;  // 20 us
 
@@ -732,12 +733,13 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
                         (uint64_t)mc * 8u < 0x7FFFFFFFull && seq_reserve(c);
     // Form choice: small scenes keep every hand-off inside one CU; large ones
     // need the wider in-order window of the multi-CU form.  On a 32k-body
-    // pile (2M sweep items per substep) the sequence form was fastest at 32
-    // workgroups (2.24 ms; 2.34 at 64, 3.49 at 128), the ticket form with
-    // coherent accesses at 64 (2.96 ms; 3.21 at 32).
+    // pile (2M sweep items per substep) the sequence form was fastest with
+    // 192 one-wave workgroups (1.65 ms; 1.68 at 128, 1.69 at 256, 1.98 at
+    // 96, 2.24 with 32 four-wave workgroups), the ticket form with coherent
+    // accesses at 64 four-wave workgroups (2.96 ms; 3.21 at 32).
     const bool seq_form = c->sweep_variant == SWEEP_SEQ;
     uint32_t wgs = c->solver_workgroups ? c->solver_workgroups
-                                        : (n_bodies >= 2048 ? (seq_form ? 32u : 64u) : 1u);
+                                        : (n_bodies >= 2048 ? (seq_form ? 192u : 64u) : 1u);
     if (wgs > 1 && seq_ok) {
         uint32_t pgrid = div_up(n_bodies > mc ? n_bodies : mc, BLOCK);
         if (pgrid > 8192) pgrid = 8192;
@@ -745,8 +747,8 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
                            c->s.seq_v, c->s.seq_c);
         {
             StageTimer sweep_timer(c, "phys/pgs_sweep");
-            hipLaunchKernelGGL(k_solve_seq<MC_THREADS>, dim3(wgs), dim3(MC_THREADS), 0, c->stream, c->s.misc, iters,
-                               c->s.cprep, c->s.rank, c->s.deg, c->s.seq_v, c->s.seq_c, c->s.rot, inv_i,
+            hipLaunchKernelGGL(k_solve_seq<SEQ_THREADS>, dim3(wgs), dim3(SEQ_THREADS), 0, c->stream, c->s.misc,
+                               iters, c->s.cprep, c->s.rank, c->s.deg, c->s.seq_v, c->s.seq_c, c->s.rot, inv_i,
                                c->s.solve_ctl, counters);
         }
         hipLaunchKernelGGL(k_seq_unpack, dim3(pgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.seq_v,
