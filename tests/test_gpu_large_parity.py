@@ -1,5 +1,5 @@
 """Parity where the automatic choices pick the multi-workgroup paths: a
-4,096-body container scene is stepped 80 frames on the device, its state is
+4,096-body container scene is stepped 240 frames on the device, its state is
 handed to the oracle, and two more frames must agree bit for bit (the
 sequence-form sweep, chosen from 2,048 bodies, and the all-pairs
 broadphase; then the LBVH broadphase from the same state)."""
@@ -23,7 +23,7 @@ def test_multi_workgroup_paths_match_oracle(broadphase):
     ctx = Context(caps_for(sc, pairs, contacts), default_params(pgs_iterations=30))
     sc.upload(ctx)
     ctx.set_broadphase(broadphase)
-    for _ in range(80):
+    for _ in range(240):
         ctx.step(c["num_bodies"], c["num_vertices"], c["num_hull_verts"])
     w = World.from_scene(sc, max_pairs=pairs, max_contacts=contacts, pgs_iterations=30)
     for k, v in w.download_state(ctx).items():
@@ -37,5 +37,5 @@ def test_multi_workgroup_paths_match_oracle(broadphase):
             assert_bitwise(st[k], w.a[k], f"{k} after frame {f}")
     got = ctx.read_counters()
     assert got == tuple(int(v) for v in w.a["ATOMIC_COUNTERS"][:2])
-    assert got[1] > 2000, "the pile should be in contact"
+    assert got[1] > 500, "the pile should be in contact"
     ctx.close()
