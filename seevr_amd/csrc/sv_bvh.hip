@@ -411,6 +411,8 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
     const bool js = jl.w != 0.0f;
     uint32_t *lstack = s_stack + threadIdx.x;
     uint32_t stack[BB_STACK - BB_LDS];
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int sp = 0;
     uint32_t node = 0;
     for (;;) {
@@ -420,26 +422,44 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
             const uint32_t c = ch[q];
             const float4 lo = nlo[c], hi = nhi[c];
             const uint32_t mi = __float_as_uint(lo.w);
-            if (mi >= j || (js && __float_as_uint(hi.w) != 0u) || boxes_apart(jl, jh, lo, hi)) continue;
-            if (c >= n - 1) {
-                if (!EMIT) {
-                    atomicAdd(cnt + mi, 1u);
-                } else {
-                    const uint32_t o = off[mi];
-                    if (o < max_pairs) {
-                        const uint32_t s = o + atomicAdd(fill + mi, 1u);
-                        if (combined) {
-                            pk[s] = (mi << bits) | j;
-                        } else {
-                            pk[s] = j;
-                            pv[s] = mi;
+            const bool keep = !(mi >= j || (js && __float_as_uint(hi.w) != 0u) || boxes_apart(jl, jh, lo, hi));
+            const bool leaf = keep && c >= n - 1;
+            if (keep && !leaf) {
+                if (sp < BB_LDS) lstack[sp++ * BLOCK] = c;
+                else if (sp < BB_STACK) stack[sp++ - BB_LDS] = c;
+            }
+            // Row counters of the lanes that found a leaf: the row of the
+            // first such lane (in a settled pile often the ground's, shared
+            // by most lanes) takes one atomic for all of its lanes; other
+            // rows take one each.
+            const uint64_t lm = __ballot(leaf);
+            if (lm) {
+                const int first = __builtin_ctzll(lm);
+                const uint32_t lead = __shfl(mi, first, 64);
+                const uint64_t same = __ballot(leaf && mi == lead);
+                uint32_t base = 0;
+                if (lane == first) {
+                    if (!EMIT) base = atomicAdd(cnt + lead, (uint32_t)__popcll(same));
+                    else if (off[lead] < max_pairs) base = atomicAdd(fill + lead, (uint32_t)__popcll(same));
+                }
+                base = __shfl(base, first, 64);
+                if (leaf) {
+                    if (!EMIT) {
+                        if (mi != lead) atomicAdd(cnt + mi, 1u);
+                    } else {
+                        const uint32_t o = off[mi];
+                        if (o < max_pairs) {
+                            const uint32_t s = o + (mi == lead ? base + (uint32_t)__popcll(same & lt)
+                                                               : atomicAdd(fill + mi, 1u));
+                            if (combined) {
+                                pk[s] = (mi << bits) | j;
+                            } else {
+                                pk[s] = j;
+                                pv[s] = mi;
+                            }
                         }
                     }
                 }
-            } else if (sp < BB_LDS) {
-                lstack[sp++ * BLOCK] = c;
-            } else if (sp < BB_STACK) {
-                stack[sp++ - BB_LDS] = c;
             }
         }
         if (sp == 0) break;
@@ -450,15 +470,23 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
 
 // Per row: clear the fill counter; rows that start below max_pairs are
 // filled, and the last of them ends the part that is sorted.
+// The offsets are an exclusive scan of the counts, so row ends off + cnt do
+// not decrease with the row, and the rows starting below max_pairs are a
+// prefix [0, r): the filled part ends at off[r-1] + cnt[r-1], found by one
+// binary search (off[0] = 0 < max_pairs).
 __global__ void k_bb_rows(uint32_t n, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
                           uint32_t *__restrict__ fill, uint32_t *misc, uint32_t max_pairs) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    fill[i] = 0u;
-    uint32_t end = (cnt[i] != 0u && off[i] < max_pairs) ? off[i] + cnt[i] : 0u;
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) end = max(end, (uint32_t)__shfl_xor((int)end, o, 64));
-    if ((threadIdx.x & 63) == 0 && end) atomicMax(misc + MISC_BPE, end);
+    if (i < n) fill[i] = 0u;
+    if (i == 0) {
+        uint32_t lo = 1, hi = n;  // first row in [1, n] whose offset is >= max_pairs (n if none)
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (off[mid] < max_pairs) lo = mid + 1;
+            else hi = mid;
+        }
+        misc[MISC_BPE] = off[lo - 1] + cnt[lo - 1];
+    }
 }
 
 __global__ void k_bb_write(const uint32_t *__restrict__ misc, const uint32_t *__restrict__ ki,
