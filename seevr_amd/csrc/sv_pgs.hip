@@ -82,7 +82,11 @@ __global__ void __launch_bounds__(BLOCK) k_prep(
         bool skip = flags & F_SKIP, dyn_a = flags & F_DYN_A, dyn_b = flags & F_DYN_B;
         if (warm && !skip && lambda[c] > 0.0f)
             prep[(size_t)c * CPREP_WORDS + P_FLAGS] = __uint_as_float(flags | F_WARM);
-        if (!skip) atomicAdd(&misc_w[MISC_NACT], 1u);
+        {  // one atomic per wave for the count of swept contacts
+            const uint64_t m = __ballot(!skip);
+            if (m && (threadIdx.x & 63) == (uint32_t)__builtin_ctzll(m))
+                atomicAdd(&misc_w[MISC_NACT], (uint32_t)__popcll(m));
+        }
         bool ia = !skip && dyn_a, ib = !skip && dyn_b;
         key[2 * c] = ia ? a : n_bodies;
         val[2 * c] = 2 * c;
