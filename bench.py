@@ -48,7 +48,8 @@ def build_workload(count: int, seed: int):
 
 PROFILE = ROOT / "profiles" / "c2_32k_container_kernels.md"
 STAGE_KERNEL = {"phys/pgs_sweep": "k_solve_seq<64>", "phys/aabb": "k_aabb", "phys/gravity": "k_gravity",
-                "phys/integrate": "k_integrate", "phys/broad_phase": "k_broadphase<false>"}
+                "phys/integrate": "k_integrate", "phys/broad_phase": "k_broadphase<false>",
+                "bvh/morton_codes": "k_morton", "bvh/radix_sort": "radix sort (k_sort_hist, k_scan_*, k_sort_scatter)"}
 
 
 def algorithmic_bytes(stage: str, n: int, n_hull: int, n_dyn: int, pairs: int, nc: int) -> float:
@@ -57,6 +58,8 @@ def algorithmic_bytes(stage: str, n: int, n_hull: int, n_dyn: int, pairs: int, n
     aabb: pos 12 + quat 16 + hull range 8 per body read, 12 per hull vertex
     read, 24 per body written.  broadphase: 28 per body read (box + static
     flag) + 8 per pair written.  gravity 52, integrate 84 per dynamic body.
+    LBVH radix sort: 16 per key per pass (key and payload read and written),
+    4 passes of 8-bit digits over 30-bit keys; Morton codes: 12 in + 4 out.
     Ordered PGS sweep: per item (iteration x contact) the two bodies'
     velocities and angular velocities read and written (96 B), the
     accumulated impulse read and written (8 B) and the contact's 20-word
@@ -72,6 +75,10 @@ def algorithmic_bytes(stage: str, n: int, n_hull: int, n_dyn: int, pairs: int, n
         return 28.0 * n + 8.0 * pairs
     if stage == "phys/pgs_sweep":
         return 184.0 * PGS_ITERS * nc
+    if stage == "bvh/radix_sort":  # 30-bit Morton keys, 8-bit digits: 4 passes of key + index in and out
+        return 16.0 * n * 4
+    if stage == "bvh/morton_codes":
+        return 16.0 * n
     return 0.0
 
 
@@ -495,14 +502,20 @@ def main() -> int:
         ", ".join(f"{k}={v[0]:.1f}x{v[1]}" for k, v in sorted(stages.items(), key=lambda kv: -kv[1][0] * kv[1][1])))
 
     # Timed region: K frames, device synchronized on both sides (every call is
-    # synchronous).  HIP events on the library stream bracket each stage in
-    # the same frames; they feed the roofline below.
+    # synchronous).  Each frame is the physics step and the LBVH over the
+    # body boxes; the metric counts the time inside the physics-step calls
+    # only, the LBVH build is timed beside it.  HIP events on the library
+    # stream bracket each stage in the same frames; they feed the roofline.
     ctx.enable_stage_timing(True)
     ctx.stage_times(reset=True)
-    t0 = time.perf_counter()
+    el = el_bvh = 0.0
     for i in range(args.steps):
+        t0 = time.perf_counter()
         ctx.step(n, nv, nh, SUBSTEPS, DT, GRAVITY)
-    el = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        ctx.bvh_build_bodies(n)
+        el_bvh += time.perf_counter() - t1
+        el += t1 - t0
     stages = ctx.stage_times(reset=True)
     ctx.enable_stage_timing(False)
     pairs, contacts = ctx.read_counters()
@@ -515,10 +528,15 @@ def main() -> int:
     nc = min(contacts, caps.max_contacts)
     roof = roofline_entry("phys/pgs_sweep", stages, n, nh, n_dyn, pairs, nc)
     hbm_stages = {k: roofline_entry(k, stages, n, nh, n_dyn, pairs, nc)
-                  for k in ("phys/aabb", "phys/broad_phase", "phys/integrate", "phys/gravity")}
+                  for k in ("phys/aabb", "phys/broad_phase", "phys/integrate", "phys/gravity", "bvh/radix_sort",
+                            "bvh/morton_codes")}
+    lbvh = {"ms_per_frame": round(1000.0 * el_bvh / args.steps, 3), "primitives": n,
+            "stage_us": {k: round(v[0], 1) for k, v in stages.items() if k.startswith("bvh/")}}
+    log(f"LBVH over body boxes: {lbvh['ms_per_frame']:.3f} ms per frame; {lbvh['stage_us']}")
     roof["copy_kernel_gbs"] = round(copy_gbs, 1)
-    total_us = sum(v[0] * v[1] for k, v in stages.items() if k != "phys/pgs_sweep")
-    breakdown = {k: round(100.0 * v[0] * v[1] / total_us, 1) for k, v in stages.items() if k != "phys/pgs_sweep"}
+    phys = {k: v for k, v in stages.items() if k.startswith("phys/") and k != "phys/pgs_sweep"}
+    total_us = sum(v[0] * v[1] for v in phys.values())
+    breakdown = {k: round(100.0 * v[0] * v[1] / total_us, 1) for k, v in phys.items()}
 
     variants = sweep_variant_section(ctx, n)
 
@@ -554,7 +572,7 @@ def main() -> int:
                    "frames_before_timing": args.warmup, "pairs_last_substep": pairs,
                    "contacts_last_substep": contacts},
         "roofline": roof, "roofline_hbm_stages": hbm_stages, "cpu_baseline": cpu, "stage_share_pct": breakdown,
-        "sweep_variants": variants, "warm_start": warm, "batched": batched, "differentiable": diff, "large_scene": large,
+        "lbvh_per_frame": lbvh, "sweep_variants": variants, "warm_start": warm, "batched": batched, "differentiable": diff, "large_scene": large,
     }
     print(json.dumps(out), flush=True)
     return 0

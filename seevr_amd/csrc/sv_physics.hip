@@ -215,14 +215,41 @@ __global__ void k_post_pairs(uint32_t *counters, uint32_t *misc, uint32_t max_pa
 
 // Step 3 (physics.comp:475-491), result per pair slot: hit flag and
 // (pos, normal, depth); contacts are then compacted in pair order.
+//
+// Divergence-sorted batches: each block takes 256 consecutive pairs (near in
+// the canonical order, so they share bodies in the caches) and reorders them
+// in LDS by support-path class (brute force for hulls of at most 32 verts,
+// lookup above: physics.comp:107-162) before the lanes take them, so a wave
+// runs pairs of one class.  Results stay in pair slots: the order of work
+// has no effect on them.
 __global__ void __launch_bounds__(BLOCK) k_narrow(const uint32_t *__restrict__ misc, const uint32_t *__restrict__ pairs,
                                                   const float *__restrict__ pos, const float *__restrict__ rot,
                                                   const uint32_t *__restrict__ hrange, const uint32_t *__restrict__ sl_off,
                                                   const uint32_t *__restrict__ sl, const float *__restrict__ hull,
                                                   uint32_t *__restrict__ flag, float *__restrict__ res) {
+    __shared__ uint32_t s_cnt[4];
+    __shared__ uint32_t s_pair[BLOCK];
     uint32_t np = misc[MISC_NP];
     Geo g{hull, sl};
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+    for (uint32_t p0 = blockIdx.x * blockDim.x; p0 < np; p0 += gridDim.x * blockDim.x) {
+        if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t mine = p0 + threadIdx.x;
+        uint32_t k = 0, rk = 0;
+        if (mine < np) {
+            k = (sl_off[pairs[2 * mine]] != 0u ? 1u : 0u) | (sl_off[pairs[2 * mine + 1]] != 0u ? 2u : 0u);
+            rk = atomicAdd(&s_cnt[k], 1u);
+        }
+        __syncthreads();
+        if (mine < np) {
+            uint32_t o = 0;
+            for (uint32_t q = 0; q < k; q++) o += s_cnt[q];
+            s_pair[o + rk] = mine;
+        }
+        __syncthreads();
+        const uint32_t m = np - p0 < (uint32_t)BLOCK ? np - p0 : (uint32_t)BLOCK;
+        if (threadIdx.x >= m) continue;
+        const uint32_t p = s_pair[threadIdx.x];
         uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
         Pair P;
         P.A = load_body(a, pos, rot, hrange, sl_off);
