@@ -362,7 +362,7 @@ def warm_start_section(ctx, n, nv, nh, factor: float = 1.0, iterations=(4, 10, 3
     return {"factor": factor, "from_same_state": out}
 
 
-def large_scene_section(frames: int = 120, timed: int = 20, compare: bool = True) -> dict:
+def large_scene_section(frames: int = 120, timed: int = 20, compare: bool = True, solver_wgs: int = 0) -> dict:
     """The 1,048,576-body scene (70% spheres of 81 hull verts, 200x200x50 m
     over a 220x1x220 m ground; 62M hull verts, so hull ranges are read as
     uint32): `frames` frames with the LBVH broadphase, the last `timed` of
@@ -382,6 +382,8 @@ def large_scene_section(frames: int = 120, timed: int = 20, compare: bool = True
                         max_contacts=6 * nb + 1024)
     ctx = Context(caps, default_params(pgs_iterations=PGS_ITERS, hull_range_u32=1))
     sc.upload(ctx)
+    if solver_wgs:
+        ctx.set_solver("ordered", solver_wgs)
     sc.close()
     log(f"large scene: {nb} bodies, {nh} hull verts, built and uploaded in {time.perf_counter() - t0:.1f} s")
     for _ in range(frames - timed):
@@ -454,6 +456,7 @@ def main() -> int:
     ap.add_argument("--no-large", action="store_true", help="skip the 1M-body section")
     ap.add_argument("--large-frames", type=int, default=120)
     ap.add_argument("--large-timed", type=int, default=20)
+    ap.add_argument("--large-solver-wgs", type=int, default=0, help="sweep workgroups of the 1M-body scene")
     ap.add_argument("--no-large-compare", action="store_true",
                     help="skip the ordered-vs-coloured comparison of the 1M-body section")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start comparison")
@@ -559,7 +562,8 @@ def main() -> int:
         diff = differentiable_section()
     large = None
     if not args.no_large:
-        large = large_scene_section(args.large_frames, args.large_timed, not args.no_large_compare)
+        large = large_scene_section(args.large_frames, args.large_timed, not args.no_large_compare,
+                                    args.large_solver_wgs)
 
     out = {
         "metric": "body-substeps/s @32k bodies", "value": round(value, 1), "unit": "body-substeps/s",

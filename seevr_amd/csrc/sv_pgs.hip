@@ -737,13 +737,15 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
                         (uint64_t)mc * 8u < 0x7FFFFFFFull && seq_reserve(c);
     // Form choice: small scenes keep every hand-off inside one CU; large ones
     // need the wider in-order window of the multi-CU form.  On a 32k-body
-    // pile (2M sweep items per substep) the sequence form was fastest with
+    // pile (1.5M sweep items per substep) the sequence form was fastest with
     // 192 one-wave workgroups (1.65 ms; 1.68 at 128, 1.69 at 256, 1.98 at
-    // 96, 2.24 with 32 four-wave workgroups), the ticket form with coherent
-    // accesses at 64 four-wave workgroups (2.96 ms; 3.21 at 32).
+    // 96, 2.24 with 32 four-wave workgroups), on the 1M-body scene (7.2M
+    // items) with 384 (1.41 ms; 1.80 at 192, 1.78 at 768); the ticket form
+    // with coherent accesses at 64 four-wave workgroups (2.96 ms; 3.21 at 32).
     const bool seq_form = c->sweep_variant == SWEEP_SEQ;
+    const uint32_t seq_wgs = n_bodies >= (1u << 18) ? 384u : 192u;
     uint32_t wgs = c->solver_workgroups ? c->solver_workgroups
-                                        : (n_bodies >= 2048 ? (seq_form ? 192u : 64u) : 1u);
+                                        : (n_bodies >= 2048 ? (seq_form ? seq_wgs : 64u) : 1u);
     if (wgs > 1 && seq_ok) {
         uint32_t pgrid = div_up(n_bodies > mc ? n_bodies : mc, BLOCK);
         if (pgrid > 8192) pgrid = 8192;
