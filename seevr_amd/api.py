@@ -86,8 +86,8 @@ class Context:
 
     # -- buffers ------------------------------------------------------------
     def buffer_bytes(self, name: str) -> int:
-        p, n = C.c_void_p(), C.c_size_t()
-        check(self.lib.sv_buffer_ptr(self.h, BUF[name], C.byref(p), C.byref(n)))
+        n = C.c_size_t()
+        check(self.lib.sv_buffer_ptr(self.h, BUF[name], None, C.byref(n)))
         return n.value
 
     def buffer_ptr(self, name: str) -> int:

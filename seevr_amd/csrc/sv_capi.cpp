@@ -299,14 +299,22 @@ int sv_set_params(sv_ctx *c, const sv_params *p) {
     if (!c || !p) return fail(SV_E_INVALID, "sv_set_params: null argument");
     if ((uint64_t)p->pgs_iterations * c->caps.max_contacts >= 0xFFFFFFFFull)
         return fail(SV_E_INVALID, "sv_set_params: pgs_iterations * max_contacts must stay below 2^32");
+    if (p->hull_range_u32 != c->params.hull_range_u32) c->sup_dirty = true;
     c->params = *p;
     return SV_OK;
+}
+
+// buffers the expanded support lookup (Scratch::sup_xyz) is derived from
+static bool touches_support(sv_buf id) {
+    return id == SV_SUPPORT_LOOKUP || id == SV_SUPPORT_LOOKUP_OFFSET || id == SV_HULL_VERTS || id == SV_GEOM_DATA ||
+           id == SV_GEOM_HULL_RANGE;
 }
 
 int sv_buffer_ptr(sv_ctx *c, sv_buf id, void **dev, size_t *bytes) {
     if (!c || !valid_buf(id)) return fail(SV_E_INVALID, "sv_buffer_ptr: bad context or buffer id %d", (int)id);
     if (dev) *dev = c->buf[id];
     if (bytes) *bytes = c->bytes[id];
+    if (dev && touches_support(id)) c->sup_external = true;
     return SV_OK;
 }
 
@@ -316,6 +324,7 @@ int sv_upload(sv_ctx *c, sv_buf id, const void *src, size_t off, size_t n) {
         return fail(SV_E_RANGE, "sv_upload: [%zu, %zu) outside buffer %d of %zu bytes", off, off + n, (int)id,
                     c->bytes[id]);
     if (!n) return SV_OK;
+    if (touches_support(id)) c->sup_dirty = true;
     HIP_OK(hipMemcpyAsync(static_cast<char *>(c->buf[id]) + off, src, n, hipMemcpyDefault, c->stream));
     return sync_and_finish(c);
 }

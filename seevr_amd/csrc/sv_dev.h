@@ -99,6 +99,10 @@ struct Body {
 struct Geo {
     const float *hull;
     const uint32_t *lookup;
+    // optional: the lookup's candidate vertices themselves, 3 floats per
+    // lookup entry (hull[start + lookup[e]]), so a lookup support needs one
+    // memory round trip instead of two; null = read through the lookup
+    const float *lxyz = nullptr;
 };
 
 SVD Body load_body(uint32_t b, const float *pos, const float *rot, const uint32_t *hrange,
@@ -139,15 +143,19 @@ SVD f3 support(const Body &B, f3 d, const Geo &g) {
         float bin_f = a / 6.2831853f * 64.0f;
         uint32_t b0 = ((uint32_t)bin_f) % 64u;
         uint32_t b1 = (b0 + 1u) % 64u;
-        uint32_t i0 = g.lookup[B.lk + ax * 64u + b0];
-        f3 v0 = ld3(g.hull, B.vs + i0);
+        f3 v0, v1;
+        if (g.lxyz) {
+            v0 = ld3(g.lxyz, B.lk + ax * 64u + b0);
+            v1 = ld3(g.lxyz, B.lk + ax * 64u + b1);
+        } else {
+            v0 = ld3(g.hull, B.vs + g.lookup[B.lk + ax * 64u + b0]);
+            v1 = ld3(g.hull, B.vs + g.lookup[B.lk + ax * 64u + b1]);
+        }
         float d0 = dot3(v0, ld);
         if (d0 > best) {
             best = d0;
             bv = v0;
         }
-        uint32_t i1 = g.lookup[B.lk + ax * 64u + b1];
-        f3 v1 = ld3(g.hull, B.vs + i1);
         float d1 = dot3(v1, ld);
         if (d1 > best) {
             best = d1;

@@ -20,6 +20,7 @@ struct Scratch {
     uint32_t *bp_stage = nullptr;   // uint[N * nseg * BP_STAGE] first partners per (row, segment)
     uint32_t *bp_ovf = nullptr;     // uint[ceil(N / 32) * nseg] tiles with a (row, segment) above BP_STAGE
     uint32_t *np_flag = nullptr;    // uint[P] narrowphase hit flags
+    float *sup_xyz = nullptr;       // float[3L] candidate vertex of each support-lookup entry (first use)
     uint32_t *np_idx = nullptr;     // uint[P] contact slot of each hit
     float *np_res = nullptr;        // float[8P] pos, normal, depth per pair
     float *cprep = nullptr;         // float[CPREP_WORDS * C] solver constants
@@ -114,6 +115,11 @@ struct sv_ctx {
     hipError_t launch_err = hipSuccess;
     // broadphase: 0 = chosen by body count, 1 = all pairs (tiled N^2), 2 = LBVH traversal
     uint32_t broadphase_mode = 0;
+    // Scratch::sup_xyz is rebuilt when the hulls, lookups or hull ranges were
+    // uploaded (dirty), when their device pointers were handed out (external
+    // writes possible: rebuilt every step), or for another body count
+    bool sup_dirty = true, sup_external = false;
+    uint32_t sup_bodies = 0;
     // warm start (extension beside the reference path): 0 = off; otherwise a
     // contact whose body pair was in contact at the previous sweep starts
     // from warm_factor times that sweep's impulse

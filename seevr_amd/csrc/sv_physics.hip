@@ -226,11 +226,12 @@ __global__ void __launch_bounds__(BLOCK) k_narrow(const uint32_t *__restrict__ m
                                                   const float *__restrict__ pos, const float *__restrict__ rot,
                                                   const uint32_t *__restrict__ hrange, const uint32_t *__restrict__ sl_off,
                                                   const uint32_t *__restrict__ sl, const float *__restrict__ hull,
-                                                  uint32_t *__restrict__ flag, float *__restrict__ res) {
+                                                  const float *__restrict__ lxyz, uint32_t *__restrict__ flag,
+                                                  float *__restrict__ res) {
     __shared__ uint32_t s_cnt[4];
     __shared__ uint32_t s_pair[BLOCK];
     uint32_t np = misc[MISC_NP];
-    Geo g{hull, sl};
+    Geo g{hull, sl, lxyz};
     for (uint32_t p0 = blockIdx.x * blockDim.x; p0 < np; p0 += gridDim.x * blockDim.x) {
         if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
         __syncthreads();
@@ -361,6 +362,26 @@ __global__ void __launch_bounds__(BLOCK) k_copy_calib(const float4 *__restrict__
     for (; i < n; i += stride) dst[i] = src[i];
 }
 
+// Candidate vertex of every support-lookup entry of bodies [0, n): one wave
+// per body, lanes over its 3 x 64 entries (data.zig:299-353 layout).
+__global__ void __launch_bounds__(BLOCK) k_sup_expand(uint32_t n, const uint32_t *__restrict__ sl_off,
+                                                      const uint32_t *__restrict__ hrange,
+                                                      const uint32_t *__restrict__ sl, const float *__restrict__ hull,
+                                                      uint32_t n_entries, float *__restrict__ out) {
+    const uint32_t b = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    if (b >= n) return;
+    const uint32_t lk = sl_off[b];
+    if (lk == 0) return;
+    const uint32_t vs = hrange[2 * b];
+    for (uint32_t e = threadIdx.x & 63; e < 192u; e += 64u) {
+        if (lk + e >= n_entries) break;
+        const uint32_t v = vs + sl[lk + e];
+        out[3 * (size_t)(lk + e)] = hull[3 * (size_t)v];
+        out[3 * (size_t)(lk + e) + 1] = hull[3 * (size_t)v + 1];
+        out[3 * (size_t)(lk + e) + 2] = hull[3 * (size_t)v + 2];
+    }
+}
+
 // XCC register of the first wave of each block (placement probe)
 __global__ void k_xcc_ids(uint32_t *out, uint32_t n) {
     if (threadIdx.x == 0 && blockIdx.x < n) out[blockIdx.x] = xcc_id();
@@ -377,6 +398,23 @@ void launch_hull_ranges(sv_ctx *c, uint32_t n) {
     if (!n) return;
     hipLaunchKernelGGL(k_hull_ranges, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, c->stream, B<float>(c, SV_GEOM_DATA),
                        B<uint32_t>(c, SV_GEOM_HULL_RANGE), (int)c->params.hull_range_u32, c->s.hrange, n);
+    const size_t L = c->caps.max_support_entries;
+    if (L < 2) return;
+    if (!c->s.sup_xyz) {
+        void *p = nullptr;
+        if (hipMalloc(&p, 3 * L * sizeof(float)) != hipSuccess) return;  // supports then read through the lookup
+        c->s.lazy.push_back(p);
+        c->s.sup_xyz = static_cast<float *>(p);
+        c->sup_dirty = true;
+    }
+    if (c->sup_dirty || c->sup_external || c->sup_bodies != n) {
+        // one wave per body: 192 lookup entries
+        hipLaunchKernelGGL(k_sup_expand, dim3(div_up(n, BLOCK / 64)), dim3(BLOCK), 0, c->stream, n,
+                           B<uint32_t>(c, SV_SUPPORT_LOOKUP_OFFSET), c->s.hrange, B<uint32_t>(c, SV_SUPPORT_LOOKUP),
+                           B<float>(c, SV_HULL_VERTS), (uint32_t)L, c->s.sup_xyz);
+        c->sup_dirty = false;
+        c->sup_bodies = n;
+    }
 }
 
 void launch_gravity(sv_ctx *c, uint32_t n, float dt, const float g[3]) {
@@ -431,7 +469,7 @@ void launch_narrowphase(sv_ctx *c) {
     hipLaunchKernelGGL(k_narrow, dim3(grid), dim3(BLOCK), 0, c->stream, c->s.misc,
                        B<uint32_t>(c, SV_COLLISION_PAIRS), B<float>(c, SV_BODY_POS), c->s.rot, c->s.hrange,
                        B<uint32_t>(c, SV_SUPPORT_LOOKUP_OFFSET), B<uint32_t>(c, SV_SUPPORT_LOOKUP),
-                       B<float>(c, SV_HULL_VERTS), c->s.np_flag, c->s.np_res);
+                       B<float>(c, SV_HULL_VERTS), c->s.sup_xyz, c->s.np_flag, c->s.np_res);
     scan_exclusive(c, c->s.np_flag, c->s.np_idx, c->s.misc + MISC_NP, mp, counters + 1);
     hipLaunchKernelGGL(k_post_contacts, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, c->caps.max_contacts);
     hipLaunchKernelGGL(k_compact_contacts, dim3(grid), dim3(BLOCK), 0, c->stream, c->s.misc,

@@ -163,6 +163,23 @@ def test_full_step_matches_oracle(settled, frames, schedule, wgs, broadphase):
     ctx.close()
 
 
+def test_full_step_with_external_geometry_pointers(settled):
+    """Once a caller holds device pointers to the hulls or support lookups
+    (and may write them itself), every step derives its lookup vertices
+    again; the result is unchanged."""
+    sc, w = _stage_world(settled)
+    ctx = new_ctx(sc, w.max_pairs, w.max_contacts)
+    assert ctx.buffer_ptr("SUPPORT_LOOKUP") and ctx.buffer_ptr("HULL_VERTS")
+    w.upload(ctx)
+    for f in range(2):
+        ctx.step(w.n, w.nv, w.nh, 10, 1 / 60, (0, -9.81, 0))
+        w.oracle_step(10, 1 / 60, (0, -9.81, 0))
+        st = w.download_state(ctx)
+        for k in ("BODY_POS", "BODY_QUAT", "BODY_VEL", "BODY_OMEGA"):
+            assert_bitwise(st[k], w.a[k], f"{k} after frame {f}")
+    ctx.close()
+
+
 def test_overflow_truncates_in_canonical_order(settled):
     sc, w = _stage_world(settled)
     w.max_pairs, w.max_contacts = 40, 15
