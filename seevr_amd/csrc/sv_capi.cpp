@@ -349,6 +349,7 @@ int sv_physics_step(sv_ctx *c, uint32_t nb, uint32_t nv, uint32_t nh, uint32_t s
     const float sub_dt = dt / (float)substeps;  // physics.zig:90
     uint32_t *counters = reinterpret_cast<uint32_t *>(c->buf[SV_ATOMIC_COUNTERS]);
     sv::launch_hull_ranges(c, nb);
+    sv::ensure_support_vertices(c, nb);
     sv::fill_u32(c, counters + 2, 0u, 1);
     for (uint32_t s = 0; s < substeps; s++) {
         { sv::StageTimer t(c, "phys/clear_counters"); sv::launch_clear_counters(c); }
@@ -395,6 +396,7 @@ int sv_physics_dispatch(sv_ctx *c, uint32_t step, uint32_t count, float dt, cons
     case 3:
         if (num_bodies > k.max_bodies) return fail(SV_E_CAPACITY, "step 3: %u bodies > capacity", num_bodies);
         sv::launch_hull_ranges(c, num_bodies);
+        sv::ensure_support_vertices(c, num_bodies);
         sv::launch_rot(c, num_bodies);
         sv::launch_narrowphase(c);
         break;

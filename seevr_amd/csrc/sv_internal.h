@@ -168,6 +168,7 @@ void prims_reserve(sv_ctx *c, uint32_t max_scan_n, uint32_t max_sort_n);
 
 // ---- sv_physics.hip ----
 void launch_hull_ranges(sv_ctx *c, uint32_t n);
+void ensure_support_vertices(sv_ctx *c, uint32_t n);
 void launch_gravity(sv_ctx *c, uint32_t n, float dt, const float g[3]);
 void launch_aabb(sv_ctx *c, uint32_t n);
 void launch_broadphase(sv_ctx *c, uint32_t n);

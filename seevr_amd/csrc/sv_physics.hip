@@ -398,8 +398,14 @@ void launch_hull_ranges(sv_ctx *c, uint32_t n) {
     if (!n) return;
     hipLaunchKernelGGL(k_hull_ranges, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, c->stream, B<float>(c, SV_GEOM_DATA),
                        B<uint32_t>(c, SV_GEOM_HULL_RANGE), (int)c->params.hull_range_u32, c->s.hrange, n);
+}
+
+// Scratch::sup_xyz for bodies [0, n) when it may be stale (after
+// launch_hull_ranges); allocates on first use, so never call it while a
+// stream is being captured.
+void ensure_support_vertices(sv_ctx *c, uint32_t n) {
     const size_t L = c->caps.max_support_entries;
-    if (L < 2) return;
+    if (!n || L < 2) return;
     if (!c->s.sup_xyz) {
         void *p = nullptr;
         if (hipMalloc(&p, 3 * L * sizeof(float)) != hipSuccess) return;  // supports then read through the lookup
