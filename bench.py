@@ -281,7 +281,7 @@ def differentiable_section(frames: int = 60, scenes: int = 256, bodies: int = 64
                     "+ U[-0.5, 0.5]"}
 
 
-def sweep_variant_section(ctx, n) -> dict:
+def sweep_variant_section(ctx, n, default_variant: str = "seq") -> dict:
     """The multi-workgroup sweep's hand-off variants from the same state (the
     pile where the timed frames left it): one substep's collision stages,
     then the sweep timed with HIP events, for each variant; all must leave
@@ -290,7 +290,7 @@ def sweep_variant_section(ctx, n) -> dict:
                                                       ("BODY_OMEGA", 3))}
     dt_sub = float(np.float32(DT) / np.float32(SUBSTEPS))
     out, ref = {}, None
-    for variant in ("seq", "coherent", "fenced"):
+    for variant in ("pipelined", "seq_basic", "coherent", "fenced"):
         for key, v in state.items():
             ctx.upload(key, v)
         ctx.set_sweep_variant(variant)
@@ -309,7 +309,7 @@ def sweep_variant_section(ctx, n) -> dict:
         ref = vel if ref is None else ref
         out[variant] = {"sweep_us": round(st.get("phys/pgs_sweep", (0.0, 0))[0], 1),
                         "bit_identical": bool(np.array_equal(vel.view(np.uint32), ref.view(np.uint32)))}
-    ctx.set_sweep_variant("seq")
+    ctx.set_sweep_variant(default_variant)
     for key, v in state.items():
         ctx.upload(key, v)
     log("sweep hand-off variants from the same state: " +
@@ -362,7 +362,8 @@ def warm_start_section(ctx, n, nv, nh, factor: float = 1.0, iterations=(4, 10, 3
     return {"factor": factor, "from_same_state": out}
 
 
-def large_scene_section(frames: int = 120, timed: int = 20, compare: bool = True, solver_wgs: int = 0) -> dict:
+def large_scene_section(frames: int = 120, timed: int = 20, compare: bool = True, solver_wgs: int = 0,
+                        sweep_variant: str = "seq") -> dict:
     """The 1,048,576-body scene (70% spheres of 81 hull verts, 200x200x50 m
     over a 220x1x220 m ground; 62M hull verts, so hull ranges are read as
     uint32): `frames` frames with the LBVH broadphase, the last `timed` of
@@ -384,6 +385,7 @@ def large_scene_section(frames: int = 120, timed: int = 20, compare: bool = True
     sc.upload(ctx)
     if solver_wgs:
         ctx.set_solver("ordered", solver_wgs)
+    ctx.set_sweep_variant(sweep_variant)
     sc.close()
     log(f"large scene: {nb} bodies, {nh} hull verts, built and uploaded in {time.perf_counter() - t0:.1f} s")
     for _ in range(frames - timed):
@@ -463,6 +465,8 @@ def main() -> int:
     ap.add_argument("--solver", default="auto", choices=["auto", "ordered", "flow", "colored"],
                     help="schedule of the contact sweep (all give the serial result bit for bit)")
     ap.add_argument("--solver-wgs", type=int, default=0, help="workgroups of the sweep kernel (0 = default)")
+    ap.add_argument("--sweep-variant", default="seq", choices=["seq", "pipelined", "seq_basic", "coherent", "fenced"],
+                    help="hand-off variant of the multi-workgroup sweep")
     ap.add_argument("--broadphase", default="auto", choices=["auto", "all_pairs", "lbvh"],
                     help="broadphase method (both give the all-pairs list bit for bit)")
     ap.add_argument("--calib", action="store_true",
@@ -487,6 +491,7 @@ def main() -> int:
     scene.upload(ctx)
     ctx.set_solver(args.solver, args.solver_wgs)
     ctx.set_broadphase(args.broadphase)
+    ctx.set_sweep_variant(args.sweep_variant)
     name, arch = ctx.device_info()
     log(f"device: {name} {arch}")
     copy_gbs = ctx.copy_bandwidth(256 << 20, 1 if args.calib else 20)
@@ -541,7 +546,7 @@ def main() -> int:
     total_us = sum(v[0] * v[1] for v in phys.values())
     breakdown = {k: round(100.0 * v[0] * v[1] / total_us, 1) for k, v in phys.items()}
 
-    variants = sweep_variant_section(ctx, n)
+    variants = sweep_variant_section(ctx, n, args.sweep_variant)
 
     cpu = None
     if not args.no_cpu:
@@ -563,7 +568,7 @@ def main() -> int:
     large = None
     if not args.no_large:
         large = large_scene_section(args.large_frames, args.large_timed, not args.no_large_compare,
-                                    args.large_solver_wgs)
+                                    args.large_solver_wgs, args.sweep_variant)
 
     out = {
         "metric": "body-substeps/s @32k bodies", "value": round(value, 1), "unit": "body-substeps/s",

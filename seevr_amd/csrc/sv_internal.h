@@ -62,6 +62,9 @@ struct Scratch {
     // (lambda, iteration) per contact
     float4 *seq_v = nullptr;  // [2N]
     uint2 *seq_c = nullptr;   // [C]
+    // pipelined sequence form: per contact, both bodies' rotation and inverse
+    // inertia and both incidences' rank and degree (7 float4)
+    float4 *seq_x = nullptr;  // [C][7]
     // warm start: (geom a, geom b) keys and impulses of the last sweep
     unsigned long long *warm_key = nullptr;  // [C]
     float *warm_lam = nullptr;               // [C]
@@ -70,7 +73,9 @@ struct Scratch {
 };
 
 // Hand-off variants of the multi-CU sweeps (sv_debug_set_sweep_variant).
-enum { SWEEP_SEQ = 0, SWEEP_COHERENT = 1, SWEEP_FENCED = 2 };
+// SWEEP_SEQ picks the pipelined sequence kernel below 2^18 bodies and the
+// basic one above; SWEEP_PIPE and SWEEP_SEQ_BASIC force one of them.
+enum { SWEEP_SEQ = 0, SWEEP_COHERENT = 1, SWEEP_FENCED = 2, SWEEP_PIPE = 3, SWEEP_SEQ_BASIC = 4 };
 
 // Words per contact in Scratch::cprep (see sv_pgs.hip).
 constexpr int CPREP_WORDS = 20;

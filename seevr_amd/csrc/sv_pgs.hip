@@ -429,6 +429,248 @@ __global__ void __launch_bounds__(THREADS) k_solve_seq(
 }
 
 // ---------------------------------------------------------------------------
+// Pipelined sequence form: the sequence-form hand-off with each lane's next
+// item fetched while it works on the current one.  A lane holds up to three
+// items: the current one (polled and applied), the next one (its constants
+// in flight or loaded) and a ticket (its position in sweep order, returned
+// by the wave's atomic on the shared counter).  In one turn of the loop a
+// lane polls and applies its current item, promotes the next item when the
+// current one is done, turns last turn's ticket into loads of the next
+// item, and the wave asks for tickets for lanes without one.  Every load a
+// turn issues is consumed one turn later, after the next poll's wait, so a
+// hand-off costs one round trip however often lanes refill; an item's
+// constants come from the contact's sweep constants and its record below,
+// whose addresses depend on the contact only.  Tickets are taken in
+// increasing order per lane, so a lane's next item and ticket always follow
+// its current item: the earliest unfinished item reaches a lane's current
+// slot within two turns and then has all its predecessors done.
+
+// Record of contact c for the pipelined form: rotation and inverse inertia of
+// bodies a and b, and (rank, degree) of both incidences.
+constexpr int SEQX_WORDS = 28;  // 7 float4
+__global__ void __launch_bounds__(BLOCK) k_seq_records(const uint32_t *__restrict__ misc,
+                                                       const float *__restrict__ prep,
+                                                       const uint32_t *__restrict__ rank,
+                                                       const uint32_t *__restrict__ deg,
+                                                       const float *__restrict__ rot,
+                                                       const float *__restrict__ inv_i, float *__restrict__ rec) {
+    const uint32_t nc = misc[MISC_NC];
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const float *P = prep + (size_t)c * CPREP_WORDS;
+        const uint32_t flags = __float_as_uint(P[P_FLAGS]);
+        if (flags & F_SKIP) continue;
+        const uint32_t a = __float_as_uint(P[P_A]), b = __float_as_uint(P[P_B]);
+        float *R = rec + (size_t)c * SEQX_WORDS;
+        for (int k = 0; k < 9; k++) R[k] = rot[(size_t)a * 9 + k];
+        for (int k = 0; k < 3; k++) R[9 + k] = inv_i[(size_t)a * 3 + k];
+        for (int k = 0; k < 9; k++) R[12 + k] = rot[(size_t)b * 9 + k];
+        for (int k = 0; k < 3; k++) R[21 + k] = inv_i[(size_t)b * 3 + k];
+        // a static side's rank is never written and its degree is zero:
+        // neither is read for a static body
+        R[24] = __uint_as_float((flags & F_DYN_A) ? rank[2 * c] : 0u);
+        R[25] = __uint_as_float(deg[a]);
+        R[26] = __uint_as_float((flags & F_DYN_B) ? rank[2 * c + 1] : 0u);
+        R[27] = __uint_as_float(deg[b]);
+    }
+}
+
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+
+// The loaded words of one item: its constants row and its record.  They are
+// decoded only a turn after the loads are issued, so issuing them waits for
+// nothing.
+struct RawItem {
+    u4v p[4];      // constants words 0-15
+    uint32_t fl;   // constants word 16 (flags)
+    u4v x[SEQX_WORDS / 4];
+};
+SVD void load_raw_item(RawItem &r, uint32_t c, __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rx) {
+    static_assert(CPREP_WORDS % 4 == 0 && SEQX_WORDS % 4 == 0 && P_FLAGS == 16 && P_IMB < 16, "row layout");
+    const int op = (int)(c * (CPREP_WORDS * 4u)), ox = (int)(c * (SEQX_WORDS * 4u));
+#pragma unroll
+    for (int k = 0; k < 4; k++) r.p[k] = __builtin_amdgcn_raw_buffer_load_b128(rp, op + 16 * k, 0, 0);
+    r.fl = __builtin_amdgcn_raw_buffer_load_b32(rp, op + 4 * P_FLAGS, 0, 0);
+#pragma unroll
+    for (int k = 0; k < SEQX_WORDS / 4; k++) r.x[k] = __builtin_amdgcn_raw_buffer_load_b128(rx, ox + 16 * k, 0, 0);
+}
+
+// Item (t, c) from its loaded words: the same values as load_item.  Returns
+// the contact's flags (F_SKIP: not an item).
+SVD uint32_t decode_item(Item &it, const RawItem &r, uint32_t c, uint32_t t) {
+    uint32_t w[16], x[SEQX_WORDS];
+#pragma unroll
+    for (int k = 0; k < 16; k++) w[k] = r.p[k / 4][k % 4];
+#pragma unroll
+    for (int k = 0; k < SEQX_WORDS; k++) x[k] = r.x[k / 4][k % 4];
+    const uint32_t flags = r.fl;
+    auto f = [](uint32_t u) { return __uint_as_float(u); };
+    it.c = c;
+    it.a = w[P_A];
+    it.b = w[P_B];
+    it.dyn_a = flags & F_DYN_A;
+    it.dyn_b = flags & F_DYN_B;
+    it.ta = it.dyn_a ? t * x[25] + x[24] : 0u;
+    it.tb = it.dyn_b ? t * x[27] + x[26] : 0u;
+    it.n = mk(f(w[P_N]), f(w[P_N + 1]), f(w[P_N + 2]));
+    it.ra = mk(f(w[P_RA]), f(w[P_RA + 1]), f(w[P_RA + 2]));
+    it.rb = mk(f(w[P_RB]), f(w[P_RB + 1]), f(w[P_RB + 2]));
+    it.kn = f(w[P_KN]);
+    it.bias = f(w[P_BIAS]);
+    it.mu = f(w[P_MU]);
+    it.ima = f(w[P_IMA]);
+    it.imb = f(w[P_IMB]);
+    it.Ra.c0 = mk(f(x[0]), f(x[1]), f(x[2]));
+    it.Ra.c1 = mk(f(x[3]), f(x[4]), f(x[5]));
+    it.Ra.c2 = mk(f(x[6]), f(x[7]), f(x[8]));
+    it.iia = mk(f(x[9]), f(x[10]), f(x[11]));
+    it.Rb.c0 = mk(f(x[12]), f(x[13]), f(x[14]));
+    it.Rb.c1 = mk(f(x[15]), f(x[16]), f(x[17]));
+    it.Rb.c2 = mk(f(x[18]), f(x[19]), f(x[20]));
+    it.iib = mk(f(x[21]), f(x[22]), f(x[23]));
+    return flags;
+}
+
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS) k_solve_seqp(
+    const uint32_t *__restrict__ misc, uint32_t iters, const float *__restrict__ prep,
+    const float4 *__restrict__ rec, float4 *sv, uint2 *sc, uint32_t *ctl, uint32_t *counters) {
+    const uint32_t nc = misc[MISC_NC];
+    const uint32_t total = iters * nc;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (total == 0) return;
+    const __amdgpu_buffer_rsrc_t rv = seq_rsrc(sv), rc = seq_rsrc(sc);
+    const __amdgpu_buffer_rsrc_t rp = seq_rsrc(prep), rx = seq_rsrc(rec);
+
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    Item it;
+    RawItem nx;
+    uint32_t t = 0, tn = 0, cn = 0, qk = 0;
+    u4 a0, a1, b0, b1;
+    u2 lc;
+    uint32_t got = 0;
+    bool has = false, has_n = false, has_k = false, fin = false;
+    // the wave's ticket request of the last turn: its lanes, and the lane
+    // that holds the counter value the atomic returned
+    uint64_t pend = 0;
+    int pend_leader = 0;
+    uint32_t pend_base = 0;
+    uint64_t last = wall_clock64();
+    for (;;) {
+        bool progressed = false;
+        if (has) {
+            const int oa = (int)(it.a * 32u), ob = (int)(it.b * 32u), oc = (int)(it.c * 8u);
+            if (!(got & 1u)) a0 = __builtin_amdgcn_raw_buffer_load_b128(rv, oa, 0, CPOL_SC1);
+            if (!(got & 2u)) a1 = __builtin_amdgcn_raw_buffer_load_b128(rv, oa + 16, 0, CPOL_SC1);
+            if (!(got & 4u)) b0 = __builtin_amdgcn_raw_buffer_load_b128(rv, ob, 0, CPOL_SC1);
+            if (!(got & 8u)) b1 = __builtin_amdgcn_raw_buffer_load_b128(rv, ob + 16, 0, CPOL_SC1);
+            if (!(got & 16u)) lc = __builtin_amdgcn_raw_buffer_load_b64(rc, oc, 0, CPOL_SC1);
+        }
+        // one wait per turn on every path: the poll's words and the loads of
+        // the last turn's next item (so promoting it below waits for nothing)
+        __builtin_amdgcn_s_waitcnt(0);
+        if (has) {
+            const int oa = (int)(it.a * 32u), ob = (int)(it.b * 32u), oc = (int)(it.c * 8u);
+            got |= ((!it.dyn_a || a0[3] == it.ta) ? 1u : 0u) | ((!it.dyn_a || a1[3] == it.ta) ? 2u : 0u) |
+                   ((!it.dyn_b || b0[3] == it.tb) ? 4u : 0u) | ((!it.dyn_b || b1[3] == it.tb) ? 8u : 0u) |
+                   (lc[1] == t ? 16u : 0u);
+            if (got == 31u) {
+                f3 la = mk(__uint_as_float(a0[0]), __uint_as_float(a0[1]), __uint_as_float(a0[2]));
+                f3 wa = mk(__uint_as_float(a1[0]), __uint_as_float(a1[1]), __uint_as_float(a1[2]));
+                f3 lb = mk(__uint_as_float(b0[0]), __uint_as_float(b0[1]), __uint_as_float(b0[2]));
+                f3 wb = mk(__uint_as_float(b1[0]), __uint_as_float(b1[1]), __uint_as_float(b1[2]));
+                const float nl = item_update(it, la, wa, lb, wb, __uint_as_float(lc[0]));
+                if (it.dyn_a) {
+                    const uint32_t k = it.ta + 1u;
+                    __builtin_amdgcn_raw_buffer_store_b128(
+                        u4{__float_as_uint(la.x), __float_as_uint(la.y), __float_as_uint(la.z), k}, rv, oa, 0,
+                        CPOL_SC1);
+                    __builtin_amdgcn_raw_buffer_store_b128(
+                        u4{__float_as_uint(wa.x), __float_as_uint(wa.y), __float_as_uint(wa.z), k}, rv, oa + 16, 0,
+                        CPOL_SC1);
+                }
+                if (it.dyn_b) {
+                    const uint32_t k = it.tb + 1u;
+                    __builtin_amdgcn_raw_buffer_store_b128(
+                        u4{__float_as_uint(lb.x), __float_as_uint(lb.y), __float_as_uint(lb.z), k}, rv, ob, 0,
+                        CPOL_SC1);
+                    __builtin_amdgcn_raw_buffer_store_b128(
+                        u4{__float_as_uint(wb.x), __float_as_uint(wb.y), __float_as_uint(wb.z), k}, rv, ob + 16, 0,
+                        CPOL_SC1);
+                }
+                __builtin_amdgcn_raw_buffer_store_b64(u2{__float_as_uint(nl), t + 1u}, rc, oc, 0, CPOL_SC1);
+                has = false;
+                progressed = true;
+            }
+        }
+        // promote the next item (a skipped contact is dropped)
+        if (!has && has_n) {
+            has_n = false;
+            if (!(decode_item(it, nx, cn, tn) & F_SKIP)) {
+                t = tn;
+                has = true;
+                got = 0;
+            }
+        }
+        // last turn's tickets
+        if (pend) {
+            const uint32_t base = __shfl(pend_base, pend_leader, 64);
+            if ((pend >> lane) & 1ull) {
+                const uint32_t q = base + (uint32_t)__popcll(pend & lt);
+                if (q >= total) fin = true;
+                else {
+                    qk = q;
+                    has_k = true;
+                }
+            }
+            pend = 0;
+        }
+        // a ticket becomes the next item: its loads are consumed next turn
+        if (has_k && !has_n) {
+            tn = qk / nc;
+            cn = qk - tn * nc;
+            load_raw_item(nx, cn, rp, rx);
+            has_n = true;
+            has_k = false;
+        }
+        // tickets for lanes that hold nothing (a lane that reserved an item
+        // while its current one waits could hold up an item that is ready):
+        // the counter value is read next turn
+        const uint64_t need = __ballot(!has && !has_n && !has_k && !fin);
+        if (need) {
+            pend = need;
+            pend_leader = __builtin_ctzll(need);
+            if (lane == pend_leader) {
+                // an offset the compiler cannot see as uniform keeps this a
+                // plain one-lane atomic whose value is read only next turn
+                // (a uniform address is rewritten into a wave-aggregated
+                // atomic that waits for the value at once)
+                uint32_t z = 0;
+                asm volatile("" : "+v"(z));
+                pend_base = __hip_atomic_fetch_add(ctl + 1 + z, (uint32_t)__popcll(need), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (!pend && !__any(has || has_n || has_k)) break;
+        if (__any(progressed)) {
+            last = wall_clock64();
+        } else {
+            const uint64_t idle = wall_clock64() - last;
+            if (idle > ABORT_CHECK_TICKS) {
+                if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+                if (idle > STALL_TICKS) {
+                    __hip_atomic_store(ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (lane == 0) atomicOr(counters + 2, (uint32_t)SV_STATUS_SOLVER_TIMEOUT);
+                    return;
+                }
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Flow form.  Item (t, c) has one predecessor per dynamic body (the previous
 // update of that body in sweep order: none for the body's first contact in
 // iteration 0) and one successor per dynamic body.  A countdown per item
@@ -657,6 +899,18 @@ bool seq_reserve(sv_ctx *c) {
     return true;
 }
 
+// Records of the pipelined sequence form (allocated on first use).
+bool seqx_reserve(sv_ctx *c) {
+    Scratch &s = c->s;
+    if (s.seq_x) return true;
+    void *x = nullptr;
+    const size_t mc = c->caps.max_contacts ? c->caps.max_contacts : 1;
+    if (hipMalloc(&x, mc * SEQX_WORDS * sizeof(float)) != hipSuccess) return false;
+    s.lazy.push_back(x);
+    s.seq_x = static_cast<float4 *>(x);
+    return true;
+}
+
 // Largest body count whose tickets fit in the single-workgroup kernel's LDS.
 uint32_t solver_lds_bodies(sv_ctx *c) {
     int lds = 0;
@@ -733,8 +987,18 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
     }
     hipLaunchKernelGGL(k_rank, dim3(rgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.inc_key[where],
                        c->s.inc_val[where], c->s.deg_off, c->s.rank);
-    const bool seq_ok = c->sweep_variant == SWEEP_SEQ && (uint64_t)n_bodies * 32u < 0x7FFFFFFFull &&
-                        (uint64_t)mc * 8u < 0x7FFFFFFFull && seq_reserve(c);
+    // Sequence kernel: the pipelined one wins on a 32k-body pile (sweep 1.65
+    // -> 1.44 ms per substep), the basic one on the 1M-body scene (1.41 ms
+    // against 3.39: the pipelined lanes idle two turns per refill, which a
+    // wide dependency graph cannot hide).
+    const bool seq_variant = c->sweep_variant == SWEEP_SEQ || c->sweep_variant == SWEEP_PIPE ||
+                             c->sweep_variant == SWEEP_SEQ_BASIC;
+    const bool pipe = c->sweep_variant == SWEEP_PIPE || (c->sweep_variant == SWEEP_SEQ && n_bodies < (1u << 18));
+    // (buffer offsets of the words and records are 31-bit)
+    const bool seq_ok = seq_variant && (uint64_t)n_bodies * 32u < 0x7FFFFFFFull &&
+                        (uint64_t)mc * 8u < 0x7FFFFFFFull &&
+                        (!pipe || (uint64_t)mc * SEQX_WORDS * 4u < 0x7FFFFFFFull) && seq_reserve(c) &&
+                        (!pipe || seqx_reserve(c));
     // Form choice: small scenes keep every hand-off inside one CU; large ones
     // need the wider in-order window of the multi-CU form.  On a 32k-body
     // pile (1.5M sweep items per substep) the sequence form was fastest with
@@ -742,7 +1006,7 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
     // 96, 2.24 with 32 four-wave workgroups), on the 1M-body scene (7.2M
     // items) with 384 (1.41 ms; 1.80 at 192, 1.78 at 768); the ticket form
     // with coherent accesses at 64 four-wave workgroups (2.96 ms; 3.21 at 32).
-    const bool seq_form = c->sweep_variant == SWEEP_SEQ;
+    const bool seq_form = seq_variant;
     const uint32_t seq_wgs = n_bodies >= (1u << 18) ? 384u : 192u;
     uint32_t wgs = c->solver_workgroups ? c->solver_workgroups
                                         : (n_bodies >= 2048 ? (seq_form ? seq_wgs : 64u) : 1u);
@@ -751,7 +1015,13 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
         if (pgrid > 8192) pgrid = 8192;
         hipLaunchKernelGGL(k_seq_pack, dim3(pgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, vel, omega, lam,
                            c->s.seq_v, c->s.seq_c);
-        {
+        if (pipe) {
+            hipLaunchKernelGGL(k_seq_records, dim3(grid), dim3(BLOCK), 0, c->stream, c->s.misc, c->s.cprep,
+                               c->s.rank, c->s.deg, c->s.rot, inv_i, reinterpret_cast<float *>(c->s.seq_x));
+            StageTimer sweep_timer(c, "phys/pgs_sweep");
+            hipLaunchKernelGGL(k_solve_seqp<SEQ_THREADS>, dim3(wgs), dim3(SEQ_THREADS), 0, c->stream, c->s.misc,
+                               iters, c->s.cprep, c->s.seq_x, c->s.seq_v, c->s.seq_c, c->s.solve_ctl, counters);
+        } else {
             StageTimer sweep_timer(c, "phys/pgs_sweep");
             hipLaunchKernelGGL(k_solve_seq<SEQ_THREADS>, dim3(wgs), dim3(SEQ_THREADS), 0, c->stream, c->s.misc,
                                iters, c->s.cprep, c->s.rank, c->s.deg, c->s.seq_v, c->s.seq_c, c->s.rot, inv_i,

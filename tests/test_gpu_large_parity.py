@@ -1,8 +1,9 @@
 """Parity where the automatic choices pick the multi-workgroup paths: a
 4,096-body container scene is stepped 240 frames on the device, its state is
 handed to the oracle, and two more frames must agree bit for bit (the
-sequence-form sweep, chosen from 2,048 bodies, and the all-pairs
-broadphase; then the LBVH broadphase from the same state)."""
+sequence-form sweep, chosen from 2,048 bodies, in its pipelined kernel, and
+the all-pairs broadphase; then the LBVH broadphase, and the basic
+sequence kernel)."""
 import numpy as np
 import pytest
 
@@ -11,8 +12,8 @@ from harness import World, assert_bitwise
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("broadphase", ["auto", "lbvh"])
-def test_multi_workgroup_paths_match_oracle(broadphase):
+@pytest.mark.parametrize("broadphase,variant", [("auto", "seq"), ("lbvh", "seq"), ("auto", "seq_basic")])
+def test_multi_workgroup_paths_match_oracle(broadphase, variant):
     from seevr_amd import Context, Scene, caps_for, default_caps, default_params
     n = 4096 + 5
     sc = Scene(default_caps(max_bodies=n, max_hull_verts=n * 289, max_vertices=n * 289, max_triangles=n * 512,
@@ -23,6 +24,7 @@ def test_multi_workgroup_paths_match_oracle(broadphase):
     ctx = Context(caps_for(sc, pairs, contacts), default_params(pgs_iterations=30))
     sc.upload(ctx)
     ctx.set_broadphase(broadphase)
+    ctx.set_sweep_variant(variant)
     for _ in range(240):
         ctx.step(c["num_bodies"], c["num_vertices"], c["num_hull_verts"])
     w = World.from_scene(sc, max_pairs=pairs, max_contacts=contacts, pgs_iterations=30)
