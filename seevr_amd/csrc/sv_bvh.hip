@@ -330,8 +330,8 @@ __global__ void k_copy_u32(uint32_t n, const uint32_t *src, uint32_t *dst) {
 // the tree with its own box and reports every leaf body i < j whose box is
 // not apart from its own and that is not static together with j.  A node is
 // skipped when j's box is apart from the node's box (the node box is the
-// exact union of the leaves, and float subtraction is monotonic, so no leaf
-// below can overlap), when its smallest index is >= j, or when j and the
+// exact union of the leaves: a bound of j's box below the node's lower bound
+// is below every leaf's, so no leaf below can overlap), when its smallest index is >= j, or when j and the
 // whole subtree are static.  The reported set is exactly the all-pairs set.
 // Pairs are counted per row i, the row counts scanned into offsets (total =
 // counters[0]), rows that start below max_pairs filled, and the filled part

@@ -132,14 +132,23 @@ SVD float sv_atan2f(float y, float x) {
 }
 
 // Broadphase separation (physics.comp:461-466): the boxes are disjoint iff on
-// some axis max_a < min_b or max_b < min_a, i.e. iff the larger of the two
-// gaps is positive on some axis (exact for finite bounds; touching boxes
-// overlap).  Only x, y, z are read.
+// some axis max_a < min_b or max_b < min_a (touching boxes overlap; a NaN
+// bound separates nothing, as in the reference).  Six compares whose lane
+// masks combine in scalar registers.  Only x, y, z are read.
 SVD bool boxes_apart(float4 alo, float4 ahi, float4 blo, float4 bhi) {
-    float sx = fmaxf(blo.x - ahi.x, alo.x - bhi.x);
-    float sy = fmaxf(blo.y - ahi.y, alo.y - bhi.y);
-    float sz = fmaxf(blo.z - ahi.z, alo.z - bhi.z);
-    return fmaxf(sx, fmaxf(sy, sz)) > 0.0f;
+    return ahi.x < blo.x || bhi.x < alo.x || ahi.y < blo.y || bhi.y < alo.y || ahi.z < blo.z || bhi.z < alo.z;
+}
+
+// The same test as one vector value for the all-pairs loop: the largest of
+// the six separations min_b - max_a, min_a - max_b is positive iff the boxes
+// are apart, for finite bounds (a difference of two floats is positive iff
+// the first is larger, with gradual underflow) and for the empty box
+// (+inf lower, -inf upper bounds), whose separations are +inf.  Six
+// subtractions and three maxima, so the lane mask comes from one compare.
+SVD float boxes_gap(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+    // left-nested so the compiler forms two three-input maxima
+    const float g = fmaxf(fmaxf(blo.x - ahi.x, alo.x - bhi.x), blo.y - ahi.y);
+    return fmaxf(fmaxf(fmaxf(g, alo.y - bhi.y), blo.z - ahi.z), alo.z - bhi.z);
 }
 
 // Device-scope release that is complete on return: the release fence writes

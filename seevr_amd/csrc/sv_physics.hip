@@ -72,13 +72,11 @@ __global__ void __launch_bounds__(BLOCK) k_aabb(uint32_t n, const float *__restr
 // Work is tiled in two dimensions so the triangle balances: a block takes
 // 32 consecutive rows (8 per wave) against one column segment of
 // bp_seg_len(n) bodies, streamed through LDS 256 boxes at a time, 64 j per
-// lane-round.  Each test is branch-free: for finite IEEE floats
-// max(lo_j - hi_i, lo_i - hi_j) > 0 exactly when hi_i < lo_j or hi_j < lo_i
-// (a difference of two floats is positive iff the first is larger, with
-// gradual underflow), so row i overlaps lane j iff the largest of the three
-// axis separations is not positive.  Lanes past the segment or n carry an
-// empty box.  A row's hits in one round form a 64-bit ballot numbered in j
-// order.
+// lane-round.  Each test is branch-free (boxes_gap: six subtractions, three
+// maxima, one compare); the compare's lane mask, and-ed in scalar registers
+// with the lanes the row may pair with (j > i on the diagonal tile, not
+// static for a static row), is the row's 64-bit hit ballot, numbered in j
+// order.  Lanes past the segment or n carry an empty box.
 //
 // The test pass writes, per (row, segment), the hit count and the first
 // BP_STAGE partners.  An exclusive scan over the counts in (row, segment)
@@ -119,7 +117,7 @@ __global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, uint32_t nseg,
     float4 rlo[BP_ROWS_PER_WAVE], rhi[BP_ROWS_PER_WAVE];
     uint32_t cnt[BP_ROWS_PER_WAVE];
     uint32_t base[BP_ROWS_PER_WAVE];
-    bool any_static = false;
+    uint32_t row_static = 0;  // bit r: row r is a static body (wave-uniform)
 #pragma unroll
     for (int r = 0; r < BP_ROWS_PER_WAVE; r++) {
         uint32_t i = row0 + r;
@@ -127,7 +125,7 @@ __global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, uint32_t nseg,
         rlo[r] = box[2 * ic];
         rhi[r] = box[2 * ic + 1];
         if (i >= n) { rlo[r] = empty_lo; rhi[r] = empty_hi; }
-        any_static |= rlo[r].w != 0.0f;
+        row_static |= (__builtin_amdgcn_readfirstlane(__float_as_uint(rlo[r].w)) != 0u ? 1u : 0u) << r;
         // keep the row boxes in vector registers (64 scalar registers would
         // spill); the empty asm only pins the register class
         asm volatile("" : "+v"(rlo[r].x), "+v"(rlo[r].y), "+v"(rlo[r].z), "+v"(rhi[r].x), "+v"(rhi[r].y),
@@ -153,13 +151,23 @@ __global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, uint32_t nseg,
             const uint32_t j = t0 + ch * 64 + lane;
             const float4 lo = s_lo[ch * 64 + lane];
             const float4 hi = s_hi[ch * 64 + lane];
+            // lanes each row may pair with: j > i on the diagonal tile, and
+            // not static when the row is static (scalar masks, set per chunk)
+            const uint64_t nonstatic = row_static ? ~__builtin_amdgcn_ballot_w64(lo.w != 0.0f) : ~0ull;
+            uint64_t allow[BP_ROWS_PER_WAVE];
+#pragma unroll
+            for (int r = 0; r < BP_ROWS_PER_WAVE; r++) {
+                allow[r] = diag ? __builtin_amdgcn_ballot_w64(j > row0 + r) : ~0ull;
+                if (row_static & (1u << r)) allow[r] &= nonstatic;
+            }
 #pragma unroll
             for (int r = 0; r < BP_ROWS_PER_WAVE; r++) {
                 const uint32_t i = row0 + r;
-                bool hit = !boxes_apart(rlo[r], rhi[r], lo, hi);
-                if (diag) hit = hit && j > i;
-                if (any_static) hit = hit && !(rlo[r].w != 0.0f && lo.w != 0.0f);
-                const uint64_t m = __ballot(hit);
+                // one compare's lane mask, and-ed with the allowed lanes in
+                // scalar registers
+                const uint64_t m =
+                    __builtin_amdgcn_ballot_w64(!(boxes_gap(rlo[r], rhi[r], lo, hi) > 0.0f)) & allow[r];
+                const bool hit = (m >> lane) & 1ull;
                 if (m) {
                     const uint32_t k = cnt[r] + (uint32_t)__popcll(m & lt);
                     if (EMIT) {
