@@ -47,7 +47,7 @@ def build_workload(count: int, seed: int):
 
 
 PROFILE = ROOT / "profiles" / "c2_32k_container_kernels.md"
-STAGE_KERNEL = {"phys/pgs_sweep": "k_solve_seq<64>", "phys/aabb": "k_aabb", "phys/gravity": "k_gravity",
+STAGE_KERNEL = {"phys/pgs_sweep": "k_solve_seqp<64>", "phys/aabb": "k_aabb", "phys/gravity": "k_gravity",
                 "phys/integrate": "k_integrate", "phys/broad_phase": "k_broadphase<false>",
                 "bvh/morton_codes": "k_morton", "bvh/radix_sort": "radix sort (k_sort_hist, k_scan_*, k_sort_scatter)"}
 
