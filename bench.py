@@ -113,21 +113,26 @@ def roofline_entry(stage, stages, n, nh, n_dyn, pairs, nc):
             "avg_us": round(us, 2), "alg_bytes": int(alg)}
 
 
-def cpu_baseline(ctx, scene, n, substeps: int) -> dict:
-    """Time the scalar CPU oracle on `substeps` substeps from the device state."""
+def cpu_baseline(ctx, scene, n, substeps: int, mt: bool = True) -> dict:
+    """Time the C oracle on `substeps` substeps from the device state: the
+    OpenMP build (every host thread for the per-body, per-row and per-pair
+    stages, the sweep on one core; the serial build's results) or, with
+    mt=False, the scalar build on one core."""
     sys.path.insert(0, str(ROOT / "tests"))
-    from harness import World
+    from harness import World, oracle
     w = World.from_scene(scene, max_pairs=ctx.caps.max_pairs, max_contacts=ctx.caps.max_contacts,
                          pgs_iterations=PGS_ITERS)
     st = w.download_state(ctx)
     for k, v in st.items():
         w.a[k][:] = v
+    cores = oracle(mt=True).threads() if mt else 1
     t0 = time.perf_counter()
-    w.oracle_step(substeps, DT * substeps / SUBSTEPS, GRAVITY)
+    w.oracle_step(substeps, DT * substeps / SUBSTEPS, GRAVITY, mt=mt)
     el = time.perf_counter() - t0
-    return {"value": n * substeps / el, "unit": "body-substeps/s", "cores": 1, "kind": "port",
+    how = f"C oracle, OpenMP build, {cores} threads (sweep serial)" if mt else "scalar C oracle, 1 core"
+    return {"value": n * substeps / el, "unit": "body-substeps/s", "cores": cores, "kind": "port",
             "sample": f"{substeps} substeps of the same scene from the state at the start of the timed "
-                      f"frames (N={n}), scalar C oracle, 1 core, {el:.1f} s"}
+                      f"frames (N={n}), {how}, {el:.1f} s"}
 
 
 def batched_section(frames: int, warmup: int, scenes: int = 4096, plain_sample: int = 32) -> dict:
@@ -449,7 +454,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=240)
     ap.add_argument("--bodies", type=int, default=32768)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-sample-substeps", type=int, default=2)
+    ap.add_argument("--cpu-sample-substeps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-engine section")
     ap.add_argument("--batched-frames", type=int, default=60)
@@ -548,11 +553,13 @@ def main() -> int:
 
     variants = sweep_variant_section(ctx, n, args.sweep_variant)
 
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu:
         log("timing the CPU oracle baseline ...")
         cpu = cpu_baseline(ctx, scene, n, args.cpu_sample_substeps)
-        log(f"cpu baseline: {cpu['value']:.0f} body-substeps/s")
+        cpu1 = cpu_baseline(ctx, scene, n, 2, mt=False)
+        log(f"cpu baseline: {cpu['value']:.0f} body-substeps/s on {cpu['cores']} threads, "
+            f"{cpu1['value']:.0f} on one core")
     warm = None
     if not args.no_warm:
         warm = warm_start_section(ctx, n, nv, nh)
@@ -580,7 +587,8 @@ def main() -> int:
                    "bodies": n, "hull_verts": nh, "substeps": SUBSTEPS, "pgs_iterations": PGS_ITERS,
                    "frames_before_timing": args.warmup, "pairs_last_substep": pairs,
                    "contacts_last_substep": contacts},
-        "roofline": roof, "roofline_hbm_stages": hbm_stages, "cpu_baseline": cpu, "stage_share_pct": breakdown,
+        "roofline": roof, "roofline_hbm_stages": hbm_stages, "cpu_baseline": cpu, "cpu_baseline_one_core": cpu1,
+        "stage_share_pct": breakdown,
         "lbvh_per_frame": lbvh, "sweep_variants": variants, "warm_start": warm, "batched": batched, "differentiable": diff, "large_scene": large,
     }
     print(json.dumps(out), flush=True)

@@ -15,11 +15,33 @@
  * brute force for AABB/broadphase/support/Karras/refit, std sort for the
  * radix sort, KATs for Morton, closed-form sphere/box MPR depths, momentum
  * and single-contact impulse identities for PGS, libm for atan2f.
+ *
+ * OR_OMP (libsvoracle_mt.so, the benchmark's multi-core CPU baseline): the
+ * per-element stages run under OpenMP, the broadphase counts each row's
+ * pairs, scans the counts and fills rows in parallel, and the narrowphase
+ * runs MPR per pair in parallel then appends contacts in pair order; the
+ * sweep stays serial.  Results equal the serial build's bit for bit
+ * (tests/test_oracle_mt.py).
  */
 #include "sv_oracle.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef OR_OMP
+#include <omp.h>
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define OMP_FOR
+#endif
+
+/* Threads the parallel stages use (1 in the serial builds). */
+int OR(threads)(void) {
+#ifdef OR_OMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 typedef or_real R;
 typedef struct { R x, y, z; } v3;
@@ -146,6 +168,7 @@ static inline void hull_info(const or_bufs *b, uint32_t bi, uint32_t *vs, uint32
 
 /* physics.comp:418-428 */
 void OR(step_gravity)(or_bufs *b, uint32_t count, R dt, const R g[3]) {
+    OMP_FOR
     for (uint32_t i = 0; i < count; i++) {
         if (b->body_inv_mass[i] <= (R)0) continue;
         for (int k = 0; k < 3; k++)
@@ -157,6 +180,7 @@ void OR(step_gravity)(or_bufs *b, uint32_t count, R dt, const R g[3]) {
 /* physics.comp:431-448 (step 1, the per-body semantics the 9/10/11
  * cascade computes in pieces). */
 void OR(step_compute_aabb)(or_bufs *b, uint32_t count) {
+    OMP_FOR
     for (uint32_t i = 0; i < count; i++) {
         m3 Rm = quat_to_mat(b->body_quat + i * 4);
         v3 p = ld3(b->body_pos, i);
@@ -174,7 +198,48 @@ void OR(step_compute_aabb)(or_bufs *b, uint32_t count) {
 }
 
 /* physics.comp:451-472, rows i in order, j ascending. */
+static inline int bp_hit(const or_bufs *b, uint32_t i, uint32_t j, v3 mna, v3 mxa) {
+    if (b->body_inv_mass[i] <= (R)0 && b->body_inv_mass[j] <= (R)0) return 0;
+    v3 mnb = ld3(b->aabb_min, j), mxb = ld3(b->aabb_max, j);
+    return !(mxa.x < mnb.x || mxb.x < mna.x || mxa.y < mnb.y || mxb.y < mna.y || mxa.z < mnb.z || mxb.z < mna.z);
+}
 void OR(step_broad_phase)(or_bufs *b, uint32_t count) {
+#ifdef OR_OMP
+    /* row counts, their scan, then rows filled at their offsets: the serial
+     * loop's list and counter */
+    uint32_t *off = (uint32_t *)malloc(((size_t)count + 1) * sizeof(uint32_t));
+    if (off) {
+#pragma omp parallel for schedule(dynamic, 16)
+        for (uint32_t i = 0; i < count; i++) {
+            v3 mna = ld3(b->aabb_min, i), mxa = ld3(b->aabb_max, i);
+            uint32_t k = 0;
+            for (uint32_t j = i + 1; j < count; j++) k += (uint32_t)bp_hit(b, i, j, mna, mxa);
+            off[i] = k;
+        }
+        uint32_t run = b->counters[0];
+        for (uint32_t i = 0; i < count; i++) {
+            uint32_t k = off[i];
+            off[i] = run;
+            run += k;
+        }
+#pragma omp parallel for schedule(dynamic, 16)
+        for (uint32_t i = 0; i < count; i++) {
+            v3 mna = ld3(b->aabb_min, i), mxa = ld3(b->aabb_max, i);
+            uint32_t idx = off[i];
+            for (uint32_t j = i + 1; j < count; j++) {
+                if (!bp_hit(b, i, j, mna, mxa)) continue;
+                if (idx < b->max_pairs) {
+                    b->pairs[(size_t)idx * 2] = i;
+                    b->pairs[(size_t)idx * 2 + 1] = j;
+                }
+                idx++;
+            }
+        }
+        b->counters[0] = run;
+        free(off);
+        return;
+    }
+#endif
     for (uint32_t i = 0; i < count; i++) {
         v3 mna = ld3(b->aabb_min, i), mxa = ld3(b->aabb_max, i);
         for (uint32_t j = i + 1; j < count; j++) {
@@ -430,6 +495,35 @@ static void add_contact(or_bufs *b, v3 cp, v3 n, R pen, uint32_t ba, uint32_t bb
 /* physics.comp:475-491 */
 void OR(step_narrow_phase)(or_bufs *b) {
     uint32_t np = b->counters[0] < b->max_pairs ? b->counters[0] : b->max_pairs;
+#ifdef OR_OMP
+    /* MPR per pair in parallel, contacts appended in pair order */
+    v3 *res = (v3 *)malloc((size_t)np * 2 * sizeof(v3));
+    R *dep = (R *)malloc((size_t)np * sizeof(R));
+    unsigned char *hit = (unsigned char *)malloc((size_t)np);
+    if (res && dep && hit) {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (uint32_t pi = 0; pi < np; pi++) {
+            v3 n = V(0, 0, 0), cp = V(0, 0, 0);
+            R depth = 0;
+            hit[pi] = (unsigned char)(mpr(b, b->pairs[pi * 2], b->pairs[pi * 2 + 1], &n, &depth, &cp) &&
+                                      depth > (R)0);
+            res[2 * (size_t)pi] = n;
+            res[2 * (size_t)pi + 1] = cp;
+            dep[pi] = depth;
+        }
+        for (uint32_t pi = 0; pi < np; pi++)
+            if (hit[pi])
+                add_contact(b, res[2 * (size_t)pi + 1], res[2 * (size_t)pi], dep[pi], b->pairs[pi * 2],
+                            b->pairs[pi * 2 + 1]);
+        free(res);
+        free(dep);
+        free(hit);
+        return;
+    }
+    free(res);
+    free(dep);
+    free(hit);
+#endif
     for (uint32_t pi = 0; pi < np; pi++) {
         uint32_t a = b->pairs[pi * 2], bb = b->pairs[pi * 2 + 1];
         v3 n, cp;
@@ -440,6 +534,7 @@ void OR(step_narrow_phase)(or_bufs *b) {
 
 /* physics.comp:494-497 */
 void OR(step_clear)(or_bufs *b) {
+    OMP_FOR
     for (uint32_t i = 0; i < b->max_contacts; i++) b->lambda_n[i] = (R)0;
 }
 
@@ -571,6 +666,7 @@ uint32_t OR(warm_save)(const or_bufs *b, uint64_t *keys, R *lam) {
 
 /* physics.comp:586-604 */
 void OR(step_integrate)(or_bufs *b, uint32_t count, R dt) {
+    OMP_FOR
     for (uint32_t i = 0; i < count; i++) {
         if (b->body_inv_mass[i] <= (R)0) continue;
         for (int k = 0; k < 3; k++) b->body_pos[i * 3 + k] += b->body_vel[i * 3 + k] * dt;
@@ -590,6 +686,7 @@ void OR(step_integrate)(or_bufs *b, uint32_t count, R dt) {
 
 /* physics.comp:607-627 */
 void OR(step_update_verts)(or_bufs *b, uint32_t count, uint32_t num_bodies) {
+    OMP_FOR
     for (uint32_t vid = 0; vid < count; vid++) {
         uint32_t lo = 0, hi = num_bodies;
         while (lo + 1 < hi) {

@@ -73,6 +73,8 @@ void OR(step_integrate)(or_bufs *b, uint32_t count, or_real dt);
 void OR(step_update_verts)(or_bufs *b, uint32_t count, uint32_t num_bodies);
 void OR(step_clear_counters)(or_bufs *b);
 /* Physics.step sequencing (physics.zig:80-117). */
+/* threads of the parallel stages: omp_get_max_threads() in the OpenMP build, else 1 */
+int OR(threads)(void);
 void OR(physics_step)(or_bufs *b, uint32_t num_bodies, uint32_t num_vertices,
                       uint32_t num_hull_verts, uint32_t substeps, float dt,
                       const float g[3]);

@@ -16,6 +16,7 @@ ROOT = Path(__file__).resolve().parents[1]
 ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "_build" / "libsvoracle.so"
 ORACLE_LIB64 = ORACLE_DIR / "_build" / "libsvoracle64.so"
+ORACLE_LIB_MT = ORACLE_DIR / "_build" / "libsvoracle_mt.so"  # OpenMP stages, same results
 
 vp = C.c_void_p
 
@@ -60,11 +61,11 @@ def build_oracle() -> None:
     subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
 
 
-def oracle(double: bool = False) -> C.CDLL:
-    key = "f64" if double else "f32"
+def oracle(double: bool = False, mt: bool = False) -> C.CDLL:
+    key = "f64" if double else ("f32mt" if mt else "f32")
     if key in _or:
         return _or[key]
-    path = ORACLE_LIB64 if double else ORACLE_LIB
+    path = ORACLE_LIB64 if double else (ORACLE_LIB_MT if mt else ORACLE_LIB)
     if not path.exists():
         build_oracle()
     lib = C.CDLL(str(path))
@@ -75,6 +76,7 @@ def oracle(double: bool = False) -> C.CDLL:
     u32 = C.c_uint32
     sig = {
         "atan2f": (C.c_float, [C.c_float, C.c_float]),
+        "threads": (C.c_int, []),
         "step_gravity": (None, [B, u32, R, C.POINTER(R)]),
         "step_compute_aabb": (None, [B, u32]),
         "step_broad_phase": (None, [B, u32]),
@@ -222,13 +224,14 @@ class World:
         self._bufs = b
         return b
 
-    def lib(self):
-        return oracle(self.double)
+    def lib(self, mt: bool = False):
+        return oracle(self.double, mt and not self.double)
 
-    def oracle_step(self, substeps=10, dt=1.0 / 60.0, g=(0.0, -9.81, 0.0)):
+    def oracle_step(self, substeps=10, dt=1.0 / 60.0, g=(0.0, -9.81, 0.0), mt: bool = False):
+        """physics_step of the oracle; mt=True runs the OpenMP build (same results)."""
         b = self.bufs()
         gg = (C.c_float * 3)(*g)
-        self.lib().physics_step(C.byref(b), self.n, self.nv, self.nh, substeps, C.c_float(dt), gg)
+        self.lib(mt).physics_step(C.byref(b), self.n, self.nv, self.nh, substeps, C.c_float(dt), gg)
 
     def oracle_step_warm(self, factor: float, substeps=10, dt=1.0 / 60.0, g=(0.0, -9.81, 0.0)):
         """physics_step with warm start; the cache lives on the World."""
