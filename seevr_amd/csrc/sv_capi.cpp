@@ -358,7 +358,9 @@ int sv_physics_step(sv_ctx *c, uint32_t nb, uint32_t nv, uint32_t nh, uint32_t s
         { sv::StageTimer t(c, "phys/broad_phase"); sv::launch_broadphase(c, nb); }
         { sv::StageTimer t(c, "phys/narrow_phase"); sv::launch_narrowphase(c); }
         { sv::StageTimer t(c, "phys/clear_impulse"); sv::launch_clear_lambda(c); }
+        c->rot_fresh = true;
         { sv::StageTimer t(c, "phys/pgs_solve"); sv::launch_solve(c, nb, sub_dt); }
+        c->rot_fresh = false;
         { sv::StageTimer t(c, "phys/integrate"); sv::launch_integrate(c, nb, sub_dt); }
     }
     { sv::StageTimer t(c, "phys/render_verts"); sv::launch_update_verts(c, nv, nb); }

@@ -116,6 +116,9 @@ struct sv_ctx {
     // (state and update count in one word), coherent tickets, or tickets with
     // L2 write-back / invalidation fences
     int sweep_variant = sv::SWEEP_SEQ;
+    // set by sv_physics_step between its AABB stage (which caches every
+    // body's rotation) and its sweep: the sweep then skips recomputing them
+    bool rot_fresh = false;
     // launch failure to report at the next synchronization
     hipError_t launch_err = hipSuccess;
     // broadphase: 0 = chosen by body count, 1 = all pairs (tiled N^2), 2 = LBVH traversal

@@ -54,12 +54,21 @@ constexpr uint64_t ABORT_CHECK_TICKS = 2000ull;  // 20 us
 template <typename T>
 inline T *B(sv_ctx *c, int id) { return reinterpret_cast<T *>(c->buf[id]); }
 
-__global__ void k_set_ninc(uint32_t *misc) { misc[MISC_NINC] = 2u * misc[MISC_NC]; }
-
-// contacts to solve: min(counters[1], max_contacts) (physics.comp:502)
-__global__ void k_solve_count(const uint32_t *counters, uint32_t *misc, uint32_t max_contacts) {
-    uint32_t t = counters[1];
-    misc[MISC_NC] = t < max_contacts ? t : max_contacts;
+// The sweep's setup in one launch: contacts to solve, min(counters[1],
+// max_contacts) (physics.comp:502), the incidence count, the cleared count
+// of swept contacts, per-body degrees and the sweep's control words.
+__global__ void __launch_bounds__(256) k_sweep_setup(const uint32_t *counters, uint32_t *misc, uint32_t max_contacts,
+                                                     uint32_t *deg, uint32_t n_bodies, uint32_t *ctl) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_bodies) deg[i] = 0u;
+    if (i == 0) {
+        const uint32_t t = counters[1];
+        const uint32_t nc = t < max_contacts ? t : max_contacts;
+        misc[MISC_NC] = nc;
+        misc[MISC_NINC] = 2u * nc;
+        misc[MISC_NACT] = 0u;
+        for (int k = 0; k < 4; k++) ctl[k] = 0u;
+    }
 }
 
 // Per-contact constants of the sweep (contact_prep), incidence keys and
@@ -285,11 +294,38 @@ SVD __amdgpu_buffer_rsrc_t seq_rsrc(const void *p) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, 0x7FFFFFFF, 0x00020000);
 }
 
-// words of the sequence form from the reference buffers
+// Record of contact c for the pipelined form: rotation and inverse inertia of
+// bodies a and b, and (rank, degree) of both incidences.
+constexpr int SEQX_WORDS = 28;  // 7 float4
+SVD void seq_record(uint32_t c, const float *__restrict__ prep, const uint32_t *__restrict__ rank,
+                    const uint32_t *__restrict__ deg, const float *__restrict__ rot,
+                    const float *__restrict__ inv_i, float *__restrict__ rec) {
+    const float *P = prep + (size_t)c * CPREP_WORDS;
+    const uint32_t flags = __float_as_uint(P[P_FLAGS]);
+    if (flags & F_SKIP) return;
+    const uint32_t a = __float_as_uint(P[P_A]), b = __float_as_uint(P[P_B]);
+    float *R = rec + (size_t)c * SEQX_WORDS;
+    for (int k = 0; k < 9; k++) R[k] = rot[(size_t)a * 9 + k];
+    for (int k = 0; k < 3; k++) R[9 + k] = inv_i[(size_t)a * 3 + k];
+    for (int k = 0; k < 9; k++) R[12 + k] = rot[(size_t)b * 9 + k];
+    for (int k = 0; k < 3; k++) R[21 + k] = inv_i[(size_t)b * 3 + k];
+    // a static side's rank is never written and its degree is zero:
+    // neither is read for a static body
+    R[24] = __uint_as_float((flags & F_DYN_A) ? rank[2 * c] : 0u);
+    R[25] = __uint_as_float(deg[a]);
+    R[26] = __uint_as_float((flags & F_DYN_B) ? rank[2 * c + 1] : 0u);
+    R[27] = __uint_as_float(deg[b]);
+}
+
+// words of the sequence form from the reference buffers, and (rec non-null)
+// the pipelined form's contact records
 __global__ void __launch_bounds__(BLOCK) k_seq_pack(const uint32_t *__restrict__ misc, uint32_t n_bodies,
                                                     const float *__restrict__ vel, const float *__restrict__ omega,
                                                     const float *__restrict__ lambda, float4 *__restrict__ sv,
-                                                    uint2 *__restrict__ sc) {
+                                                    uint2 *__restrict__ sc, const float *__restrict__ prep,
+                                                    const uint32_t *__restrict__ rank,
+                                                    const uint32_t *__restrict__ deg, const float *__restrict__ rot,
+                                                    const float *__restrict__ inv_i, float *__restrict__ rec) {
     const uint32_t nc = misc[MISC_NC];
     const uint32_t m = n_bodies > nc ? n_bodies : nc;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
@@ -297,7 +333,10 @@ __global__ void __launch_bounds__(BLOCK) k_seq_pack(const uint32_t *__restrict__
             sv[2 * i] = make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0f);
             sv[2 * i + 1] = make_float4(omega[3 * i], omega[3 * i + 1], omega[3 * i + 2], 0.0f);
         }
-        if (i < nc) sc[i] = make_uint2(__float_as_uint(lambda[i]), 0u);
+        if (i < nc) {
+            sc[i] = make_uint2(__float_as_uint(lambda[i]), 0u);
+            if (rec) seq_record(i, prep, rank, deg, rot, inv_i, rec);
+        }
     }
 }
 
@@ -444,35 +483,6 @@ __global__ void __launch_bounds__(THREADS) k_solve_seq(
 // increasing order per lane, so a lane's next item and ticket always follow
 // its current item: the earliest unfinished item reaches a lane's current
 // slot within two turns and then has all its predecessors done.
-
-// Record of contact c for the pipelined form: rotation and inverse inertia of
-// bodies a and b, and (rank, degree) of both incidences.
-constexpr int SEQX_WORDS = 28;  // 7 float4
-__global__ void __launch_bounds__(BLOCK) k_seq_records(const uint32_t *__restrict__ misc,
-                                                       const float *__restrict__ prep,
-                                                       const uint32_t *__restrict__ rank,
-                                                       const uint32_t *__restrict__ deg,
-                                                       const float *__restrict__ rot,
-                                                       const float *__restrict__ inv_i, float *__restrict__ rec) {
-    const uint32_t nc = misc[MISC_NC];
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-        const float *P = prep + (size_t)c * CPREP_WORDS;
-        const uint32_t flags = __float_as_uint(P[P_FLAGS]);
-        if (flags & F_SKIP) continue;
-        const uint32_t a = __float_as_uint(P[P_A]), b = __float_as_uint(P[P_B]);
-        float *R = rec + (size_t)c * SEQX_WORDS;
-        for (int k = 0; k < 9; k++) R[k] = rot[(size_t)a * 9 + k];
-        for (int k = 0; k < 3; k++) R[9 + k] = inv_i[(size_t)a * 3 + k];
-        for (int k = 0; k < 9; k++) R[12 + k] = rot[(size_t)b * 9 + k];
-        for (int k = 0; k < 3; k++) R[21 + k] = inv_i[(size_t)b * 3 + k];
-        // a static side's rank is never written and its degree is zero:
-        // neither is read for a static body
-        R[24] = __uint_as_float((flags & F_DYN_A) ? rank[2 * c] : 0u);
-        R[25] = __uint_as_float(deg[a]);
-        R[26] = __uint_as_float((flags & F_DYN_B) ? rank[2 * c + 1] : 0u);
-        R[27] = __uint_as_float(deg[b]);
-    }
-}
 
 typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 
@@ -926,10 +936,10 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
     uint32_t *counters = B<uint32_t>(c, SV_ATOMIC_COUNTERS);
     uint32_t mc = c->caps.max_contacts;
     if (mc == 0 || n_bodies == 0) return;
-    hipLaunchKernelGGL(k_solve_count, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, mc);
-    launch_rot(c, n_bodies);
-    fill_u32(c, c->s.deg, 0u, n_bodies);
-    fill_u32(c, c->s.misc + MISC_NACT, 0u, 1);
+    hipLaunchKernelGGL(k_sweep_setup, dim3(div_up(n_bodies, 256)), dim3(256), 0, c->stream, counters, c->s.misc, mc,
+                       c->s.deg, n_bodies, c->s.solve_ctl);
+    // (the step's AABB stage has just cached every body's rotation)
+    if (!c->rot_fresh) launch_rot(c, n_bodies);
     uint32_t iters = c->params.pgs_iterations;
     bool flow = c->solver_flow;
     if (flow) {
@@ -954,7 +964,6 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
         if (!launch_solve_colored(c, n_bodies)) c->launch_err = hipErrorLaunchFailure;
         return;
     }
-    hipLaunchKernelGGL(k_set_ninc, dim3(1), dim3(1), 0, c->stream, c->s.misc);
     int key_bits = 32 - __builtin_clz(n_bodies);  // keys are in [0, n_bodies]
     int where = radix_sort(c, c->s.inc_key[0], c->s.inc_val[0], c->s.inc_key[1], c->s.inc_val[1],
                            c->s.misc + MISC_NINC, 2u * mc, key_bits);
@@ -966,7 +975,6 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
     }
     uint32_t rgrid = div_up(2u * (uint64_t)mc, BLOCK);
     if (rgrid > 8192) rgrid = 8192;
-    fill_u32(c, c->s.solve_ctl, 0u, 4);
     float *vel = B<float>(c, SV_BODY_VEL), *omega = B<float>(c, SV_BODY_OMEGA);
     float *inv_i = B<float>(c, SV_BODY_INV_INERTIA), *lam = B<float>(c, SV_CONTACT_LAMBDA_N);
     if (flow) {
@@ -1014,10 +1022,9 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
         uint32_t pgrid = div_up(n_bodies > mc ? n_bodies : mc, BLOCK);
         if (pgrid > 8192) pgrid = 8192;
         hipLaunchKernelGGL(k_seq_pack, dim3(pgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, vel, omega, lam,
-                           c->s.seq_v, c->s.seq_c);
+                           c->s.seq_v, c->s.seq_c, c->s.cprep, c->s.rank, c->s.deg, c->s.rot, inv_i,
+                           pipe ? reinterpret_cast<float *>(c->s.seq_x) : nullptr);
         if (pipe) {
-            hipLaunchKernelGGL(k_seq_records, dim3(grid), dim3(BLOCK), 0, c->stream, c->s.misc, c->s.cprep,
-                               c->s.rank, c->s.deg, c->s.rot, inv_i, reinterpret_cast<float *>(c->s.seq_x));
             StageTimer sweep_timer(c, "phys/pgs_sweep");
             hipLaunchKernelGGL(k_solve_seqp<SEQ_THREADS>, dim3(wgs), dim3(SEQ_THREADS), 0, c->stream, c->s.misc,
                                iters, c->s.cprep, c->s.seq_x, c->s.seq_v, c->s.seq_c, c->s.solve_ctl, counters);
