@@ -235,6 +235,10 @@ int sv_device_info(sv_ctx *ctx, char *name, size_t name_len, char *arch, size_t 
 int sv_stage_times(sv_ctx *ctx, int reset, const char **names, double *avg_us,
                    uint64_t *calls, int cap);
 int sv_enable_stage_timing(sv_ctx *ctx, int enable);
+/* Time only the stage named `stage` while timing is enabled (NULL or "" =
+ * every stage).  Each timed stage adds two events to the stream, which cost
+ * the step time; a measurement of one kernel then perturbs the step less. */
+int sv_stage_timing_only(sv_ctx *ctx, const char *stage);
 
 /* Schedule of the contact sweep (step 5).  No counterpart in the
  * reference, which runs the sweep on one invocation (physics.comp:523-581).

@@ -159,7 +159,9 @@ class Context:
         check(self.lib.sv_debug_copy_bandwidth(self.h, nbytes, iters, C.byref(out)))
         return out.value
 
-    def enable_stage_timing(self, on: bool = True) -> None:
+    def enable_stage_timing(self, on: bool = True, only: str | None = None) -> None:
+        """HIP-event timing of the library's stages; `only` names the one stage to time."""
+        check(self.lib.sv_stage_timing_only(self.h, (only or "").encode()))
         check(self.lib.sv_enable_stage_timing(self.h, int(on)))
 
     SOLVERS = {"auto": 0, "ordered": 1, "flow": 2, "colored": 3}

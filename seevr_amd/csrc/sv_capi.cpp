@@ -156,6 +156,7 @@ int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
 
 int timing_begin(sv_ctx *c, const char *name) {
     if (!c->timing) return -1;
+    if (!c->timing_only.empty() && c->timing_only != name) return -1;
     int stage = -1;
     for (size_t i = 0; i < c->stage_names.size(); i++)
         if (c->stage_names[i] == name) stage = (int)i;
@@ -478,6 +479,12 @@ int sv_device_info(sv_ctx *c, char *name, size_t name_len, char *arch, size_t ar
 int sv_enable_stage_timing(sv_ctx *c, int enable) {
     if (!c) return fail(SV_E_INVALID, "sv_enable_stage_timing: null context");
     c->timing = enable != 0;
+    return SV_OK;
+}
+
+int sv_stage_timing_only(sv_ctx *c, const char *stage) {
+    if (!c) return fail(SV_E_INVALID, "sv_stage_timing_only: null context");
+    c->timing_only = stage ? stage : "";
     return SV_OK;
 }
 

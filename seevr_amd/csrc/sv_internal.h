@@ -134,6 +134,7 @@ struct sv_ctx {
     float warm_factor = 0.0f;
     // stage timing
     bool timing = false;
+    std::string timing_only;  // non-empty: the one stage timed
     std::vector<std::string> stage_names;
     std::vector<double> stage_ms;
     std::vector<uint64_t> stage_calls;
