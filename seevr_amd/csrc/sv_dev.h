@@ -201,9 +201,12 @@ SVD void expand_portal(Portal &s, f3 v4, f3 s4a, f3 s4b) {
     } else {
         slot = dot3(s.v[3], v4v0) > 0.0f ? 2 : 1;
     }
-    if (slot == 1) { s.v[1] = v4; s.sa[1] = s4a; s.sb[1] = s4b; }
-    else if (slot == 2) { s.v[2] = v4; s.sa[2] = s4a; s.sb[2] = s4b; }
-    else { s.v[3] = v4; s.sa[3] = s4a; s.sb[3] = s4b; }
+    // one select per element: stores chosen by slot would become a store at
+    // a per-lane index, which moves the portal out of registers
+    const bool k1 = slot == 1, k2 = slot == 2, k3 = slot == 3;
+    s.v[1] = sel3(k1, v4, s.v[1]); s.sa[1] = sel3(k1, s4a, s.sa[1]); s.sb[1] = sel3(k1, s4b, s.sb[1]);
+    s.v[2] = sel3(k2, v4, s.v[2]); s.sa[2] = sel3(k2, s4a, s.sa[2]); s.sb[2] = sel3(k2, s4b, s.sb[2]);
+    s.v[3] = sel3(k3, v4, s.v[3]); s.sa[3] = sel3(k3, s4a, s.sa[3]); s.sb[3] = sel3(k3, s4b, s.sb[3]);
 }
 
 // physics.comp:224-247: barycentric contact point, averaged over A and B.
@@ -261,14 +264,13 @@ SVD int discover_portal(const Pair &P, const Geo &g, Portal &s) {
         f3 v3c, s3a, s3b;
         mink(P, g, dir, v3c, s3a, s3b);
         if (dot3(v3c, dir) < CCD_EPS) return -1;
-        if (dot3(cross3(s.v[1], v3c), s.v[0]) < -CCD_EPS) {
-            s.v[2] = v3c; s.sa[2] = s3a; s.sb[2] = s3b;
-            dir = norm3(cross3(s.v[1] - s.v[0], s.v[2] - s.v[0]));
-            trials++;
-            continue;
-        }
-        if (dot3(cross3(v3c, s.v[2]), s.v[0]) < -CCD_EPS) {
-            s.v[1] = v3c; s.sa[1] = s3a; s.sb[1] = s3b;
+        // replace v2 if the origin is outside (v0, v1, v3c), else v1 if it is
+        // outside (v0, v3c, v2); selects per element (see expand_portal)
+        const bool r2 = dot3(cross3(s.v[1], v3c), s.v[0]) < -CCD_EPS;
+        const bool r1 = !r2 && dot3(cross3(v3c, s.v[2]), s.v[0]) < -CCD_EPS;
+        if (r2 || r1) {
+            s.v[2] = sel3(r2, v3c, s.v[2]); s.sa[2] = sel3(r2, s3a, s.sa[2]); s.sb[2] = sel3(r2, s3b, s.sb[2]);
+            s.v[1] = sel3(r1, v3c, s.v[1]); s.sa[1] = sel3(r1, s3a, s.sa[1]); s.sb[1] = sel3(r1, s3b, s.sb[1]);
             dir = norm3(cross3(s.v[1] - s.v[0], s.v[2] - s.v[0]));
             trials++;
             continue;

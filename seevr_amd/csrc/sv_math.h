@@ -34,6 +34,9 @@ SVD f3 cross3(f3 a, f3 b) {
 }
 SVD float len3(f3 a) { return __builtin_sqrtf(dot3(a, a)); }
 SVD f3 norm3(f3 a) { float inv = 1.0f / __builtin_sqrtf(dot3(a, a)); return inv * a; }
+// c ? a : b per component (a select of whole structs can become a select of
+// their addresses, which puts them in memory)
+SVD f3 sel3(bool c, f3 a, f3 b) { return mk(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z); }
 SVD f3 vmin3(f3 a, f3 b) { return mk(gmin(a.x, b.x), gmin(a.y, b.y), gmin(a.z, b.z)); }
 SVD f3 vmax3(f3 a, f3 b) { return mk(gmax(a.x, b.x), gmax(a.y, b.y), gmax(a.z, b.z)); }
 
