@@ -47,6 +47,7 @@ def build_workload(count: int, seed: int):
 
 
 PROFILE = ROOT / "profiles" / "c2_32k_container_kernels.md"
+VALU_PEAK = 256 * 4 * 16 * 2.4e9  # non-packed FP32 lane operations per second
 STAGE_FRAMES = 5  # frames after the timed ones whose stages are all timed
 STAGE_KERNEL = {"phys/pgs_sweep": "k_solve_seqp<64>", "phys/aabb": "k_aabb", "phys/gravity": "k_gravity",
                 "phys/integrate": "k_integrate", "phys/broad_phase": "k_broadphase<false>",
@@ -557,6 +558,19 @@ def main() -> int:
     hbm_stages = {k: roofline_entry(k, stages, n, nh, n_dyn, pairs, nc)
                   for k in ("phys/aabb", "phys/broad_phase", "phys/integrate", "phys/gravity", "bvh/radix_sort",
                             "bvh/morton_codes")}
+    # the all-pairs test loop is bound by vector issue, not HBM: its tests/s
+    # against the non-packed FP32 issue rate (256 CUs x 4 SIMDs x 16 lanes x
+    # 2.4 GHz; packed FP32 is not used), at 10 vector instructions per test
+    # (six subtractions, three maxima, one compare)
+    bp_us = stages.get("phys/broad_phase", (0.0, 0))[0]
+    tests = n * (n - 1) / 2.0
+    if bp_us > 0:
+        lane_ops = 10.0 * tests / (bp_us * 1e-6)
+        hbm_stages["phys/broad_phase"]["valu"] = {
+            "tests_per_launch": int(tests), "tests_per_s": round(tests / (bp_us * 1e-6), 1),
+            "vector_ops_per_test": 10, "lane_ops_per_s": round(lane_ops, 1), "peak_lane_ops_per_s": VALU_PEAK,
+            "frac": round(lane_ops / VALU_PEAK, 4),
+            "note": "whole stage time (count pass, scan, placement, emit pass) against the test loop's issue"}
     lbvh = {"ms_per_frame": round(1000.0 * el_bvh / args.steps, 3), "primitives": n,
             "stage_us": {k: round(v[0], 1) for k, v in stages.items() if k.startswith("bvh/")}}
     log(f"LBVH over body boxes: {lbvh['ms_per_frame']:.3f} ms per frame; {lbvh['stage_us']}")
