@@ -267,10 +267,10 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
     if (e == hipSuccess) e = dalloc(s.deg_off, N + 1);
     if (e == hipSuccess) e = dalloc(s.rank, 2 * C + 2);
     if (e == hipSuccess) e = dalloc(s.ticket, N + 1);
-    if (e == hipSuccess) e = dalloc(s.solve_ctl, 4);
+    if (e == hipSuccess) e = dalloc(s.solve_ctl, sv::SOLVE_CTL_WORDS);
     if (e == hipSuccess) e = dalloc(s.misc, 64);
     if (e == hipSuccess) e = hipMemset(s.misc, 0, 64 * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(s.solve_ctl, 0, 4 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(s.solve_ctl, 0, sv::SOLVE_CTL_WORDS * sizeof(uint32_t));
     if (e != hipSuccess) {
         free_ctx(c);
         return fail(SV_E_NOMEM, "sv_create: device allocation failed: %s", hipGetErrorString(e));

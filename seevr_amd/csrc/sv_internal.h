@@ -30,7 +30,8 @@ struct Scratch {
     uint32_t *deg_off = nullptr;    // uint[N + 1]
     uint32_t *rank = nullptr;       // uint[2C] rank of each incidence in its body
     uint32_t *ticket = nullptr;     // uint[N] completed updates per body (solver)
-    uint32_t *solve_ctl = nullptr;  // uint[4]: [0] abort flag, [1..3] solver counters
+    uint32_t *solve_ctl = nullptr;  // uint[SOLVE_CTL_WORDS]: [0] abort flag, [1..3] solver counters,
+                                    // [64 + 16 s] ticket counter of sweep stream s
     uint32_t *flow_count = nullptr; // uint[iters * C] flow-form countdowns (allocated on first use)
     uint32_t flow_count_cap = 0;
     uint32_t *flow_queue = nullptr; // uint[iters * C] flow-form ready queue
@@ -76,6 +77,9 @@ struct Scratch {
 // SWEEP_SEQ picks the pipelined sequence kernel below 2^18 bodies and the
 // basic one above; SWEEP_PIPE and SWEEP_SEQ_BASIC force one of them.
 enum { SWEEP_SEQ = 0, SWEEP_COHERENT = 1, SWEEP_FENCED = 2, SWEEP_PIPE = 3, SWEEP_SEQ_BASIC = 4 };
+
+// Control words of the sweeps (Scratch::solve_ctl).
+constexpr uint32_t SOLVE_CTL_WORDS = 1024;
 
 // Words per contact in Scratch::cprep (see sv_pgs.hip).
 constexpr int CPREP_WORDS = 20;

@@ -48,6 +48,7 @@ constexpr int BLOCK = 256;
 constexpr int SOLVE_THREADS = 1024;
 constexpr int MC_THREADS = 256;
 constexpr int SEQ_THREADS = 64;  // sequence form: one wave per workgroup
+constexpr uint32_t SEQ_BACKOFF_MAX = 8;  // pipelined form: most turns a lane skips polling
 constexpr uint64_t STALL_TICKS = 200000000ull;  // 2 s of the 100 MHz wall clock
 constexpr uint64_t ABORT_CHECK_TICKS = 2000ull;  // 20 us
 
@@ -67,8 +68,9 @@ __global__ void __launch_bounds__(256) k_sweep_setup(const uint32_t *counters, u
         misc[MISC_NC] = nc;
         misc[MISC_NINC] = 2u * nc;
         misc[MISC_NACT] = 0u;
-        for (int k = 0; k < 4; k++) ctl[k] = 0u;
     }
+    if (blockIdx.x == 0)
+        for (uint32_t k = threadIdx.x; k < SOLVE_CTL_WORDS; k += blockDim.x) ctl[k] = 0u;
 }
 
 // Per-contact constants of the sweep (contact_prep), incidence keys and
@@ -364,7 +366,8 @@ template <int THREADS>
 __global__ void __launch_bounds__(THREADS) k_solve_seq(
     const uint32_t *__restrict__ misc, uint32_t iters, const float *__restrict__ prep,
     const uint32_t *__restrict__ rank, const uint32_t *__restrict__ deg, float4 *sv, uint2 *sc,
-    const float *__restrict__ rot, const float *__restrict__ inv_i, uint32_t *ctl, uint32_t *counters) {
+    const float *__restrict__ rot, const float *__restrict__ inv_i, uint32_t *ctl, uint32_t *counters,
+    uint32_t streams) {
     const uint32_t nc = misc[MISC_NC];
     const uint32_t total = iters * nc;
     const int lane = threadIdx.x & 63;
@@ -383,6 +386,10 @@ __global__ void __launch_bounds__(THREADS) k_solve_seq(
     u2 lc;
     uint32_t got = 0;
     bool has = false, fin = false;
+    // ticket streams, as in the pipelined kernel (seq_ticket)
+    const uint32_t lg = streams > 1u ? 31u - __builtin_clz(streams) : 0u;
+    const uint32_t stream = blockIdx.x & (streams - 1u);
+    uint32_t *tk = ctl + 64 + 16 * stream;
     uint64_t last = wall_clock64();
     for (;;) {
         uint64_t need = __ballot(!has && !fin);
@@ -391,10 +398,11 @@ __global__ void __launch_bounds__(THREADS) k_solve_seq(
             int leader = __builtin_ctzll(need);
             uint32_t base = 0;
             if (lane == leader)
-                base = __hip_atomic_fetch_add(ctl + 1, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                base = __hip_atomic_fetch_add(tk, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             base = __shfl(base, leader, 64);
             if (!has && !fin) {
-                uint32_t q = base + (uint32_t)__popcll(need & lt);
+                const uint32_t r = base + (uint32_t)__popcll(need & lt);
+                const uint32_t q = ((r >> 6) << (6 + lg)) + (stream << 6) + (r & 63u);
                 if (q >= total) fin = true;
                 else {
                     has = load_item(it, q, nc, prep, rank, deg, rot, inv_i);
@@ -543,7 +551,7 @@ SVD uint32_t decode_item(Item &it, const RawItem &r, uint32_t c, uint32_t t) {
 template <int THREADS>
 __global__ void __launch_bounds__(THREADS) k_solve_seqp(
     const uint32_t *__restrict__ misc, uint32_t iters, const float *__restrict__ prep,
-    const float4 *__restrict__ rec, float4 *sv, uint2 *sc, uint32_t *ctl, uint32_t *counters) {
+    const float4 *__restrict__ rec, float4 *sv, uint2 *sc, uint32_t *ctl, uint32_t *counters, uint32_t streams) {
     const uint32_t nc = misc[MISC_NC];
     const uint32_t total = iters * nc;
     const int lane = threadIdx.x & 63;
@@ -566,10 +574,27 @@ __global__ void __launch_bounds__(THREADS) k_solve_seqp(
     uint64_t pend = 0;
     int pend_leader = 0;
     uint32_t pend_base = 0;
+    // ticket streams: with 8, stream s (workgroups s, s+8, ...; about one
+    // XCD each) hands out the items of blocks s, s+8, s+16, ... of 64
+    // consecutive items, in order, from its own counter (one counter served
+    // every wave's refills one atomic at a time); the earliest unfinished
+    // item is then always held, or next, in its stream
+    const uint32_t lg = streams > 1u ? 31u - __builtin_clz(streams) : 0u;  // streams: a power of two
+    const uint32_t stream = blockIdx.x & (streams - 1u);
+    uint32_t *tk = ctl + 64 + 16 * stream;
+    // turns this lane skips polling: the counts it last read leave at least
+    // `backoff` + 2 updates of a body (or iterations of the contact) before
+    // its item can run, and a body takes at most about one update per turn
+    // (each waits for the previous one's words), so earlier polls would only
+    // load the memory system every hand-off goes through
+    uint32_t backoff = 0;
     uint64_t last = wall_clock64();
     for (;;) {
         bool progressed = false;
-        if (has) {
+        bool polled = false;
+        if (has && backoff) backoff--;
+        else if (has) {
+            polled = true;
             const int oa = (int)(it.a * 32u), ob = (int)(it.b * 32u), oc = (int)(it.c * 8u);
             if (!(got & 1u)) a0 = __builtin_amdgcn_raw_buffer_load_b128(rv, oa, 0, CPOL_SC1);
             if (!(got & 2u)) a1 = __builtin_amdgcn_raw_buffer_load_b128(rv, oa + 16, 0, CPOL_SC1);
@@ -612,6 +637,14 @@ __global__ void __launch_bounds__(THREADS) k_solve_seqp(
                 __builtin_amdgcn_raw_buffer_store_b64(u2{__float_as_uint(nl), t + 1u}, rc, oc, 0, CPOL_SC1);
                 has = false;
                 progressed = true;
+            } else if (polled) {
+                uint32_t d = 0;
+                if (!(got & 1u)) d = max(d, it.ta - a0[3]);
+                if (!(got & 2u)) d = max(d, it.ta - a1[3]);
+                if (!(got & 4u)) d = max(d, it.tb - b0[3]);
+                if (!(got & 8u)) d = max(d, it.tb - b1[3]);
+                if (!(got & 16u)) d = max(d, t - lc[1]);
+                backoff = d >= 3u ? min(d - 2u, SEQ_BACKOFF_MAX) : 0u;
             }
         }
         // promote the next item (a skipped contact is dropped)
@@ -621,13 +654,15 @@ __global__ void __launch_bounds__(THREADS) k_solve_seqp(
                 t = tn;
                 has = true;
                 got = 0;
+                backoff = 0;
             }
         }
         // last turn's tickets
         if (pend) {
             const uint32_t base = __shfl(pend_base, pend_leader, 64);
             if ((pend >> lane) & 1ull) {
-                const uint32_t q = base + (uint32_t)__popcll(pend & lt);
+                const uint32_t r = base + (uint32_t)__popcll(pend & lt);
+                const uint32_t q = ((r >> 6) << (6 + lg)) + (stream << 6) + (r & 63u);
                 if (q >= total) fin = true;
                 else {
                     qk = q;
@@ -658,7 +693,7 @@ __global__ void __launch_bounds__(THREADS) k_solve_seqp(
                 // atomic that waits for the value at once)
                 uint32_t z = 0;
                 asm volatile("" : "+v"(z));
-                pend_base = __hip_atomic_fetch_add(ctl + 1 + z, (uint32_t)__popcll(need), __ATOMIC_RELAXED,
+                pend_base = __hip_atomic_fetch_add(tk + z, (uint32_t)__popcll(need), __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT);
             }
         }
@@ -929,6 +964,17 @@ uint32_t solver_lds_bodies(sv_ctx *c) {
     return avail > 0 ? (uint32_t)(avail / 4) : 0u;
 }
 
+// Ticket streams of the sequence-form sweeps: 8 (one counter per XCD's
+// share of the workgroups), fewer for fewer workgroups so every stream has
+// some.  With one counter for every wave's refills the 32k-pile sweep took
+// 1.44 ms at 192 workgroups and 1.90 at 256; with 8 it takes 1.30 and 1.28
+// (16 streams: 1.28), and the 1M-body sweep 1.11 ms instead of 1.79.
+uint32_t seq_streams(uint32_t wgs) {
+    uint32_t s = 8u;
+    while (s > 1u && s > wgs) s >>= 1;
+    return s;
+}
+
 // Step 5 (physics.comp:500-583) for the contacts counters[1] names.
 // Reads lambda_n as the starting impulses (step 4 clears them, or, with
 // warm start, carries them over).
@@ -995,10 +1041,10 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
     }
     hipLaunchKernelGGL(k_rank, dim3(rgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.inc_key[where],
                        c->s.inc_val[where], c->s.deg_off, c->s.rank);
-    // Sequence kernel: the pipelined one wins on a 32k-body pile (sweep 1.65
-    // -> 1.44 ms per substep), the basic one on the 1M-body scene (1.41 ms
-    // against 3.39: the pipelined lanes idle two turns per refill, which a
-    // wide dependency graph cannot hide).
+    // Sequence kernel: the pipelined one wins on a 32k-body pile (sweep 1.28
+    // ms per substep against 1.4 for the basic one), the basic one on the
+    // 1M-body scene (1.11 ms against 1.46: the pipelined lanes idle two
+    // turns per refill, which a wide dependency graph cannot hide).
     const bool seq_variant = c->sweep_variant == SWEEP_SEQ || c->sweep_variant == SWEEP_PIPE ||
                              c->sweep_variant == SWEEP_SEQ_BASIC;
     const bool pipe = c->sweep_variant == SWEEP_PIPE || (c->sweep_variant == SWEEP_SEQ && n_bodies < (1u << 18));
@@ -1008,14 +1054,15 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
                         (!pipe || (uint64_t)mc * SEQX_WORDS * 4u < 0x7FFFFFFFull) && seq_reserve(c) &&
                         (!pipe || seqx_reserve(c));
     // Form choice: small scenes keep every hand-off inside one CU; large ones
-    // need the wider in-order window of the multi-CU form.  On a 32k-body
-    // pile (1.5M sweep items per substep) the sequence form was fastest with
-    // 192 one-wave workgroups (1.65 ms; 1.68 at 128, 1.69 at 256, 1.98 at
-    // 96, 2.24 with 32 four-wave workgroups), on the 1M-body scene (7.2M
-    // items) with 384 (1.41 ms; 1.80 at 192, 1.78 at 768); the ticket form
-    // with coherent accesses at 64 four-wave workgroups (2.96 ms; 3.21 at 32).
+    // need the wider in-order window of the multi-CU form.  With 8 ticket
+    // streams the pipelined kernel on a 32k-body pile (1.5M sweep items per
+    // substep) was fastest with 256 one-wave workgroups (1.28 ms; 1.30 at
+    // 192, 1.30 at 320, 1.33 at 384, 1.68 at 128), the basic kernel on the
+    // 1M-body scene (7.2M items) with 640 (1.11 ms; 1.39 at 384, 1.17 at
+    // 512, 1.14 at 768; the pipelined one 1.46 at 512); the ticket form with
+    // coherent accesses at 64 four-wave workgroups (2.96 ms; 3.21 at 32).
     const bool seq_form = seq_variant;
-    const uint32_t seq_wgs = n_bodies >= (1u << 18) ? 384u : 192u;
+    const uint32_t seq_wgs = n_bodies >= (1u << 18) ? 640u : 256u;
     uint32_t wgs = c->solver_workgroups ? c->solver_workgroups
                                         : (n_bodies >= 2048 ? (seq_form ? seq_wgs : 64u) : 1u);
     if (wgs > 1 && seq_ok) {
@@ -1027,12 +1074,13 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
         if (pipe) {
             StageTimer sweep_timer(c, "phys/pgs_sweep");
             hipLaunchKernelGGL(k_solve_seqp<SEQ_THREADS>, dim3(wgs), dim3(SEQ_THREADS), 0, c->stream, c->s.misc,
-                               iters, c->s.cprep, c->s.seq_x, c->s.seq_v, c->s.seq_c, c->s.solve_ctl, counters);
+                               iters, c->s.cprep, c->s.seq_x, c->s.seq_v, c->s.seq_c, c->s.solve_ctl, counters,
+                               seq_streams(wgs));
         } else {
             StageTimer sweep_timer(c, "phys/pgs_sweep");
             hipLaunchKernelGGL(k_solve_seq<SEQ_THREADS>, dim3(wgs), dim3(SEQ_THREADS), 0, c->stream, c->s.misc,
                                iters, c->s.cprep, c->s.rank, c->s.deg, c->s.seq_v, c->s.seq_c, c->s.rot, inv_i,
-                               c->s.solve_ctl, counters);
+                               c->s.solve_ctl, counters, seq_streams(wgs));
         }
         hipLaunchKernelGGL(k_seq_unpack, dim3(pgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.seq_v,
                            c->s.seq_c, vel, omega, lam);
