@@ -49,7 +49,7 @@ def build_workload(count: int, seed: int):
 PROFILE = ROOT / "profiles" / "c2_32k_container_kernels.md"
 VALU_PEAK = 256 * 4 * 16 * 2.4e9  # non-packed FP32 lane operations per second
 STAGE_FRAMES = 5  # frames after the timed ones whose stages are all timed
-STAGE_KERNEL = {"phys/pgs_sweep": "k_solve_seqp<64>", "phys/aabb": "k_aabb", "phys/gravity": "k_gravity",
+STAGE_KERNEL = {"phys/pgs_sweep": "k_solve_seqp2<64>", "phys/aabb": "k_aabb", "phys/gravity": "k_gravity",
                 "phys/integrate": "k_integrate", "phys/broad_phase": "k_broadphase<false>",
                 "bvh/morton_codes": "k_morton", "bvh/radix_sort": "radix sort (k_sort_hist, k_scan_*, k_sort_scatter)"}
 
@@ -297,7 +297,7 @@ def sweep_variant_section(ctx, n, default_variant: str = "seq") -> dict:
                                                       ("BODY_OMEGA", 3))}
     dt_sub = float(np.float32(DT) / np.float32(SUBSTEPS))
     out, ref = {}, None
-    for variant in ("pipelined", "seq_basic", "coherent", "fenced"):
+    for variant in ("paired", "pipelined", "seq_basic", "coherent", "fenced"):
         for key, v in state.items():
             ctx.upload(key, v)
         ctx.set_sweep_variant(variant)
@@ -475,7 +475,7 @@ def main() -> int:
     ap.add_argument("--solver", default="auto", choices=["auto", "ordered", "flow", "colored"],
                     help="schedule of the contact sweep (all give the serial result bit for bit)")
     ap.add_argument("--solver-wgs", type=int, default=0, help="workgroups of the sweep kernel (0 = default)")
-    ap.add_argument("--sweep-variant", default="seq", choices=["seq", "pipelined", "seq_basic", "coherent", "fenced"],
+    ap.add_argument("--sweep-variant", default="seq", choices=["seq", "pipelined", "paired", "seq_basic", "coherent", "fenced"],
                     help="hand-off variant of the multi-workgroup sweep")
     ap.add_argument("--broadphase", default="auto", choices=["auto", "all_pairs", "lbvh"],
                     help="broadphase method (both give the all-pairs list bit for bit)")

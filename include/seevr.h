@@ -217,12 +217,13 @@ int sv_debug_xcc_ids(sv_ctx *ctx, uint32_t nblocks, uint32_t *out);
 
 /* Hand-off variant of the multi-workgroup sweeps (all give the same result):
  * 0 = sequence words (default: each body's velocities and update count in
- * one 16-byte word, one round trip per hand-off; the pipelined kernel below
- * 2^18 bodies, the basic one above), 1 = tickets with device-coherent
- * accesses, 2 = tickets with L2 write-back / invalidation fences, 3 =
- * sequence words, pipelined kernel (each lane's next item loaded while it
- * works on the current one), 4 = sequence words, basic kernel.  For tests
- * and measurements. */
+ * one 16-byte word, one round trip per hand-off; the paired pipelined
+ * kernel below 2^18 bodies, the basic one above), 1 = tickets with
+ * device-coherent accesses, 2 = tickets with L2 write-back / invalidation
+ * fences, 3 = sequence words, pipelined kernel (each lane's next item loaded
+ * while it works on the current one), 4 = sequence words, basic kernel, 5 =
+ * sequence words, pipelined kernel with two lanes per item (one per body).
+ * For tests and measurements. */
 int sv_debug_set_sweep_variant(sv_ctx *ctx, int variant);
 
 /* Thread-local message of the last failure ("" if none). */

@@ -189,7 +189,7 @@ class Context:
         """Broadphase method: "auto", "all_pairs" or "lbvh" (identical pair lists)."""
         check(self.lib.sv_set_broadphase(self.h, self.BROADPHASES[method]))
 
-    SWEEP_VARIANTS = {"seq": 0, "coherent": 1, "fenced": 2, "pipelined": 3, "seq_basic": 4}
+    SWEEP_VARIANTS = {"seq": 0, "coherent": 1, "fenced": 2, "pipelined": 3, "seq_basic": 4, "paired": 5}
 
     def set_sweep_variant(self, variant: str) -> None:
         """Hand-off variant of the multi-workgroup sweeps (same results)."""

@@ -530,7 +530,7 @@ int sv_set_warm_start(sv_ctx *c, float factor) {
 
 int sv_debug_set_sweep_variant(sv_ctx *c, int variant) {
     if (!c) return fail(SV_E_INVALID, "sv_debug_set_sweep_variant: null context");
-    if (variant < sv::SWEEP_SEQ || variant > sv::SWEEP_SEQ_BASIC)
+    if (variant < sv::SWEEP_SEQ || variant > sv::SWEEP_PAIR)
         return fail(SV_E_INVALID, "sv_debug_set_sweep_variant: unknown variant %d", variant);
     c->sweep_variant = variant;
     return SV_OK;

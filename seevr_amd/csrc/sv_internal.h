@@ -76,7 +76,7 @@ struct Scratch {
 // Hand-off variants of the multi-CU sweeps (sv_debug_set_sweep_variant).
 // SWEEP_SEQ picks the pipelined sequence kernel below 2^18 bodies and the
 // basic one above; SWEEP_PIPE and SWEEP_SEQ_BASIC force one of them.
-enum { SWEEP_SEQ = 0, SWEEP_COHERENT = 1, SWEEP_FENCED = 2, SWEEP_PIPE = 3, SWEEP_SEQ_BASIC = 4 };
+enum { SWEEP_SEQ = 0, SWEEP_COHERENT = 1, SWEEP_FENCED = 2, SWEEP_PIPE = 3, SWEEP_SEQ_BASIC = 4, SWEEP_PAIR = 5 };
 
 // Control words of the sweeps (Scratch::solve_ctl).
 constexpr uint32_t SOLVE_CTL_WORDS = 1024;

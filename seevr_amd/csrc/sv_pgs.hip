@@ -716,6 +716,263 @@ __global__ void __launch_bounds__(THREADS) k_solve_seqp(
 }
 
 // ---------------------------------------------------------------------------
+// Paired pipelined form: the pipelined sequence kernel with two lanes per
+// item, lane 2p for body a and lane 2p+1 for body b.  Each lane polls and
+// writes its own body's words (and both poll the contact's word), loads only
+// its own body's rotation and inverse inertia, and runs its body's half of
+// item_update; the two halves meet through a lane swap (DPP quad permute
+// 1,0,3,2) wherever item_update combines both bodies (the normal velocity,
+// the tangent, the friction mass), and the terms both need are computed by
+// both lanes from the same operands in the same order, so the update is the
+// one item_update computes, bit for bit.  A turn's dependent arithmetic is
+// then about 300 instructions instead of 500; a wave holds 32 items.
+
+SVD uint32_t pair_swap(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);  // quad_perm(1, 0, 3, 2)
+}
+SVD float pair_swapf(float v) { return __uint_as_float(pair_swap(__float_as_uint(v))); }
+SVD f3 pair_swap3(f3 v) { return mk(pair_swapf(v.x), pair_swapf(v.y), pair_swapf(v.z)); }
+
+// One lane's share of an item: the contact's shared constants and its own
+// body's.
+struct Half {
+    f3 n, r;           // contact normal, own lever arm
+    float kn, bias, mu, ima, imb, im;
+    m3 R;              // own rotation
+    f3 ii;             // own inverse inertia
+    uint32_t c, body, k;  // contact, own body, own body's awaited count
+    bool dyn;
+};
+struct RawHalf {
+    u4v p0, p2, p3;  // constants words 0-3, 8-11, 12-15 (normal, k_n, bias, mu, inverse masses)
+    f3 r;            // own lever arm
+    uint32_t im;     // own inverse mass
+    uint32_t body;   // own body
+    uint32_t fl;     // flags
+    u4v x[3];        // own body's rotation and inverse inertia
+    uint2 rd;        // own incidence's (rank, degree)
+};
+// Every own-body word is read at an offset that depends on the side, so the
+// decode selects nothing by side (a select between two elements of a
+// register array, by lane, becomes a per-lane index into the array).
+SVD void load_raw_half(RawHalf &r, uint32_t c, uint32_t side, const float *__restrict__ prep,
+                       const float *__restrict__ rec) {
+    const float *p = prep + (size_t)c * CPREP_WORDS;
+    const float *x = rec + (size_t)c * SEQX_WORDS;
+    const u4v *p4 = reinterpret_cast<const u4v *>(p);
+    r.p0 = p4[0];
+    r.p2 = p4[2];
+    r.p3 = p4[3];
+    const float *rr = p + P_RA + 3 * side;
+    r.r = mk(rr[0], rr[1], rr[2]);
+    r.im = __float_as_uint(p[P_IMA + side]);
+    r.body = __float_as_uint(p[P_A + side]);
+    r.fl = __float_as_uint(p[P_FLAGS]);
+    const u4v *x4 = reinterpret_cast<const u4v *>(x + 12 * side);
+#pragma unroll
+    for (int k = 0; k < 3; k++) r.x[k] = x4[k];
+    r.rd = reinterpret_cast<const uint2 *>(x + 24)[side];
+}
+SVD uint32_t decode_half(Half &h, const RawHalf &r, uint32_t c, uint32_t t, uint32_t side) {
+    static_assert(F_DYN_A == 2u && F_DYN_B == 4u && P_N == 0 && P_RA == 3 && P_RB == 6 && P_KN == 9 &&
+                      P_BIAS == 10 && P_MU == 11 && P_IMA == 12 && P_IMB == 13 && P_A == 14 && P_B == 15,
+                  "constants row layout");
+    auto f = [](uint32_t u) { return __uint_as_float(u); };
+    const uint32_t flags = r.fl;
+    h.c = c;
+    h.n = mk(f(r.p0[0]), f(r.p0[1]), f(r.p0[2]));
+    h.kn = f(r.p2[1]);
+    h.bias = f(r.p2[2]);
+    h.mu = f(r.p2[3]);
+    h.ima = f(r.p3[0]);
+    h.imb = f(r.p3[1]);
+    h.r = r.r;
+    h.im = f(r.im);
+    h.body = r.body;
+    h.R.c0 = mk(f(r.x[0][0]), f(r.x[0][1]), f(r.x[0][2]));
+    h.R.c1 = mk(f(r.x[0][3]), f(r.x[1][0]), f(r.x[1][1]));
+    h.R.c2 = mk(f(r.x[1][2]), f(r.x[1][3]), f(r.x[2][0]));
+    h.ii = mk(f(r.x[2][1]), f(r.x[2][2]), f(r.x[2][3]));
+    h.dyn = (flags >> (1u + side)) & 1u;  // F_DYN_A = 2, F_DYN_B = 4
+    h.k = h.dyn ? t * r.rd.y + r.rd.x : 0u;
+    return flags;
+}
+
+// item_update (sv_dev.h) split over the pair: l, w are this lane's body's
+// velocity and angular velocity; side 0 is body a, side 1 body b.  Both
+// lanes return the new impulse.
+SVD float half_update(const Half &h, bool side, f3 &l, f3 &w, float old) {
+    f3 vs = l + cross3(w, h.r);
+    f3 vo = pair_swap3(vs);
+    f3 va = side ? vo : vs, vb = side ? vs : vo;
+    const float vn = dot3(vb - va, h.n);
+    float dln = (-vn + h.bias) / h.kn;
+    const float nl = gmax(old + dln, 0.0f);
+    dln = nl - old;
+    const f3 imp = dln * h.n;
+    // a: la - ima imp, wa - I(ra x imp); b: lb + imb imp, wb + I(rb x imp)
+    const f3 d = h.im * imp;
+    l = l + (side ? d : -d);
+    const f3 e = apply_inv_I(h.R, h.ii, cross3(h.r, imp));
+    w = w + (side ? e : -e);
+
+    vs = l + cross3(w, h.r);
+    vo = pair_swap3(vs);
+    va = side ? vo : vs;
+    vb = side ? vs : vo;
+    const f3 dv = vb - va;
+    const f3 vt = dv - dot3(dv, h.n) * h.n;
+    const float vtl = len3(vt);
+    if (vtl > 1e-6f) {
+        const f3 td = mk(vt.x / vtl, vt.y / vtl, vt.z / vtl);
+        const f3 rt = cross3(h.r, td);
+        const float T = dot3(rt, apply_inv_I(h.R, h.ii, rt));
+        const float To = pair_swapf(T);
+        const float Ta = side ? To : T, Tb = side ? T : To;
+        const float kt = h.ima + h.imb + Ta + Tb;
+        const float dlt = gmin(vtl / kt, h.mu * nl);
+        const f3 impt = dlt * td;
+        // a: la + ima impt, wa + I(ra x impt); b: lb - imb impt, wb - I(rb x impt)
+        const f3 d2 = h.im * impt;
+        l = l + (side ? -d2 : d2);
+        const f3 e2 = apply_inv_I(h.R, h.ii, cross3(h.r, impt));
+        w = w + (side ? -e2 : e2);
+    }
+    return nl;
+}
+
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS) k_solve_seqp2(
+    const uint32_t *__restrict__ misc, uint32_t iters, const float *__restrict__ prep,
+    const float4 *__restrict__ rec, float4 *sv, uint2 *sc, uint32_t *ctl, uint32_t *counters, uint32_t streams) {
+    const uint32_t nc = misc[MISC_NC];
+    const uint32_t total = iters * nc;
+    const int lane = threadIdx.x & 63;
+    const uint32_t side = (uint32_t)lane & 1u;
+    const int lane0 = lane & ~1;  // the pair's first lane
+    const uint64_t ltp = (lane0 == 0) ? 0ull : (~0ull >> (64 - lane0));
+    if (total == 0) return;
+    const __amdgpu_buffer_rsrc_t rv = seq_rsrc(sv), rc = seq_rsrc(sc);
+    const float *recf = reinterpret_cast<const float *>(rec);
+    // ticket streams (seq_streams), one ticket per pair
+    const uint32_t lg = streams > 1u ? 31u - __builtin_clz(streams) : 0u;
+    const uint32_t stream = blockIdx.x & (streams - 1u);
+    uint32_t *tk = ctl + 64 + 16 * stream;
+
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    // both lanes of a pair hold the same item and take the same branches
+    Half h;
+    RawHalf nx;
+    uint32_t t = 0, tn = 0, cn = 0, qk = 0;
+    u4 v0, v1;  // own body's (v, k), (w, k)
+    u2 lc;      // the contact's (lambda, t)
+    uint32_t got = 0;
+    // lane state as bits of one vector register (as separate booleans they
+    // are lane masks in scalar registers, which this kernel runs out of)
+    constexpr uint32_t HAS = 1, HAS_N = 2, HAS_K = 4, FIN = 8;
+    uint32_t st = 0;
+    uint64_t pend = 0;
+    int pend_leader = 0;
+    uint32_t pend_base = 0;
+    uint64_t last = wall_clock64();
+    for (;;) {
+        bool progressed = false;
+        if (st & HAS) {
+            const int ob = (int)(h.body * 32u), oc = (int)(h.c * 8u);
+            if (!(got & 1u)) v0 = __builtin_amdgcn_raw_buffer_load_b128(rv, ob, 0, CPOL_SC1);
+            if (!(got & 2u)) v1 = __builtin_amdgcn_raw_buffer_load_b128(rv, ob + 16, 0, CPOL_SC1);
+            if (!(got & 4u)) lc = __builtin_amdgcn_raw_buffer_load_b64(rc, oc, 0, CPOL_SC1);
+        }
+        // one wait per turn on every path (polls and the next item's loads)
+        __builtin_amdgcn_s_waitcnt(0);
+        if (st & HAS) {
+            // a static body's words never change: any count will do
+            got |= ((!h.dyn || v0[3] == h.k) ? 1u : 0u) | ((!h.dyn || v1[3] == h.k) ? 2u : 0u) |
+                   (lc[1] == t ? 4u : 0u);
+            const uint32_t both = got & pair_swap(got);
+            if (both == 7u) {
+                f3 l = mk(__uint_as_float(v0[0]), __uint_as_float(v0[1]), __uint_as_float(v0[2]));
+                f3 w = mk(__uint_as_float(v1[0]), __uint_as_float(v1[1]), __uint_as_float(v1[2]));
+                const float nl = half_update(h, side != 0u, l, w, __uint_as_float(lc[0]));
+                if (h.dyn) {
+                    const int ob = (int)(h.body * 32u);
+                    const uint32_t k = h.k + 1u;
+                    __builtin_amdgcn_raw_buffer_store_b128(
+                        u4{__float_as_uint(l.x), __float_as_uint(l.y), __float_as_uint(l.z), k}, rv, ob, 0, CPOL_SC1);
+                    __builtin_amdgcn_raw_buffer_store_b128(
+                        u4{__float_as_uint(w.x), __float_as_uint(w.y), __float_as_uint(w.z), k}, rv, ob + 16, 0,
+                        CPOL_SC1);
+                }
+                if (!side)
+                    __builtin_amdgcn_raw_buffer_store_b64(u2{__float_as_uint(nl), t + 1u}, rc, (int)(h.c * 8u), 0,
+                                                          CPOL_SC1);
+                st &= ~HAS;
+                progressed = true;
+            }
+        }
+        // promote the next item (a skipped contact is dropped)
+        if ((st & (HAS | HAS_N)) == HAS_N) {
+            st &= ~HAS_N;
+            if (!(decode_half(h, nx, cn, tn, side) & F_SKIP)) {
+                t = tn;
+                st |= HAS;
+                got = 0;
+            }
+        }
+        // last turn's tickets: one per pair, counted on the pairs' first lanes
+        if (pend) {
+            const uint32_t base = __shfl(pend_base, pend_leader, 64);
+            if ((pend >> lane0) & 1ull) {
+                const uint32_t r = base + (uint32_t)__popcll(pend & ltp);
+                const uint32_t q = ((r >> 6) << (6 + lg)) + (stream << 6) + (r & 63u);
+                if (q >= total) st |= FIN;
+                else {
+                    qk = q;
+                    st |= HAS_K;
+                }
+            }
+            pend = 0;
+        }
+        if ((st & (HAS_K | HAS_N)) == HAS_K) {
+            tn = qk / nc;
+            cn = qk - tn * nc;
+            load_raw_half(nx, cn, side, prep, recf);
+            st = (st & ~HAS_K) | HAS_N;
+        }
+        // tickets for pairs that hold nothing
+        const uint64_t need = __ballot(side == 0u && st == 0u);
+        if (need) {
+            pend = need;
+            pend_leader = __builtin_ctzll(need);
+            if (lane == pend_leader) {
+                // an offset the compiler cannot see as uniform keeps this a
+                // plain one-lane atomic whose value is read only next turn
+                uint32_t z = 0;
+                asm volatile("" : "+v"(z));
+                pend_base = __hip_atomic_fetch_add(tk + z, (uint32_t)__popcll(need), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (!pend && !__any((st & (HAS | HAS_N | HAS_K)) != 0u)) break;
+        if (__any(progressed)) {
+            last = wall_clock64();
+        } else {
+            const uint64_t idle = wall_clock64() - last;
+            if (idle > ABORT_CHECK_TICKS) {
+                if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+                if (idle > STALL_TICKS) {
+                    __hip_atomic_store(ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (lane == 0) atomicOr(counters + 2, (uint32_t)SV_STATUS_SOLVER_TIMEOUT);
+                    return;
+                }
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Flow form.  Item (t, c) has one predecessor per dynamic body (the previous
 // update of that body in sweep order: none for the body's first contact in
 // iteration 0) and one successor per dynamic body.  A countdown per item
@@ -1041,13 +1298,16 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
     }
     hipLaunchKernelGGL(k_rank, dim3(rgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.inc_key[where],
                        c->s.inc_val[where], c->s.deg_off, c->s.rank);
-    // Sequence kernel: the pipelined one wins on a 32k-body pile (sweep 1.28
-    // ms per substep against 1.4 for the basic one), the basic one on the
-    // 1M-body scene (1.11 ms against 1.46: the pipelined lanes idle two
-    // turns per refill, which a wide dependency graph cannot hide).
+    // Sequence kernel: the paired pipelined one wins on a 32k-body pile
+    // (sweep 1.05 ms per substep; 1.28 one lane per item, 1.4 basic), the
+    // basic one on the 1M-body scene (1.11 ms; 1.46 pipelined, 2.09 paired:
+    // the pipelined lanes idle two turns per refill, which a wide dependency
+    // graph cannot hide).
     const bool seq_variant = c->sweep_variant == SWEEP_SEQ || c->sweep_variant == SWEEP_PIPE ||
-                             c->sweep_variant == SWEEP_SEQ_BASIC;
-    const bool pipe = c->sweep_variant == SWEEP_PIPE || (c->sweep_variant == SWEEP_SEQ && n_bodies < (1u << 18));
+                             c->sweep_variant == SWEEP_SEQ_BASIC || c->sweep_variant == SWEEP_PAIR;
+    const bool small = n_bodies < (1u << 18);
+    const bool paired = c->sweep_variant == SWEEP_PAIR || (c->sweep_variant == SWEEP_SEQ && small);
+    const bool pipe = c->sweep_variant == SWEEP_PIPE || paired;
     // (buffer offsets of the words and records are 31-bit)
     const bool seq_ok = seq_variant && (uint64_t)n_bodies * 32u < 0x7FFFFFFFull &&
                         (uint64_t)mc * 8u < 0x7FFFFFFFull &&
@@ -1055,14 +1315,17 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
                         (!pipe || seqx_reserve(c));
     // Form choice: small scenes keep every hand-off inside one CU; large ones
     // need the wider in-order window of the multi-CU form.  With 8 ticket
-    // streams the pipelined kernel on a 32k-body pile (1.5M sweep items per
-    // substep) was fastest with 256 one-wave workgroups (1.28 ms; 1.30 at
-    // 192, 1.30 at 320, 1.33 at 384, 1.68 at 128), the basic kernel on the
+    // streams the paired kernel on a 32k-body pile (1.5M sweep items per
+    // substep) was fastest with 512 one-wave workgroups (1.05 ms; 1.01 at
+    // 384 and 1.03 at 448 at the first timed frame but slower over the timed
+    // frames, 1.10 at 576, 1.30 at 768), the one-lane pipelined kernel with
+    // 256 (1.28 ms; 1.30 at 192, 1.30 at 320, 1.33 at 384), the basic kernel on the
     // 1M-body scene (7.2M items) with 640 (1.11 ms; 1.39 at 384, 1.17 at
     // 512, 1.14 at 768; the pipelined one 1.46 at 512); the ticket form with
     // coherent accesses at 64 four-wave workgroups (2.96 ms; 3.21 at 32).
     const bool seq_form = seq_variant;
-    const uint32_t seq_wgs = n_bodies >= (1u << 18) ? 640u : 256u;
+    // (the paired kernel holds 32 items per wave: twice the workgroups)
+    const uint32_t seq_wgs = (n_bodies >= (1u << 18) ? 640u : 256u) * (paired ? 2u : 1u);
     uint32_t wgs = c->solver_workgroups ? c->solver_workgroups
                                         : (n_bodies >= 2048 ? (seq_form ? seq_wgs : 64u) : 1u);
     if (wgs > 1 && seq_ok) {
@@ -1073,7 +1336,8 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
                            pipe ? reinterpret_cast<float *>(c->s.seq_x) : nullptr);
         if (pipe) {
             StageTimer sweep_timer(c, "phys/pgs_sweep");
-            hipLaunchKernelGGL(k_solve_seqp<SEQ_THREADS>, dim3(wgs), dim3(SEQ_THREADS), 0, c->stream, c->s.misc,
+            hipLaunchKernelGGL(paired ? k_solve_seqp2<SEQ_THREADS> : k_solve_seqp<SEQ_THREADS>, dim3(wgs),
+                               dim3(SEQ_THREADS), 0, c->stream, c->s.misc,
                                iters, c->s.cprep, c->s.seq_x, c->s.seq_v, c->s.seq_c, c->s.solve_ctl, counters,
                                seq_streams(wgs));
         } else {

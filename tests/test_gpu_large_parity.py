@@ -1,9 +1,9 @@
 """Parity where the automatic choices pick the multi-workgroup paths: a
 4,096-body container scene is stepped 240 frames on the device, its state is
 handed to the oracle, and two more frames must agree bit for bit (the
-sequence-form sweep, chosen from 2,048 bodies, in its pipelined kernel, and
-the all-pairs broadphase; then the LBVH broadphase, and the basic
-sequence kernel)."""
+sequence-form sweep, chosen from 2,048 bodies, in its paired pipelined
+kernel, and the all-pairs broadphase; then the LBVH broadphase, the basic
+sequence kernel and the one-lane pipelined kernel)."""
 import numpy as np
 import pytest
 
@@ -12,7 +12,7 @@ from harness import World, assert_bitwise
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("broadphase,variant", [("auto", "seq"), ("lbvh", "seq"), ("auto", "seq_basic")])
+@pytest.mark.parametrize("broadphase,variant", [("auto", "seq"), ("lbvh", "seq"), ("auto", "seq_basic"), ("auto", "pipelined")])
 def test_multi_workgroup_paths_match_oracle(broadphase, variant):
     from seevr_amd import Context, Scene, caps_for, default_caps, default_params
     n = 4096 + 5

@@ -98,7 +98,8 @@ def test_broadphase_narrowphase_bitwise(settled):
 # (schedule, workgroups, hand-off variant of the multi-workgroup forms)
 SOLVER_FORMS = [("auto", 0, "seq"), ("ordered", 1, "seq"), ("ordered", 8, "seq"), ("ordered", 64, "seq"),
                 ("ordered", 8, "coherent"), ("ordered", 8, "fenced"), ("ordered", 8, "pipelined"),
-                ("ordered", 64, "pipelined"), ("ordered", 8, "seq_basic"), ("ordered", 64, "seq_basic"), ("flow", 1, "seq"), ("flow", 64, "coherent"), ("flow", 64, "fenced")]
+                ("ordered", 64, "pipelined"), ("ordered", 8, "seq_basic"), ("ordered", 64, "seq_basic"),
+                ("ordered", 8, "paired"), ("ordered", 64, "paired"), ("flow", 1, "seq"), ("flow", 64, "coherent"), ("flow", 64, "fenced")]
 
 
 @pytest.mark.parametrize("schedule,wgs,variant", SOLVER_FORMS)
@@ -146,7 +147,8 @@ def test_integrate_verts_bitwise(settled):
 @pytest.mark.parametrize("frames,schedule,wgs,broadphase,variant",
                          [(1, "auto", 0, "auto", "seq"), (8, "auto", 0, "auto", "seq"), (8, "flow", 0, "auto", "seq"),
                           (8, "auto", 0, "lbvh", "seq"), (8, "ordered", 64, "auto", "seq"),
-                          (8, "ordered", 64, "auto", "pipelined"), (8, "ordered", 64, "auto", "seq_basic")])
+                          (8, "ordered", 64, "auto", "pipelined"), (8, "ordered", 64, "auto", "seq_basic"),
+                          (8, "ordered", 64, "auto", "paired")])
 def test_full_step_matches_oracle(settled, frames, schedule, wgs, broadphase, variant):
     sc, w = _stage_world(settled)
     ctx = new_ctx(sc, w.max_pairs, w.max_contacts)
