@@ -334,8 +334,9 @@ __global__ void k_copy_u32(uint32_t n, const uint32_t *src, uint32_t *dst) {
 // is below every leaf's, so no leaf below can overlap), when its smallest index is >= j, or when j and the
 // whole subtree are static.  The reported set is exactly the all-pairs set.
 // Pairs are counted per row i, the row counts scanned into offsets (total =
-// counters[0]), rows that start below max_pairs filled, and the filled part
-// sorted by (i, j): the first max_pairs entries are the all-pairs kernel's.
+// counters[0]), the pairs of rows that start below max_pairs appended in any
+// order, and that part sorted by (i, j): the first max_pairs entries are the
+// all-pairs kernel's.
 
 __global__ void k_bb_centroids(uint32_t n, const float4 *__restrict__ box, float *__restrict__ cent,
                                uint32_t *__restrict__ ids) {
@@ -392,20 +393,54 @@ SVD uint32_t xcd_block(uint32_t b, uint32_t g) {
 // conflict free); deeper ones in a private array.
 constexpr int BB_LDS = 16;
 
+// Rows a counting lane stages for k_bb_scatter (more: that lane walks again).
+constexpr int BB_STAGE = 32;
+
+// Append the pairs (mi, j) of the lanes with `found` set whose row mi starts
+// below max_pairs (the rows the capped list keeps) at the end of the pair
+// buffer: one atomic on the append counter per wave.  The pairs land in no
+// particular order; the pair sort that follows orders them.
+SVD void bb_append(bool found, uint32_t mi, uint32_t j, uint64_t lt, const uint32_t *__restrict__ off,
+                   uint32_t *tail, uint32_t *__restrict__ pk, uint32_t *__restrict__ pv, uint32_t max_pairs, int bits,
+                   bool combined) {
+    const bool keep = found && off[mi] < max_pairs;
+    const uint64_t km = __ballot(keep);
+    if (!km) return;
+    const int first = __builtin_ctzll(km);
+    uint32_t base = 0;
+    if (keep && __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0u)) == 0u)
+        base = atomicAdd(tail, (uint32_t)__popcll(km));
+    base = __shfl(base, first, 64);
+    if (!keep) return;
+    const uint32_t s = base + (uint32_t)__popcll(km & lt);
+    if (combined) {
+        pk[s] = (mi << bits) | j;
+    } else {
+        pk[s] = j;
+        pv[s] = mi;
+    }
+}
+
 // One lane per body j, in Morton order so neighbouring lanes walk similar
-// paths.  COUNT: bump the row count of every reported i.  EMIT: place the
-// pair in row i's slot range if the row starts below max_pairs.
+// paths.  COUNT: bump the row count of every reported i, and, with a
+// staging buffer, also keep the first BB_STAGE reported rows of lane t in
+// stage[k * n + t] and their number in scnt[t], so that k_bb_scatter appends
+// them without a second walk.  EMIT: append the pairs whose row starts below
+// max_pairs; with scnt given, only lanes that found more than BB_STAGE rows
+// walk (k_bb_scatter appended the others).
 template <bool EMIT>
 __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *__restrict__ ids,
                                                     const float4 *__restrict__ box, const float4 *__restrict__ nlo,
                                                     const float4 *__restrict__ nhi, const uint32_t *__restrict__ left,
                                                     const uint32_t *__restrict__ right, uint32_t *cnt,
-                                                    const uint32_t *__restrict__ off, uint32_t *fill,
+                                                    const uint32_t *__restrict__ off, uint32_t *tail,
                                                     uint32_t *__restrict__ pk, uint32_t *__restrict__ pv,
-                                                    uint32_t max_pairs, int bits, bool combined) {
+                                                    uint32_t max_pairs, int bits, bool combined,
+                                                    uint32_t *__restrict__ stage, uint32_t *__restrict__ scnt) {
     __shared__ uint32_t s_stack[BB_LDS * BLOCK];
     uint32_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
+    if (EMIT && scnt != nullptr && scnt[t] <= (uint32_t)BB_STAGE) return;
     const uint32_t j = ids[t];
     const float4 jl = box[2 * j], jh = box[2 * j + 1];
     const bool js = jl.w != 0.0f;
@@ -414,7 +449,7 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int sp = 0;
-    uint32_t node = 0;
+    uint32_t node = 0, nf = 0;
     for (;;) {
         const uint32_t ch[2] = {left[node], right[node]};
 #pragma unroll
@@ -428,36 +463,20 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
                 if (sp < BB_LDS) lstack[sp++ * BLOCK] = c;
                 else if (sp < BB_STACK) stack[sp++ - BB_LDS] = c;
             }
-            // Row counters of the lanes that found a leaf: the row of the
-            // first such lane (in a settled pile often the ground's, shared
-            // by most lanes) takes one atomic for all of its lanes; other
-            // rows take one each.
-            const uint64_t lm = __ballot(leaf);
-            if (lm) {
-                const int first = __builtin_ctzll(lm);
-                const uint32_t lead = __shfl(mi, first, 64);
-                const uint64_t same = __ballot(leaf && mi == lead);
-                uint32_t base = 0;
-                if (lane == first) {
-                    if (!EMIT) base = atomicAdd(cnt + lead, (uint32_t)__popcll(same));
-                    else if (off[lead] < max_pairs) base = atomicAdd(fill + lead, (uint32_t)__popcll(same));
-                }
-                base = __shfl(base, first, 64);
-                if (leaf) {
-                    if (!EMIT) {
+            if (EMIT) {
+                bb_append(leaf, mi, j, lt, off, tail, pk, pv, max_pairs, bits, combined);
+            } else {
+                // one atomic for the first finding lane's row and its lanes
+                const uint64_t lm = __ballot(leaf);
+                if (lm) {
+                    const int first = __builtin_ctzll(lm);
+                    const uint32_t lead = __shfl(mi, first, 64);
+                    const uint64_t same = __ballot(leaf && mi == lead);
+                    if (lane == first) atomicAdd(cnt + lead, (uint32_t)__popcll(same));
+                    if (leaf) {
                         if (mi != lead) atomicAdd(cnt + mi, 1u);
-                    } else {
-                        const uint32_t o = off[mi];
-                        if (o < max_pairs) {
-                            const uint32_t s = o + (mi == lead ? base + (uint32_t)__popcll(same & lt)
-                                                               : atomicAdd(fill + mi, 1u));
-                            if (combined) {
-                                pk[s] = (mi << bits) | j;
-                            } else {
-                                pk[s] = j;
-                                pv[s] = mi;
-                            }
-                        }
+                        if (stage != nullptr && nf < (uint32_t)BB_STAGE) stage[nf * n + t] = mi;
+                        ++nf;
                     }
                 }
             }
@@ -466,27 +485,82 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
         --sp;
         node = sp < BB_LDS ? lstack[sp * BLOCK] : stack[sp - BB_LDS];
     }
+    if (!EMIT && stage != nullptr) scnt[t] = nf;
 }
 
-// Per row: clear the fill counter; rows that start below max_pairs are
-// filled, and the last of them ends the part that is sorted.
-// The offsets are an exclusive scan of the counts, so row ends off + cnt do
-// not decrease with the row, and the rows starting below max_pairs are a
-// prefix [0, r): the filled part ends at off[r-1] + cnt[r-1], found by one
-// binary search (off[0] = 0 < max_pairs).
-__global__ void k_bb_rows(uint32_t n, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
-                          uint32_t *__restrict__ fill, uint32_t *misc, uint32_t max_pairs) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) fill[i] = 0u;
-    if (i == 0) {
-        uint32_t lo = 1, hi = n;  // first row in [1, n] whose offset is >= max_pairs (n if none)
-        while (lo < hi) {
-            const uint32_t mid = lo + (hi - lo) / 2;
-            if (off[mid] < max_pairs) lo = mid + 1;
-            else hi = mid;
-        }
-        misc[MISC_BPE] = off[lo - 1] + cnt[lo - 1];
+// Lane t appends the rows its walk staged (scnt[t] <= BB_STAGE); lanes with
+// more rows are left to the EMIT walk.  One append atomic per workgroup: the
+// lanes' kept counts are scanned in the workgroup (all rows are kept when
+// the total fits max_pairs; otherwise each staged row's offset is checked).
+__global__ void __launch_bounds__(BLOCK) k_bb_scatter(uint32_t n, const uint32_t *__restrict__ ids,
+                                                      const uint32_t *__restrict__ stage,
+                                                      const uint32_t *__restrict__ scnt,
+                                                      const uint32_t *__restrict__ off,
+                                                      const uint32_t *__restrict__ counters, uint32_t *tail,
+                                                      uint32_t *__restrict__ pk, uint32_t *__restrict__ pv,
+                                                      uint32_t max_pairs, int bits, bool combined) {
+    __shared__ uint32_t s_wave[BLOCK / 64 + 1];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t m = t < n ? scnt[t] : 0u;
+    if (m > (uint32_t)BB_STAGE) m = 0;
+    const bool all_kept = counters[0] <= max_pairs;
+    uint32_t kc = m;
+    if (!all_kept) {
+        kc = 0;
+        for (uint32_t k = 0; k < m; k++) kc += off[stage[k * n + t]] < max_pairs ? 1u : 0u;
     }
+    uint32_t inc = kc;  // inclusive scan over the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += v;
+    }
+    if (lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < BLOCK / 64; w++) {
+            const uint32_t v = s_wave[w];
+            s_wave[w] = acc;
+            acc += v;
+        }
+        s_wave[BLOCK / 64] = acc ? atomicAdd(tail, acc) : 0u;
+    }
+    __syncthreads();
+    if (kc == 0) return;
+    uint32_t o = s_wave[BLOCK / 64] + s_wave[wave] + inc - kc;
+    const uint32_t j = ids[t];
+    for (uint32_t k = 0; k < m; k++) {
+        const uint32_t mi = stage[k * n + t];
+        if (!all_kept && off[mi] >= max_pairs) continue;
+        if (combined) {
+            pk[o] = (mi << bits) | j;
+        } else {
+            pk[o] = j;
+            pv[o] = mi;
+        }
+        ++o;
+    }
+}
+
+// Clear the append counter and find where the kept part of the pair list
+// ends: rows that start below max_pairs are kept.  The offsets are an
+// exclusive scan of the counts, so row ends off + cnt do not decrease with
+// the row, and the kept rows are a prefix [0, r): the kept part ends at
+// off[r-1] + cnt[r-1], found by one binary search (off[0] = 0 < max_pairs).
+// One thread.
+__global__ void k_bb_rows(uint32_t n, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
+                          uint32_t *__restrict__ tail, uint32_t *misc, uint32_t max_pairs) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    tail[0] = 0u;
+    uint32_t lo = 1, hi = n;  // first row in [1, n] whose offset is >= max_pairs (n if none)
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (off[mid] < max_pairs) lo = mid + 1;
+        else hi = mid;
+    }
+    misc[MISC_BPE] = off[lo - 1] + cnt[lo - 1];
 }
 
 __global__ void k_bb_write(const uint32_t *__restrict__ misc, const uint32_t *__restrict__ ki,
@@ -596,6 +670,8 @@ bool bb_reserve(sv_ctx *c) {
     if (e == hipSuccess) e = lazy_alloc(c, s.bb_cnt, N + 1);
     if (e == hipSuccess) e = lazy_alloc(c, s.bb_off, N + 1);
     if (e == hipSuccess) e = lazy_alloc(c, s.bb_fill, N + 1);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_stage, (size_t)BB_STAGE * N + 1);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_scnt, N + 1);
     s.bb_ready = e == hipSuccess;
     return s.bb_ready;
 }
@@ -634,14 +710,15 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
     fill_u32(c, s.bb_cnt, 0u, n);
     hipLaunchKernelGGL(k_bb_pairs<false>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_lo, s.bb_hi, s.bb_left,
                        s.bb_right, s.bb_cnt, (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr,
-                       (uint32_t *)nullptr, mp, bits, combined);
+                       (uint32_t *)nullptr, mp, bits, combined, s.bb_stage, s.bb_scnt);
     scan_exclusive(c, s.bb_cnt, s.bb_off, nullptr, n, counters);
     hipLaunchKernelGGL(k_bb_post, dim3(1), dim3(1), 0, st, counters, s.misc, mp);
-    fill_u32(c, s.misc + MISC_BPE, 0u, 1);
-    hipLaunchKernelGGL(k_bb_rows, dim3(g), dim3(BLOCK), 0, st, n, s.bb_cnt, s.bb_off, s.bb_fill, s.misc, mp);
+    hipLaunchKernelGGL(k_bb_rows, dim3(1), dim3(64), 0, st, n, s.bb_cnt, s.bb_off, s.bb_fill, s.misc, mp);
+    hipLaunchKernelGGL(k_bb_scatter, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bb_stage, s.bb_scnt, s.bb_off, counters,
+                       s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits, combined);
     hipLaunchKernelGGL(k_bb_pairs<true>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_lo, s.bb_hi, s.bb_left,
                        s.bb_right, (uint32_t *)nullptr, s.bb_off, s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits,
-                       combined);
+                       combined, (uint32_t *)nullptr, s.bb_scnt);
     const uint32_t cap = mp + n;
     const uint32_t *ki, *kj;
     if (combined) {

@@ -51,6 +51,8 @@ struct Scratch {
     uint32_t *bb_mk[2] = {nullptr, nullptr}, *bb_mv[2] = {nullptr, nullptr};  // [N] Morton sort
     uint32_t *bb_cnt = nullptr, *bb_off = nullptr, *bb_fill = nullptr;         // [N + 1] per-row pairs
     uint32_t *bb_pk[2] = {nullptr, nullptr}, *bb_pv[2] = {nullptr, nullptr};  // [P + N] pair sort
+    uint32_t *bb_stage = nullptr;   // uint[BB_STAGE * N] rows staged by each counting lane
+    uint32_t *bb_scnt = nullptr;    // uint[N] rows found by each counting lane
     bool bb_ready = false;
     // level-ordered BVH refit (allocated on first use)
     uint32_t *lv_depth = nullptr, *lv_order = nullptr;  // [max(T, N)] node depth, nodes by depth

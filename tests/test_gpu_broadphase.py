@@ -92,6 +92,39 @@ def test_all_static_and_two_bodies():
     assert_bitwise(got, want, "two-body scene")
 
 
+def clump_world(max_pairs):
+    """box_world(3000) with 60 dynamic bodies (indices from 1000) sharing one box."""
+    sc, w = box_world(3000, 31, max_pairs=max_pairs)
+    lo = w.a["BODY_AABB_MIN"].reshape(-1, 3)
+    hi = w.a["BODY_AABB_MAX"].reshape(-1, 3)
+    clump = np.arange(1000, 3000, 2000 // 60)[:60]
+    lo[clump] = lo[clump[0]]
+    hi[clump] = hi[clump[0]]
+    w.a["BODY_INV_MASS"][clump] = 1.0
+    return sc, w, clump
+
+
+@pytest.mark.parametrize("cut", [False, True])
+def test_lbvh_rows_beyond_staging(cut):
+    """Lanes that find more rows than the counting walk stages walk again.
+
+    The clump's body with the largest index finds 59 rows; most other bodies
+    find few.  With `cut`, the cap ends the list inside the clump's rows.
+    """
+    _, wf, clump = clump_world(400000)
+    _, full = oracle_broadphase(wf)
+    per_j = np.bincount(full[:, 1], minlength=wf.n)
+    assert per_j.max() >= 59 and (per_j[per_j > 0] <= 32).mean() > 0.5 and (per_j > 32).sum() > 0
+    cap = 400000
+    if cut:
+        cap = int(np.searchsorted(full[:, 0], clump[len(clump) // 2])) + 7
+    sc, w, _ = clump_world(cap)
+    want_cnt, want = oracle_broadphase(w)
+    got_cnt, got = device_broadphase(sc, w, "lbvh")
+    assert int(got_cnt[0]) == int(want_cnt[0])
+    assert_bitwise(got, want, f"lbvh pairs with a clump, cap={cap}")
+
+
 def test_lbvh_equals_all_pairs_at_scale():
     """40k bodies: the two device methods agree (the oracle pins both above)."""
     sc, w = box_world(40000, 21, max_pairs=2_000_000)
