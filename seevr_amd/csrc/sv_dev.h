@@ -88,6 +88,49 @@ SVD void body_aabb_wave(uint32_t b, int lane, const float *__restrict__ pos, con
     }
 }
 
+// The same box with 16 lanes (one DPP row) per body, four bodies per wave.
+// Lane g of the row takes vertices g, g + 16, ...; each round loads 6 per
+// lane before using any (indices past the last vertex load the last one
+// again, which leaves min and max as they are), and the row reduces by
+// right shifts of 1, 2, 4, 8 lanes, so lane 15 of the row ends with the
+// box.  min and max select one of their operands, so any order of
+// reduction gives the serial loop's box.
+constexpr int AABB_ROW = 16, AABB_UNROLL = 6;
+
+template <int CTRL>
+SVD float dpp_keep(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+SVD void body_aabb_row(uint32_t b, int g, const float *__restrict__ pos, const float *__restrict__ quat,
+                       const uint32_t *__restrict__ hrange, const float *__restrict__ hull, m3 &R, f3 &mn,
+                       f3 &mx) {
+    R = quat_mat_at(quat, b);
+    const f3 p = ld3(pos, b);
+    const uint32_t vs = hrange[2 * b], vc = hrange[2 * b + 1];
+    mn = mk(1e10f, 1e10f, 1e10f);
+    mx = mk(-1e10f, -1e10f, -1e10f);
+    for (uint32_t v0 = (uint32_t)g; v0 < vc; v0 += AABB_ROW * AABB_UNROLL) {
+        f3 hv[AABB_UNROLL];
+#pragma unroll
+        for (int k = 0; k < AABB_UNROLL; k++) hv[k] = ld3(hull, vs + min(v0 + (uint32_t)(AABB_ROW * k), vc - 1u));
+#pragma unroll
+        for (int k = 0; k < AABB_UNROLL; k++) {
+            const f3 wv = p + mvec(R, hv[k]);
+            mn = vmin3(mn, wv);
+            mx = vmax3(mx, wv);
+        }
+    }
+#define SV_ROW_STEP(CTRL)                                                                          \
+    mn = vmin3(mn, mk(dpp_keep<CTRL>(mn.x), dpp_keep<CTRL>(mn.y), dpp_keep<CTRL>(mn.z)));        \
+    mx = vmax3(mx, mk(dpp_keep<CTRL>(mx.x), dpp_keep<CTRL>(mx.y), dpp_keep<CTRL>(mx.z)));
+    SV_ROW_STEP(0x111)  // row_shr:1
+    SV_ROW_STEP(0x112)  // row_shr:2
+    SV_ROW_STEP(0x114)  // row_shr:4
+    SV_ROW_STEP(0x118)  // row_shr:8
+#undef SV_ROW_STEP
+}
+
 // ---------------------------------------------------------------------------
 // MPR (physics.comp:107-399) for one pair.
 struct Body {

@@ -40,8 +40,9 @@ __global__ void k_gravity(uint32_t n, float dt, float gx, float gy, float gz, fl
 }
 
 // Per-body AABB, the semantics of physics.comp:431-448: min/max over
-// p + R*v of the body's hull verts, seeded with +-1e10.  One wave per body,
-// lanes stride the hull (coalesced 12-byte loads), then a shuffle tree.
+// p + R*v of the body's hull verts, seeded with +-1e10.  One DPP row (16
+// lanes) per body, four bodies per wave: lanes stride the hull, six 12-byte
+// loads in flight per lane, then a row reduction.
 // Also caches the body's rotation columns and a packed broadphase box
 // (min.xyz, static flag), (max.xyz, 0) for the later stages.
 __global__ void __launch_bounds__(BLOCK) k_aabb(uint32_t n, const float *__restrict__ pos,
@@ -49,13 +50,14 @@ __global__ void __launch_bounds__(BLOCK) k_aabb(uint32_t n, const float *__restr
                                                 const float *__restrict__ hull, const float *__restrict__ inv_mass,
                                                 float *__restrict__ amin_out, float *__restrict__ amax_out,
                                                 float *__restrict__ rot, float4 *__restrict__ bpbox) {
-    uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    int lane = threadIdx.x & 63;
+    // a DPP row per body: rows only read lanes of their own row
+    const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) / AABB_ROW;
+    const int g = threadIdx.x & (AABB_ROW - 1);
     if (b >= n) return;
     m3 R;
     f3 mn, mx;
-    body_aabb_wave(b, lane, pos, quat, hrange, hull, R, mn, mx);
-    if (lane == 0) {
+    body_aabb_row(b, g, pos, quat, hrange, hull, R, mn, mx);
+    if (g == AABB_ROW - 1) {
         st3(amin_out, b, mn);
         st3(amax_out, b, mx);
         st_rot(rot, b, R);
@@ -440,7 +442,7 @@ void launch_gravity(sv_ctx *c, uint32_t n, float dt, const float g[3]) {
 
 void launch_aabb(sv_ctx *c, uint32_t n) {
     if (!n) return;
-    hipLaunchKernelGGL(k_aabb, dim3(div_up((uint64_t)n * 64, BLOCK)), dim3(BLOCK), 0, c->stream, n,
+    hipLaunchKernelGGL(k_aabb, dim3(div_up((uint64_t)n * AABB_ROW, BLOCK)), dim3(BLOCK), 0, c->stream, n,
                        B<float>(c, SV_BODY_POS), B<float>(c, SV_BODY_QUAT), c->s.hrange, B<float>(c, SV_HULL_VERTS),
                        B<float>(c, SV_BODY_INV_MASS), B<float>(c, SV_BODY_AABB_MIN), B<float>(c, SV_BODY_AABB_MAX),
                        c->s.rot, c->s.bpbox);
