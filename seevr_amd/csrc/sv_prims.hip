@@ -116,11 +116,16 @@ __global__ void k_fill(uint32_t *p, uint32_t value, size_t n) {
     for (; i < n; i += stride) p[i] = value;
 }
 
-// Per-tile 256-bin digit histogram, digit-major: hist[d * nblocks + blk].
+// Per-tile 256-bin digit histogram, digit-major over the tiles that hold
+// keys: hist[d * nb + blk], nb = tiles of the key count read on the device
+// (<= nblocks, the launch), so the scan between the two kernels covers
+// 256 * nb entries, whose number block 0 writes to *hist_n.
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t *keys, const uint32_t *n_dev, uint32_t n_max,
-                                                            int shift, uint32_t *hist, uint32_t nblocks) {
+                                                            int shift, uint32_t *hist, uint32_t *hist_n) {
     __shared__ uint32_t h[256];
-    uint32_t n = read_n(n_dev, n_max);
+    const uint32_t n = read_n(n_dev, n_max), nb = (n + SORT_TILE - 1) / SORT_TILE;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *hist_n = 256u * nb;
+    if (blockIdx.x >= nb) return;
     h[threadIdx.x] = 0;
     __syncthreads();
     uint32_t base = blockIdx.x * SORT_TILE;
@@ -129,26 +134,26 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t *keys
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
 // Stable scatter.  Element order inside a tile: wave w owns
 // [w*ROUNDS*64, (w+1)*ROUNDS*64), round r the next 64, lane = offset.
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                                                                uint32_t *vout, const uint32_t *n_dev, uint32_t n_max,
-                                                               int shift, const uint32_t *offs, uint32_t nblocks) {
+                                                               int shift, const uint32_t *offs) {
     __shared__ uint32_t wcnt[SORT_WAVES][256];
     __shared__ uint32_t gbase[256];
-    uint32_t n = read_n(n_dev, n_max);
+    const uint32_t n = read_n(n_dev, n_max), nb = (n + SORT_TILE - 1) / SORT_TILE;
+    if (blockIdx.x >= nb) return;
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int d = threadIdx.x; d < 256; d += SORT_THREADS) {
-        gbase[d] = offs[d * nblocks + blockIdx.x];
+        gbase[d] = offs[d * nb + blockIdx.x];
 #pragma unroll
         for (int q = 0; q < SORT_WAVES; q++) wcnt[q][d] = 0;
     }
     __syncthreads();
     uint32_t base = blockIdx.x * SORT_TILE + w * (SORT_ROUNDS * 64);
-    if (blockIdx.x * SORT_TILE >= n) return;
     uint32_t key[SORT_ROUNDS], val[SORT_ROUNDS], loc[SORT_ROUNDS];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
@@ -235,17 +240,17 @@ int radix_sort(sv_ctx *c, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1
     if (n_max == 0 || key_bits <= 0) return 0;
     uint32_t nblocks = div_up(n_max, SORT_TILE);
     uint32_t hist_n = 256u * nblocks;
-    ensure_u32(c->s.sort_hist, c->s.sort_hist_cap, 2u * hist_n);
-    uint32_t *hist = c->s.sort_hist, *offs = c->s.sort_hist + hist_n;
+    ensure_u32(c->s.sort_hist, c->s.sort_hist_cap, 2u * hist_n + 1);
+    uint32_t *hist = c->s.sort_hist, *offs = c->s.sort_hist + hist_n, *used = c->s.sort_hist + 2u * hist_n;
     int passes = (key_bits + 7) / 8;
     uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
     for (int p = 0; p < passes; p++) {
         int shift = p * 8;
         hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, c->stream, ka, n_dev, n_max, shift,
-                           hist, nblocks);
-        scan_exclusive(c, hist, offs, nullptr, hist_n, nullptr);
+                           hist, used);
+        scan_exclusive(c, hist, offs, used, hist_n, nullptr);
         hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(SORT_THREADS), 0, c->stream, ka, va, kb, vb, n_dev,
-                           n_max, shift, offs, nblocks);
+                           n_max, shift, offs);
         uint32_t *t = ka; ka = kb; kb = t;
         t = va; va = vb; vb = t;
     }
