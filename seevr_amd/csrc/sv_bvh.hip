@@ -488,10 +488,20 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
     if (!EMIT && stage != nullptr) scnt[t] = nf;
 }
 
-// Lane t appends the rows its walk staged (scnt[t] <= BB_STAGE); lanes with
-// more rows are left to the EMIT walk.  One append atomic per workgroup: the
-// lanes' kept counts are scanned in the workgroup (all rows are kept when
-// the total fits max_pairs; otherwise each staged row's offset is checked).
+// Thread x of a workgroup appends the rows staged by lanes t = base + x +
+// k * BLOCK, k < BB_SCATTER_ITEMS (scnt[t] <= BB_STAGE); lanes with more
+// rows are left to the EMIT walk.  One append atomic per workgroup (the
+// atomics of one counter serialise, so a workgroup takes many lanes): the
+// waves' kept counts are scanned in the workgroup (all rows are kept when
+// the total fits max_pairs; otherwise each staged row's offset is checked),
+// and each wave writes its entries in rounds across its lanes.
+constexpr int BB_SCATTER_ITEMS = 8;
+
+SVD uint32_t bb_staged(const uint32_t *__restrict__ scnt, uint32_t t, uint32_t n) {
+    const uint32_t m = t < n ? scnt[t] : 0u;
+    return m > (uint32_t)BB_STAGE ? 0u : m;
+}
+
 __global__ void __launch_bounds__(BLOCK) k_bb_scatter(uint32_t n, const uint32_t *__restrict__ ids,
                                                       const uint32_t *__restrict__ stage,
                                                       const uint32_t *__restrict__ scnt,
@@ -500,23 +510,21 @@ __global__ void __launch_bounds__(BLOCK) k_bb_scatter(uint32_t n, const uint32_t
                                                       uint32_t *__restrict__ pk, uint32_t *__restrict__ pv,
                                                       uint32_t max_pairs, int bits, bool combined) {
     __shared__ uint32_t s_wave[BLOCK / 64 + 1];
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t m = t < n ? scnt[t] : 0u;
-    if (m > (uint32_t)BB_STAGE) m = 0;
+    const uint32_t base = blockIdx.x * (BLOCK * BB_SCATTER_ITEMS) + threadIdx.x;
     const bool all_kept = counters[0] <= max_pairs;
-    uint32_t kc = m;
-    if (!all_kept) {
-        kc = 0;
-        for (uint32_t k = 0; k < m; k++) kc += off[stage[k * n + t]] < max_pairs ? 1u : 0u;
+    uint32_t tot = 0;
+    for (int it = 0; it < BB_SCATTER_ITEMS; it++) {
+        const uint32_t t = base + (uint32_t)(it * BLOCK), m = bb_staged(scnt, t, n);
+        if (all_kept) {
+            tot += m;
+        } else {
+            for (uint32_t k = 0; k < m; k++) tot += off[stage[k * n + t]] < max_pairs ? 1u : 0u;
+        }
     }
-    uint32_t inc = kc;  // inclusive scan over the wave
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t v = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += v;
-    }
-    if (lane == 63) s_wave[wave] = inc;
+    for (int d = 32; d >= 1; d >>= 1) tot += __shfl_xor(tot, d, 64);
+    if (lane == 0) s_wave[wave] = tot;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t acc = 0;
@@ -528,19 +536,31 @@ __global__ void __launch_bounds__(BLOCK) k_bb_scatter(uint32_t n, const uint32_t
         s_wave[BLOCK / 64] = acc ? atomicAdd(tail, acc) : 0u;
     }
     __syncthreads();
-    if (kc == 0) return;
-    uint32_t o = s_wave[BLOCK / 64] + s_wave[wave] + inc - kc;
-    const uint32_t j = ids[t];
-    for (uint32_t k = 0; k < m; k++) {
-        const uint32_t mi = stage[k * n + t];
-        if (!all_kept && off[mi] >= max_pairs) continue;
-        if (combined) {
-            pk[o] = (mi << bits) | j;
-        } else {
-            pk[o] = j;
-            pv[o] = mi;
+    uint32_t o = s_wave[BLOCK / 64] + s_wave[wave];  // the wave's first slot
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int it = 0; it < BB_SCATTER_ITEMS; it++) {
+        const uint32_t t = base + (uint32_t)(it * BLOCK), m = bb_staged(scnt, t, n);
+        const uint32_t j = m ? ids[t] : 0u;
+        for (uint32_t k0 = 0; __ballot(k0 < m); k0 += 4) {
+            uint32_t mi[4];  // four staged rows in flight per lane
+#pragma unroll
+            for (int q = 0; q < 4; q++) mi[q] = k0 + q < m ? stage[(k0 + q) * n + t] : 0u;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const bool keep = k0 + q < m && (all_kept || off[mi[q]] < max_pairs);
+                const uint64_t km = __ballot(keep);
+                if (keep) {
+                    const uint32_t s = o + (uint32_t)__popcll(km & lt);
+                    if (combined) {
+                        pk[s] = (mi[q] << bits) | j;
+                    } else {
+                        pk[s] = j;
+                        pv[s] = mi[q];
+                    }
+                }
+                o += (uint32_t)__popcll(km);
+            }
         }
-        ++o;
     }
 }
 
@@ -714,7 +734,7 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
     scan_exclusive(c, s.bb_cnt, s.bb_off, nullptr, n, counters);
     hipLaunchKernelGGL(k_bb_post, dim3(1), dim3(1), 0, st, counters, s.misc, mp);
     hipLaunchKernelGGL(k_bb_rows, dim3(1), dim3(64), 0, st, n, s.bb_cnt, s.bb_off, s.bb_fill, s.misc, mp);
-    hipLaunchKernelGGL(k_bb_scatter, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bb_stage, s.bb_scnt, s.bb_off, counters,
+    hipLaunchKernelGGL(k_bb_scatter, dim3(div_up(n, BLOCK * BB_SCATTER_ITEMS)), dim3(BLOCK), 0, st, n, ids, s.bb_stage, s.bb_scnt, s.bb_off, counters,
                        s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits, combined);
     hipLaunchKernelGGL(k_bb_pairs<true>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_lo, s.bb_hi, s.bb_left,
                        s.bb_right, (uint32_t *)nullptr, s.bb_off, s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits,
