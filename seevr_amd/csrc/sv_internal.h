@@ -187,7 +187,9 @@ void ensure_support_vertices(sv_ctx *c, uint32_t n);
 void launch_gravity(sv_ctx *c, uint32_t n, float dt, const float g[3]);
 void launch_aabb(sv_ctx *c, uint32_t n);
 void launch_broadphase(sv_ctx *c, uint32_t n);
-void launch_narrowphase(sv_ctx *c);
+// after_broadphase: the same substep's broadphase has set the pair count;
+// clear_lambda: also clear the contact impulses (the next step's work).
+void launch_narrowphase(sv_ctx *c, bool after_broadphase = false, bool clear_lambda = false);
 void launch_clear_lambda(sv_ctx *c);
 void launch_integrate(sv_ctx *c, uint32_t n, float dt);
 void launch_update_verts(sv_ctx *c, uint32_t nv, uint32_t nb);

@@ -278,22 +278,27 @@ __global__ void __launch_bounds__(BLOCK) k_narrow(const uint32_t *__restrict__ m
     }
 }
 
-// counters[1] = total contacts (written by the scan); clamp, flag overflow.
-__global__ void k_post_contacts(uint32_t *counters, uint32_t *misc, uint32_t max_contacts) {
-    uint32_t t = counters[1];
-    misc[MISC_NC] = t < max_contacts ? t : max_contacts;
-    if (t > max_contacts) counters[2] |= SV_STATUS_CONTACT_OVERFLOW;
-}
-
-// add_contact (physics.comp:403-412) at the canonical slot.
-__global__ void __launch_bounds__(BLOCK) k_compact_contacts(const uint32_t *__restrict__ misc,
+// add_contact (physics.comp:403-412) at the canonical slot.  Thread 0 also
+// clamps the contact count (counters[1] holds the total) and flags an
+// overflow, and with `lambda` given the kernel clears the impulses of every
+// contact slot (the next step, physics.comp:494-497, which reads nothing
+// written here).
+__global__ void __launch_bounds__(BLOCK) k_compact_contacts(uint32_t *__restrict__ misc, uint32_t *__restrict__ counters,
                                                             const uint32_t *__restrict__ pairs,
                                                             const uint32_t *__restrict__ flag,
                                                             const uint32_t *__restrict__ idx,
                                                             const float *__restrict__ res, uint32_t max_contacts,
                                                             float *__restrict__ cpos, float *__restrict__ cnrm,
                                                             float *__restrict__ cpen, uint32_t *__restrict__ ga,
-                                                            uint32_t *__restrict__ gb) {
+                                                            uint32_t *__restrict__ gb, float *__restrict__ lambda) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t t = counters[1];
+        misc[MISC_NC] = t < max_contacts ? t : max_contacts;
+        if (t > max_contacts) counters[2] |= SV_STATUS_CONTACT_OVERFLOW;
+    }
+    if (lambda)
+        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < max_contacts; k += gridDim.x * blockDim.x)
+            lambda[k] = 0.0f;
     uint32_t np = misc[MISC_NP];
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
         if (!flag[p]) continue;
@@ -474,10 +479,12 @@ void launch_broadphase(sv_ctx *c, uint32_t n) {
                        (uint32_t *)nullptr, c->s.row_off, c->s.bp_stage, c->s.bp_ovf, pairs, c->caps.max_pairs);
 }
 
-void launch_narrowphase(sv_ctx *c) {
+void launch_narrowphase(sv_ctx *c, bool after_broadphase, bool clear_lambda) {
     uint32_t *counters = B<uint32_t>(c, SV_ATOMIC_COUNTERS);
-    // pairs to process: min(counters[0], max_pairs) (physics.comp:477)
-    hipLaunchKernelGGL(k_post_pairs, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, c->caps.max_pairs);
+    // pairs to process: min(counters[0], max_pairs) (physics.comp:477); the
+    // broadphase of the same substep has computed it already
+    if (!after_broadphase)
+        hipLaunchKernelGGL(k_post_pairs, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, c->caps.max_pairs);
     uint32_t mp = c->caps.max_pairs;
     uint32_t grid = div_up(mp, BLOCK);
     if (grid > 8192) grid = 8192;
@@ -487,12 +494,12 @@ void launch_narrowphase(sv_ctx *c) {
                        B<uint32_t>(c, SV_SUPPORT_LOOKUP_OFFSET), B<uint32_t>(c, SV_SUPPORT_LOOKUP),
                        B<float>(c, SV_HULL_VERTS), c->s.sup_xyz, c->s.np_flag, c->s.np_res);
     scan_exclusive(c, c->s.np_flag, c->s.np_idx, c->s.misc + MISC_NP, mp, counters + 1);
-    hipLaunchKernelGGL(k_post_contacts, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, c->caps.max_contacts);
-    hipLaunchKernelGGL(k_compact_contacts, dim3(grid), dim3(BLOCK), 0, c->stream, c->s.misc,
+    hipLaunchKernelGGL(k_compact_contacts, dim3(grid), dim3(BLOCK), 0, c->stream, c->s.misc, counters,
                        B<uint32_t>(c, SV_COLLISION_PAIRS), c->s.np_flag, c->s.np_idx, c->s.np_res,
                        c->caps.max_contacts, B<float>(c, SV_CONTACT_POS), B<float>(c, SV_CONTACT_NORMAL),
                        B<float>(c, SV_CONTACT_PENETRATION), B<uint32_t>(c, SV_CONTACT_GEOM_A),
-                       B<uint32_t>(c, SV_CONTACT_GEOM_B));
+                       B<uint32_t>(c, SV_CONTACT_GEOM_B),
+                       clear_lambda ? B<float>(c, SV_CONTACT_LAMBDA_N) : (float *)nullptr);
 }
 
 void launch_clear_lambda(sv_ctx *c) {
