@@ -84,31 +84,26 @@ __global__ void __launch_bounds__(BLOCK) k_aabb(uint32_t n, const float *__restr
 // BP_STAGE partners.  An exclusive scan over the counts in (row, segment)
 // order gives every (row, segment) its slot range in lexicographic (i, j)
 // order; k_bp_place copies staged partners there, and only tiles holding a
-// (row, segment) with more than BP_STAGE hits are tested again to place the
-// rest (EMIT form).
+// (row, segment) with more than BP_STAGE hits, which the test pass lists,
+// are tested again to place the rest (EMIT form).
 constexpr int BP_ROWS_PER_WAVE = 8;
 constexpr int BP_TILE = BLOCK;  // 256 boxes per LDS tile
 
 template <bool EMIT>
-__global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, uint32_t nseg, uint32_t seg_len,
-                                                      const float4 *__restrict__ box, uint32_t *__restrict__ cnt_out,
-                                                      const uint32_t *__restrict__ off,
-                                                      uint32_t *__restrict__ stage, uint32_t *__restrict__ tile_ovf,
-                                                      uint32_t *__restrict__ pairs, uint32_t max_pairs) {
-    __shared__ float4 s_lo[BP_TILE];
-    __shared__ float4 s_hi[BP_TILE];
-    const uint32_t seg = blockIdx.y;
-    const uint32_t blk_row0 = blockIdx.x * BP_ROWS;
+SVD void bp_tile(uint32_t bx, uint32_t seg, uint32_t n, uint32_t nseg, uint32_t seg_len,
+                 const float4 *__restrict__ box, uint32_t *__restrict__ cnt_out, const uint32_t *__restrict__ off,
+                 uint32_t *__restrict__ stage, uint32_t *__restrict__ ovf_list, uint32_t *__restrict__ pairs,
+                 uint32_t max_pairs, float4 *s_lo, float4 *s_hi, uint32_t *s_ovf) {
+    const uint32_t blk_row0 = bx * BP_ROWS;
     const uint32_t j_lo = max(seg * seg_len, blk_row0 + 1);
     const uint32_t j_hi = min(seg * seg_len + seg_len, n);
-    const uint32_t tile = blockIdx.x * nseg + seg;
-    if (EMIT) {
-        if (j_lo >= j_hi || tile_ovf[tile] == 0) return;
-    } else if (j_lo >= j_hi) {  // no j > i in this segment: zero its counts
+    const uint32_t tile = bx * nseg + seg;
+    if (j_lo >= j_hi) {  // no j > i in this segment: zero its counts
         uint32_t i = blk_row0 + threadIdx.x;
-        if (threadIdx.x < BP_ROWS && i < n) cnt_out[(size_t)i * nseg + seg] = 0;
+        if (!EMIT && threadIdx.x < BP_ROWS && i < n) cnt_out[(size_t)i * nseg + seg] = 0;
         return;
     }
+    if (!EMIT && threadIdx.x == 0) *s_ovf = 0u;
     const int lane = threadIdx.x & 63;
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t row0 = blk_row0 + w * BP_ROWS_PER_WAVE;
@@ -188,15 +183,45 @@ __global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, uint32_t nseg,
             }
         }
     }
-    if (!EMIT && lane == 0) {
-        bool ovf = false;
+    if (!EMIT) {
+        if (lane == 0) {
+            bool ovf = false;
 #pragma unroll
-        for (int r = 0; r < BP_ROWS_PER_WAVE; r++) {
-            uint32_t i = row0 + r;
-            if (i < n) cnt_out[(size_t)i * nseg + seg] = cnt[r];
-            ovf |= cnt[r] > BP_STAGE;
+            for (int r = 0; r < BP_ROWS_PER_WAVE; r++) {
+                uint32_t i = row0 + r;
+                if (i < n) cnt_out[(size_t)i * nseg + seg] = cnt[r];
+                ovf |= cnt[r] > BP_STAGE;
+            }
+            if (ovf) *s_ovf = 1u;
         }
-        if (ovf) tile_ovf[tile] = 1u;
+        __syncthreads();
+        // list the tile for the EMIT pass: ovf_list[0] counts, tiles follow
+        if (threadIdx.x == 0 && *s_ovf) ovf_list[1 + atomicAdd(ovf_list, 1u)] = tile;
+    }
+}
+
+// Count pass: one block per (row tile, column segment).  EMIT pass: a
+// smaller grid walks the tiles the count pass listed.
+template <bool EMIT>
+__global__ void __launch_bounds__(BLOCK) k_broadphase(uint32_t n, uint32_t nseg, uint32_t seg_len,
+                                                      const float4 *__restrict__ box, uint32_t *__restrict__ cnt_out,
+                                                      const uint32_t *__restrict__ off,
+                                                      uint32_t *__restrict__ stage, uint32_t *__restrict__ ovf_list,
+                                                      uint32_t *__restrict__ pairs, uint32_t max_pairs) {
+    __shared__ float4 s_lo[BP_TILE];
+    __shared__ float4 s_hi[BP_TILE];
+    __shared__ uint32_t s_ovf;
+    if (!EMIT) {
+        bp_tile<false>(blockIdx.x, blockIdx.y, n, nseg, seg_len, box, cnt_out, off, stage, ovf_list, pairs, max_pairs,
+                       s_lo, s_hi, &s_ovf);
+        return;
+    }
+    const uint32_t listed = ovf_list[0];
+    for (uint32_t q = blockIdx.x; q < listed; q += gridDim.x) {
+        const uint32_t tile = ovf_list[1 + q];
+        __syncthreads();  // the previous tile's LDS reads are done
+        bp_tile<true>(tile / nseg, tile % nseg, n, nseg, seg_len, box, cnt_out, off, stage, ovf_list, pairs, max_pairs,
+                      s_lo, s_hi, &s_ovf);
     }
 }
 
@@ -466,7 +491,7 @@ void launch_broadphase(sv_ctx *c, uint32_t n) {
     const uint32_t seg_len = bp_seg_len(n), nseg = bp_nseg(n);
     const uint32_t nrb = div_up(n, BP_ROWS);
     uint32_t *pairs = B<uint32_t>(c, SV_COLLISION_PAIRS);
-    fill_u32(c, c->s.bp_ovf, 0u, (size_t)nrb * nseg);
+    fill_u32(c, c->s.bp_ovf, 0u, 1);
     hipLaunchKernelGGL(k_broadphase<false>, dim3(nrb, nseg), dim3(BLOCK), 0, c->stream, n, nseg, seg_len, c->s.bpbox,
                        c->s.row_count, (const uint32_t *)nullptr, c->s.bp_stage, c->s.bp_ovf, (uint32_t *)nullptr,
                        c->caps.max_pairs);
@@ -475,7 +500,8 @@ void launch_broadphase(sv_ctx *c, uint32_t n) {
     hipLaunchKernelGGL(k_post_pairs, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, c->caps.max_pairs);
     hipLaunchKernelGGL(k_bp_place, dim3(div_up((uint64_t)ncnt * BP_STAGE, BLOCK)), dim3(BLOCK), 0, c->stream, n,
                        nseg, c->s.row_count, c->s.row_off, c->s.bp_stage, pairs, c->caps.max_pairs);
-    hipLaunchKernelGGL(k_broadphase<true>, dim3(nrb, nseg), dim3(BLOCK), 0, c->stream, n, nseg, seg_len, c->s.bpbox,
+    const uint32_t emit_grid = nrb * nseg < 512u ? nrb * nseg : 512u;
+    hipLaunchKernelGGL(k_broadphase<true>, dim3(emit_grid), dim3(BLOCK), 0, c->stream, n, nseg, seg_len, c->s.bpbox,
                        (uint32_t *)nullptr, c->s.row_off, c->s.bp_stage, c->s.bp_ovf, pairs, c->caps.max_pairs);
 }
 
