@@ -353,8 +353,7 @@ int sv_physics_step(sv_ctx *c, uint32_t nb, uint32_t nv, uint32_t nh, uint32_t s
     sv::ensure_support_vertices(c, nb);
     sv::fill_u32(c, counters + 2, 0u, 1);
     for (uint32_t s = 0; s < substeps; s++) {
-        { sv::StageTimer t(c, "phys/clear_counters"); sv::launch_clear_counters(c); }
-        { sv::StageTimer t(c, "phys/gravity"); sv::launch_gravity(c, nb, sub_dt, gravity); }
+        { sv::StageTimer t(c, "phys/gravity"); sv::launch_gravity(c, nb, sub_dt, gravity, true); }
         { sv::StageTimer t(c, "phys/aabb"); sv::launch_aabb(c, nb); }
         { sv::StageTimer t(c, "phys/broad_phase"); sv::launch_broadphase(c, nb); }
         // without warm start the impulse clear rides on the contact compaction

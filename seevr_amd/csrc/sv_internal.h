@@ -184,7 +184,8 @@ void prims_reserve(sv_ctx *c, uint32_t max_scan_n, uint32_t max_sort_n);
 // ---- sv_physics.hip ----
 void launch_hull_ranges(sv_ctx *c, uint32_t n);
 void ensure_support_vertices(sv_ctx *c, uint32_t n);
-void launch_gravity(sv_ctx *c, uint32_t n, float dt, const float g[3]);
+// clear_counters: also clear counters[0..1] (the full step's first dispatch)
+void launch_gravity(sv_ctx *c, uint32_t n, float dt, const float g[3], bool clear_counters = false);
 void launch_aabb(sv_ctx *c, uint32_t n);
 void launch_broadphase(sv_ctx *c, uint32_t n);
 // after_broadphase: the same substep's broadphase has set the pair count;

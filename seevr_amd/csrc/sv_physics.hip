@@ -32,9 +32,17 @@ __global__ void k_hull_ranges(const float *geom_data, const uint32_t *user, int 
 }
 
 // Step 0 (physics.comp:418-428): damped velocities plus gravity, dynamic only.
+// With `counters` given, thread 0 also clears the pair and contact counters
+// (the full step's clear, physics.zig's first dispatch, touches nothing
+// gravity reads or writes).
 __global__ void k_gravity(uint32_t n, float dt, float gx, float gy, float gz, float ld, float ad,
-                          const float *__restrict__ inv_mass, float *__restrict__ vel, float *__restrict__ omega) {
+                          const float *__restrict__ inv_mass, float *__restrict__ vel, float *__restrict__ omega,
+                          uint32_t *__restrict__ counters) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (counters && i == 0) {
+        counters[0] = 0;
+        counters[1] = 0;
+    }
     if (i >= n) return;
     gravity_body(i, dt, gx, gy, gz, ld, ad, inv_mass, vel, omega);
 }
@@ -463,11 +471,15 @@ void ensure_support_vertices(sv_ctx *c, uint32_t n) {
     }
 }
 
-void launch_gravity(sv_ctx *c, uint32_t n, float dt, const float g[3]) {
-    if (!n) return;
+void launch_gravity(sv_ctx *c, uint32_t n, float dt, const float g[3], bool clear_counters) {
+    if (!n) {
+        if (clear_counters) launch_clear_counters(c);
+        return;
+    }
     hipLaunchKernelGGL(k_gravity, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, c->stream, n, dt, g[0], g[1], g[2],
                        c->params.linear_damping, c->params.angular_damping, B<float>(c, SV_BODY_INV_MASS),
-                       B<float>(c, SV_BODY_VEL), B<float>(c, SV_BODY_OMEGA));
+                       B<float>(c, SV_BODY_VEL), B<float>(c, SV_BODY_OMEGA),
+                       clear_counters ? B<uint32_t>(c, SV_ATOMIC_COUNTERS) : (uint32_t *)nullptr);
 }
 
 void launch_aabb(sv_ctx *c, uint32_t n) {
