@@ -358,7 +358,7 @@ int sv_physics_step(sv_ctx *c, uint32_t nb, uint32_t nv, uint32_t nh, uint32_t s
         { sv::StageTimer t(c, "phys/broad_phase"); sv::launch_broadphase(c, nb); }
         // without warm start the impulse clear rides on the contact compaction
         const bool warm = c->warm_factor > 0.0f;
-        { sv::StageTimer t(c, "phys/narrow_phase"); sv::launch_narrowphase(c, true, !warm); }
+        { sv::StageTimer t(c, "phys/narrow_phase"); sv::launch_narrowphase(c, true, !warm, nb); }
         if (warm) { sv::StageTimer t(c, "phys/clear_impulse"); sv::launch_clear_lambda(c); }
         c->rot_fresh = true;
         { sv::StageTimer t(c, "phys/pgs_solve"); sv::launch_solve(c, nb, sub_dt); }

@@ -125,6 +125,7 @@ struct sv_ctx {
     // set by sv_physics_step between its AABB stage (which caches every
     // body's rotation) and its sweep: the sweep then skips recomputing them
     bool rot_fresh = false;
+    bool sweep_setup_done = false;  // the contact compaction has done k_sweep_setup's work
     // launch failure to report at the next synchronization
     hipError_t launch_err = hipSuccess;
     // broadphase: 0 = chosen by body count, 1 = all pairs (tiled N^2), 2 = LBVH traversal
@@ -189,8 +190,10 @@ void launch_gravity(sv_ctx *c, uint32_t n, float dt, const float g[3], bool clea
 void launch_aabb(sv_ctx *c, uint32_t n);
 void launch_broadphase(sv_ctx *c, uint32_t n);
 // after_broadphase: the same substep's broadphase has set the pair count;
-// clear_lambda: also clear the contact impulses (the next step's work).
-void launch_narrowphase(sv_ctx *c, bool after_broadphase = false, bool clear_lambda = false);
+// clear_lambda: also clear the contact impulses (the next step's work);
+// sweep_bodies > 0: also do the sweep's setup for that many bodies.
+void launch_narrowphase(sv_ctx *c, bool after_broadphase = false, bool clear_lambda = false,
+                        uint32_t sweep_bodies = 0);
 void launch_clear_lambda(sv_ctx *c);
 void launch_integrate(sv_ctx *c, uint32_t n, float dt);
 void launch_update_verts(sv_ctx *c, uint32_t nv, uint32_t nb);

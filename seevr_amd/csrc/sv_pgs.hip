@@ -1238,9 +1238,12 @@ uint32_t seq_streams(uint32_t wgs) {
 void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
     uint32_t *counters = B<uint32_t>(c, SV_ATOMIC_COUNTERS);
     uint32_t mc = c->caps.max_contacts;
+    const bool setup_done = c->sweep_setup_done;  // by the full step's contact compaction
+    c->sweep_setup_done = false;
     if (mc == 0 || n_bodies == 0) return;
-    hipLaunchKernelGGL(k_sweep_setup, dim3(div_up(n_bodies, 256)), dim3(256), 0, c->stream, counters, c->s.misc, mc,
-                       c->s.deg, n_bodies, c->s.solve_ctl);
+    if (!setup_done)
+        hipLaunchKernelGGL(k_sweep_setup, dim3(div_up(n_bodies, 256)), dim3(256), 0, c->stream, counters, c->s.misc,
+                           mc, c->s.deg, n_bodies, c->s.solve_ctl);
     // (the step's AABB stage has just cached every body's rotation)
     if (!c->rot_fresh) launch_rot(c, n_bodies);
     uint32_t iters = c->params.pgs_iterations;

@@ -323,11 +323,23 @@ __global__ void __launch_bounds__(BLOCK) k_compact_contacts(uint32_t *__restrict
                                                             const float *__restrict__ res, uint32_t max_contacts,
                                                             float *__restrict__ cpos, float *__restrict__ cnrm,
                                                             float *__restrict__ cpen, uint32_t *__restrict__ ga,
-                                                            uint32_t *__restrict__ gb, float *__restrict__ lambda) {
+                                                            uint32_t *__restrict__ gb, float *__restrict__ lambda,
+                                                            uint32_t *__restrict__ deg, uint32_t n_bodies,
+                                                            uint32_t *__restrict__ ctl) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const uint32_t t = counters[1];
-        misc[MISC_NC] = t < max_contacts ? t : max_contacts;
+        const uint32_t nc = t < max_contacts ? t : max_contacts;
+        misc[MISC_NC] = nc;
         if (t > max_contacts) counters[2] |= SV_STATUS_CONTACT_OVERFLOW;
+        if (deg) {
+            misc[MISC_NINC] = 2u * nc;
+            misc[MISC_NACT] = 0u;
+        }
+    }
+    if (deg) {  // the sweep's setup (k_sweep_setup): degrees and control words
+        const uint32_t stride = gridDim.x * blockDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
+        for (uint32_t b = t0; b < n_bodies; b += stride) deg[b] = 0u;
+        for (uint32_t k = t0; k < SOLVE_CTL_WORDS; k += stride) ctl[k] = 0u;
     }
     if (lambda)
         for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < max_contacts; k += gridDim.x * blockDim.x)
@@ -517,7 +529,7 @@ void launch_broadphase(sv_ctx *c, uint32_t n) {
                        (uint32_t *)nullptr, c->s.row_off, c->s.bp_stage, c->s.bp_ovf, pairs, c->caps.max_pairs);
 }
 
-void launch_narrowphase(sv_ctx *c, bool after_broadphase, bool clear_lambda) {
+void launch_narrowphase(sv_ctx *c, bool after_broadphase, bool clear_lambda, uint32_t sweep_bodies) {
     uint32_t *counters = B<uint32_t>(c, SV_ATOMIC_COUNTERS);
     // pairs to process: min(counters[0], max_pairs) (physics.comp:477); the
     // broadphase of the same substep has computed it already
@@ -537,7 +549,9 @@ void launch_narrowphase(sv_ctx *c, bool after_broadphase, bool clear_lambda) {
                        c->caps.max_contacts, B<float>(c, SV_CONTACT_POS), B<float>(c, SV_CONTACT_NORMAL),
                        B<float>(c, SV_CONTACT_PENETRATION), B<uint32_t>(c, SV_CONTACT_GEOM_A),
                        B<uint32_t>(c, SV_CONTACT_GEOM_B),
-                       clear_lambda ? B<float>(c, SV_CONTACT_LAMBDA_N) : (float *)nullptr);
+                       clear_lambda ? B<float>(c, SV_CONTACT_LAMBDA_N) : (float *)nullptr,
+                       sweep_bodies ? c->s.deg : (uint32_t *)nullptr, sweep_bodies, c->s.solve_ctl);
+    c->sweep_setup_done = sweep_bodies > 0;
 }
 
 void launch_clear_lambda(sv_ctx *c) {
