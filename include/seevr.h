@@ -211,6 +211,12 @@ int sv_debug_atan2(sv_ctx *ctx, const float *y, const float *x, float *out, uint
  * read + write bandwidth in GB/s. */
 int sv_debug_copy_bandwidth(sv_ctx *ctx, size_t nbytes, uint32_t iters, double *gbps);
 
+/* Host only (no device work): words the all-pairs broadphase needs at n
+ * bodies and what a context with capacity max_bodies allocates for them:
+ * out[0] (row, segment) cells needed, out[1] allocated, out[2] overflow-list
+ * words needed, out[3] allocated.  A sizing check for tests. */
+int sv_debug_broadphase_scratch(uint32_t max_bodies, uint32_t n, uint64_t out[4]);
+
 /* XCC (XCD) hardware register read by the first wave of each of `nblocks`
  * one-wave blocks (a placement probe: which XCD runs which workgroup). */
 int sv_debug_xcc_ids(sv_ctx *ctx, uint32_t nblocks, uint32_t *out);

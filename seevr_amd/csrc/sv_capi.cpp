@@ -250,11 +250,12 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
     if (e == hipSuccess) e = dalloc(s.hrange, 2 * N);
     if (e == hipSuccess) e = dalloc(s.rot, 9 * N);
     if (e == hipSuccess) e = dalloc(s.bpbox, 2 * N);
-    const size_t NSEG = sv::bp_nseg((uint32_t)N);
-    if (e == hipSuccess) e = dalloc(s.row_count, N * NSEG + 1);
-    if (e == hipSuccess) e = dalloc(s.row_off, N * NSEG + 1);
-    if (e == hipSuccess) e = dalloc(s.bp_stage, N * NSEG * sv::BP_STAGE);
-    if (e == hipSuccess) e = dalloc(s.bp_ovf, (N / sv::BP_ROWS + 1) * NSEG);
+    // broadphase cells for every n <= N (bp_nseg is not monotonic in n)
+    const size_t CELLS = sv::bp_cells_cap((uint32_t)N);
+    if (e == hipSuccess) e = dalloc(s.row_count, CELLS);
+    if (e == hipSuccess) e = dalloc(s.row_off, CELLS);
+    if (e == hipSuccess) e = dalloc(s.bp_stage, CELLS * sv::BP_STAGE);
+    if (e == hipSuccess) e = dalloc(s.bp_ovf, sv::bp_ovf_words_cap((uint32_t)N));
     if (e == hipSuccess) e = dalloc(s.np_flag, P + 1);
     if (e == hipSuccess) e = dalloc(s.np_idx, P + 1);
     if (e == hipSuccess) e = dalloc(s.np_res, 8 * P + 8);
@@ -276,7 +277,7 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
         return fail(SV_E_NOMEM, "sv_create: device allocation failed: %s", hipGetErrorString(e));
     }
     size_t NP = std::max<size_t>(k.max_triangles, N);
-    sv::prims_reserve(c, (uint32_t)std::max<size_t>(N * NSEG + 1, P), (uint32_t)std::max<size_t>(2 * C, NP));
+    sv::prims_reserve(c, (uint32_t)std::max<size_t>(CELLS, P), (uint32_t)std::max<size_t>(2 * C, NP));
     if (!s.scan_part || !s.sort_hist) {
         free_ctx(c);
         return fail(SV_E_NOMEM, "sv_create: scratch allocation failed");
@@ -558,6 +559,19 @@ int sv_stage_times(sv_ctx *c, int reset, const char **names, double *avg_us, uin
         std::fill(c->stage_calls.begin(), c->stage_calls.end(), 0);
     }
     return n;
+}
+
+// Host-only: the all-pairs broadphase's need at n bodies against what a
+// context of capacity max_bodies allocates.
+int sv_debug_broadphase_scratch(uint32_t max_bodies, uint32_t n, uint64_t out[4]) {
+    if (!out || n > max_bodies) return fail(SV_E_INVALID, "sv_debug_broadphase_scratch: bad argument");
+    const bool all_pairs = n <= sv::BP_ALL_PAIRS_MAX_BODIES;
+    const uint64_t nseg = sv::bp_nseg(n);
+    out[0] = all_pairs ? (uint64_t)n * nseg + 1 : 0;                             // cells (+ scan total)
+    out[1] = sv::bp_cells_cap(max_bodies);
+    out[2] = all_pairs ? 1 + (uint64_t)sv::div_up(n, sv::BP_ROWS) * nseg : 0;  // overflow list
+    out[3] = sv::bp_ovf_words_cap(max_bodies);
+    return SV_OK;
 }
 
 // Evaluate the library's atan2f on the device (host arrays in and out).

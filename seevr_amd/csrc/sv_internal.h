@@ -15,10 +15,12 @@ struct Scratch {
     uint32_t *hrange = nullptr;   // uint[2N] hull (start, count) read by the kernels
     float *rot = nullptr;         // float[9N] per-body rotation columns of this substep
     float4 *bpbox = nullptr;      // float4[2N] broadphase boxes: (min, static), (max, 0)
+    // broadphase cells: nseg = bp_nseg_max(N), the most segments any n <= N uses
     uint32_t *row_count = nullptr;  // uint[N * nseg + 1] broadphase hits per (row, segment)
     uint32_t *row_off = nullptr;    // uint[N * nseg + 1] their exclusive scan
     uint32_t *bp_stage = nullptr;   // uint[N * nseg * BP_STAGE] first partners per (row, segment)
-    uint32_t *bp_ovf = nullptr;     // uint[ceil(N / 32) * nseg] tiles with a (row, segment) above BP_STAGE
+    uint32_t *bp_ovf = nullptr;     // uint[1 + ceil(N / 32) * nseg]: [0] = number of listed tiles, then the
+                                    // (row tile, segment) tiles holding a cell with more than BP_STAGE hits
     uint32_t *np_flag = nullptr;    // uint[P] narrowphase hit flags
     float *sup_xyz = nullptr;       // float[3L] candidate vertex of each support-lookup entry (first use)
     uint32_t *np_idx = nullptr;     // uint[P] contact slot of each hit
@@ -96,6 +98,24 @@ inline uint32_t bp_seg_len(uint32_t n) {
     return s < 2048u ? 2048u : s;
 }
 inline uint32_t bp_nseg(uint32_t n) { return n ? (n + bp_seg_len(n) - 1) / bp_seg_len(n) : 1u; }
+// bp_nseg is not monotonic in n (at n = 131072 it is 64, at n = 131073 it
+// is 57), so scratch sized for a capacity uses the largest value over every
+// n <= cap: up to 2^17 bodies the segment length is 2048 and bp_nseg(n) =
+// ceil(n / 2048); above, the segment length is at least n / 64.
+inline uint32_t bp_nseg_max(uint32_t cap) {
+    const uint32_t s = cap ? (cap + 2047u) / 2048u : 1u;
+    return s < 64u ? s : 64u;
+}
+// (row, segment) cells and overflow-list words (one count word followed by
+// at most one entry per (row tile, segment)) for any n <= cap bodies.
+// The all-pairs kernels index cells with 32-bit words: they take at most
+// 2^25 bodies (larger scenes use the LBVH broadphase, the same pair list).
+constexpr uint32_t BP_ALL_PAIRS_MAX_BODIES = 1u << 25;
+inline uint32_t bp_rows_cap(uint32_t cap) { return cap < BP_ALL_PAIRS_MAX_BODIES ? cap : BP_ALL_PAIRS_MAX_BODIES; }
+inline size_t bp_cells_cap(uint32_t cap) { return (size_t)bp_rows_cap(cap) * bp_nseg_max(cap) + 1; }
+inline size_t bp_ovf_words_cap(uint32_t cap) {
+    return ((size_t)bp_rows_cap(cap) + BP_ROWS - 1) / BP_ROWS * bp_nseg_max(cap) + 1;
+}
 
 }  // namespace sv
 

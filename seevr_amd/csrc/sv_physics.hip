@@ -509,9 +509,14 @@ void launch_broadphase(sv_ctx *c, uint32_t n) {
         hipLaunchKernelGGL(k_post_pairs, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, c->caps.max_pairs);
         return;
     }
-    const bool lbvh = c->broadphase_mode == 2 || (c->broadphase_mode == 0 && n >= BP_LBVH_MIN_BODIES);
+    const bool lbvh = c->broadphase_mode == 2 || (c->broadphase_mode == 0 && n >= BP_LBVH_MIN_BODIES) ||
+                      n > BP_ALL_PAIRS_MAX_BODIES;
     // same pair list either way; without LBVH scratch the all-pairs kernel runs
     if (lbvh && launch_broadphase_lbvh(c, n)) return;
+    if (n > BP_ALL_PAIRS_MAX_BODIES) {  // no LBVH scratch and too many rows for the cells
+        c->launch_err = hipErrorOutOfMemory;
+        return;
+    }
     const uint32_t seg_len = bp_seg_len(n), nseg = bp_nseg(n);
     const uint32_t nrb = div_up(n, BP_ROWS);
     uint32_t *pairs = B<uint32_t>(c, SV_COLLISION_PAIRS);
