@@ -2,8 +2,7 @@
 // bounds, 30-bit Morton codes, stable LSD radix sort (8-bit digits, wave
 // ballot ranking; any digit width gives the same stable order as the
 // reference's eight 4-bit passes), Karras 2012 hierarchy, leaf boxes and the
-// bottom-up refit (one depth level per launch; the arrival-flag climb of the
-// reference remains as the fallback).
+// bottom-up refit (the reference's arrival climb, one launch).
 #include "sv_internal.h"
 #include "sv_math.h"
 
@@ -205,114 +204,73 @@ __global__ void k_clear_flags(uint32_t n, uint32_t *flags) {
     if (i + 1 < n) flags[i] = 0;
 }
 
-// Arrival at a parent during the refit: returns the flag's previous value
-// (0 for the first arrival).  The release fence publishes this thread's
-// child box before the flag update and the acquire fence makes the other
-// child's box visible to the second arrival.  The release is written out
-// (rather than an acq_rel RMW) so that it carries the wait for the box
-// stores to complete.
-SVD uint32_t arrive_second(uint32_t *flag) {
-    release_agent();
-    uint32_t old = __hip_atomic_fetch_or(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return old;
-}
-
-// bvh.comp:379-397, with the arrival above: the first arrival's writes of
-// its child box are published before its flag, and the second arrival reads
-// both children only after seeing it.
-__global__ void k_propagate(uint32_t n, const uint32_t *__restrict__ parent_of, const uint32_t *__restrict__ left,
-                            const uint32_t *__restrict__ right, uint32_t *flags, float *nmin, float *nmax) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t p = parent_of[n - 1 + i];
-    // Node prefix lengths strictly decrease toward the root and lie in
-    // [0, 64], so a climb takes at most 65 steps; the bound only guards a
-    // corrupt parent array.
-    for (int depth = 0; p != 0xFFFFFFFFu && depth < 80; depth++) {
-        if (arrive_second(flags + p) == 0u) return;
-        uint32_t li = left[p], ri = right[p];
-        st3(nmin, p, vmin3(ld3(nmin, li), ld3(nmin, ri)));
-        st3(nmax, p, vmax3(ld3(nmax, li), ld3(nmax, ri)));
-        p = parent_of[p];
-    }
-}
-
 // ---------------------------------------------------------------------------
-// Level-ordered refit: the same node boxes as the bottom-up climb above
-// (each internal node = the union of its left and right child, in that
-// order), computed one depth level at a time from the deepest, so no
-// thread ever waits on another and no device-scope fence is needed inside
-// a level: kernel boundaries order the levels.  A node's depth is its count
-// of ancestors (parent pointers only); nodes are bucketed by depth with a
-// per-block LDS histogram.
-constexpr uint32_t MAX_DEPTH = 128;  // Karras trees over 30-bit codes + index bits are < 64 deep
+// Bottom-up refit, bvh.comp:379-397: a thread per leaf climbs; at each parent
+// the first arrival stops and the second writes the union of the two child
+// boxes (left child first, as the reference's min/max read them) and goes on.
+// One launch, no host read.  The box hand-off between the two arrivals uses
+// device-coherent accesses (agent-scope relaxed atomics: SC1 loads and stores
+// served at the point of coherence, so another XCD's L2 holds no stale copy):
+// a climber waits until its box stores have completed before it bumps the
+// parent's arrival counter, and the second arrival reads the sibling's box
+// only after its own bump returned the first's.  No cache write-back or
+// invalidation is needed, which made the fenced climb slow (3.8 ms at 1M
+// leaves).  The continuing thread keeps its own child's box in registers.
+// A node's key prefix strictly lengthens toward the leaves (at most 64
+// bits), so a climb takes at most 65 steps; the bound only guards a corrupt
+// parent array.
+SVD float ldc(const float *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+SVD void stc(float *p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+SVD f3 ld3c(const float *p, uint32_t i) { return mk(ldc(p + 3 * (size_t)i), ldc(p + 3 * (size_t)i + 1), ldc(p + 3 * (size_t)i + 2)); }
+SVD void st3c(float *p, uint32_t i, f3 v) {
+    stc(p + 3 * (size_t)i, v.x);
+    stc(p + 3 * (size_t)i + 1, v.y);
+    stc(p + 3 * (size_t)i + 2, v.z);
+}
+SVD float4 ld4c(const float4 *p, uint32_t i) {
+    const float *q = reinterpret_cast<const float *>(p + i);
+    return make_float4(ldc(q), ldc(q + 1), ldc(q + 2), ldc(q + 3));
+}
+SVD void st4c(float4 *p, uint32_t i, float4 v) {
+    float *q = reinterpret_cast<float *>(p + i);
+    stc(q, v.x); stc(q + 1, v.y); stc(q + 2, v.z); stc(q + 3, v.w);
+}
+// Arrival at parent p after this thread's stores completed: true for the
+// second of the two arrivals.  The caller's topology reads are inputs of the
+// first empty asm, so they are issued before the bump and overlap the store
+// completion; the second keeps the sibling loads below the bump's return.
+SVD bool arrive_second(uint32_t *flag, uint32_t li, uint32_t ri, uint32_t pp) {
+    asm volatile("" ::"v"(li), "v"(ri), "v"(pp) : "memory");
+    __builtin_amdgcn_s_waitcnt(0);
+    const uint32_t old = __hip_atomic_fetch_add(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    asm volatile("" ::: "memory");
+    return old != 0u;
+}
 
-__global__ void __launch_bounds__(BLOCK) k_node_depth(uint32_t n, const uint32_t *__restrict__ parent,
-                                                      uint32_t *__restrict__ depth, uint32_t *hist, uint32_t *maxd) {
-    __shared__ uint32_t s_h[MAX_DEPTH];
-    for (uint32_t t = threadIdx.x; t < MAX_DEPTH; t += BLOCK) s_h[t] = 0u;
-    __syncthreads();
+// float3 node arrays of the reference layout (SV_BVH_AABB_MIN/MAX)
+__global__ void __launch_bounds__(BLOCK) k_propagate(uint32_t n, const uint32_t *__restrict__ parent_of,
+                                                     const uint32_t *__restrict__ left,
+                                                     const uint32_t *__restrict__ right, uint32_t *flags,
+                                                     float *nmin, float *nmax) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i + 1 < n) {
-        uint32_t d = 0, p = i;
-        while (d < MAX_DEPTH - 1) {
-            const uint32_t q = parent[p];
-            if (q == 0xFFFFFFFFu) break;
-            p = q;
-            d++;
-        }
-        depth[i] = d;
-        atomicAdd(&s_h[d], 1u);
-    }
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < MAX_DEPTH; t += BLOCK)
-        if (s_h[t]) {
-            atomicAdd(hist + t, s_h[t]);
-            atomicMax(maxd, t);
-        }
-}
-
-__global__ void k_level_offsets(const uint32_t *__restrict__ hist, uint32_t *__restrict__ off,
-                                uint32_t *__restrict__ fill) {
-    uint32_t run = 0;
-    for (uint32_t d = 0; d < MAX_DEPTH; d++) {
-        off[d] = run;
-        fill[d] = 0u;
-        run += hist[d];
-    }
-    off[MAX_DEPTH] = run;
-}
-
-__global__ void __launch_bounds__(BLOCK) k_node_scatter(uint32_t n, const uint32_t *__restrict__ depth,
-                                                        const uint32_t *__restrict__ off, uint32_t *fill,
-                                                        uint32_t *__restrict__ order) {
-    __shared__ uint32_t s_cnt[MAX_DEPTH], s_base[MAX_DEPTH];
-    for (uint32_t t = threadIdx.x; t < MAX_DEPTH; t += BLOCK) s_cnt[t] = 0u;
-    __syncthreads();
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    uint32_t d = 0, local = 0;
-    const bool valid = i + 1 < n;
-    if (valid) {
-        d = depth[i];
-        local = atomicAdd(&s_cnt[d], 1u);
-    }
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < MAX_DEPTH; t += BLOCK)
-        if (s_cnt[t]) s_base[t] = atomicAdd(fill + t, s_cnt[t]);
-    __syncthreads();
-    if (valid) order[off[d] + s_base[d] + local] = i;
-}
-
-__global__ void __launch_bounds__(BLOCK) k_refit_level(const uint32_t *__restrict__ off, uint32_t level,
-                                                       const uint32_t *__restrict__ order,
-                                                       const uint32_t *__restrict__ left,
-                                                       const uint32_t *__restrict__ right, float *nmin, float *nmax) {
-    const uint32_t b1 = off[level + 1];
-    for (uint32_t k = off[level] + blockIdx.x * BLOCK + threadIdx.x; k < b1; k += gridDim.x * BLOCK) {
-        const uint32_t p = order[k], li = left[p], ri = right[p];
-        st3(nmin, p, vmin3(ld3(nmin, li), ld3(nmin, ri)));
-        st3(nmax, p, vmax3(ld3(nmax, li), ld3(nmax, ri)));
+    if (i >= n) return;
+    uint32_t node = n - 1 + i;
+    f3 mn = ld3(nmin, node), mx = ld3(nmax, node);  // leaf box of the previous launch
+    uint32_t p = parent_of[node];
+    for (int depth = 0; p != 0xFFFFFFFFu && depth < 80; depth++) {
+        // topology is immutable: read it before the arrival, so the
+        // sibling's box is the only round trip after it
+        const uint32_t li = left[p], ri = right[p], pp = parent_of[p];
+        if (!arrive_second(flags + p, li, ri, pp)) return;
+        const bool is_left = li == node;
+        const uint32_t sib = is_left ? ri : li;
+        const f3 smn = ld3c(nmin, sib), smx = ld3c(nmax, sib);
+        mn = is_left ? vmin3(mn, smn) : vmin3(smn, mn);
+        mx = is_left ? vmax3(mx, smx) : vmax3(smx, mx);
+        st3c(nmin, p, mn);
+        st3c(nmax, p, mx);
+        node = p;
+        p = pp;
     }
 }
 
@@ -359,22 +317,32 @@ __global__ void k_bb_leaves(uint32_t n, const uint32_t *__restrict__ ids, const 
     nhi[leaf] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(lo.w != 0.0f ? 1u : 0u));
 }
 
-// One depth level of the broadphase tree's refit (see k_refit_level): the
-// union of the children, the smaller index minimum and the AND of the
-// all-static flags.
-__global__ void __launch_bounds__(BLOCK) k_bb_refit_level(const uint32_t *__restrict__ off, uint32_t level,
-                                                          const uint32_t *__restrict__ order,
-                                                          const uint32_t *__restrict__ left,
-                                                          const uint32_t *__restrict__ right, float4 *nlo,
-                                                          float4 *nhi) {
-    const uint32_t b1 = off[level + 1];
-    for (uint32_t k = off[level] + blockIdx.x * BLOCK + threadIdx.x; k < b1; k += gridDim.x * BLOCK) {
-        const uint32_t p = order[k], li = left[p], ri = right[p];
-        const float4 al = nlo[li], ah = nhi[li], bl = nlo[ri], bh = nhi[ri];
+// The broadphase tree's refit (see k_propagate): the union of the children,
+// the smaller index minimum and the AND of the all-static flags.
+__global__ void __launch_bounds__(BLOCK) k_bb_propagate(uint32_t n, const uint32_t *__restrict__ parent_of,
+                                                        const uint32_t *__restrict__ left,
+                                                        const uint32_t *__restrict__ right, uint32_t *flags,
+                                                        float4 *nlo, float4 *nhi) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    uint32_t node = n - 1 + i;
+    float4 lo = nlo[node], hi = nhi[node];
+    uint32_t p = parent_of[node];
+    for (int depth = 0; p != 0xFFFFFFFFu && depth < 80; depth++) {
+        const uint32_t li = left[p], ri = right[p], pp = parent_of[p];
+        if (!arrive_second(flags + p, li, ri, pp)) return;
+        const bool is_left = li == node;
+        const uint32_t sib = is_left ? ri : li;
+        const float4 sl = ld4c(nlo, sib), sh = ld4c(nhi, sib);
+        const float4 al = is_left ? lo : sl, ah = is_left ? hi : sh, bl = is_left ? sl : lo, bh = is_left ? sh : hi;
         const uint32_t mi = min(__float_as_uint(al.w), __float_as_uint(bl.w));
         const uint32_t st = __float_as_uint(ah.w) & __float_as_uint(bh.w);
-        nlo[p] = make_float4(gmin(al.x, bl.x), gmin(al.y, bl.y), gmin(al.z, bl.z), __uint_as_float(mi));
-        nhi[p] = make_float4(gmax(ah.x, bh.x), gmax(ah.y, bh.y), gmax(ah.z, bh.z), __uint_as_float(st));
+        lo = make_float4(gmin(al.x, bl.x), gmin(al.y, bl.y), gmin(al.z, bl.z), __uint_as_float(mi));
+        hi = make_float4(gmax(ah.x, bh.x), gmax(ah.y, bh.y), gmax(ah.z, bh.z), __uint_as_float(st));
+        st4c(nlo, p, lo);
+        st4c(nhi, p, hi);
+        node = p;
+        p = pp;
     }
 }
 
@@ -622,51 +590,6 @@ hipError_t lazy_alloc(sv_ctx *c, T *&p, size_t count) {
     return e;
 }
 
-// Level scratch (depth, order, histogram, offsets, fill, max depth) for
-// trees of up to max(max_triangles, max_bodies) leaves.
-bool level_reserve(sv_ctx *c) {
-    Scratch &s = c->s;
-    if (s.lv_depth) return true;
-    size_t n = c->caps.max_triangles > c->caps.max_bodies ? c->caps.max_triangles : c->caps.max_bodies;
-    hipError_t e = lazy_alloc(c, s.lv_depth, n + 1);
-    if (e == hipSuccess) e = lazy_alloc(c, s.lv_order, n + 1);
-    if (e == hipSuccess) e = lazy_alloc(c, s.lv_small, 4 * (size_t)(MAX_DEPTH + 1));
-    if (e != hipSuccess) s.lv_depth = nullptr;
-    return e == hipSuccess;
-}
-
-// Refit of an n-leaf Karras tree level by level (k_refit_level); float3
-// node arrays (nmin, nmax) or the broadphase's float4 nodes (nlo, nhi).
-bool refit_by_levels(sv_ctx *c, uint32_t n, const uint32_t *parent, const uint32_t *left, const uint32_t *right,
-                     float *nmin, float *nmax, float4 *nlo, float4 *nhi) {
-    if (n < 2) return true;
-    if (!level_reserve(c)) return false;
-    Scratch &s = c->s;
-    hipStream_t st = c->stream;
-    uint32_t *hist = s.lv_small, *off = hist + (MAX_DEPTH + 1), *fill = off + (MAX_DEPTH + 1),
-             *maxd = fill + (MAX_DEPTH + 1);
-    const uint32_t g = div_up(n, BLOCK);
-    fill_u32(c, hist, 0u, MAX_DEPTH + 1);
-    fill_u32(c, maxd, 0u, 1);
-    hipLaunchKernelGGL(k_node_depth, dim3(g), dim3(BLOCK), 0, st, n, parent, s.lv_depth, hist, maxd);
-    hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(1), 0, st, hist, off, fill);
-    hipLaunchKernelGGL(k_node_scatter, dim3(g), dim3(BLOCK), 0, st, n, s.lv_depth, off, fill, s.lv_order);
-    uint32_t deepest = 0;
-    if (hipMemcpyAsync(&deepest, maxd, sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return false;
-    uint32_t lg = g > 2048 ? 2048 : g;
-    for (int64_t L = (int64_t)deepest; L >= 0; L--) {
-        if (nlo)
-            hipLaunchKernelGGL(k_bb_refit_level, dim3(lg), dim3(BLOCK), 0, st, off, (uint32_t)L, s.lv_order, left,
-                               right, nlo, nhi);
-        else
-            hipLaunchKernelGGL(k_refit_level, dim3(lg), dim3(BLOCK), 0, st, off, (uint32_t)L, s.lv_order, left, right,
-                               nmin, nmax);
-    }
-    return true;
-}
-
 bool bb_reserve(sv_ctx *c) {
     if (c->s.bb_ready) return true;
     Scratch &s = c->s;
@@ -722,8 +645,9 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
     hipLaunchKernelGGL(k_hierarchy, dim3(g), dim3(BLOCK), 0, st, n, s.bb_mk[w], s.bb_left, s.bb_right, s.bb_count,
                        s.bb_parent);
     hipLaunchKernelGGL(k_bb_leaves, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_lo, s.bb_hi);
-    if (!refit_by_levels(c, n, s.bb_parent, s.bb_left, s.bb_right, nullptr, nullptr, s.bb_lo, s.bb_hi))
-        return false;
+    hipLaunchKernelGGL(k_clear_flags, dim3(g), dim3(BLOCK), 0, st, n, s.bb_flags);
+    hipLaunchKernelGGL(k_bb_propagate, dim3(g), dim3(BLOCK), 0, st, n, s.bb_parent, s.bb_left, s.bb_right,
+                       s.bb_flags, s.bb_lo, s.bb_hi);
     // count, scan, fill, sort
     const int bits = 32 - __builtin_clz(n - 1);  // body indices < 2^bits
     const bool combined = 2 * bits <= 32;
@@ -812,15 +736,10 @@ void bvh_stage(sv_ctx *c, uint32_t step, uint32_t n, bool bodies) {
     case 6:
         hipLaunchKernelGGL(k_clear_flags, dim3(g), dim3(BLOCK), 0, s, n, B<uint32_t>(c, SV_BVH_FLAGS));
         break;
-    case 7:
-        // level-ordered refit; the arrival-flag climb (bvh.comp:379-397) only
-        // when the level scratch cannot be allocated (same boxes either way)
-        if (!refit_by_levels(c, n, B<uint32_t>(c, SV_BVH_PARENT), B<uint32_t>(c, SV_BVH_LEFT),
-                             B<uint32_t>(c, SV_BVH_RIGHT), B<float>(c, SV_BVH_AABB_MIN), B<float>(c, SV_BVH_AABB_MAX),
-                             nullptr, nullptr))
-            hipLaunchKernelGGL(k_propagate, dim3(g), dim3(BLOCK), 0, s, n, B<uint32_t>(c, SV_BVH_PARENT),
-                               B<uint32_t>(c, SV_BVH_LEFT), B<uint32_t>(c, SV_BVH_RIGHT), B<uint32_t>(c, SV_BVH_FLAGS),
-                               B<float>(c, SV_BVH_AABB_MIN), B<float>(c, SV_BVH_AABB_MAX));
+    case 7:  // needs step 6's cleared arrival counters
+        hipLaunchKernelGGL(k_propagate, dim3(g), dim3(BLOCK), 0, s, n, B<uint32_t>(c, SV_BVH_PARENT),
+                           B<uint32_t>(c, SV_BVH_LEFT), B<uint32_t>(c, SV_BVH_RIGHT), B<uint32_t>(c, SV_BVH_FLAGS),
+                           B<float>(c, SV_BVH_AABB_MIN), B<float>(c, SV_BVH_AABB_MAX));
         break;
     default:
         break;  // 8..11: folded into the stages above

@@ -56,9 +56,6 @@ struct Scratch {
     uint32_t *bb_stage = nullptr;   // uint[BB_STAGE * N] rows staged by each counting lane
     uint32_t *bb_scnt = nullptr;    // uint[N] rows found by each counting lane
     bool bb_ready = false;
-    // level-ordered BVH refit (allocated on first use)
-    uint32_t *lv_depth = nullptr, *lv_order = nullptr;  // [max(T, N)] node depth, nodes by depth
-    uint32_t *lv_small = nullptr;                       // histogram, offsets, fill, deepest level
     // graph-coloured sweep and residual (allocated on first use)
     unsigned long long *col_claim = nullptr;  // [N] colour-round claims per body
     uint32_t *col_color = nullptr;            // [C] colour of each contact

@@ -71,7 +71,7 @@ def solve(w):
 def scalar_gs(vels, contacts, iters, inv_mass=None):
     """Exact rational restatement of the reduced equations (module docstring)."""
     v = [Fraction(0)] + [Fraction(x) for x in vels]
-    im = [Fraction(0)] + [Fraction(1) if inv_mass is None else Fraction(x) for x in inv_mass]
+    im = [Fraction(0)] + [Fraction(x) for x in (inv_mass if inv_mass is not None else [1] * len(vels))]
     lam = [Fraction(0)] * len(contacts)
     for _ in range(iters):
         for k, (a, b) in enumerate(contacts):
