@@ -161,6 +161,17 @@ int sv_set_params(sv_ctx *ctx, const sv_params *params);
 
 /* Device address and size of a buffer (descriptor binding, vulkan.zig:465). */
 int sv_buffer_ptr(sv_ctx *ctx, sv_buf id, void **dev, size_t *bytes);
+/* Bind caller-owned device memory as buffer `id`, the way Physics.init and
+ * BVH.init bind the buffers that Data owns through descriptor sets
+ * (src/physics.zig:37-68, src/bvh.zig:32-51, src/vulkan.zig:465-506).  The
+ * memory must hold at least the bytes the capacities need (sv_buffer_ptr
+ * reports them before binding), be 16-byte aligned, stay valid until the
+ * context is destroyed or the id is bound again, and keep the std430 layout.
+ * The library frees its own allocation of that id and never frees bound
+ * memory.  Every later stage reads and writes the bound memory in place, so
+ * a renderer can read SV_VERTICES without a copy.  Waits for the context's
+ * queued work first. */
+int sv_bind_buffer(sv_ctx *ctx, sv_buf id, void *dev, size_t nbytes);
 /* Staging upload / readback (vulkan.zig:304-328). Offsets in bytes. */
 int sv_upload(sv_ctx *ctx, sv_buf id, const void *src, size_t offset, size_t nbytes);
 int sv_download(sv_ctx *ctx, sv_buf id, void *dst, size_t offset, size_t nbytes);

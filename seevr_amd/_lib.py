@@ -68,7 +68,7 @@ class SceneCounts(C.Structure):
 
 EXPORTS = [
     "sv_default_caps", "sv_default_params", "sv_create", "sv_destroy", "sv_get_params", "sv_set_params",
-    "sv_buffer_ptr", "sv_upload", "sv_download", "sv_physics_step", "sv_read_counters",
+    "sv_buffer_ptr", "sv_bind_buffer", "sv_upload", "sv_download", "sv_physics_step", "sv_read_counters",
     "sv_physics_dispatch", "sv_bvh_build", "sv_bvh_build_bodies", "sv_bvh_dispatch", "sv_debug_atan2",
     "sv_debug_copy_bandwidth", "sv_debug_broadphase_scratch", "sv_debug_xcc_ids", "sv_debug_set_sweep_variant", "sv_last_error", "sv_device_info", "sv_stage_times", "sv_enable_stage_timing", "sv_stage_timing_only", "sv_set_solver", "sv_solver_residual", "sv_debug_color_counts", "sv_set_broadphase", "sv_set_warm_start", "sv_scene_create",
     "sv_scene_destroy", "sv_scene_add_material", "sv_scene_add_box", "sv_scene_add_sphere",
@@ -107,6 +107,7 @@ def load(path: Path | str | None = None) -> C.CDLL:
         "sv_get_params": (C.c_int, [vp, C.POINTER(Params)]),
         "sv_set_params": (C.c_int, [vp, C.POINTER(Params)]),
         "sv_buffer_ptr": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(sz)]),
+        "sv_bind_buffer": (C.c_int, [vp, C.c_int, vp, sz]),
         "sv_upload": (C.c_int, [vp, C.c_int, vp, sz, sz]),
         "sv_download": (C.c_int, [vp, C.c_int, vp, sz, sz]),
         "sv_physics_step": (C.c_int, [vp, u32, u32, u32, u32, f32, fp]),

@@ -95,6 +95,10 @@ class Context:
         check(self.lib.sv_buffer_ptr(self.h, BUF[name], C.byref(p), C.byref(n)))
         return p.value or 0
 
+    def bind_buffer(self, name: str, dev_ptr: int, nbytes: int) -> None:
+        """Use caller-owned device memory as buffer `name` (sv_bind_buffer)."""
+        check(self.lib.sv_bind_buffer(self.h, BUF[name], C.c_void_p(dev_ptr), nbytes))
+
     def upload(self, name: str, arr, offset_elems: int = 0) -> None:
         a = np.ascontiguousarray(arr, dtype=_np_dtype(name))
         check(self.lib.sv_upload(self.h, BUF[name], a.ctypes.data_as(C.c_void_p), offset_elems * 4, a.nbytes))

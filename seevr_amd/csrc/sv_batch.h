@@ -137,6 +137,7 @@ struct sv_batch {
     uint32_t g_nv = 0, g_substeps = 0;
     float g_dt = 0.0f;
     sv_params g_params{};
+    uint64_t g_epoch = 0;  // the context's bind_epoch when the graph was captured
     // differentiable form (sv_diff.hip), sized by sv_batch_diff_reserve
     uint32_t d_frames = 0, d_substeps = 0, d_iters = 0;
     float *d_ckpt = nullptr;     // [(frames + 1)][13N] state at each frame start (pos, quat, vel, omega)

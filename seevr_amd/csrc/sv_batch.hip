@@ -588,7 +588,7 @@ int sv_batch_step(sv_batch *b, uint32_t nv, uint32_t substeps, float dt) {
         return sv::batch_finish(b);
     }
     if (!b->graph_valid || b->g_nv != nv || b->g_substeps != substeps || b->g_dt != dt ||
-        !same_params(b->g_params, c->params)) {
+        !same_params(b->g_params, c->params) || b->g_epoch != c->bind_epoch) {
         drop_graph(b);
         hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return dev_fail(e, "sv_batch_step: begin capture");
@@ -605,6 +605,7 @@ int sv_batch_step(sv_batch *b, uint32_t nv, uint32_t substeps, float dt) {
         b->g_substeps = substeps;
         b->g_dt = dt;
         b->g_params = c->params;
+        b->g_epoch = c->bind_epoch;
     }
     hipError_t e = hipGraphLaunch(b->exec, c->stream);
     if (e != hipSuccess) return dev_fail(e, "sv_batch_step: graph launch");

@@ -12,6 +12,7 @@ namespace {
 
 constexpr int BLOCK = 256;
 constexpr int BOUNDS_ITEMS = 8;
+static_assert(BLOCK * BOUNDS_ITEMS == BOUNDS_TILE, "bounds partials sizing");
 
 template <typename T>
 inline T *B(sv_ctx *c, int id) { return reinterpret_cast<T *>(c->buf[id]); }

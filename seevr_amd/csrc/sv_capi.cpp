@@ -102,7 +102,7 @@ void free_ctx(sv_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int i = 0; i < SV_BUF_COUNT; i++)
-        if (c->buf[i]) (void)hipFree(c->buf[i]);
+        if (c->buf[i] && c->owned[i]) (void)hipFree(c->buf[i]);
     sv::Scratch &s = c->s;
     void *ptrs[] = {s.hrange, s.rot, s.bpbox, s.row_count, s.row_off, s.bp_stage, s.bp_ovf, s.np_flag, s.np_idx, s.np_res,
                     s.cprep, s.inc_key[0], s.inc_key[1], s.inc_val[0], s.inc_val[1], s.deg, s.deg_off,
@@ -242,6 +242,7 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
     for (int i = 0; i < SV_BUF_COUNT && e == hipSuccess; i++) {
         if (c->bytes[i] == 0) continue;
         e = hipMalloc(&c->buf[i], c->bytes[i]);
+        if (e == hipSuccess) c->owned[i] = true;
         if (e == hipSuccess) e = hipMemset(c->buf[i], 0, c->bytes[i]);
     }
     const sv_caps &k = c->caps;
@@ -317,6 +318,25 @@ int sv_buffer_ptr(sv_ctx *c, sv_buf id, void **dev, size_t *bytes) {
     if (dev) *dev = c->buf[id];
     if (bytes) *bytes = c->bytes[id];
     if (dev && touches_support(id)) c->sup_external = true;
+    return SV_OK;
+}
+
+int sv_bind_buffer(sv_ctx *c, sv_buf id, void *dev, size_t nbytes) {
+    if (!c || !valid_buf(id) || !dev) return fail(SV_E_INVALID, "sv_bind_buffer: bad context, buffer id or pointer");
+    size_t need[SV_BUF_COUNT];
+    buffer_sizes(c->caps, need);
+    if (nbytes < need[id])
+        return fail(SV_E_CAPACITY, "sv_bind_buffer: buffer %d needs %zu bytes for the capacities, got %zu", (int)id,
+                    need[id], nbytes);
+    if (reinterpret_cast<uintptr_t>(dev) % 16u)
+        return fail(SV_E_INVALID, "sv_bind_buffer: device address must be 16-byte aligned");
+    HIP_OK(hipStreamSynchronize(c->stream));  // no queued work still reads the old buffer
+    if (c->buf[id] && c->owned[id]) HIP_OK(hipFree(c->buf[id]));
+    c->buf[id] = dev;
+    c->bytes[id] = nbytes;
+    c->owned[id] = false;
+    c->bind_epoch++;
+    if (touches_support(id)) c->sup_dirty = c->sup_external = true;  // the caller may write it at any time
     return SV_OK;
 }
 
@@ -436,7 +456,10 @@ int sv_physics_dispatch(sv_ctx *c, uint32_t step, uint32_t count, float dt, cons
     return step == 5 ? check_status(c) : SV_OK;
 }
 
-static int bvh_run(sv_ctx *c, uint32_t n, bool bodies) {
+}  // extern "C"
+
+namespace sv {
+int bvh_build_run(sv_ctx *c, uint32_t n, bool bodies) {
     static const char *names[8] = {"bvh/centroids", "bvh/scene_bounds", "bvh/morton_codes", "bvh/radix_sort",
                                    "bvh/build_tree", "bvh/leaf_aabbs", "bvh/clear_flags", "bvh/propagate"};
     for (uint32_t st = 0; st < 8; st++) {
@@ -445,19 +468,22 @@ static int bvh_run(sv_ctx *c, uint32_t n, bool bodies) {
     }
     return sync_and_finish(c);
 }
+}  // namespace sv
+
+extern "C" {
 
 int sv_bvh_build(sv_ctx *c, uint32_t n) {
     if (!c) return fail(SV_E_INVALID, "sv_bvh_build: null context");
     if (n == 0) return SV_OK;  // bvh.zig:64
     if (n > c->caps.max_triangles) return fail(SV_E_CAPACITY, "sv_bvh_build: %u triangles > capacity", n);
-    return bvh_run(c, n, false);
+    return sv::bvh_build_run(c, n, false);
 }
 
 int sv_bvh_build_bodies(sv_ctx *c, uint32_t n) {
     if (!c) return fail(SV_E_INVALID, "sv_bvh_build_bodies: null context");
     if (n == 0) return SV_OK;
     if (n > c->caps.max_bodies) return fail(SV_E_CAPACITY, "sv_bvh_build_bodies: %u bodies > capacity", n);
-    return bvh_run(c, n, true);
+    return sv::bvh_build_run(c, n, true);
 }
 
 int sv_bvh_dispatch(sv_ctx *c, uint32_t step, uint32_t n) {

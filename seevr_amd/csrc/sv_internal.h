@@ -79,6 +79,18 @@ struct Scratch {
 // basic one above; SWEEP_PIPE and SWEEP_SEQ_BASIC force one of them.
 enum { SWEEP_SEQ = 0, SWEEP_COHERENT = 1, SWEEP_FENCED = 2, SWEEP_PIPE = 3, SWEEP_SEQ_BASIC = 4, SWEEP_PAIR = 5 };
 
+// Keys per radix-sort tile and values per scan tile (sv_prims.hip), and the
+// scratch words the sort (digit histograms, their scan, a count word) and
+// the scan (one partial per tile) take for n keys or values.
+constexpr uint32_t SORT_TILE_KEYS = 2048;
+constexpr uint32_t SCAN_TILE_VALUES = 1024;
+constexpr size_t sort_tiles(size_t n) { return (n + SORT_TILE_KEYS - 1) / SORT_TILE_KEYS; }
+constexpr size_t sort_hist_words(size_t n) { return 2 * 256 * sort_tiles(n) + 1; }
+constexpr size_t scan_part_words(size_t n) { return (n + SCAN_TILE_VALUES - 1) / SCAN_TILE_VALUES + 1; }
+// Centroids per partial of the LBVH's bounds reduction (sv_bvh.hip): the
+// partials take 6 floats per tile.
+constexpr uint32_t BOUNDS_TILE = 2048;
+
 // Control words of the sweeps (Scratch::solve_ctl).
 constexpr uint32_t SOLVE_CTL_WORDS = 1024;
 
@@ -123,6 +135,14 @@ struct sv_ctx {
     sv_params params{};
     void *buf[SV_BUF_COUNT] = {};
     size_t bytes[SV_BUF_COUNT] = {};
+    // false for a buffer the caller bound (sv_bind_buffer): not freed here
+    bool owned[SV_BUF_COUNT] = {};
+    // bumped by every sv_bind_buffer: captured graphs hold buffer addresses
+    // and are rebuilt when it changed
+    uint64_t bind_epoch = 0;
+    // scan and sort scratch is caller-provided and sized up front: never
+    // grown or freed (a view context of an allocation-free entry point)
+    bool scratch_fixed = false;
     sv::Scratch s;
     // workgroups of the ordered PGS kernel: 0 = chosen by body count,
     // 1 = single-CU form, > 1 = multi-CU form with that many workgroups
@@ -256,6 +276,11 @@ void bvh_stage(sv_ctx *c, uint32_t step, uint32_t n, bool bodies);
 bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n);
 // Body count from which the automatic broadphase choice is the LBVH.
 constexpr uint32_t BP_LBVH_MIN_BODIES = 1u << 17;
+
+// ---- sv_capi.cpp ----
+// BVH.build's stage sequence (bvh.zig:63-172) over n triangles or bodies,
+// then the context's synchronization and error check.
+int bvh_build_run(sv_ctx *c, uint32_t n, bool bodies);
 
 // ---- errors (sv_capi.cpp): sets sv_last_error() and returns code ----
 int set_error(int code, const char *msg);

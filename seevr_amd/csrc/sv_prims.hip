@@ -12,11 +12,13 @@ namespace {
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 1024
+static_assert(SCAN_TILE == SCAN_TILE_VALUES, "scan scratch sizing");
 
 constexpr int SORT_THREADS = 256;
 constexpr int SORT_WAVES = SORT_THREADS / 64;
 constexpr int SORT_ROUNDS = 8;
 constexpr int SORT_TILE = SORT_THREADS * SORT_ROUNDS;  // 2048
+static_assert(SORT_TILE == SORT_TILE_KEYS, "sort scratch sizing");
 
 __device__ __forceinline__ uint32_t read_n(const uint32_t *n_dev, uint32_t n_max) {
     if (!n_dev) return n_max;
@@ -212,6 +214,18 @@ void ensure_u32(uint32_t *&p, uint32_t &cap, uint32_t need) {
     if (hipMalloc(&p, (size_t)need * sizeof(uint32_t)) == hipSuccess) cap = need;
 }
 
+// Scratch of at least `need` words: grown when the context owns it; a
+// fixed (caller-provided) one that is too small is a launch error instead.
+static bool reserve_fixed(sv_ctx *c, uint32_t *&p, uint32_t &cap, uint32_t need) {
+    if (c->scratch_fixed) {
+        if (need <= cap) return true;
+        c->launch_err = hipErrorInvalidValue;
+        return false;
+    }
+    ensure_u32(p, cap, need);
+    return p != nullptr && need <= cap;
+}
+
 void prims_reserve(sv_ctx *c, uint32_t max_scan_n, uint32_t max_sort_n) {
     uint32_t sort_blocks = div_up(max_sort_n, SORT_TILE);
     uint32_t hist_n = 256u * sort_blocks;
@@ -227,7 +241,7 @@ void scan_exclusive(sv_ctx *c, const uint32_t *in, uint32_t *out, const uint32_t
         return;
     }
     uint32_t nparts = div_up(n_max, SCAN_TILE);
-    ensure_u32(c->s.scan_part, c->s.scan_part_cap, nparts);
+    if (!reserve_fixed(c, c->s.scan_part, c->s.scan_part_cap, nparts)) return;
     hipLaunchKernelGGL(k_scan_partials, dim3(nparts), dim3(SCAN_THREADS), 0, c->stream, in, n_dev, n_max,
                        c->s.scan_part);
     hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(SCAN_THREADS), 0, c->stream, c->s.scan_part, nparts, total_dev);
@@ -240,7 +254,7 @@ int radix_sort(sv_ctx *c, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1
     if (n_max == 0 || key_bits <= 0) return 0;
     uint32_t nblocks = div_up(n_max, SORT_TILE);
     uint32_t hist_n = 256u * nblocks;
-    ensure_u32(c->s.sort_hist, c->s.sort_hist_cap, 2u * hist_n + 1);
+    if (!reserve_fixed(c, c->s.sort_hist, c->s.sort_hist_cap, 2u * hist_n + 1)) return 0;
     uint32_t *hist = c->s.sort_hist, *offs = c->s.sort_hist + hist_n, *used = c->s.sort_hist + 2u * hist_n;
     int passes = (key_bits + 7) / 8;
     uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;

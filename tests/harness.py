@@ -359,3 +359,59 @@ def make_scene(workload: int, count: int, seed: int = 1, segments: int = 0, extr
     sc = Scene(big)
     sc.build_workload(workload, count, segments, seed)
     return sc
+
+
+class HipMem:
+    """Caller-owned device memory and streams for tests of entry points that
+    take device pointers, straight from the HIP runtime the library loaded
+    (torch would bring its own copy of the runtime)."""
+
+    _hip = None
+
+    def __init__(self):
+        if HipMem._hip is None:
+            from seevr_amd import _lib
+            _lib.load()  # the library's runtime is the process's libamdhip64
+            h = C.CDLL("libamdhip64.so.7")
+            for name, args in {"hipMalloc": [C.POINTER(C.c_void_p), C.c_size_t], "hipFree": [C.c_void_p],
+                               "hipMemcpy": [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int],
+                               "hipStreamCreate": [C.POINTER(C.c_void_p)], "hipStreamDestroy": [C.c_void_p],
+                               "hipDeviceSynchronize": []}.items():
+                fn = getattr(h, name)
+                fn.restype = C.c_int
+                fn.argtypes = args
+            HipMem._hip = h
+        self.hip = HipMem._hip
+        self.blocks = []
+        self.streams = []
+
+    def alloc(self, nbytes: int, fill=None) -> int:
+        p = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(p), max(nbytes, 16)) == 0, "hipMalloc"
+        self.blocks.append(p.value)
+        if fill is not None:
+            self.upload(p.value, np.full(max(nbytes, 16) // 4, fill, np.int32))
+        return p.value
+
+    def upload(self, dev: int, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr)
+        assert self.hip.hipMemcpy(C.c_void_p(dev), a.ctypes.data_as(C.c_void_p), a.nbytes, 1) == 0  # H2D
+
+    def download(self, dev: int, count: int, dtype) -> np.ndarray:
+        out = np.empty(count, dtype)
+        assert self.hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(dev), out.nbytes, 2) == 0  # D2H
+        return out
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        assert self.hip.hipStreamCreate(C.byref(s)) == 0
+        self.streams.append(s.value)
+        return s.value
+
+    def close(self) -> None:
+        self.hip.hipDeviceSynchronize()
+        for s in self.streams:
+            self.hip.hipStreamDestroy(C.c_void_p(s))
+        for p in self.blocks:
+            self.hip.hipFree(C.c_void_p(p))
+        self.blocks, self.streams = [], []
