@@ -207,6 +207,39 @@ int sv_bvh_build(sv_ctx *ctx, uint32_t num_tris);
 /* The same LBVH over body AABBs (primitive i = body i, centroid =
  * (min + max) / 2, leaf box = body AABB); reads SV_BODY_AABB_MIN/MAX. */
 int sv_bvh_build_bodies(sv_ctx *ctx, uint32_t num_bodies);
+/* Output arrays of a body LBVH built into caller memory, in the layout of
+ * the SV_BVH_* buffers above (n = bodies, 2n - 1 nodes). */
+typedef struct sv_bvh_arrays {
+    float *centroids;        /* float[3n]                                    */
+    uint32_t *morton;        /* uint[n] sorted Morton codes                  */
+    uint32_t *sort_indices;  /* uint[n] body of each sorted position         */
+    uint32_t *prim_indices;  /* uint[n]                                      */
+    float *scene_bounds;     /* float[6]                                     */
+    float *node_min;         /* float[3(2n-1)]                               */
+    float *node_max;         /* float[3(2n-1)]                               */
+    uint32_t *left, *right, *count, *parent;  /* uint[2n-1]                  */
+} sv_bvh_arrays;
+
+/* Largest body count of sv_bvh_build_bodies_noalloc: its scratch is a fixed
+ * array of the library's code object (about 14 MB per GPU, held until the
+ * process ends). */
+#define SV_NOALLOC_MAX_BODIES (1u << 20)
+
+/* sv_bvh_build_bodies without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * caller passes the body boxes (float[3n] min and max, device memory) and
+ * the output arrays; everything else the build needs (sort buffers and
+ * digit histograms, scan partials, bounds partials, arrival counters) lives
+ * in one static device array of the library.  Same results as
+ * sv_bvh_build_bodies, bit for bit.  Runs on `stream` (a hipStream_t; NULL =
+ * the null stream) of the current device and returns after it finished.
+ * num_bodies above SV_NOALLOC_MAX_BODIES is SV_E_CAPACITY.  Every call on a
+ * GPU shares that GPU's scratch array: calls that overlap on one GPU (other
+ * host threads, other streams) race and must not be issued. */
+int sv_bvh_build_bodies_noalloc(const float *aabb_min, const float *aabb_max, uint32_t num_bodies,
+                                const sv_bvh_arrays *out, void *stream);
+
 /* One BVH stage, addressed like bvh.comp's push constants {step, num_tris}
  * (src/shaders/bvh.comp:401-416): 0 centroids, 1 scene bounds, 2 Morton,
  * 3 whole radix sort, 4 hierarchy, 5 leaf boxes, 6 clear flags,

@@ -66,10 +66,17 @@ class SceneCounts(C.Structure):
                 ("num_support_entries", C.c_uint32)]
 
 
+class BvhArrays(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("centroids", "morton", "sort_indices", "prim_indices", "scene_bounds",
+                                          "node_min", "node_max", "left", "right", "count", "parent")]
+
+
+NOALLOC_MAX_BODIES = 1 << 20
+
 EXPORTS = [
     "sv_default_caps", "sv_default_params", "sv_create", "sv_destroy", "sv_get_params", "sv_set_params",
     "sv_buffer_ptr", "sv_bind_buffer", "sv_upload", "sv_download", "sv_physics_step", "sv_read_counters",
-    "sv_physics_dispatch", "sv_bvh_build", "sv_bvh_build_bodies", "sv_bvh_dispatch", "sv_debug_atan2",
+    "sv_physics_dispatch", "sv_bvh_build", "sv_bvh_build_bodies", "sv_bvh_build_bodies_noalloc", "sv_bvh_dispatch", "sv_debug_atan2",
     "sv_debug_copy_bandwidth", "sv_debug_broadphase_scratch", "sv_debug_xcc_ids", "sv_debug_set_sweep_variant", "sv_last_error", "sv_device_info", "sv_stage_times", "sv_enable_stage_timing", "sv_stage_timing_only", "sv_set_solver", "sv_solver_residual", "sv_debug_color_counts", "sv_set_broadphase", "sv_set_warm_start", "sv_scene_create",
     "sv_scene_destroy", "sv_scene_add_material", "sv_scene_add_box", "sv_scene_add_sphere",
     "sv_scene_set_body_state", "sv_scene_get_counts", "sv_scene_array", "sv_scene_upload",
@@ -116,6 +123,7 @@ def load(path: Path | str | None = None) -> C.CDLL:
         "sv_bvh_build": (C.c_int, [vp, u32]),
         "sv_bvh_build_bodies": (C.c_int, [vp, u32]),
         "sv_bvh_dispatch": (C.c_int, [vp, u32, u32]),
+        "sv_bvh_build_bodies_noalloc": (C.c_int, [vp, vp, u32, C.POINTER(BvhArrays), vp]),
         "sv_debug_atan2": (C.c_int, [vp, fp, fp, fp, u32]),
         "sv_debug_copy_bandwidth": (C.c_int, [vp, sz, u32, C.POINTER(C.c_double)]),
         "sv_debug_xcc_ids": (C.c_int, [vp, u32, C.POINTER(u32)]),

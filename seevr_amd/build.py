@@ -23,7 +23,7 @@ CSRC = PKG / "csrc"
 OUT = PKG / "libseevr_hip.so"
 OBJ = PKG / "_build"
 
-HIP_SOURCES = ["sv_prims.hip", "sv_physics.hip", "sv_pgs.hip", "sv_bvh.hip", "sv_batch.hip", "sv_diff.hip", "sv_color.hip", "sv_warm.hip", "sv_capi.cpp"]
+HIP_SOURCES = ["sv_prims.hip", "sv_physics.hip", "sv_pgs.hip", "sv_bvh.hip", "sv_batch.hip", "sv_diff.hip", "sv_color.hip", "sv_warm.hip", "sv_noalloc.hip", "sv_capi.cpp"]
 HOST_SOURCES = ["sv_scene.cpp"]
 
 ARCH = os.environ.get("SEEVR_OFFLOAD_ARCH", "gfx950")
