@@ -220,7 +220,8 @@ void ensure_u32(uint32_t *&p, uint32_t &cap, uint32_t need);
 void prims_reserve(sv_ctx *c, uint32_t max_scan_n, uint32_t max_sort_n);
 
 // ---- sv_physics.hip ----
-void launch_hull_ranges(sv_ctx *c, uint32_t n);
+// exact: read SV_GEOM_HULL_RANGE whatever sv_params.hull_range_u32 says
+void launch_hull_ranges(sv_ctx *c, uint32_t n, bool exact = false);
 void ensure_support_vertices(sv_ctx *c, uint32_t n);
 // clear_counters: also clear counters[0..1] (the full step's first dispatch)
 void launch_gravity(sv_ctx *c, uint32_t n, float dt, const float g[3], bool clear_counters = false);

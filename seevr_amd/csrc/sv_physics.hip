@@ -454,10 +454,10 @@ __global__ void k_atan2(const float *y, const float *x, float *out, uint32_t n) 
 
 }  // namespace
 
-void launch_hull_ranges(sv_ctx *c, uint32_t n) {
+void launch_hull_ranges(sv_ctx *c, uint32_t n, bool exact) {
     if (!n) return;
     hipLaunchKernelGGL(k_hull_ranges, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, c->stream, B<float>(c, SV_GEOM_DATA),
-                       B<uint32_t>(c, SV_GEOM_HULL_RANGE), (int)c->params.hull_range_u32, c->s.hrange, n);
+                       B<uint32_t>(c, SV_GEOM_HULL_RANGE), (exact || c->params.hull_range_u32) ? 1 : 0, c->s.hrange, n);
 }
 
 // Scratch::sup_xyz for bodies [0, n) when it may be stale (after

@@ -249,9 +249,10 @@ class World:
         self.warm_lam = np.zeros(max(self.max_contacts, 1), np.float64 if self.double else np.float32)
         self.warm_n = 0
 
-    def oracle_stage(self, step: int, count: int = None, dt: float = 0.0, g=(0.0, 0.0, 0.0)):
+    def oracle_stage(self, step: int, count: int = None, dt: float = 0.0, g=(0.0, 0.0, 0.0), mt: bool = False):
+        """One stage by its reference step id; mt=True runs the OpenMP build (same results)."""
         b = self.bufs()
-        lib = self.lib()
+        lib = self.lib(mt)
         R = C.c_double if self.double else C.c_float
         count = self.n if count is None else count
         if step == 0:
