@@ -415,7 +415,10 @@ int sv_scene_build_batch(sv_scene *scene, uint32_t first, uint32_t count, uint32
  * take scenes, largest first, from a queue); a frame of `substeps` launches
  * plus the render-vertex update is replayed as one captured hipGraph.  Every
  * scene's results equal sv_physics_step on that scene alone, bit for bit,
- * including its pair and contact order and overflow truncation. */
+ * including its pair and contact order and overflow truncation.  Hull
+ * ranges are read from SV_GEOM_HULL_RANGE (sv_scene_upload fills it): the
+ * concatenated hulls of a few thousand scenes pass 2^24 vertices, where
+ * geom_data's float offsets are not exact. */
 #define SV_BATCH_MAX_SCENE_BODIES 512
 #define SV_BATCH_MAX_SCENE_CONTACTS 2048
 typedef struct sv_batch sv_batch;

@@ -44,15 +44,6 @@ static inline v3 ld3(const R *p, uint32_t i) { return V(p[3 * i], p[3 * i + 1], 
 static inline void st3(R *p, uint32_t i, v3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
 static inline void acc3(R *p, uint32_t i, v3 v) { p[3 * i] += v.x; p[3 * i + 1] += v.y; p[3 * i + 2] += v.z; }
 
-/* physics.comp:74-81 */
-static m3 quat_mat(const R *q) {
-    R x = q[0], y = q[1], z = q[2], w = q[3];
-    m3 m;
-    m.c0 = V(1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y));
-    m.c1 = V(2 * (x * y - w * z), 1 - 2 * (x * x + z * z), 2 * (y * z + w * x));
-    m.c2 = V(2 * (x * z + w * y), 2 * (y * z - w * x), 1 - 2 * (x * x + y * y));
-    return m;
-}
 static v3 mvec(m3 m, v3 v) {
     return V(m.c0.x * v.x + m.c1.x * v.y + m.c2.x * v.z, m.c0.y * v.x + m.c1.y * v.y + m.c2.y * v.z,
              m.c0.z * v.x + m.c1.z * v.y + m.c2.z * v.z);
@@ -71,6 +62,47 @@ static void inv_I_rbar(m3 *Rb, m3 R_, v3 d, v3 x, v3 yb, R s) {
     add_outer(Rb, yb, hmul(d, mtvec(R_, x)), s);
     add_outer(Rb, x, hmul(d, mtvec(R_, yb)), s);
 }
+
+/* Precision of the forward recomputation.  By default every forward value
+ * is FP64.  adj_set_forward32(1) rounds the result of every forward
+ * operation to float.  For float operands that is the float operation
+ * itself (a double result of +, -, *, / or sqrt rounded once to float is the
+ * correctly rounded float result, since 53 >= 2 * 24 + 2), so the forward
+ * values and every branch they decide (impulse clamp, stick or slip, the
+ * tangent threshold) are those of the FP32 sweep of physics.comp; the
+ * reverse stays FP64.  That is the exact derivative of the FP32 computation,
+ * evaluated in double. */
+static int g_f32 = 0;
+void adj_set_forward32(int on) { g_f32 = on != 0; }
+static inline R rf(R x) { return g_f32 ? (R)(float)x : x; }
+static inline v3 fadd(v3 a, v3 b) { return V(rf(a.x + b.x), rf(a.y + b.y), rf(a.z + b.z)); }
+static inline v3 fsub(v3 a, v3 b) { return V(rf(a.x - b.x), rf(a.y - b.y), rf(a.z - b.z)); }
+static inline v3 fscl(R s, v3 a) { return V(rf(s * a.x), rf(s * a.y), rf(s * a.z)); }
+static inline v3 fhmul(v3 a, v3 b) { return V(rf(a.x * b.x), rf(a.y * b.y), rf(a.z * b.z)); }
+static inline R fdot(v3 a, v3 b) { return rf(rf(rf(a.x * b.x) + rf(a.y * b.y)) + rf(a.z * b.z)); }
+static inline v3 fcross(v3 a, v3 b) {
+    return V(rf(rf(a.y * b.z) - rf(a.z * b.y)), rf(rf(a.z * b.x) - rf(a.x * b.z)), rf(rf(a.x * b.y) - rf(a.y * b.x)));
+}
+static inline R fsqrt(R x) { return rf(sqrt(x)); }
+/* physics.comp:74-81, each operation rounded like the forward */
+static m3 fquat_mat(const R *q) {
+    R x = q[0], y = q[1], z = q[2], w = q[3];
+    m3 m;
+    m.c0 = V(rf(1 - rf(2 * rf(rf(y * y) + rf(z * z)))), rf(2 * rf(rf(x * y) + rf(w * z))),
+             rf(2 * rf(rf(x * z) - rf(w * y))));
+    m.c1 = V(rf(2 * rf(rf(x * y) - rf(w * z))), rf(1 - rf(2 * rf(rf(x * x) + rf(z * z)))),
+             rf(2 * rf(rf(y * z) + rf(w * x))));
+    m.c2 = V(rf(2 * rf(rf(x * z) + rf(w * y))), rf(2 * rf(rf(y * z) - rf(w * x))),
+             rf(1 - rf(2 * rf(rf(x * x) + rf(y * y)))));
+    return m;
+}
+static v3 fmvec(m3 m, v3 v) {
+    return V(rf(rf(rf(m.c0.x * v.x) + rf(m.c1.x * v.y)) + rf(m.c2.x * v.z)),
+             rf(rf(rf(m.c0.y * v.x) + rf(m.c1.y * v.y)) + rf(m.c2.y * v.z)),
+             rf(rf(rf(m.c0.z * v.x) + rf(m.c1.z * v.y)) + rf(m.c2.z * v.z)));
+}
+static v3 fmtvec(m3 m, v3 v) { return V(fdot(m.c0, v), fdot(m.c1, v), fdot(m.c2, v)); }
+static v3 finv_I(m3 R_, v3 d, v3 v) { return fmvec(R_, fhmul(d, fmtvec(R_, v))); }
 
 typedef struct adj_scene {
     uint32_t n;
@@ -100,19 +132,20 @@ static cst contact_cst(const adj_scene *s, uint32_t c, const R *vel1, const R *w
     uint32_t a = s->ca[c], b = s->cb[c];
     k.n = ld3(s->cn, c);
     v3 cp = ld3(s->cp, c);
-    k.ra = sub(cp, ld3(s->pos, a));
-    k.rb = sub(cp, ld3(s->pos, b));
-    v3 va = add(ld3(vel1, a), cross(ld3(w1, a), k.ra));
-    v3 vb = add(ld3(vel1, b), cross(ld3(w1, b), k.rb));
-    R vn = dot(sub(vb, va), k.n);
+    k.ra = fsub(cp, ld3(s->pos, a));
+    k.rb = fsub(cp, ld3(s->pos, b));
+    v3 va = fadd(ld3(vel1, a), fcross(ld3(w1, a), k.ra));
+    v3 vb = fadd(ld3(vel1, b), fcross(ld3(w1, b), k.rb));
+    R vn = fdot(fsub(vb, va), k.n);
     R e = gmax(s->rest[a], s->rest[b]);
-    k.bias = s->baumgarte * gmax(s->pen[c] - s->slop, 0) / s->dt + e * gmax(-vn, 0);
+    k.bias = rf(rf(rf(rf(s->baumgarte) * gmax(rf(s->pen[c] - rf(s->slop)), 0)) / s->dt) + rf(e * gmax(-vn, 0)));
     k.ima = s->inv_mass[a];
     k.imb = s->inv_mass[b];
-    m3 Ra = quat_mat(s->quat + 4 * a), Rb = quat_mat(s->quat + 4 * b);
-    v3 ran = cross(k.ra, k.n), rbn = cross(k.rb, k.n);
-    k.kn = k.ima + k.imb + dot(ran, inv_I(Ra, ld3(s->inv_i, a), ran)) + dot(rbn, inv_I(Rb, ld3(s->inv_i, b), rbn));
-    k.mu = sqrt(s->fric[a] * s->fric[b]);
+    m3 Ra = fquat_mat(s->quat + 4 * a), Rb = fquat_mat(s->quat + 4 * b);
+    v3 ran = fcross(k.ra, k.n), rbn = fcross(k.rb, k.n);
+    k.kn = rf(rf(rf(k.ima + k.imb) + fdot(ran, finv_I(Ra, ld3(s->inv_i, a), ran))) +
+              fdot(rbn, finv_I(Rb, ld3(s->inv_i, b), rbn)));
+    k.mu = fsqrt(rf(s->fric[a] * s->fric[b]));
     k.skip = (k.ima == 0 && k.imb == 0) || a == b;
     k.dyn_a = body_dyn(s, a);
     k.dyn_b = body_dyn(s, b);
@@ -130,8 +163,8 @@ void adj_forward(adj_scene *s, R *tape) {
     for (uint32_t i = 0; i < n; i++) {
         if (s->inv_mass[i] <= 0) continue;
         for (int k = 0; k < 3; k++) {
-            s->vel[3 * i + k] = s->vel[3 * i + k] * s->ld + s->g[k] * s->dt;
-            s->omega[3 * i + k] *= s->ad;
+            s->vel[3 * i + k] = rf(rf(s->vel[3 * i + k] * rf(s->ld)) + rf(s->g[k] * s->dt));
+            s->omega[3 * i + k] = rf(s->omega[3 * i + k] * rf(s->ad));
         }
     }
     if (tape) {
@@ -153,35 +186,35 @@ void adj_forward(adj_scene *s, R *tape) {
                 st3(tp, 0, la); st3(tp, 1, wa); st3(tp, 2, lb); st3(tp, 3, wb);
                 tp[12] = lam[c];
             }
-            m3 Ra = quat_mat(s->quat + 4 * a), Rb = quat_mat(s->quat + 4 * b);
+            m3 Ra = fquat_mat(s->quat + 4 * a), Rb = fquat_mat(s->quat + 4 * b);
             v3 iia = ld3(s->inv_i, a), iib = ld3(s->inv_i, b);
-            v3 va = add(la, cross(wa, k.ra)), vb = add(lb, cross(wb, k.rb));
-            R vn = dot(sub(vb, va), k.n);
-            R dln = (-vn + k.bias) / k.kn;
+            v3 va = fadd(la, fcross(wa, k.ra)), vb = fadd(lb, fcross(wb, k.rb));
+            R vn = fdot(fsub(vb, va), k.n);
+            R dln = rf(rf(-vn + k.bias) / k.kn);
             R old = lam[c];
-            R nl = gmax(old + dln, 0);
-            dln = nl - old;
+            R nl = gmax(rf(old + dln), 0);
+            dln = rf(nl - old);
             lam[c] = nl;
-            v3 imp = scl(dln, k.n);
-            la = sub(la, scl(k.ima, imp));
-            lb = add(lb, scl(k.imb, imp));
-            wa = sub(wa, inv_I(Ra, iia, cross(k.ra, imp)));
-            wb = add(wb, inv_I(Rb, iib, cross(k.rb, imp)));
-            va = add(la, cross(wa, k.ra));
-            vb = add(lb, cross(wb, k.rb));
-            v3 dv = sub(vb, va);
-            v3 vt = sub(dv, scl(dot(dv, k.n), k.n));
-            R vtl = sqrt(dot(vt, vt));
-            if (vtl > 1e-6) {
-                v3 td = V(vt.x / vtl, vt.y / vtl, vt.z / vtl);
-                v3 rat = cross(k.ra, td), rbt = cross(k.rb, td);
-                R kt = k.ima + k.imb + dot(rat, inv_I(Ra, iia, rat)) + dot(rbt, inv_I(Rb, iib, rbt));
-                R dlt = gmin(vtl / kt, k.mu * nl);
-                v3 impt = scl(dlt, td);
-                la = add(la, scl(k.ima, impt));
-                lb = sub(lb, scl(k.imb, impt));
-                wa = add(wa, inv_I(Ra, iia, cross(k.ra, impt)));
-                wb = sub(wb, inv_I(Rb, iib, cross(k.rb, impt)));
+            v3 imp = fscl(dln, k.n);
+            la = fsub(la, fscl(k.ima, imp));
+            lb = fadd(lb, fscl(k.imb, imp));
+            wa = fsub(wa, finv_I(Ra, iia, fcross(k.ra, imp)));
+            wb = fadd(wb, finv_I(Rb, iib, fcross(k.rb, imp)));
+            va = fadd(la, fcross(wa, k.ra));
+            vb = fadd(lb, fcross(wb, k.rb));
+            v3 dv = fsub(vb, va);
+            v3 vt = fsub(dv, fscl(fdot(dv, k.n), k.n));
+            R vtl = fsqrt(fdot(vt, vt));
+            if (vtl > rf(1e-6)) {
+                v3 td = V(rf(vt.x / vtl), rf(vt.y / vtl), rf(vt.z / vtl));
+                v3 rat = fcross(k.ra, td), rbt = fcross(k.rb, td);
+                R kt = rf(rf(rf(k.ima + k.imb) + fdot(rat, finv_I(Ra, iia, rat))) + fdot(rbt, finv_I(Rb, iib, rbt)));
+                R dlt = gmin(rf(vtl / kt), rf(k.mu * nl));
+                v3 impt = fscl(dlt, td);
+                la = fadd(la, fscl(k.ima, impt));
+                lb = fsub(lb, fscl(k.imb, impt));
+                wa = fadd(wa, finv_I(Ra, iia, fcross(k.ra, impt)));
+                wb = fsub(wb, finv_I(Rb, iib, fcross(k.rb, impt)));
             }
             if (k.dyn_a) { st3(s->vel, a, la); st3(s->omega, a, wa); }
             if (k.dyn_b) { st3(s->vel, b, lb); st3(s->omega, b, wb); }
@@ -193,17 +226,18 @@ void adj_forward(adj_scene *s, R *tape) {
     /* integrate */
     for (uint32_t i = 0; i < n; i++) {
         if (s->inv_mass[i] <= 0) continue;
-        for (int k = 0; k < 3; k++) s->pos[3 * i + k] += s->vel[3 * i + k] * s->dt;
+        for (int k = 0; k < 3; k++) s->pos[3 * i + k] = rf(s->pos[3 * i + k] + rf(s->vel[3 * i + k] * s->dt));
         R ox = s->omega[3 * i], oy = s->omega[3 * i + 1], oz = s->omega[3 * i + 2];
         R *q = s->quat + 4 * i;
         R qx = q[0], qy = q[1], qz = q[2], qw = q[3];
-        R dx = ox * qw + oy * qz - oz * qy;
-        R dy = -ox * qz + oy * qw + oz * qx;
-        R dz = ox * qy - oy * qx + oz * qw;
-        R dw = -ox * qx - oy * qy - oz * qz;
-        qx += 0.5 * dx * s->dt; qy += 0.5 * dy * s->dt; qz += 0.5 * dz * s->dt; qw += 0.5 * dw * s->dt;
-        R l = sqrt(qx * qx + qy * qy + qz * qz + qw * qw);
-        if (l > 1e-8) { qx /= l; qy /= l; qz /= l; qw /= l; }
+        R dx = rf(rf(rf(ox * qw) + rf(oy * qz)) - rf(oz * qy));
+        R dy = rf(rf(rf(-ox * qz) + rf(oy * qw)) + rf(oz * qx));
+        R dz = rf(rf(rf(ox * qy) - rf(oy * qx)) + rf(oz * qw));
+        R dw = rf(rf(rf(-ox * qx) - rf(oy * qy)) - rf(oz * qz));
+        qx = rf(qx + rf(rf(0.5 * dx) * s->dt)); qy = rf(qy + rf(rf(0.5 * dy) * s->dt));
+        qz = rf(qz + rf(rf(0.5 * dz) * s->dt)); qw = rf(qw + rf(rf(0.5 * dw) * s->dt));
+        R l = fsqrt(rf(rf(rf(rf(qx * qx) + rf(qy * qy)) + rf(qz * qz)) + rf(qw * qw)));
+        if (l > rf(1e-8)) { qx = rf(qx / l); qy = rf(qy / l); qz = rf(qz / l); qw = rf(qw / l); }
         q[0] = qx; q[1] = qy; q[2] = qz; q[3] = qw;
     }
 }
@@ -221,32 +255,32 @@ static void item_vjp(const cst *k, m3 Ra, m3 Rb, v3 iia, v3 iib, v3 la, v3 wa, v
                      v3 gwa2, v3 glb2, v3 gwb2, R gnl, item_adj *o) {
     memset(o, 0, sizeof(*o));
     /* forward recomputation */
-    v3 va = add(la, cross(wa, k->ra)), vb = add(lb, cross(wb, k->rb));
-    R vn = dot(sub(vb, va), k->n);
-    R dln = (-vn + k->bias) / k->kn;
-    R x = old + dln;
+    v3 va = fadd(la, fcross(wa, k->ra)), vb = fadd(lb, fcross(wb, k->rb));
+    R vn = fdot(fsub(vb, va), k->n);
+    R dln = rf(rf(-vn + k->bias) / k->kn);
+    R x = rf(old + dln);
     R nl = gmax(x, 0);
-    R d2 = nl - old;
-    v3 imp = scl(d2, k->n);
-    v3 la1 = sub(la, scl(k->ima, imp)), lb1 = add(lb, scl(k->imb, imp));
-    v3 ua = cross(k->ra, imp), ub = cross(k->rb, imp);
-    v3 wa1 = sub(wa, inv_I(Ra, iia, ua)), wb1 = add(wb, inv_I(Rb, iib, ub));
-    v3 va2 = add(la1, cross(wa1, k->ra)), vb2 = add(lb1, cross(wb1, k->rb));
-    v3 dv = sub(vb2, va2);
-    v3 vt = sub(dv, scl(dot(dv, k->n), k->n));
-    R vtl = sqrt(dot(vt, vt));
+    R d2 = rf(nl - old);
+    v3 imp = fscl(d2, k->n);
+    v3 la1 = fsub(la, fscl(k->ima, imp)), lb1 = fadd(lb, fscl(k->imb, imp));
+    v3 ua = fcross(k->ra, imp), ub = fcross(k->rb, imp);
+    v3 wa1 = fsub(wa, finv_I(Ra, iia, ua)), wb1 = fadd(wb, finv_I(Rb, iib, ub));
+    v3 va2 = fadd(la1, fcross(wa1, k->ra)), vb2 = fadd(lb1, fcross(wb1, k->rb));
+    v3 dv = fsub(vb2, va2);
+    v3 vt = fsub(dv, fscl(fdot(dv, k->n), k->n));
+    R vtl = fsqrt(fdot(vt, vt));
     /* reverse */
     v3 gla1 = gla2, gwa1 = gwa2, glb1 = glb2, gwb1 = gwb2, gdv = V(0, 0, 0);
     R gnl_acc = gnl;
-    if (vtl > 1e-6) {
-        v3 td = V(vt.x / vtl, vt.y / vtl, vt.z / vtl);
-        v3 rat = cross(k->ra, td), rbt = cross(k->rb, td);
-        v3 Irat = inv_I(Ra, iia, rat), Irbt = inv_I(Rb, iib, rbt);
-        R kt = k->ima + k->imb + dot(rat, Irat) + dot(rbt, Irbt);
-        R s1 = vtl / kt, s2 = k->mu * nl;
+    if (vtl > rf(1e-6)) {
+        v3 td = V(rf(vt.x / vtl), rf(vt.y / vtl), rf(vt.z / vtl));
+        v3 rat = fcross(k->ra, td), rbt = fcross(k->rb, td);
+        v3 Irat = finv_I(Ra, iia, rat), Irbt = finv_I(Rb, iib, rbt);
+        R kt = rf(rf(rf(k->ima + k->imb) + fdot(rat, Irat)) + fdot(rbt, Irbt));
+        R s1 = rf(vtl / kt), s2 = rf(k->mu * nl);
         R dlt = gmin(s1, s2);
-        v3 impt = scl(dlt, td);
-        v3 ta = cross(k->ra, impt), tb = cross(k->rb, impt);
+        v3 impt = fscl(dlt, td);
+        v3 ta = fcross(k->ra, impt), tb = fcross(k->rb, impt);
         v3 gta = inv_I(Ra, iia, gwa2);
         inv_I_rbar(&o->Ra, Ra, iia, ta, gwa2, 1);
         v3 gimpt = cross(gta, k->ra);
@@ -343,15 +377,16 @@ void adj_backward(const adj_scene *s, const R *tape, R *ap, R *aq, R *av, R *aw,
         const R *q = s->quat + 4 * i;
         R ox = w2[3 * i], oy = w2[3 * i + 1], oz = w2[3 * i + 2];
         R qx = q[0], qy = q[1], qz = q[2], qw = q[3];
-        R dx = ox * qw + oy * qz - oz * qy;
-        R dy = -ox * qz + oy * qw + oz * qx;
-        R dz = ox * qy - oy * qx + oz * qw;
-        R dw = -ox * qx - oy * qy - oz * qz;
-        R px = qx + 0.5 * dx * dt, py = qy + 0.5 * dy * dt, pz = qz + 0.5 * dz * dt, pw = qw + 0.5 * dw * dt;
-        R l = sqrt(px * px + py * py + pz * pz + pw * pw);
+        R dx = rf(rf(rf(ox * qw) + rf(oy * qz)) - rf(oz * qy));
+        R dy = rf(rf(rf(-ox * qz) + rf(oy * qw)) + rf(oz * qx));
+        R dz = rf(rf(rf(ox * qy) - rf(oy * qx)) + rf(oz * qw));
+        R dw = rf(rf(rf(-ox * qx) - rf(oy * qy)) - rf(oz * qz));
+        R px = rf(qx + rf(rf(0.5 * dx) * dt)), py = rf(qy + rf(rf(0.5 * dy) * dt)), pz = rf(qz + rf(rf(0.5 * dz) * dt)),
+          pw = rf(qw + rf(rf(0.5 * dw) * dt));
+        R l = fsqrt(rf(rf(rf(rf(px * px) + rf(py * py)) + rf(pz * pz)) + rf(pw * pw)));
         R g[4] = {aq[4 * i], aq[4 * i + 1], aq[4 * i + 2], aq[4 * i + 3]};
-        if (l > 1e-8) {
-            R ux = px / l, uy = py / l, uz = pz / l, uw = pw / l;
+        if (l > rf(1e-8)) {
+            R ux = rf(px / l), uy = rf(py / l), uz = rf(pz / l), uw = rf(pw / l);
             R d = g[0] * ux + g[1] * uy + g[2] * uz + g[3] * uw;
             g[0] = (g[0] - d * ux) / l; g[1] = (g[1] - d * uy) / l;
             g[2] = (g[2] - d * uz) / l; g[3] = (g[3] - d * uw) / l;
@@ -378,7 +413,7 @@ void adj_backward(const adj_scene *s, const R *tape, R *ap, R *aq, R *av, R *aw,
             if (k->skip) continue;
             uint32_t a = s->ca[c], b = s->cb[c];
             const R *tp = tape + ((size_t)t * nc + c) * 13;
-            m3 Ra = quat_mat(s->quat + 4 * a), Rb = quat_mat(s->quat + 4 * b);
+            m3 Ra = fquat_mat(s->quat + 4 * a), Rb = fquat_mat(s->quat + 4 * b);
             v3 zero = V(0, 0, 0);
             item_adj o;
             item_vjp(k, Ra, Rb, ld3(s->inv_i, a), ld3(s->inv_i, b), ld3(tp, 0), ld3(tp, 1), ld3(tp, 2), ld3(tp, 3),
@@ -409,19 +444,19 @@ void adj_backward(const adj_scene *s, const R *tape, R *ap, R *aq, R *av, R *aw,
         const R *A = acc + 9 * c;
         v3 gra = V(A[0], A[1], A[2]), grb = V(A[3], A[4], A[5]);
         R gkn = A[6], gbias = A[7], gmu = A[8];
-        v3 va = add(ld3(v1, a), cross(ld3(w1, a), k->ra));
-        v3 vb = add(ld3(v1, b), cross(ld3(w1, b), k->rb));
-        R vn = dot(sub(vb, va), k->n);
+        v3 va = fadd(ld3(v1, a), fcross(ld3(w1, a), k->ra));
+        v3 vb = fadd(ld3(v1, b), fcross(ld3(w1, b), k->rb));
+        R vn = fdot(fsub(vb, va), k->n);
         R e = gmax(s->rest[a], s->rest[b]);
         R term = -vn < 0 ? 0 : -vn;
         R ge = gbias * term;
         R gvn = -vn < 0 ? 0 : -gbias * e;
         if (s->rest[a] < s->rest[b]) ae[b] += ge; else ae[a] += ge;
         v3 gvb = scl(gvn, k->n), gva = scl(-gvn, k->n);
-        m3 Ra = quat_mat(s->quat + 4 * a), Rb = quat_mat(s->quat + 4 * b);
-        v3 ran = cross(k->ra, k->n), rbn = cross(k->rb, k->n);
-        v3 gran = scl(2 * gkn, inv_I(Ra, ld3(s->inv_i, a), ran));
-        v3 grbn = scl(2 * gkn, inv_I(Rb, ld3(s->inv_i, b), rbn));
+        m3 Ra = fquat_mat(s->quat + 4 * a), Rb = fquat_mat(s->quat + 4 * b);
+        v3 ran = fcross(k->ra, k->n), rbn = fcross(k->rb, k->n);
+        v3 gran = scl(2 * gkn, finv_I(Ra, ld3(s->inv_i, a), ran));
+        v3 grbn = scl(2 * gkn, finv_I(Rb, ld3(s->inv_i, b), rbn));
         gra = add(gra, cross(gva, ld3(w1, a)));
         grb = add(grb, cross(gvb, ld3(w1, b)));
         gra = add(gra, cross(k->n, gran));

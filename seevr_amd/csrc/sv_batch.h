@@ -146,8 +146,11 @@ struct sv_batch {
     float *d_prep = nullptr;     // [substeps][S * cs][CPREP_WORDS] sweep constants
     float *d_items = nullptr;    // [substeps][S][iters * cs][TAPE_ITEM] sweep item inputs
     uint32_t *d_nc = nullptr;    // [substeps][S] contacts solved
-    float *d_adj = nullptr;      // [13N] adjoint of the state
-    float *d_contrib = nullptr;  // [S * 2cs][18] per-incidence adjoint contributions
+    double *d_adj = nullptr;     // [13N] adjoint of the state (the reverse pass runs in FP64)
+    float *d_adj32 = nullptr;    // [13N] the caller's FP32 adjoint in and out
+    double *d_contrib = nullptr; // [S * 2cs][20] per-incidence adjoint contributions
+    double *d_amat = nullptr;    // [2N] friction | restitution adjoints per geom, FP64 sums
+    float *d_amat32 = nullptr;   // [2N] the same rounded to float for the output
     hipStream_t d_stream = nullptr;  // adjoint kernels (default priority), beside the context stream
     hipEvent_t d_ev[5] = {};
 };

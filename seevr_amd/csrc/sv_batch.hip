@@ -401,10 +401,14 @@ BatchArgs batch_args(sv_batch *b, float dt, uint32_t *queue) {
     return A;
 }
 
-// Substeps of every scene on the context stream (no render vertices).
+// Substeps of every scene on the context stream (no render vertices).  The
+// scenes' hulls are concatenated, so their offsets pass 2^24 for a few
+// thousand scenes, where geom_data's float offsets (data.zig:468-473) are
+// no longer exact: the batch always reads the exact SV_GEOM_HULL_RANGE (a
+// scene alone has small, exact offsets, and must give the same result).
 void enqueue_substeps(sv_batch *b, uint32_t substeps, float sub_dt, uint32_t threads, uint32_t wgs_per_cu) {
     sv_ctx *c = b->ctx;
-    launch_hull_ranges(c, b->num_bodies);
+    launch_hull_ranges(c, b->num_bodies, true);
     fill_u32(c, b->queue, 0u, substeps);
     uint32_t grid = (uint32_t)b->cus * wgs_per_cu;
     if (grid > b->num_scenes) grid = b->num_scenes;
@@ -463,7 +467,7 @@ void free_batch(sv_batch *b) {
     drop_graph(b);
     void *ptrs[] = {b->sc_body, b->sc_order, b->sc_cnt, b->queue, b->abort_flag, b->sc_grav, b->snap,
                     b->d_ckpt, b->d_state, b->d_w2, b->d_prep, b->d_items, b->d_nc, b->d_adj,
-                    b->d_contrib};
+                    b->d_contrib, b->d_amat, b->d_amat32, b->d_adj32};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : b->d_ev)

@@ -23,7 +23,11 @@
 //   constants   bias (restitution, pre-sweep velocities), k_n (positions and
 //               rotations), mu (friction), r = p_contact - p_body; per-body
 //               sums are gathered in the sorted incidence order (so they are
-//               deterministic); material sums use atomics
+//               deterministic), and so are the friction and restitution
+//               adjoints: each incidence carries its own geom's share and
+//               the body's thread adds them, in that order, to an FP64 sum
+//               kept over the whole backward pass (geoms are 1:1 with
+//               bodies, src/data.zig:462-479)
 //   rotations   R = quat_mat(q0) -> quaternion adjoint
 //   gravity     v1 = v0 ld + g dt, w1 = w0 ad
 // The FP64 CPU restatement of the same adjoint is the checker.
@@ -38,7 +42,7 @@ namespace {
 constexpr uint32_t ACC = 9;       // per-contact constant adjoints: ra 3, rb 3, kn, bias, mu
 constexpr uint32_t DC_MAX = 512;  // contacts per scene in the differentiable form (LDS tile)
 constexpr uint32_t DINC_MAX = 2 * DC_MAX;
-constexpr uint32_t CONTRIB = 18;  // per incidence: p 3, v 3, w 3, R 9
+constexpr uint32_t CONTRIB = 20;  // per incidence: p 3, v 3, w 3, R 9, own geom's friction and restitution
 
 struct AdjArgs {
     const uint32_t *sc_body;
@@ -53,42 +57,76 @@ struct AdjArgs {
     const float *items;  // [S][iters * cs][TAPE_ITEM]
     const uint32_t *nc;  // [S]
     const float *inv_mass, *inv_i, *fric, *rest;
-    float *adj;          // [13N] in: after the substep, out: before it
-    float *afric, *arest;
-    float *contrib;
+    double *adj;         // [13N] in: after the substep, out: before it
+    double *afric, *arest;  // [N] per geom (= body), summed over every substep of the backward pass
+    double *contrib;
     uint32_t n, cs, iters;
     float dt, ld, ad;
 };
 
+// FP32 R diag(d) R^T v, as the forward sweep evaluates it (apply_inv_I)
 SVD f3 inv_I3(const m3 &R, f3 d, f3 v) { return mvec(R, hmul(d, mtvec(R, v))); }
-SVD void add_outer(m3 &A, f3 u, f3 w, float s) {
+
+// ---------------------------------------------------------------------------
+// Adjoint values are FP64.  The forward quantities an adjoint needs are
+// recomputed in FP32 from the recorded inputs, exactly as the forward sweep
+// computed them, so every branch (impulse clamp, stick or slip, the tangent
+// threshold) is the forward's own; only the reverse accumulation runs in
+// double.  The friction and restitution gradients are sums of terms that
+// cancel by orders of magnitude (a friction gradient of 1e-5 from terms of
+// 1e-2), which FP32 adjoint values cannot resolve to 1e-3.
+struct g3 {
+    double x, y, z;
+};
+SVD g3 gk(double x, double y, double z) { g3 r; r.x = x; r.y = y; r.z = z; return r; }
+SVD g3 G(f3 a) { return gk(a.x, a.y, a.z); }
+SVD g3 gzero() { return gk(0.0, 0.0, 0.0); }
+SVD g3 operator+(g3 a, g3 b) { return gk(a.x + b.x, a.y + b.y, a.z + b.z); }
+SVD g3 operator-(g3 a, g3 b) { return gk(a.x - b.x, a.y - b.y, a.z - b.z); }
+SVD g3 operator-(g3 a) { return gk(-a.x, -a.y, -a.z); }
+SVD g3 operator*(double s, g3 a) { return gk(s * a.x, s * a.y, s * a.z); }
+SVD double gdot(g3 a, g3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+SVD g3 gcross(g3 a, g3 b) { return gk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+SVD g3 ghmul(f3 a, g3 b) { return gk(a.x * b.x, a.y * b.y, a.z * b.z); }
+SVD g3 ldg(const double *p, uint32_t i) { return gk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+SVD void stg(double *p, uint32_t i, g3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
+// R (an FP32 rotation) times v, and its transpose
+SVD g3 gmvec(const m3 &m, g3 v) { return v.x * G(m.c0) + v.y * G(m.c1) + v.z * G(m.c2); }
+SVD g3 gmtvec(const m3 &m, g3 v) { return gk(gdot(G(m.c0), v), gdot(G(m.c1), v), gdot(G(m.c2), v)); }
+SVD g3 inv_I3g(const m3 &R, f3 d, g3 v) { return gmvec(R, ghmul(d, gmtvec(R, v))); }
+
+struct gm3 {
+    g3 c0, c1, c2;
+};
+SVD gm3 gm3_zero() {
+    gm3 m;
+    m.c0 = m.c1 = m.c2 = gzero();
+    return m;
+}
+SVD void add_outer(gm3 &A, g3 u, g3 w, double s) {
     A.c0 = A.c0 + (s * w.x) * u;
     A.c1 = A.c1 + (s * w.y) * u;
     A.c2 = A.c2 + (s * w.z) * u;
 }
 // adjoint of R in y = R diag(d) R^T x for the output adjoint yb, times s
-SVD void inv_I_rbar(m3 &Rb, const m3 &R, f3 d, f3 x, f3 yb, float s) {
-    add_outer(Rb, yb, hmul(d, mtvec(R, x)), s);
-    add_outer(Rb, x, hmul(d, mtvec(R, yb)), s);
-}
-SVD m3 m3_zero() {
-    m3 m;
-    m.c0 = m.c1 = m.c2 = mk(0.0f, 0.0f, 0.0f);
-    return m;
+SVD void inv_I_rbar(gm3 &Rb, const m3 &R, f3 d, g3 x, g3 yb, double s) {
+    add_outer(Rb, yb, ghmul(d, gmtvec(R, x)), s);
+    add_outer(Rb, x, ghmul(d, gmtvec(R, yb)), s);
 }
 
 struct ItemAdj {
-    f3 la, wa, lb, wb, ra, rb;
-    float old, kn, bias, mu;
-    m3 Ra, Rb;
+    g3 la, wa, lb, wb, ra, rb;
+    double old, kn, bias, mu;
+    gm3 Ra, Rb;
 };
 
-// Adjoint of apply_item (physics.comp:525-580) at recorded inputs.
-SVD void item_vjp(const Item &k, f3 la, f3 wa, f3 lb, f3 wb, float old, f3 gla2, f3 gwa2, f3 glb2, f3 gwb2,
-                  float gnl, ItemAdj &o) {
-    o.ra = o.rb = mk(0.0f, 0.0f, 0.0f);
-    o.kn = o.bias = o.mu = 0.0f;
-    o.Ra = o.Rb = m3_zero();
+// Adjoint of apply_item (physics.comp:525-580) at recorded inputs: the
+// forward part in FP32 (the sweep's values), the reverse part in FP64.
+SVD void item_vjp(const Item &k, f3 la, f3 wa, f3 lb, f3 wb, float old, g3 gla2, g3 gwa2, g3 glb2, g3 gwb2,
+                  double gnl, ItemAdj &o) {
+    o.ra = o.rb = gzero();
+    o.kn = o.bias = o.mu = 0.0;
+    o.Ra = o.Rb = gm3_zero();
     const f3 va = la + cross3(wa, k.ra), vb = lb + cross3(wb, k.rb);
     const float vn = dot3(vb - va, k.n);
     const float dln = (-vn + k.bias) / k.kn;
@@ -103,8 +141,9 @@ SVD void item_vjp(const Item &k, f3 la, f3 wa, f3 lb, f3 wb, float old, f3 gla2,
     const f3 dv = vb2 - va2;
     const f3 vt = dv - dot3(dv, k.n) * k.n;
     const float vtl = len3(vt);
-    f3 gla1 = gla2, gwa1 = gwa2, glb1 = glb2, gwb1 = gwb2, gdv = mk(0.0f, 0.0f, 0.0f);
-    float gnl_acc = gnl;
+    const g3 n = G(k.n), ra = G(k.ra), rb = G(k.rb);
+    g3 gla1 = gla2, gwa1 = gwa2, glb1 = glb2, gwb1 = gwb2, gdv = gzero();
+    double gnl_acc = gnl;
     if (vtl > 1e-6f) {
         const f3 td = mk(vt.x / vtl, vt.y / vtl, vt.z / vtl);
         const f3 rat = cross3(k.ra, td), rbt = cross3(k.rb, td);
@@ -114,70 +153,74 @@ SVD void item_vjp(const Item &k, f3 la, f3 wa, f3 lb, f3 wb, float old, f3 gla2,
         const float dlt = gmin(s1, s2);
         const f3 impt = dlt * td;
         const f3 ta = cross3(k.ra, impt), tb = cross3(k.rb, impt);
-        const f3 gta = inv_I3(k.Ra, k.iia, gwa2);
-        inv_I_rbar(o.Ra, k.Ra, k.iia, ta, gwa2, 1.0f);
-        f3 gimpt = cross3(gta, k.ra);
-        o.ra = o.ra + cross3(impt, gta);
-        const f3 gtb = -inv_I3(k.Rb, k.iib, gwb2);
-        inv_I_rbar(o.Rb, k.Rb, k.iib, tb, gwb2, -1.0f);
-        gimpt = gimpt + cross3(gtb, k.rb);
-        o.rb = o.rb + cross3(impt, gtb);
-        gimpt = (gimpt + k.ima * gla2) - k.imb * glb2;
-        const float gdlt = dot3(gimpt, td);
-        f3 gtd = dlt * gimpt;
-        float gs1 = 0.0f, gs2 = 0.0f;
+        const g3 tdg = G(td), impg = G(impt);
+        const g3 gta = inv_I3g(k.Ra, k.iia, gwa2);
+        inv_I_rbar(o.Ra, k.Ra, k.iia, G(ta), gwa2, 1.0);
+        g3 gimpt = gcross(gta, ra);
+        o.ra = o.ra + gcross(impg, gta);
+        const g3 gtb = -inv_I3g(k.Rb, k.iib, gwb2);
+        inv_I_rbar(o.Rb, k.Rb, k.iib, G(tb), gwb2, -1.0);
+        gimpt = gimpt + gcross(gtb, rb);
+        o.rb = o.rb + gcross(impg, gtb);
+        gimpt = (gimpt + (double)k.ima * gla2) - (double)k.imb * glb2;
+        const double gdlt = gdot(gimpt, tdg);
+        g3 gtd = (double)dlt * gimpt;
+        double gs1 = 0.0, gs2 = 0.0;
         if (s2 < s1) gs2 = gdlt; else gs1 = gdlt;
-        o.mu += gs2 * nl;
-        gnl_acc += gs2 * k.mu;
-        float gvtl = gs1 / kt;
-        const float gkt = -gs1 * s1 / kt;
-        const f3 grat = (2.0f * gkt) * Irat, grbt = (2.0f * gkt) * Irbt;
-        inv_I_rbar(o.Ra, k.Ra, k.iia, rat, rat, gkt);
-        inv_I_rbar(o.Rb, k.Rb, k.iib, rbt, rbt, gkt);
-        o.ra = o.ra + cross3(td, grat);
-        gtd = gtd + cross3(grat, k.ra);
-        o.rb = o.rb + cross3(td, grbt);
-        gtd = gtd + cross3(grbt, k.rb);
-        f3 gvt = (1.0f / vtl) * gtd;
-        gvtl += -dot3(gtd, vt) / (vtl * vtl);
-        gvt = gvt + (gvtl / vtl) * vt;
-        gdv = gvt - dot3(gvt, k.n) * k.n;
+        o.mu += gs2 * (double)nl;
+        gnl_acc += gs2 * (double)k.mu;
+        double gvtl = gs1 / (double)kt;
+        const double gkt = -gs1 * (double)s1 / (double)kt;
+        const g3 grat = (2.0 * gkt) * G(Irat), grbt = (2.0 * gkt) * G(Irbt);
+        inv_I_rbar(o.Ra, k.Ra, k.iia, G(rat), G(rat), gkt);
+        inv_I_rbar(o.Rb, k.Rb, k.iib, G(rbt), G(rbt), gkt);
+        o.ra = o.ra + gcross(tdg, grat);
+        gtd = gtd + gcross(grat, ra);
+        o.rb = o.rb + gcross(tdg, grbt);
+        gtd = gtd + gcross(grbt, rb);
+        const double il = 1.0 / (double)vtl;
+        const g3 vtg = G(vt);
+        g3 gvt = il * gtd;
+        gvtl += -gdot(gtd, vtg) * il * il;
+        gvt = gvt + (gvtl * il) * vtg;
+        gdv = gvt - gdot(gvt, n) * n;
     }
-    const f3 gvb2 = gdv, gva2 = -gdv;
+    const g3 gvb2 = gdv, gva2 = -gdv;
     gla1 = gla1 + gva2;
-    gwa1 = gwa1 + cross3(k.ra, gva2);
-    o.ra = o.ra + cross3(gva2, wa1);
+    gwa1 = gwa1 + gcross(ra, gva2);
+    o.ra = o.ra + gcross(gva2, G(wa1));
     glb1 = glb1 + gvb2;
-    gwb1 = gwb1 + cross3(k.rb, gvb2);
-    o.rb = o.rb + cross3(gvb2, wb1);
-    f3 gwa = gwa1;
-    const f3 gua = -inv_I3(k.Ra, k.iia, gwa1);
-    inv_I_rbar(o.Ra, k.Ra, k.iia, ua, gwa1, -1.0f);
-    f3 gimp = cross3(gua, k.ra);
-    o.ra = o.ra + cross3(imp, gua);
-    f3 gwb = gwb1;
-    const f3 gub = inv_I3(k.Rb, k.iib, gwb1);
-    inv_I_rbar(o.Rb, k.Rb, k.iib, ub, gwb1, 1.0f);
-    gimp = gimp + cross3(gub, k.rb);
-    o.rb = o.rb + cross3(imp, gub);
-    f3 gla = gla1, glb = glb1;
-    gimp = (gimp - k.ima * gla1) + k.imb * glb1;
-    const float gd2 = dot3(gimp, k.n);
+    gwb1 = gwb1 + gcross(rb, gvb2);
+    o.rb = o.rb + gcross(gvb2, G(wb1));
+    g3 gwa = gwa1;
+    const g3 gua = -inv_I3g(k.Ra, k.iia, gwa1);
+    inv_I_rbar(o.Ra, k.Ra, k.iia, G(ua), gwa1, -1.0);
+    const g3 impg = G(imp);
+    g3 gimp = gcross(gua, ra);
+    o.ra = o.ra + gcross(impg, gua);
+    g3 gwb = gwb1;
+    const g3 gub = inv_I3g(k.Rb, k.iib, gwb1);
+    inv_I_rbar(o.Rb, k.Rb, k.iib, G(ub), gwb1, 1.0);
+    gimp = gimp + gcross(gub, rb);
+    o.rb = o.rb + gcross(impg, gub);
+    g3 gla = gla1, glb = glb1;
+    gimp = (gimp - (double)k.ima * gla1) + (double)k.imb * glb1;
+    const double gd2 = gdot(gimp, n);
     gnl_acc += gd2;
-    float gold = -gd2;
-    const float gx = x < 0.0f ? 0.0f : gnl_acc;
+    double gold = -gd2;
+    const double gx = x < 0.0f ? 0.0 : gnl_acc;
     gold += gx;
-    const float gdln = gx;
-    o.bias += gdln / k.kn;
-    const float gvn = -gdln / k.kn;
-    o.kn += -gdln * dln / k.kn;
-    const f3 gvb = gvn * k.n, gva = -gvn * k.n;
+    const double gdln = gx;
+    o.bias += gdln / (double)k.kn;
+    const double gvn = -gdln / (double)k.kn;
+    o.kn += -gdln * (double)dln / (double)k.kn;
+    const g3 gvb = gvn * n, gva = -gvn * n;
     gla = gla + gva;
-    gwa = gwa + cross3(k.ra, gva);
-    o.ra = o.ra + cross3(gva, wa);
+    gwa = gwa + gcross(ra, gva);
+    o.ra = o.ra + gcross(gva, G(wa));
     glb = glb + gvb;
-    gwb = gwb + cross3(k.rb, gvb);
-    o.rb = o.rb + cross3(gvb, wb);
+    gwb = gwb + gcross(rb, gvb);
+    o.rb = o.rb + gcross(gvb, G(wb));
     o.la = gla;
     o.wa = gwa;
     o.lb = glb;
@@ -186,49 +229,50 @@ SVD void item_vjp(const Item &k, f3 la, f3 wa, f3 lb, f3 wb, float old, f3 gla2,
 }
 
 // quaternion adjoint from the adjoint A of R = quat_mat(q) (physics.comp:74-81)
-SVD void quat_mat_vjp(const float *q, const m3 &A, float *qb) {
-    const float x = q[0], y = q[1], z = q[2], w = q[3];
-    qb[0] += A.c0.y * 2.0f * y + A.c0.z * 2.0f * z + A.c1.x * 2.0f * y - A.c1.y * 4.0f * x + A.c1.z * 2.0f * w +
-             A.c2.x * 2.0f * z - A.c2.y * 2.0f * w - A.c2.z * 4.0f * x;
-    qb[1] += -A.c0.x * 4.0f * y + A.c0.y * 2.0f * x - A.c0.z * 2.0f * w + A.c1.x * 2.0f * x + A.c1.z * 2.0f * z +
-             A.c2.x * 2.0f * w + A.c2.y * 2.0f * z - A.c2.z * 4.0f * y;
-    qb[2] += -A.c0.x * 4.0f * z + A.c0.y * 2.0f * w + A.c0.z * 2.0f * x - A.c1.x * 2.0f * w - A.c1.y * 4.0f * z +
-             A.c1.z * 2.0f * y + A.c2.x * 2.0f * x + A.c2.y * 2.0f * y;
-    qb[3] += A.c0.y * 2.0f * z - A.c0.z * 2.0f * y - A.c1.x * 2.0f * z + A.c1.z * 2.0f * x + A.c2.x * 2.0f * y -
-             A.c2.y * 2.0f * x;
+SVD void quat_mat_vjp(const float *q, const gm3 &A, double *qb) {
+    const double x = q[0], y = q[1], z = q[2], w = q[3];
+    qb[0] += A.c0.y * 2.0 * y + A.c0.z * 2.0 * z + A.c1.x * 2.0 * y - A.c1.y * 4.0 * x + A.c1.z * 2.0 * w +
+             A.c2.x * 2.0 * z - A.c2.y * 2.0 * w - A.c2.z * 4.0 * x;
+    qb[1] += -A.c0.x * 4.0 * y + A.c0.y * 2.0 * x - A.c0.z * 2.0 * w + A.c1.x * 2.0 * x + A.c1.z * 2.0 * z +
+             A.c2.x * 2.0 * w + A.c2.y * 2.0 * z - A.c2.z * 4.0 * y;
+    qb[2] += -A.c0.x * 4.0 * z + A.c0.y * 2.0 * w + A.c0.z * 2.0 * x - A.c1.x * 2.0 * w - A.c1.y * 4.0 * z +
+             A.c1.z * 2.0 * y + A.c2.x * 2.0 * x + A.c2.y * 2.0 * y;
+    qb[3] += A.c0.y * 2.0 * z - A.c0.z * 2.0 * y - A.c1.x * 2.0 * z + A.c1.z * 2.0 * x + A.c2.x * 2.0 * y -
+             A.c2.y * 2.0 * x;
 }
 
-SVD void acc_rot(float *r9, const m3 &A) {
+SVD void acc_rot(double *r9, const gm3 &A) {
     r9[0] += A.c0.x; r9[1] += A.c0.y; r9[2] += A.c0.z;
     r9[3] += A.c1.x; r9[4] += A.c1.y; r9[5] += A.c1.z;
     r9[6] += A.c2.x; r9[7] += A.c2.y; r9[8] += A.c2.z;
 }
-SVD m3 ld_m3(const float *r9) {
-    m3 m;
-    m.c0 = mk(r9[0], r9[1], r9[2]);
-    m.c1 = mk(r9[3], r9[4], r9[5]);
-    m.c2 = mk(r9[6], r9[7], r9[8]);
+SVD gm3 ld_gm3(const double *r9) {
+    gm3 m;
+    m.c0 = gk(r9[0], r9[1], r9[2]);
+    m.c1 = gk(r9[3], r9[4], r9[5]);
+    m.c2 = gk(r9[6], r9[7], r9[8]);
     return m;
 }
 
 template <int T>
 __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
-    __shared__ float s_pb[3 * NB_MAX], s_qb[4 * NB_MAX], s_vb[3 * NB_MAX], s_wb[3 * NB_MAX];
-    __shared__ float s_Rb[9 * NB_MAX];
+    __shared__ double s_pb[3 * NB_MAX], s_qb[4 * NB_MAX], s_vb[3 * NB_MAX], s_wb[3 * NB_MAX];
+    __shared__ double s_Rb[9 * NB_MAX];
     __shared__ float s_v1[3 * NB_MAX], s_w1[3 * NB_MAX];
     __shared__ uint32_t s_key[DINC_MAX];
     __shared__ uint16_t s_rank[DINC_MAX];
     __shared__ uint16_t s_start[NB_MAX], s_deg[NB_MAX];
     __shared__ uint32_t s_rt[NB_MAX];
-    __shared__ float s_lb[DC_MAX];
-    __shared__ float s_acc[DC_MAX * ACC];
+    __shared__ double s_lb[DC_MAX];
+    __shared__ double s_acc[DC_MAX * ACC];
     __shared__ uint32_t s_q, s_next;
     const int tid = threadIdx.x, lane = tid & 63;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const size_t N = A.n;
     const float *pos0 = A.state, *quat0 = A.state + 3 * N, *vel0 = A.state + 7 * N, *omg0 = A.state + 10 * N;
-    float *ap = A.adj, *aq = A.adj + 3 * N, *av = A.adj + 7 * N, *aw = A.adj + 10 * N;
+    double *ap = A.adj, *aq = A.adj + 3 * N, *av = A.adj + 7 * N, *aw = A.adj + 10 * N;
     const float dt = A.dt;
+    (void)pos0;
 
     for (;;) {
         if (tid == 0) {
@@ -245,7 +289,7 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
         const uint32_t nc = A.nc[sc];
         const size_t cseg = (size_t)sc * A.cs;
 
-        // post-gravity velocities, and the integrate adjoint
+        // post-gravity velocities (FP32, as the forward), and the integrate adjoint
         for (uint32_t i = tid; i < ns; i += T) {
             const uint32_t b = bs + i;
             const bool dyn = A.inv_mass[b] > 0.0f;
@@ -256,10 +300,11 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
             }
             st3(s_v1, i, v0);
             st3(s_w1, i, w0);
-            f3 gp = ld3(ap, b), gv = ld3(av, b), gw = ld3(aw, b);
-            float g[4] = {aq[4 * b], aq[4 * b + 1], aq[4 * b + 2], aq[4 * b + 3]};
+            g3 gp = ldg(ap, b), gv = ldg(av, b), gw = ldg(aw, b);
+            double g[4] = {aq[4 * b], aq[4 * b + 1], aq[4 * b + 2], aq[4 * b + 3]};
             if (dyn) {
-                gv = gv + dt * gp;
+                const double hdt = dt;
+                gv = gv + hdt * gp;
                 const float *q = quat0 + 4 * (size_t)b;
                 const float ox = A.w2[3 * b], oy = A.w2[3 * b + 1], oz = A.w2[3 * b + 2];
                 const float qx = q[0], qy = q[1], qz = q[2], qw = q[3];
@@ -271,29 +316,30 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
                             pw = qw + 0.5f * dw * dt;
                 const float l = __builtin_sqrtf(px * px + py * py + pz * pz + pw * pw);
                 if (l > 1e-8f) {
-                    const float ux = px / l, uy = py / l, uz = pz / l, uw = pw / l;
-                    const float d = g[0] * ux + g[1] * uy + g[2] * uz + g[3] * uw;
-                    g[0] = (g[0] - d * ux) / l;
-                    g[1] = (g[1] - d * uy) / l;
-                    g[2] = (g[2] - d * uz) / l;
-                    g[3] = (g[3] - d * uw) / l;
+                    const double ux = px / l, uy = py / l, uz = pz / l, uw = pw / l, il = 1.0 / (double)l;
+                    const double d = g[0] * ux + g[1] * uy + g[2] * uz + g[3] * uw;
+                    g[0] = (g[0] - d * ux) * il;
+                    g[1] = (g[1] - d * uy) * il;
+                    g[2] = (g[2] - d * uz) * il;
+                    g[3] = (g[3] - d * uw) * il;
                 }
-                const float hx = 0.5f * dt * g[0], hy = 0.5f * dt * g[1], hz = 0.5f * dt * g[2], hw = 0.5f * dt * g[3];
-                gw = gw + mk(hx * qw - hy * qz + hz * qy - hw * qx, hx * qz + hy * qw - hz * qx - hw * qy,
+                const double hh = 0.5 * hdt;
+                const double hx = hh * g[0], hy = hh * g[1], hz = hh * g[2], hw = hh * g[3];
+                gw = gw + gk(hx * qw - hy * qz + hz * qy - hw * qx, hx * qz + hy * qw - hz * qx - hw * qy,
                              -hx * qy + hy * qx + hz * qw - hw * qz);
-                const float n0 = g[0] + hy * oz - hz * oy - hw * ox;
-                const float n1 = g[1] - hx * oz + hz * ox - hw * oy;
-                const float n2 = g[2] + hx * oy - hy * ox - hw * oz;
-                const float n3 = g[3] + hx * ox + hy * oy + hz * oz;
+                const double n0 = g[0] + hy * oz - hz * oy - hw * ox;
+                const double n1 = g[1] - hx * oz + hz * ox - hw * oy;
+                const double n2 = g[2] + hx * oy - hy * ox - hw * oz;
+                const double n3 = g[3] + hx * ox + hy * oy + hz * oz;
                 g[0] = n0; g[1] = n1; g[2] = n2; g[3] = n3;
             }
-            st3(s_pb, i, gp);
-            st3(s_vb, i, gv);
-            st3(s_wb, i, gw);
+            stg(s_pb, i, gp);
+            stg(s_vb, i, gv);
+            stg(s_wb, i, gw);
 #pragma unroll
             for (int k = 0; k < 4; k++) s_qb[4 * i + k] = g[k];
 #pragma unroll
-            for (int k = 0; k < 9; k++) s_Rb[9 * i + k] = 0.0f;
+            for (int k = 0; k < 9; k++) s_Rb[9 * i + k] = 0.0;
             s_rt[i] = 0u;
             s_deg[i] = 0;  // bodies without incidences keep 0
         }
@@ -305,9 +351,9 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
             const bool live = !(flags & F_SKIP);
             s_key[2 * c] = (live && (flags & F_DYN_A)) ? (((a - bs) << 12) | (2 * c)) : 0xFFFFFFFFu;
             s_key[2 * c + 1] = (live && (flags & F_DYN_B)) ? (((b - bs) << 12) | (2 * c + 1)) : 0xFFFFFFFFu;
-            s_lb[c] = 0.0f;
+            s_lb[c] = 0.0;
 #pragma unroll
-            for (int k = 0; k < (int)ACC; k++) s_acc[c * ACC + k] = 0.0f;
+            for (int k = 0; k < (int)ACC; k++) s_acc[c * ACC + k] = 0.0;
         }
         uint32_t L = 2;
         while (L < 2 * nc) L <<= 1;
@@ -388,23 +434,23 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
                                                                        __HIP_MEMORY_SCOPE_WORKGROUP) == rb_pos);
                     if (ready) {
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                        const f3 z = mk(0.0f, 0.0f, 0.0f);
+                        const g3 z = gzero();
                         ItemAdj o;
                         item_vjp(it, t_la, t_wa, t_lb, t_wb, t_old,
-                                 it.dyn_a ? ld3(s_vb, it.a) : z, it.dyn_a ? ld3(s_wb, it.a) : z,
-                                 it.dyn_b ? ld3(s_vb, it.b) : z, it.dyn_b ? ld3(s_wb, it.b) : z, s_lb[it.c], o);
+                                 it.dyn_a ? ldg(s_vb, it.a) : z, it.dyn_a ? ldg(s_wb, it.a) : z,
+                                 it.dyn_b ? ldg(s_vb, it.b) : z, it.dyn_b ? ldg(s_wb, it.b) : z, s_lb[it.c], o);
                         if (it.dyn_a) {
-                            st3(s_vb, it.a, o.la);
-                            st3(s_wb, it.a, o.wa);
+                            stg(s_vb, it.a, o.la);
+                            stg(s_wb, it.a, o.wa);
                             acc_rot(s_Rb + 9 * it.a, o.Ra);
                         }
                         if (it.dyn_b) {
-                            st3(s_vb, it.b, o.lb);
-                            st3(s_wb, it.b, o.wb);
+                            stg(s_vb, it.b, o.lb);
+                            stg(s_wb, it.b, o.wb);
                             acc_rot(s_Rb + 9 * it.b, o.Rb);
                         }
                         s_lb[it.c] = o.old;
-                        float *acc = s_acc + it.c * ACC;
+                        double *acc = s_acc + it.c * ACC;
                         acc[0] += o.ra.x; acc[1] += o.ra.y; acc[2] += o.ra.z;
                         acc[3] += o.rb.x; acc[4] += o.rb.y; acc[5] += o.rb.z;
                         acc[6] += o.kn; acc[7] += o.bias; acc[8] += o.mu;
@@ -440,9 +486,9 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
             const uint32_t ga_b = __float_as_uint(P[P_A]), gb_b = __float_as_uint(P[P_B]);
             const uint32_t gga = __float_as_uint(P[P_GEOM_A]), ggb = __float_as_uint(P[P_GEOM_B]);
             const uint32_t a = ga_b - bs, b = gb_b - bs;
-            const float *acc = s_acc + c * ACC;
-            f3 gra = mk(acc[0], acc[1], acc[2]), grb = mk(acc[3], acc[4], acc[5]);
-            const float gkn = acc[6], gbias = acc[7], gmu = acc[8];
+            const double *acc = s_acc + c * ACC;
+            g3 gra = gk(acc[0], acc[1], acc[2]), grb = gk(acc[3], acc[4], acc[5]);
+            const double gkn = acc[6], gbias = acc[7], gmu = acc[8];
             const f3 n = mk(P[P_N], P[P_N + 1], P[P_N + 2]);
             const f3 ra = mk(P[P_RA], P[P_RA + 1], P[P_RA + 2]), rb = mk(P[P_RB], P[P_RB + 1], P[P_RB + 2]);
             const f3 w1a = ld3(s_w1, a), w1b = ld3(s_w1, b);
@@ -451,74 +497,115 @@ __global__ void __launch_bounds__(T) k_batch_adjoint(AdjArgs A) {
             const float ea = A.rest[gga], eb = A.rest[ggb];
             const float e = gmax(ea, eb);
             const float term = -vn < 0.0f ? 0.0f : -vn;
-            const float ge = gbias * term;
-            const float gvn = -vn < 0.0f ? 0.0f : -gbias * e;
-            atomicAdd(ea < eb ? A.arest + ggb : A.arest + gga, ge);
-            const f3 gvb = gvn * n, gva = -gvn * n;
+            const double ge = gbias * (double)term;
+            const double gvn = -vn < 0.0f ? 0.0 : -gbias * (double)e;
+            // e = max(e_a, e_b) (physics.comp:519): the larger one takes it
+            // (a on ties, as GLSL max returns its first argument)
+            const double ge_a = ea < eb ? 0.0 : ge, ge_b = ea < eb ? ge : 0.0;
+            const g3 ng = G(n), rag = G(ra), rbg = G(rb);
+            const g3 gvb = gvn * ng, gva = -gvn * ng;
             const m3 Ra = quat_mat_at(quat0, ga_b), Rb = quat_mat_at(quat0, gb_b);
             const f3 iia = ld3(A.inv_i, ga_b), iib = ld3(A.inv_i, gb_b);
             const f3 ran = cross3(ra, n), rbn = cross3(rb, n);
-            const f3 gran = (2.0f * gkn) * inv_I3(Ra, iia, ran), grbn = (2.0f * gkn) * inv_I3(Rb, iib, rbn);
-            gra = gra + cross3(gva, w1a) + cross3(n, gran);
-            grb = grb + cross3(gvb, w1b) + cross3(n, grbn);
+            const g3 gran = (2.0 * gkn) * G(inv_I3(Ra, iia, ran)), grbn = (2.0 * gkn) * G(inv_I3(Rb, iib, rbn));
+            gra = gra + gcross(gva, G(w1a)) + gcross(ng, gran);
+            grb = grb + gcross(gvb, G(w1b)) + gcross(ng, grbn);
+            // mu = sqrt(f_a f_b) (physics.comp:557): dmu/df_a = f_b / (2 mu)
             const float mu = P[P_MU];
-            if (mu > 0.0f) {
-                atomicAdd(A.afric + gga, gmu * A.fric[ggb] / (2.0f * mu));
-                atomicAdd(A.afric + ggb, gmu * A.fric[gga] / (2.0f * mu));
-            }
-            float *ca = A.contrib + (2 * cseg + 2 * c) * CONTRIB;
-            float *cb = ca + CONTRIB;
-            m3 Rba = m3_zero(), Rbb = m3_zero();
-            inv_I_rbar(Rba, Ra, iia, ran, ran, gkn);
-            inv_I_rbar(Rbb, Rb, iib, rbn, rbn, gkn);
-            st3(ca, 0, -gra);
-            st3(ca, 1, gva);
-            st3(ca, 2, cross3(ra, gva));
-            st3(cb, 0, -grb);
-            st3(cb, 1, gvb);
-            st3(cb, 2, cross3(rb, gvb));
+            const double gf_a = mu > 0.0f ? gmu * (double)A.fric[ggb] / (2.0 * (double)mu) : 0.0;
+            const double gf_b = mu > 0.0f ? gmu * (double)A.fric[gga] / (2.0 * (double)mu) : 0.0;
+            double *ca = A.contrib + (2 * cseg + 2 * c) * CONTRIB;
+            double *cb = ca + CONTRIB;
+            gm3 Rba = gm3_zero(), Rbb = gm3_zero();
+            inv_I_rbar(Rba, Ra, iia, G(ran), G(ran), gkn);
+            inv_I_rbar(Rbb, Rb, iib, G(rbn), G(rbn), gkn);
+            stg(ca, 0, -gra);
+            stg(ca, 1, gva);
+            stg(ca, 2, gcross(rag, gva));
+            stg(cb, 0, -grb);
+            stg(cb, 1, gvb);
+            stg(cb, 2, gcross(rbg, gvb));
 #pragma unroll
             for (int k = 0; k < 9; k++) {
-                ca[9 + k] = 0.0f;
-                cb[9 + k] = 0.0f;
+                ca[9 + k] = 0.0;
+                cb[9 + k] = 0.0;
             }
             acc_rot(ca + 9, Rba);
             acc_rot(cb + 9, Rbb);
+            ca[18] = gf_a;
+            ca[19] = ge_a;
+            cb[18] = gf_b;
+            cb[19] = ge_b;
         }
         __syncthreads();
 
         // per body: gather in incidence order, rotation and gravity adjoints
         for (uint32_t i = tid; i < ns; i += T) {
             const uint32_t b = bs + i;
-            f3 gp = ld3(s_pb, i), gv = ld3(s_vb, i), gw = ld3(s_wb, i);
-            float *R9 = s_Rb + 9 * i;
+            g3 gp = ldg(s_pb, i), gv = ldg(s_vb, i), gw = ldg(s_wb, i);
+            double *R9 = s_Rb + 9 * i;
             const uint32_t deg = s_deg[i];
             if (deg) {
                 const uint32_t k0 = s_start[i];
+                double gfr = A.afric[b], gre = A.arest[b];
                 for (uint32_t p = k0; p < k0 + deg; p++) {
                     const uint32_t slot = s_key[p] & 0xFFFu;
-                    const float *cc = A.contrib + (2 * cseg + slot) * CONTRIB;
-                    gp = gp + ld3(cc, 0);
-                    gv = gv + ld3(cc, 1);
-                    gw = gw + ld3(cc, 2);
+                    const double *cc = A.contrib + (2 * cseg + slot) * CONTRIB;
+                    gp = gp + ldg(cc, 0);
+                    gv = gv + ldg(cc, 1);
+                    gw = gw + ldg(cc, 2);
 #pragma unroll
                     for (int k = 0; k < 9; k++) R9[k] += cc[9 + k];
+                    gfr += cc[18];
+                    gre += cc[19];
                 }
+                A.afric[b] = gfr;
+                A.arest[b] = gre;
+            } else if (A.inv_mass[b] == 0.0f && A.inv_i[3 * b] == 0.0f && A.inv_i[3 * b + 1] == 0.0f &&
+                       A.inv_i[3 * b + 2] == 0.0f) {
+                // a static body has no incidences (it orders nothing): its
+                // material shares are gathered from the contacts, in order
+                double gfr = A.afric[b], gre = A.arest[b];
+                for (uint32_t c = 0; c < nc; c++) {
+                    const float *P = A.prep + (cseg + c) * CPREP_WORDS;
+                    if (__float_as_uint(P[P_FLAGS]) & F_SKIP) continue;
+                    const double *ca = A.contrib + (2 * cseg + 2 * c) * CONTRIB;
+                    if (__float_as_uint(P[P_A]) == b) {
+                        gfr += ca[18];
+                        gre += ca[19];
+                    }
+                    if (__float_as_uint(P[P_B]) == b) {
+                        gfr += ca[CONTRIB + 18];
+                        gre += ca[CONTRIB + 19];
+                    }
+                }
+                A.afric[b] = gfr;
+                A.arest[b] = gre;
             }
-            float qb[4] = {s_qb[4 * i], s_qb[4 * i + 1], s_qb[4 * i + 2], s_qb[4 * i + 3]};
-            quat_mat_vjp(quat0 + 4 * (size_t)b, ld_m3(R9), qb);
+            double qb[4] = {s_qb[4 * i], s_qb[4 * i + 1], s_qb[4 * i + 2], s_qb[4 * i + 3]};
+            quat_mat_vjp(quat0 + 4 * (size_t)b, ld_gm3(R9), qb);
             if (A.inv_mass[b] > 0.0f) {
-                gv = A.ld * gv;
-                gw = A.ad * gw;
+                gv = (double)A.ld * gv;
+                gw = (double)A.ad * gw;
             }
-            st3(ap, b, gp);
-            st3(av, b, gv);
-            st3(aw, b, gw);
+            stg(ap, b, gp);
+            stg(av, b, gv);
+            stg(aw, b, gw);
 #pragma unroll
             for (int k = 0; k < 4; k++) aq[4 * b + k] = qb[k];
         }
         __syncthreads();
     }
+}
+
+__global__ void k_to_float(const double *in, float *out, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
+__global__ void k_to_double(const float *in, double *out, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (double)in[i];
 }
 
 typedef void (*AdjKernel)(AdjArgs);
@@ -587,7 +674,10 @@ int sv_batch_diff_reserve(sv_batch *b, uint32_t max_frames, uint32_t substeps) {
     for (int k = 0; k < 5 && e == hipSuccess; k++)
         if (!b->d_ev[k]) e = hipEventCreateWithFlags(&b->d_ev[k], hipEventDisableTiming);
     if (e == hipSuccess) e = dmalloc(b->d_adj, 13 * N);
+    if (e == hipSuccess) e = dmalloc(b->d_adj32, 13 * N);
     if (e == hipSuccess) e = dmalloc(b->d_contrib, 2 * S * cs * sv::CONTRIB);
+    if (e == hipSuccess) e = dmalloc(b->d_amat, 2 * N);
+    if (e == hipSuccess) e = dmalloc(b->d_amat32, 2 * N);
     if (e != hipSuccess) {
         b->d_frames = 0;
         return set_error(SV_E_NOMEM, "sv_batch_diff_reserve: device allocation failed");
@@ -628,19 +718,18 @@ int sv_batch_diff_backward(sv_batch *b, uint32_t frames, uint32_t substeps, floa
     const size_t iters = b->d_iters;
     const float sub_dt = dt / (float)substeps;
     hipError_t e = hipSuccess;
-    if (grad_state) e = hipMemcpyAsync(b->d_adj, grad_state, N13 * sizeof(float), hipMemcpyDefault, st);
-    else sv::fill_u32(c, reinterpret_cast<uint32_t *>(b->d_adj), 0u, N13);
-    float *afric = nullptr, *arest = nullptr;
-    if (e == hipSuccess) e = hipMalloc(&afric, (N ? N : 1) * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&arest, (N ? N : 1) * sizeof(float));
-    if (e != hipSuccess) {
-        if (afric) (void)hipFree(afric);
-        if (arest) (void)hipFree(arest);
-        return sv::batch_dev_fail(e, "sv_batch_diff_backward: setup");
+    if (grad_state) {
+        e = hipMemcpyAsync(b->d_adj32, grad_state, N13 * sizeof(float), hipMemcpyDefault, st);
+        if (e == hipSuccess)
+            hipLaunchKernelGGL(sv::k_to_double, dim3(sv::div_up(N13, 256)), dim3(256), 0, st, b->d_adj32, b->d_adj,
+                               (uint32_t)N13);
+    } else {
+        sv::fill_u32(c, reinterpret_cast<uint32_t *>(b->d_adj), 0u, 2 * N13);  // N13 doubles
     }
-    sv::fill_u32(c, reinterpret_cast<uint32_t *>(afric), 0u, N);
-    sv::fill_u32(c, reinterpret_cast<uint32_t *>(arest), 0u, N);
-    sv::launch_hull_ranges(c, b->num_bodies);
+    if (e != hipSuccess) return sv::batch_dev_fail(e, "sv_batch_diff_backward: setup");
+    double *afric = b->d_amat, *arest = b->d_amat + N;
+    sv::fill_u32(c, reinterpret_cast<uint32_t *>(b->d_amat), 0u, 4 * N);  // 2N doubles
+    sv::launch_hull_ranges(c, b->num_bodies, true);  // exact offsets, as the forward (enqueue_substeps)
     uint32_t grid = (uint32_t)b->cus * b->wgs_per_cu;
     if (grid > S) grid = (uint32_t)S;
     if (grid == 0) grid = 1;
@@ -704,16 +793,22 @@ int sv_batch_diff_backward(sv_batch *b, uint32_t frames, uint32_t substeps, floa
         }
         if (e == hipSuccess) e = hipEventRecord(ev_adj[k], ast);
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(out_state, b->d_adj, N13 * sizeof(float), hipMemcpyDefault, ast);
-    if (e == hipSuccess) e = hipMemcpyAsync(out_friction, afric, N * sizeof(float), hipMemcpyDefault, ast);
-    if (e == hipSuccess) e = hipMemcpyAsync(out_restitution, arest, N * sizeof(float), hipMemcpyDefault, ast);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(sv::k_to_float, dim3(sv::div_up(N13, 256)), dim3(256), 0, ast, b->d_adj, b->d_adj32,
+                           (uint32_t)N13);
+        e = hipMemcpyAsync(out_state, b->d_adj32, N13 * sizeof(float), hipMemcpyDefault, ast);
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(sv::k_to_float, dim3(sv::div_up(2 * N, 256)), dim3(256), 0, ast, b->d_amat, b->d_amat32,
+                           (uint32_t)(2 * N));
+        e = hipMemcpyAsync(out_friction, b->d_amat32, N * sizeof(float), hipMemcpyDefault, ast);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out_restitution, b->d_amat32 + N, N * sizeof(float), hipMemcpyDefault, ast);
     if (e == hipSuccess) e = hipEventRecord(ev_misc, ast);
     if (e == hipSuccess) e = hipStreamWaitEvent(st, ev_misc, 0);
     if (e == hipSuccess) e = copy_state(b, b->d_ckpt + (size_t)frames * N13, false);
     int rc = e == hipSuccess ? sv::batch_finish(b) : sv::batch_dev_fail(e, "sv_batch_diff_backward");
     (void)hipStreamSynchronize(ast);
-    (void)hipFree(afric);
-    (void)hipFree(arest);
     return rc;
 }
 

@@ -42,6 +42,8 @@ def lib() -> C.CDLL:
         _lib.adj_forward.argtypes = [C.POINTER(AdjScene), dp]
         _lib.adj_backward.restype = None
         _lib.adj_backward.argtypes = [C.POINTER(AdjScene), dp, dp, dp, dp, dp, dp, dp]
+        _lib.adj_set_forward32.restype = None
+        _lib.adj_set_forward32.argtypes = [C.c_int]
     return _lib
 
 
@@ -104,6 +106,69 @@ class Problem:
             lib().adj_backward(C.byref(sc), _p(tape), _p(adj["pos"]), _p(adj["quat"]), _p(adj["vel"]),
                                _p(adj["omega"]), _p(af), _p(ae))
         return adj, af, ae
+
+
+    def grad_along(self, starts: list[dict], fric, rest, gout: dict, forward32: bool = True):
+        """The adjoint linearised along a given trajectory: starts[s] is the
+        state at the start of substep s (for instance an FP32 rollout's, the
+        device's own), each substep's Jacobian is taken there, and gout is the
+        adjoint of the state after the last substep.  forward32 evaluates the
+        forward values and their branches as the FP32 sweep does
+        (adj_set_forward32), the reverse in FP64."""
+        fric = np.ascontiguousarray(fric, np.float64)
+        rest = np.ascontiguousarray(rest, np.float64)
+        L = lib()
+        L.adj_set_forward32(1 if forward32 else 0)
+        try:
+            tapes = []
+            for st0, c in zip(starts, self.contacts):
+                st = {k: np.array(st0[k], np.float64) for k in STATE}
+                sc = self._scene(st, fric, rest, c)
+                tape = np.zeros(L.adj_tape_doubles(C.byref(sc)), np.float64)
+                L.adj_forward(C.byref(sc), _p(tape))
+                tapes.append(tape)
+            adj = {k: np.array(gout[k], np.float64) for k in STATE}
+            af = np.zeros(self.n)
+            ae = np.zeros(self.n)
+            for st0, tape, c in zip(reversed(starts), reversed(tapes), reversed(self.contacts)):
+                st = {k: np.array(st0[k], np.float64) for k in STATE}
+                sc = self._scene(st, fric, rest, c)
+                L.adj_backward(C.byref(sc), _p(tape), _p(adj["pos"]), _p(adj["quat"]), _p(adj["vel"]),
+                               _p(adj["omega"]), _p(af), _p(ae))
+            return adj, af, ae
+        finally:
+            L.adj_set_forward32(0)
+
+
+def record_fp32(world, substeps: int, dt_sub: float, g) -> tuple[dict, list[dict], dict, list[dict]]:
+    """From an FP32 oracle World (the device's trajectory, bit for bit), run
+    `substeps` oracle substeps recording each substep's contacts and start
+    state.  Returns (initial state, contacts, statics, starts) in FP64."""
+    a = world.a
+    n = world.n
+    starts, contacts = [], []
+    statics = {"inv_mass": a["BODY_INV_MASS"][:n].astype(np.float64),
+               "inv_i": a["BODY_INV_INERTIA"][: 3 * n].astype(np.float64),
+               "fric": a["GEOM_FRICTION"][:n].astype(np.float64), "rest": a["GEOM_RESTITUTION"][:n].astype(np.float64)}
+    for _ in range(substeps):
+        starts.append({"pos": a["BODY_POS"][: 3 * n].astype(np.float64), "quat": a["BODY_QUAT"][: 4 * n].astype(np.float64),
+                       "vel": a["BODY_VEL"][: 3 * n].astype(np.float64),
+                       "omega": a["BODY_OMEGA"][: 3 * n].astype(np.float64)})
+        world.oracle_stage(8)
+        world.oracle_stage(0, dt=dt_sub, g=g)
+        world.oracle_stage(1)
+        world.oracle_stage(2)
+        world.oracle_stage(3)
+        nc = world.ncontacts()
+        contacts.append({"nc": nc, "a": a["CONTACT_GEOM_A"][:nc].astype(np.uint32).copy(),
+                         "b": a["CONTACT_GEOM_B"][:nc].astype(np.uint32).copy(),
+                         "cp": a["CONTACT_POS"][: 3 * nc].astype(np.float64),
+                         "n": a["CONTACT_NORMAL"][: 3 * nc].astype(np.float64),
+                         "pen": a["CONTACT_PENETRATION"][:nc].astype(np.float64)})
+        world.oracle_stage(4)
+        world.oracle_stage(5, dt=dt_sub)
+        world.oracle_stage(6, dt=dt_sub)
+    return dict(starts[0]), contacts, statics, starts
 
 
 def record_base(world, substeps: int, dt_sub: float, g) -> tuple[dict, list[dict], dict]:
