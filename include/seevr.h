@@ -53,7 +53,8 @@ enum {
 enum {
     SV_STATUS_PAIR_OVERFLOW = 1u << 0,    /* pairs found > max_pairs        */
     SV_STATUS_CONTACT_OVERFLOW = 1u << 1, /* contacts found > max_contacts  */
-    SV_STATUS_SOLVER_TIMEOUT = 1u << 2    /* ordered PGS wait expired       */
+    SV_STATUS_SOLVER_TIMEOUT = 1u << 2,   /* ordered PGS wait expired       */
+    SV_STATUS_SORT_TIMEOUT = 1u << 3      /* radix-sort look-back expired   */
 };
 
 /* ---- buffer ids --------------------------------------------------------
@@ -221,7 +222,7 @@ typedef struct sv_bvh_arrays {
 } sv_bvh_arrays;
 
 /* Largest body count of sv_bvh_build_bodies_noalloc: its scratch is a fixed
- * array of the library's code object (about 14 MB per GPU, held until the
+ * array of the library's code object (about 15 MB per GPU, held until the
  * process ends). */
 #define SV_NOALLOC_MAX_BODIES (1u << 20)
 

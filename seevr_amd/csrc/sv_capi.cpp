@@ -143,6 +143,10 @@ int check_status(sv_ctx *c) {
                      hipMemcpyDeviceToHost));
     if (status & SV_STATUS_SOLVER_TIMEOUT)
         return fail(SV_E_TIMEOUT, "ordered PGS made no progress for 2 s and was aborted");
+    if (status & SV_STATUS_SORT_TIMEOUT) {
+        (void)sv::sort_take_abort(c);
+        return fail(SV_E_TIMEOUT, "radix sort look-back made no progress for 2 s and was aborted");
+    }
     return SV_OK;
 }
 
@@ -466,7 +470,10 @@ int bvh_build_run(sv_ctx *c, uint32_t n, bool bodies) {
         sv::StageTimer t(c, names[st]);
         sv::bvh_stage(c, st, n, bodies);
     }
-    return sync_and_finish(c);
+    int rc = sync_and_finish(c);
+    if (rc == SV_OK && sv::sort_take_abort(c))
+        rc = fail(SV_E_TIMEOUT, "bvh build: radix sort look-back made no progress for 2 s and was aborted");
+    return rc;
 }
 }  // namespace sv
 

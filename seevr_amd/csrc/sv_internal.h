@@ -80,12 +80,29 @@ struct Scratch {
 enum { SWEEP_SEQ = 0, SWEEP_COHERENT = 1, SWEEP_FENCED = 2, SWEEP_PIPE = 3, SWEEP_SEQ_BASIC = 4, SWEEP_PAIR = 5 };
 
 // Keys per radix-sort tile and values per scan tile (sv_prims.hip), and the
-// scratch words the sort (digit histograms, their scan, a count word) and
-// the scan (one partial per tile) take for n keys or values.
-constexpr uint32_t SORT_TILE_KEYS = 2048;
+// scratch words the sort and the scan (one partial per tile) take for n
+// keys or values.  Sort scratch, in this order: per pass the digit counts
+// being accumulated, per pass the digit offsets, control words (done
+// counter, tile counter per pass, abort word), and per pass, tile and digit
+// one look-back status word.
+constexpr uint32_t SORT_TILE_SMALL = 2048, SORT_TILE_LARGE = 8192;  // keys per tile, by sort size
+constexpr uint32_t SORT_SMALL_MAX = 1u << 18;                       // largest sort on small tiles
 constexpr uint32_t SCAN_TILE_VALUES = 1024;
-constexpr size_t sort_tiles(size_t n) { return (n + SORT_TILE_KEYS - 1) / SORT_TILE_KEYS; }
-constexpr size_t sort_hist_words(size_t n) { return 2 * 256 * sort_tiles(n) + 1; }
+constexpr uint32_t SORT_MAX_PASSES = 4;             // 32-bit keys, 8-bit digits
+constexpr uint32_t SORT_MAX_KEYS = (1u << 30) - 1;  // counts share a word with 2 flag bits
+constexpr size_t SORT_ACC_OFF = 0, SORT_GOFF_OFF = SORT_MAX_PASSES * 256, SORT_CTL_OFF = 2 * SORT_MAX_PASSES * 256;
+constexpr size_t SORT_CTL_WORDS = 64;
+constexpr size_t SORT_STATUS_OFF = SORT_CTL_OFF + SORT_CTL_WORDS;
+enum : uint32_t { SORT_CTL_DONE = 0, SORT_CTL_TILE = 1, SORT_CTL_ABORT = 8 };
+// Status words are sized for the most tiles any sort of at most n keys
+// uses (small tiles up to SORT_SMALL_MAX keys, large above), so that
+// scratch sized for a capacity serves every smaller sort.
+constexpr size_t sort_tiles(size_t n) {
+    const size_t small = ((n < SORT_SMALL_MAX ? n : SORT_SMALL_MAX) + SORT_TILE_SMALL - 1) / SORT_TILE_SMALL;
+    const size_t large = (n + SORT_TILE_LARGE - 1) / SORT_TILE_LARGE;
+    return small > large ? small : large;
+}
+constexpr size_t sort_hist_words(size_t n) { return SORT_STATUS_OFF + (size_t)SORT_MAX_PASSES * 256 * sort_tiles(n); }
 constexpr size_t scan_part_words(size_t n) { return (n + SCAN_TILE_VALUES - 1) / SCAN_TILE_VALUES + 1; }
 // Centroids per partial of the LBVH's bounds reduction (sv_bvh.hip): the
 // partials take 6 floats per tile.
@@ -214,6 +231,9 @@ void scan_exclusive(sv_ctx *c, const uint32_t *in, uint32_t *out, const uint32_t
 // and 1 when in (k1, v1).
 int radix_sort(sv_ctx *c, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1,
                const uint32_t *n_dev, uint32_t n_max, int key_bits);
+// After a synchronisation: whether a sort's look-back wait expired since the
+// last call (the sort's abort word, cleared by the call).
+bool sort_take_abort(sv_ctx *c);
 void fill_u32(sv_ctx *c, uint32_t *p, uint32_t value, size_t n);
 void ensure_u32(uint32_t *&p, uint32_t &cap, uint32_t need);
 // Allocate scan/sort scratch for the largest sizes a context can use.

@@ -15,20 +15,22 @@ namespace {
 
 constexpr size_t align64(size_t words) { return (words + 63) / 64 * 64; }  // 256-byte parts
 
-// Word offsets of the parts of the scratch array for n bodies (each part
-// grows with n, so the layout for any n <= the maximum fits the array).
+// Word offsets of the parts of the scratch array for n bodies.  The sort's
+// scratch comes first at its size for the maximum, so that its control
+// words (a completion counter and tile counters that every sort leaves at
+// zero) sit at one address for every n; the other parts grow with n, so the
+// layout for any n <= the maximum fits the array.
 struct Carve {
-    size_t morton_temp, sort_temp, flags, bounds, hist, scan, total;
+    size_t hist, morton_temp, sort_temp, flags, bounds, total;
 };
 constexpr Carve carve(size_t n) {
     Carve c{};
     size_t o = 0;
+    c.hist = o; o += align64(sv::sort_hist_words(SV_NOALLOC_MAX_BODIES));             // sort histograms, control, look-back
     c.morton_temp = o; o += align64(n);                                                // sort ping-pong keys
     c.sort_temp = o; o += align64(n);                                                  // sort ping-pong values
     c.flags = o; o += align64(n);                                                      // refit arrival counters
     c.bounds = o; o += align64(6 * ((n + sv::BOUNDS_TILE - 1) / sv::BOUNDS_TILE));     // bounds partials
-    c.hist = o; o += align64(sv::sort_hist_words(n));                                  // digit histograms
-    c.scan = o; o += align64(sv::scan_part_words(256 * sv::sort_tiles(n)));            // their scan's partials
     c.total = o;
     return c;
 }
@@ -108,9 +110,7 @@ extern "C" int sv_bvh_build_bodies_noalloc(const float *aabb_min, const float *a
     v.buf[SV_BVH_SORT_TEMP] = base + cv.sort_temp;
     v.buf[SV_BVH_FLAGS] = base + cv.flags;  // cleared by the build's step 6
     v.buf[SV_BVH_TEMP] = base + cv.bounds;
-    v.s.sort_hist = base + cv.hist;  // every word read is written by the same pass first
-    v.s.sort_hist_cap = (uint32_t)sv::sort_hist_words(n);
-    v.s.scan_part = base + cv.scan;
-    v.s.scan_part_cap = (uint32_t)sv::scan_part_words(256 * sv::sort_tiles(n));
+    v.s.sort_hist = base + cv.hist;  // zero at load; every sort leaves its control words at zero
+    v.s.sort_hist_cap = (uint32_t)sv::sort_hist_words(SV_NOALLOC_MAX_BODIES);
     return sv::bvh_build_run(&v, n, true);
 }

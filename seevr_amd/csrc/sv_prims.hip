@@ -1,8 +1,10 @@
 // sv_prims.hip - device-wide exclusive scan and stable LSD radix sort used
 // by the broadphase, narrowphase compaction, the ordered solver and the
-// LBVH.  Wave64 scans use cross-lane shuffles; the sort ranks keys with
-// wave ballots (8 ballots give each lane the mask of lanes holding the same
-// 8-bit digit), keeps per-wave digit counters in LDS and scatters stably.
+// LBVH.  Wave64 scans use cross-lane shuffles; the sort takes one launch for
+// every pass's digit histograms and one launch per pass (decoupled
+// look-back), ranks keys with wave ballots (8 ballots give each lane the
+// mask of lanes holding the same 8-bit digit) and writes each tile out
+// digit by digit from LDS.
 #include "sv_internal.h"
 
 namespace sv {
@@ -14,11 +16,16 @@ constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 1024
 static_assert(SCAN_TILE == SCAN_TILE_VALUES, "scan scratch sizing");
 
-constexpr int SORT_THREADS = 256;
-constexpr int SORT_WAVES = SORT_THREADS / 64;
-constexpr int SORT_ROUNDS = 8;
-constexpr int SORT_TILE = SORT_THREADS * SORT_ROUNDS;  // 2048
-static_assert(SORT_TILE == SORT_TILE_KEYS, "sort scratch sizing");
+// Radix-sort tiles: 256 threads x 8 rounds (2048 keys) up to
+// SORT_SMALL_MAX keys, where short look-back chains and more workgroups
+// matter; 512 threads x 16 rounds (8192 keys) above, for longer write runs
+// and fewer look-backs.
+static_assert(256 * 8 == SORT_TILE_SMALL && 512 * 16 == SORT_TILE_LARGE, "sort scratch sizing");
+constexpr int HIST_THREADS = 256;
+constexpr int HIST_ITEMS = 16;                         // keys per thread and iteration
+constexpr int HIST_CHUNK = HIST_THREADS * HIST_ITEMS;  // 4096
+constexpr uint32_t HIST_BLOCKS = 256;
+constexpr int LB_WINDOW = 16;                          // predecessors read at once in the look-back
 
 __device__ __forceinline__ uint32_t read_n(const uint32_t *n_dev, uint32_t n_max) {
     if (!n_dev) return n_max;
@@ -118,88 +125,270 @@ __global__ void k_fill(uint32_t *p, uint32_t value, size_t n) {
     for (; i < n; i += stride) p[i] = value;
 }
 
-// Per-tile 256-bin digit histogram, digit-major over the tiles that hold
-// keys: hist[d * nb + blk], nb = tiles of the key count read on the device
-// (<= nblocks, the launch), so the scan between the two kernels covers
-// 256 * nb entries, whose number block 0 writes to *hist_n.
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t *keys, const uint32_t *n_dev, uint32_t n_max,
-                                                            int shift, uint32_t *hist, uint32_t *hist_n) {
-    __shared__ uint32_t h[256];
-    const uint32_t n = read_n(n_dev, n_max), nb = (n + SORT_TILE - 1) / SORT_TILE;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *hist_n = 256u * nb;
-    if (blockIdx.x >= nb) return;
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    uint32_t base = blockIdx.x * SORT_TILE;
-    for (int k = 0; k < SORT_ROUNDS; k++) {
-        uint32_t i = base + k * SORT_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort, 8-bit digits, one launch per pass (decoupled
+// look-back).  Element order inside a tile of THREADS x ROUNDS keys: wave w owns
+// [w * ROUNDS * 64, (w + 1) * ROUNDS * 64), round r the next 64, lane =
+// offset, so a wave's ranks follow element order.
+//
+// k_os_hist: every digit histogram of the sort in one read of the keys
+// (a digit histogram does not depend on the order of the keys).  Blocks
+// count in LDS (a wave whose 64 keys share a digit adds 64 once) and add
+// their counts to per-pass accumulators with device-scope atomics; the last
+// block to finish (a device-scope counter) turns them into each pass's
+// exclusive digit offsets and leaves the accumulators, the pass tile
+// counters and the done counter at zero for the next sort.  The same launch
+// zeroes the look-back status words the passes use.
+//
+// k_os_pass: a workgroup takes the next tile index from the pass counter
+// (tiles are taken in launch order, so every tile a workgroup waits for is
+// held by a running workgroup), ranks its keys stably (8 wave ballots give
+// each lane the lanes holding its digit), publishes its digit counts as
+// "aggregate" words, and finds its base per digit by looking back over the
+// predecessors' words (flag and count in one 32-bit word, device-coherent,
+// LB_WINDOW of them read at once) until an inclusive prefix; then publishes
+// its own inclusive prefix.  A wait is bounded (2 s, a broken invariant):
+// it then sets the abort word and the SV_STATUS_SORT_TIMEOUT bit of the
+// context's status word.  Keys, then values, are staged digit-sorted in LDS
+// and written out in runs, so consecutive lanes write consecutive addresses.
+constexpr uint32_t OS_FLAG_A = 1u << 30, OS_FLAG_P = 2u << 30, OS_VALUE = (1u << 30) - 1u;
+constexpr uint64_t OS_STALL_TICKS = 200000000ull;  // 2 s of the 100 MHz wall clock
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void st_agent(uint32_t *p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Exclusive scan of one value per digit, held by threads 0..255 of a
+// block of nthreads >= 256 (every thread calls it).
+template <int NTHREADS>
+__device__ __forceinline__ uint32_t digit_excl_scan(uint32_t v) {
+    __shared__ uint32_t wsum[4];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t inc = 0;
+    if (tid < 256) {
+        inc = wave_incl_scan(v);
+        if (lane == 63) wsum[w] = inc;
     }
     __syncthreads();
-    hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+    uint32_t base = 0;
+    if (tid < 256) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (i < w) base += wsum[i];
+    }
+    __syncthreads();
+    return base + inc - v;
 }
 
-// Stable scatter.  Element order inside a tile: wave w owns
-// [w*ROUNDS*64, (w+1)*ROUNDS*64), round r the next 64, lane = offset.
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
-                                                               uint32_t *vout, const uint32_t *n_dev, uint32_t n_max,
-                                                               int shift, const uint32_t *offs) {
-    __shared__ uint32_t wcnt[SORT_WAVES][256];
-    __shared__ uint32_t gbase[256];
-    const uint32_t n = read_n(n_dev, n_max), nb = (n + SORT_TILE - 1) / SORT_TILE;
-    if (blockIdx.x >= nb) return;
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int d = threadIdx.x; d < 256; d += SORT_THREADS) {
-        gbase[d] = offs[d * nb + blockIdx.x];
+__global__ void __launch_bounds__(HIST_THREADS) k_os_hist(const uint32_t *__restrict__ keys, const uint32_t *n_dev,
+                                                          uint32_t n_max, int passes, uint32_t tile_keys,
+                                                          uint32_t *acc, uint32_t *__restrict__ goff, uint32_t *ctl,
+                                                          uint32_t *__restrict__ status) {
+    __shared__ uint32_t h[SORT_MAX_PASSES][256];
+    __shared__ uint32_t s_last;
+    const uint32_t n = read_n(n_dev, n_max), nb = (n + tile_keys - 1) / tile_keys;
+    const int tid = threadIdx.x;
+    // the passes' look-back words start at zero
+    const size_t nstat = (size_t)passes * nb * 256;
+    for (size_t i = (size_t)blockIdx.x * HIST_THREADS + tid; i < nstat; i += (size_t)gridDim.x * HIST_THREADS)
+        status[i] = 0u;
+    for (int p = 0; p < (int)SORT_MAX_PASSES; p++) h[p][tid] = 0u;
+    __syncthreads();
+    for (uint32_t base = blockIdx.x * HIST_CHUNK; base < n; base += gridDim.x * HIST_CHUNK) {
+        uint32_t key[HIST_ITEMS];
 #pragma unroll
-        for (int q = 0; q < SORT_WAVES; q++) wcnt[q][d] = 0;
+        for (int k = 0; k < HIST_ITEMS; k++) {
+            const uint32_t i = base + k * HIST_THREADS + tid;
+            key[k] = i < n ? keys[i] : 0u;
+        }
+        const bool full = base + HIST_CHUNK <= n;  // uniform over the block
+#pragma unroll
+        for (int k = 0; k < HIST_ITEMS; k++) {
+            const bool valid = base + k * HIST_THREADS + tid < n;
+            for (int p = 0; p < passes; p++) {
+                const uint32_t d = (key[k] >> (8 * p)) & 255u;
+                if (full) {
+                    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+                    if (__all(d == d0)) {
+                        if ((tid & 63) == 0) atomicAdd(&h[p][d0], 64u);
+                        continue;
+                    }
+                }
+                if (valid) atomicAdd(&h[p][d], 1u);
+            }
+        }
     }
     __syncthreads();
-    uint32_t base = blockIdx.x * SORT_TILE + w * (SORT_ROUNDS * 64);
-    uint32_t key[SORT_ROUNDS], val[SORT_ROUNDS], loc[SORT_ROUNDS];
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int p = 0; p < passes; p++) {
+        const uint32_t c = h[p][tid];
+        if (c) atomicAdd(acc + p * 256 + tid, c);
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // this thread's additions have been performed
+    __syncthreads();                // ... and every thread's
+    if (tid == 0) s_last = atomicAdd(ctl + SORT_CTL_DONE, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+    for (int p = 0; p < passes; p++) {
+        const uint32_t tot = ld_agent(acc + p * 256 + tid);
+        st_agent(acc + p * 256 + tid, 0u);
+        uint32_t all;
+        goff[p * 256 + tid] = block_excl_scan(tot, &all);
+    }
+    if (tid == 0) {
+        ctl[SORT_CTL_DONE] = 0u;
+        for (int p = 0; p < (int)SORT_MAX_PASSES; p++) ctl[SORT_CTL_TILE + p] = 0u;
+    }
+}
+
+template <int THREADS, int ROUNDS>
+__global__ void __launch_bounds__(THREADS) k_os_pass(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                     const uint32_t *n_dev, uint32_t n_max, int pass,
+                                                     const uint32_t *__restrict__ goff, uint32_t *ctl,
+                                                     uint32_t *status, uint32_t *counters) {
+    constexpr int WAVES = THREADS / 64, TILE = THREADS * ROUNDS;
+    __shared__ uint32_t s_buf[TILE];
+    __shared__ uint32_t wcnt[WAVES][256];
+    __shared__ uint32_t s_lo[256], s_gb[256];
+    __shared__ uint32_t s_tile;
+    const uint32_t n = read_n(n_dev, n_max), nb = (n + TILE - 1) / TILE;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int shift = 8 * pass;
+    if (tid == 0) s_tile = atomicAdd(ctl + SORT_CTL_TILE + pass, 1u);
+    for (int i = tid; i < WAVES * 256; i += THREADS) (&wcnt[0][0])[i] = 0u;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= nb) return;  // workgroups beyond the keys read on the device
+
+    // stable ranks inside the tile: the ballots of every round first (no
+    // dependence between rounds), then the wave's running digit counters
+    const uint32_t base = tile * TILE + w * (ROUNDS * 64);
+    uint32_t key[ROUNDS], val[ROUNDS], pos[ROUNDS];
 #pragma unroll
-    for (int r = 0; r < SORT_ROUNDS; r++) {
-        uint32_t i = base + r * 64 + lane;
-        bool valid = i < n;
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t i = base + r * 64 + lane;
+        const bool valid = i < n;
         key[r] = valid ? kin[i] : 0u;
         val[r] = valid ? vin[i] : 0u;
-        uint32_t d = (key[r] >> shift) & 255u;
+    }
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const bool valid = base + r * 64 + lane < n;
+        const uint32_t d = (key[r] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bal = __ballot(bit);
-            peers &= bit ? bal : ~bal;
+        for (int bit = 0; bit < 8; bit++) {
+            const bool on = (d >> bit) & 1u;
+            const uint64_t bal = __ballot(on);
+            peers &= on ? bal : ~bal;
         }
-        uint32_t rank = (uint32_t)__popcll(peers & lt);
-        uint32_t cur = wcnt[w][d];
-        loc[r] = cur + rank;
+        // rank among the round's lanes with this digit | their number << 8;
+        // all ones for a lane past the keys
+        pos[r] = valid ? (uint32_t)__popcll(peers & lt) | ((uint32_t)__popcll(peers) << 8) : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t d = (key[r] >> shift) & 255u;
+        const bool valid = pos[r] != 0xFFFFFFFFu;
+        const uint32_t rank = pos[r] & 255u, num = pos[r] >> 8;
+        const uint32_t cur = wcnt[w][d];
+        if (valid) pos[r] = cur + rank;
         __builtin_amdgcn_wave_barrier();
-        if (valid && rank == 0) wcnt[w][d] = cur + (uint32_t)__popcll(peers);
+        if (valid && rank == 0) wcnt[w][d] = cur + num;
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < 256; d += SORT_THREADS) {
-        uint32_t run = 0;
+    // digit d = tid (< 256): the waves' offsets and the tile's count
+    const uint32_t d = (uint32_t)tid;
+    uint32_t cnt = 0;
+    uint32_t *mine = status + ((size_t)pass * nb + tile) * 256 + d;
+    if (tid < 256) {
 #pragma unroll
-        for (int q = 0; q < SORT_WAVES; q++) {
-            uint32_t t = wcnt[q][d];
-            wcnt[q][d] = run;
-            run += t;
+        for (int q = 0; q < WAVES; q++) {
+            const uint32_t t = wcnt[q][d];
+            wcnt[q][d] = cnt;
+            cnt += t;
+        }
+        st_agent(mine, (tile == 0 ? OS_FLAG_P : OS_FLAG_A) | cnt);
+    }
+    const uint32_t lo = digit_excl_scan<THREADS>(cnt);
+    if (tid < 256) {
+        s_lo[d] = lo;
+        // look back over the predecessors' words of digit d
+        uint32_t excl = 0;
+        int32_t t = (int32_t)tile - 1;  // nearest predecessor not yet added
+        uint64_t start = 0;
+        const uint32_t *col = status + (size_t)pass * nb * 256 + d;
+        while (t >= 0) {
+            uint32_t v[LB_WINDOW];
+#pragma unroll
+            for (int j = 0; j < LB_WINDOW; j++) v[j] = (t - j >= 0) ? ld_agent(col + (size_t)(t - j) * 256) : OS_FLAG_P;
+            int j = 0;
+            bool done = false;
+#pragma unroll
+            for (int q = 0; q < LB_WINDOW; q++) {
+                if (j == q) {
+                    const uint32_t x = v[q];
+                    if (x & OS_FLAG_P) {
+                        excl += x & OS_VALUE;
+                        done = true;
+                    } else if (x & OS_FLAG_A) {
+                        excl += x & OS_VALUE;
+                        j = q + 1;
+                    }
+                }
+            }
+            if (done) break;
+            t -= j;
+            if (j == LB_WINDOW) continue;
+            // tile t has not published: the workgroup holding it is running
+            if (start == 0) start = wall_clock64();
+            if (wall_clock64() - start > OS_STALL_TICKS) {
+                st_agent(ctl + SORT_CTL_ABORT, 1u);
+                if (counters) atomicOr(counters + 2, (uint32_t)SV_STATUS_SORT_TIMEOUT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (tile > 0) st_agent(mine, OS_FLAG_P | (excl + cnt));
+        s_gb[d] = goff[pass * 256 + d] + excl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        if (pos[r] != 0xFFFFFFFFu) {
+            const uint32_t dd = (key[r] >> shift) & 255u;
+            pos[r] += s_lo[dd] + wcnt[w][dd];
+        }
+    }
+    // keys staged digit-sorted in LDS (element order kept per digit) and
+    // written out in runs; then the values through the same buffer
+    const uint32_t left = n - tile * TILE;
+    const uint32_t m = left < (uint32_t)TILE ? left : (uint32_t)TILE;
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++)
+        if (pos[r] != 0xFFFFFFFFu) s_buf[pos[r]] = key[r];
+    __syncthreads();
+    uint32_t dst[ROUNDS];
+#pragma unroll
+    for (int k = 0; k < ROUNDS; k++) {
+        const uint32_t i = k * THREADS + tid;
+        dst[k] = 0xFFFFFFFFu;
+        if (i < m) {
+            const uint32_t kk = s_buf[i];
+            const uint32_t dd = (kk >> shift) & 255u;
+            dst[k] = s_gb[dd] + (i - s_lo[dd]);
+            kout[dst[k]] = kk;
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < SORT_ROUNDS; r++) {
-        uint32_t i = base + r * 64 + lane;
-        if (i < n) {
-            uint32_t d = (key[r] >> shift) & 255u;
-            uint32_t pos = gbase[d] + wcnt[w][d] + loc[r];
-            kout[pos] = key[r];
-            vout[pos] = val[r];
-        }
-    }
+    for (int r = 0; r < ROUNDS; r++)
+        if (pos[r] != 0xFFFFFFFFu) s_buf[pos[r]] = val[r];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ROUNDS; k++)
+        if (dst[k] != 0xFFFFFFFFu) vout[dst[k]] = s_buf[k * THREADS + tid];
 }
 
 }  // namespace
@@ -211,7 +400,15 @@ void ensure_u32(uint32_t *&p, uint32_t &cap, uint32_t need) {
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
-    if (hipMalloc(&p, (size_t)need * sizeof(uint32_t)) == hipSuccess) cap = need;
+    // zeroed: the sort's control words must start at zero
+    if (hipMalloc(&p, (size_t)need * sizeof(uint32_t)) == hipSuccess) {
+        if (hipMemset(p, 0, (size_t)need * sizeof(uint32_t)) == hipSuccess) {
+            cap = need;
+        } else {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+    }
 }
 
 // Scratch of at least `need` words: grown when the context owns it; a
@@ -227,11 +424,8 @@ static bool reserve_fixed(sv_ctx *c, uint32_t *&p, uint32_t &cap, uint32_t need)
 }
 
 void prims_reserve(sv_ctx *c, uint32_t max_scan_n, uint32_t max_sort_n) {
-    uint32_t sort_blocks = div_up(max_sort_n, SORT_TILE);
-    uint32_t hist_n = 256u * sort_blocks;
-    uint32_t scan_n = max_scan_n > hist_n ? max_scan_n : hist_n;
-    ensure_u32(c->s.scan_part, c->s.scan_part_cap, div_up(scan_n, SCAN_TILE) + 1);
-    ensure_u32(c->s.sort_hist, c->s.sort_hist_cap, 2u * hist_n + 1);
+    ensure_u32(c->s.scan_part, c->s.scan_part_cap, (uint32_t)scan_part_words(max_scan_n));
+    ensure_u32(c->s.sort_hist, c->s.sort_hist_cap, (uint32_t)sort_hist_words(max_sort_n));
 }
 
 void scan_exclusive(sv_ctx *c, const uint32_t *in, uint32_t *out, const uint32_t *n_dev, uint32_t n_max,
@@ -252,23 +446,44 @@ void scan_exclusive(sv_ctx *c, const uint32_t *in, uint32_t *out, const uint32_t
 int radix_sort(sv_ctx *c, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, const uint32_t *n_dev,
                uint32_t n_max, int key_bits) {
     if (n_max == 0 || key_bits <= 0) return 0;
-    uint32_t nblocks = div_up(n_max, SORT_TILE);
-    uint32_t hist_n = 256u * nblocks;
-    if (!reserve_fixed(c, c->s.sort_hist, c->s.sort_hist_cap, 2u * hist_n + 1)) return 0;
-    uint32_t *hist = c->s.sort_hist, *offs = c->s.sort_hist + hist_n, *used = c->s.sort_hist + 2u * hist_n;
-    int passes = (key_bits + 7) / 8;
+    if (key_bits > 32 || n_max > SORT_MAX_KEYS) {
+        c->launch_err = hipErrorInvalidValue;
+        return 0;
+    }
+    const bool small = n_max <= SORT_SMALL_MAX;
+    const uint32_t tile_keys = small ? SORT_TILE_SMALL : SORT_TILE_LARGE;
+    const uint32_t nblocks = div_up(n_max, tile_keys);
+    if (!reserve_fixed(c, c->s.sort_hist, c->s.sort_hist_cap, (uint32_t)sort_hist_words(n_max))) return 0;
+    uint32_t *acc = c->s.sort_hist + SORT_ACC_OFF, *goff = c->s.sort_hist + SORT_GOFF_OFF,
+             *ctl = c->s.sort_hist + SORT_CTL_OFF, *status = c->s.sort_hist + SORT_STATUS_OFF;
+    uint32_t *counters = reinterpret_cast<uint32_t *>(c->buf[SV_ATOMIC_COUNTERS]);
+    const int passes = (key_bits + 7) / 8;
+    const uint32_t chunks = div_up(n_max, HIST_CHUNK);
+    const uint32_t hb = chunks < HIST_BLOCKS ? chunks : HIST_BLOCKS;
+    hipLaunchKernelGGL(k_os_hist, dim3(hb), dim3(HIST_THREADS), 0, c->stream, k0, n_dev, n_max, passes, tile_keys,
+                       acc, goff, ctl, status);
     uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
     for (int p = 0; p < passes; p++) {
-        int shift = p * 8;
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, c->stream, ka, n_dev, n_max, shift,
-                           hist, used);
-        scan_exclusive(c, hist, offs, used, hist_n, nullptr);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(SORT_THREADS), 0, c->stream, ka, va, kb, vb, n_dev,
-                           n_max, shift, offs);
+        if (small)
+            hipLaunchKernelGGL((k_os_pass<256, 8>), dim3(nblocks), dim3(256), 0, c->stream, ka, va, kb, vb, n_dev,
+                               n_max, p, goff, ctl, status, counters);
+        else
+            hipLaunchKernelGGL((k_os_pass<512, 16>), dim3(nblocks), dim3(512), 0, c->stream, ka, va, kb, vb, n_dev,
+                               n_max, p, goff, ctl, status, counters);
         uint32_t *t = ka; ka = kb; kb = t;
         t = va; va = vb; vb = t;
     }
     return passes & 1;
+}
+
+bool sort_take_abort(sv_ctx *c) {
+    if (!c->s.sort_hist || c->s.sort_hist_cap < SORT_STATUS_OFF) return false;
+    uint32_t *w = c->s.sort_hist + SORT_CTL_OFF + SORT_CTL_ABORT;
+    uint32_t v = 0;
+    if (hipMemcpy(&v, w, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess || v == 0) return false;
+    const uint32_t zero = 0;
+    (void)hipMemcpy(w, &zero, sizeof(zero), hipMemcpyHostToDevice);
+    return true;
 }
 
 void fill_u32(sv_ctx *c, uint32_t *p, uint32_t value, size_t n) {
