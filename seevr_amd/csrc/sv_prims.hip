@@ -24,7 +24,7 @@ static_assert(256 * 8 == SORT_TILE_SMALL && 512 * 16 == SORT_TILE_LARGE, "sort s
 constexpr int HIST_THREADS = 256;
 constexpr int HIST_ITEMS = 16;                         // keys per thread and iteration
 constexpr int HIST_CHUNK = HIST_THREADS * HIST_ITEMS;  // 4096
-constexpr uint32_t HIST_BLOCKS = 256;
+constexpr uint32_t HIST_BLOCKS = 512;
 constexpr int LB_WINDOW = 16;                          // predecessors read at once in the look-back
 
 __device__ __forceinline__ uint32_t read_n(const uint32_t *n_dev, uint32_t n_max) {
@@ -133,7 +133,7 @@ __global__ void k_fill(uint32_t *p, uint32_t value, size_t n) {
 //
 // k_os_hist: every digit histogram of the sort in one read of the keys
 // (a digit histogram does not depend on the order of the keys).  Blocks
-// count in LDS (a wave whose 64 keys share a digit adds 64 once) and add
+// count in LDS (one addition per run of equal digits in a wave) and add
 // their counts to per-pass accumulators with device-scope atomics; the last
 // block to finish (a device-scope counter) turns them into each pass's
 // exclusive digit offsets and leaves the accumulators, the pass tile
@@ -179,41 +179,50 @@ __device__ __forceinline__ uint32_t digit_excl_scan(uint32_t v) {
     return base + inc - v;
 }
 
+template <int PASSES>
 __global__ void __launch_bounds__(HIST_THREADS) k_os_hist(const uint32_t *__restrict__ keys, const uint32_t *n_dev,
-                                                          uint32_t n_max, int passes, uint32_t tile_keys,
-                                                          uint32_t *acc, uint32_t *__restrict__ goff, uint32_t *ctl,
+                                                          uint32_t n_max, uint32_t tile_keys, uint32_t *acc,
+                                                          uint32_t *__restrict__ goff, uint32_t *ctl,
                                                           uint32_t *__restrict__ status) {
+    constexpr int passes = PASSES;
     __shared__ uint32_t h[SORT_MAX_PASSES][256];
     __shared__ uint32_t s_last;
     const uint32_t n = read_n(n_dev, n_max), nb = (n + tile_keys - 1) / tile_keys;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
+    // blocks beyond the keys on the device (at least one block works) leave at once
+    uint32_t active = (n + HIST_CHUNK - 1) / HIST_CHUNK;
+    active = active < 1u ? 1u : active > gridDim.x ? gridDim.x : active;
+    if (blockIdx.x >= active) return;
     // the passes' look-back words start at zero
     const size_t nstat = (size_t)passes * nb * 256;
-    for (size_t i = (size_t)blockIdx.x * HIST_THREADS + tid; i < nstat; i += (size_t)gridDim.x * HIST_THREADS)
+    for (size_t i = (size_t)blockIdx.x * HIST_THREADS + tid; i < nstat; i += (size_t)active * HIST_THREADS)
         status[i] = 0u;
     for (int p = 0; p < (int)SORT_MAX_PASSES; p++) h[p][tid] = 0u;
     __syncthreads();
-    for (uint32_t base = blockIdx.x * HIST_CHUNK; base < n; base += gridDim.x * HIST_CHUNK) {
+    for (uint32_t base = blockIdx.x * HIST_CHUNK; base < n; base += active * HIST_CHUNK) {
         uint32_t key[HIST_ITEMS];
 #pragma unroll
         for (int k = 0; k < HIST_ITEMS; k++) {
             const uint32_t i = base + k * HIST_THREADS + tid;
             key[k] = i < n ? keys[i] : 0u;
         }
-        const bool full = base + HIST_CHUNK <= n;  // uniform over the block
+        // one addition per run of equal digits in consecutive lanes (the
+        // lanes holding keys are a prefix of the wave)
 #pragma unroll
         for (int k = 0; k < HIST_ITEMS; k++) {
             const bool valid = base + k * HIST_THREADS + tid < n;
+            const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
+#pragma unroll
             for (int p = 0; p < passes; p++) {
                 const uint32_t d = (key[k] >> (8 * p)) & 255u;
-                if (full) {
-                    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-                    if (__all(d == d0)) {
-                        if ((tid & 63) == 0) atomicAdd(&h[p][d0], 64u);
-                        continue;
-                    }
+                const uint32_t prev = __shfl_up(d, 1, 64);
+                const bool head = valid && (lane == 0 || d != prev);
+                const uint64_t heads = __ballot(head);
+                if (head) {
+                    const uint64_t above = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+                    const uint32_t next = above ? (uint32_t)__builtin_ctzll(above) : nvalid;
+                    atomicAdd(&h[p][d], next - (uint32_t)lane);
                 }
-                if (valid) atomicAdd(&h[p][d], 1u);
             }
         }
     }
@@ -224,7 +233,7 @@ __global__ void __launch_bounds__(HIST_THREADS) k_os_hist(const uint32_t *__rest
     }
     __builtin_amdgcn_s_waitcnt(0);  // this thread's additions have been performed
     __syncthreads();                // ... and every thread's
-    if (tid == 0) s_last = atomicAdd(ctl + SORT_CTL_DONE, 1u) == gridDim.x - 1;
+    if (tid == 0) s_last = atomicAdd(ctl + SORT_CTL_DONE, 1u) == active - 1;
     __syncthreads();
     if (!s_last) return;
     for (int p = 0; p < passes; p++) {
@@ -253,11 +262,12 @@ __global__ void __launch_bounds__(THREADS) k_os_pass(const uint32_t *__restrict_
     const uint32_t n = read_n(n_dev, n_max), nb = (n + TILE - 1) / TILE;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int shift = 8 * pass;
+    // exactly the workgroups below the device's tile count take tiles
+    if (blockIdx.x >= nb) return;
     if (tid == 0) s_tile = atomicAdd(ctl + SORT_CTL_TILE + pass, 1u);
     for (int i = tid; i < WAVES * 256; i += THREADS) (&wcnt[0][0])[i] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
-    if (tile >= nb) return;  // workgroups beyond the keys read on the device
 
     // stable ranks inside the tile: the ballots of every round first (no
     // dependence between rounds), then the wave's running digit counters
@@ -275,12 +285,14 @@ __global__ void __launch_bounds__(THREADS) k_os_pass(const uint32_t *__restrict_
     for (int r = 0; r < ROUNDS; r++) {
         const bool valid = base + r * 64 + lane < n;
         const uint32_t d = (key[r] >> shift) & 255u;
+        // lanes whose digit bits all equal this lane's: bal XNOR (bit ? ~0 : 0)
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 8; bit++) {
-            const bool on = (d >> bit) & 1u;
-            const uint64_t bal = __ballot(on);
-            peers &= on ? bal : ~bal;
+            const uint32_t b = (d >> bit) & 1u;
+            const uint64_t bal = __ballot(b);
+            const uint64_t m = (uint64_t)(int64_t)(-(int32_t)b);
+            peers &= ~(bal ^ m);
         }
         // rank among the round's lanes with this digit | their number << 8;
         // all ones for a lane past the keys
@@ -460,8 +472,9 @@ int radix_sort(sv_ctx *c, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1
     const int passes = (key_bits + 7) / 8;
     const uint32_t chunks = div_up(n_max, HIST_CHUNK);
     const uint32_t hb = chunks < HIST_BLOCKS ? chunks : HIST_BLOCKS;
-    hipLaunchKernelGGL(k_os_hist, dim3(hb), dim3(HIST_THREADS), 0, c->stream, k0, n_dev, n_max, passes, tile_keys,
-                       acc, goff, ctl, status);
+    auto hist = passes == 1 ? k_os_hist<1> : passes == 2 ? k_os_hist<2> : passes == 3 ? k_os_hist<3> : k_os_hist<4>;
+    hipLaunchKernelGGL(hist, dim3(hb), dim3(HIST_THREADS), 0, c->stream, k0, n_dev, n_max, tile_keys, acc, goff, ctl,
+                       status);
     uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
     for (int p = 0; p < passes; p++) {
         if (small)
