@@ -251,6 +251,14 @@ int sv_bvh_dispatch(sv_ctx *ctx, uint32_t step, uint32_t num_tris);
  * check of the function the support lookup bins with). */
 int sv_debug_atan2(sv_ctx *ctx, const float *y, const float *x, float *out, uint32_t n);
 
+/* The library's stable LSD radix sort (the one the LBVH, the broadphase
+ * pair order and the sweep's body incidences use) on host arrays: sorts the
+ * first n of n_max (key, value) pairs by the low key_bits bits of the key,
+ * with n read on the device as the pipeline passes it (n <= n_max <=
+ * 2^30 - 1, 1 <= key_bits <= 32).  A test entry point. */
+int sv_debug_radix_sort(sv_ctx *ctx, const uint32_t *keys, const uint32_t *vals, uint32_t n, uint32_t n_max,
+                        uint32_t key_bits, uint32_t *keys_out, uint32_t *vals_out);
+
 /* Copy `nbytes` device to device `iters` times with a plain copy kernel
  * (the calibration kernel of the HBM byte counters) and report the average
  * read + write bandwidth in GB/s. */

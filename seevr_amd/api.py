@@ -199,6 +199,20 @@ class Context:
         """Hand-off variant of the multi-workgroup sweeps (same results)."""
         check(self.lib.sv_debug_set_sweep_variant(self.h, self.SWEEP_VARIANTS[variant]))
 
+    def debug_radix_sort(self, keys, vals, key_bits: int, n_max: int | None = None):
+        """The library's stable radix sort of (keys, vals) by the low key_bits
+        bits, with the count read on the device from an n_max-sized buffer."""
+        k = np.ascontiguousarray(keys, np.uint32)
+        v = np.ascontiguousarray(vals, np.uint32)
+        n = k.size
+        ko = np.zeros(n, np.uint32)
+        vo = np.zeros(n, np.uint32)
+        u = C.POINTER(C.c_uint32)
+        check(self.lib.sv_debug_radix_sort(self.h, k.ctypes.data_as(u), v.ctypes.data_as(u), n,
+                                           n if n_max is None else n_max, key_bits, ko.ctypes.data_as(u),
+                                           vo.ctypes.data_as(u)))
+        return ko, vo
+
     def xcc_ids(self, nblocks: int) -> np.ndarray:
         """XCD each of nblocks one-wave blocks ran on (placement probe)."""
         out = np.zeros(nblocks, np.uint32)

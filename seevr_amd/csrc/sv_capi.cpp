@@ -625,6 +625,36 @@ int sv_debug_atan2(sv_ctx *c, const float *y, const float *x, float *out, uint32
     return SV_OK;
 }
 
+int sv_debug_radix_sort(sv_ctx *c, const uint32_t *keys, const uint32_t *vals, uint32_t n, uint32_t n_max,
+                        uint32_t key_bits, uint32_t *keys_out, uint32_t *vals_out) {
+    if (!c || n > n_max || n_max > sv::SORT_MAX_KEYS || key_bits < 1 || key_bits > 32 ||
+        (n && (!keys || !vals || !keys_out || !vals_out)))
+        return fail(SV_E_INVALID, "sv_debug_radix_sort: bad argument");
+    if (!n_max) return SV_OK;
+    uint32_t *d = nullptr;
+    HIP_OK(hipMalloc(&d, (4 * (size_t)n_max + 1) * sizeof(uint32_t)));
+    uint32_t *k0 = d, *v0 = d + n_max, *k1 = d + 2 * (size_t)n_max, *v1 = d + 3 * (size_t)n_max, *dn = d + 4 * (size_t)n_max;
+    hipError_t e = hipMemset(d, 0xFF, 4 * (size_t)n_max * sizeof(uint32_t));  // past n: never read as keys
+    if (e == hipSuccess && n) e = hipMemcpy(k0, keys, n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && n) e = hipMemcpy(v0, vals, n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dn, &n, sizeof(n), hipMemcpyHostToDevice);
+    int rc = SV_OK;
+    if (e == hipSuccess) {
+        const int where = sv::radix_sort(c, k0, v0, k1, v1, dn, n_max, (int)key_bits);
+        rc = sync_and_finish(c);
+        if (rc == SV_OK && sv::sort_take_abort(c))
+            rc = fail(SV_E_TIMEOUT, "sv_debug_radix_sort: look-back made no progress for 2 s and was aborted");
+        if (rc == SV_OK && n) {
+            e = hipMemcpy(keys_out, where ? k1 : k0, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(vals_out, where ? v1 : v0, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+        }
+    }
+    (void)hipFree(d);
+    if (rc != SV_OK) return rc;
+    if (e != hipSuccess) return fail(SV_E_DEVICE, "sv_debug_radix_sort: %s", hipGetErrorString(e));
+    return SV_OK;
+}
+
 // Copy `nbytes` device-to-device `iters` times with k_copy_calib and report
 // the average bandwidth (read + write bytes per second).
 int sv_debug_xcc_ids(sv_ctx *c, uint32_t nblocks, uint32_t *out) {

@@ -181,7 +181,8 @@ __device__ __forceinline__ uint32_t digit_excl_scan(uint32_t v) {
 
 template <int PASSES>
 __global__ void __launch_bounds__(HIST_THREADS) k_os_hist(const uint32_t *__restrict__ keys, const uint32_t *n_dev,
-                                                          uint32_t n_max, uint32_t tile_keys, uint32_t *acc,
+                                                          uint32_t n_max, uint32_t top_mask, uint32_t tile_keys,
+                                                          uint32_t *acc,
                                                           uint32_t *__restrict__ goff, uint32_t *ctl,
                                                           uint32_t *__restrict__ status) {
     constexpr int passes = PASSES;
@@ -214,7 +215,7 @@ __global__ void __launch_bounds__(HIST_THREADS) k_os_hist(const uint32_t *__rest
             const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
 #pragma unroll
             for (int p = 0; p < passes; p++) {
-                const uint32_t d = (key[k] >> (8 * p)) & 255u;
+                const uint32_t d = (key[k] >> (8 * p)) & (p == passes - 1 ? top_mask : 255u);
                 const uint32_t prev = __shfl_up(d, 1, 64);
                 const bool head = valid && (lane == 0 || d != prev);
                 const uint64_t heads = __ballot(head);
@@ -251,7 +252,7 @@ __global__ void __launch_bounds__(HIST_THREADS) k_os_hist(const uint32_t *__rest
 template <int THREADS, int ROUNDS>
 __global__ void __launch_bounds__(THREADS) k_os_pass(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                      uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                     const uint32_t *n_dev, uint32_t n_max, int pass,
+                                                     const uint32_t *n_dev, uint32_t n_max, int pass, uint32_t dmask,
                                                      const uint32_t *__restrict__ goff, uint32_t *ctl,
                                                      uint32_t *status, uint32_t *counters) {
     constexpr int WAVES = THREADS / 64, TILE = THREADS * ROUNDS;
@@ -284,7 +285,7 @@ __global__ void __launch_bounds__(THREADS) k_os_pass(const uint32_t *__restrict_
 #pragma unroll
     for (int r = 0; r < ROUNDS; r++) {
         const bool valid = base + r * 64 + lane < n;
-        const uint32_t d = (key[r] >> shift) & 255u;
+        const uint32_t d = (key[r] >> shift) & dmask;
         // lanes whose digit bits all equal this lane's: bal XNOR (bit ? ~0 : 0)
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -300,7 +301,7 @@ __global__ void __launch_bounds__(THREADS) k_os_pass(const uint32_t *__restrict_
     }
 #pragma unroll
     for (int r = 0; r < ROUNDS; r++) {
-        const uint32_t d = (key[r] >> shift) & 255u;
+        const uint32_t d = (key[r] >> shift) & dmask;
         const bool valid = pos[r] != 0xFFFFFFFFu;
         const uint32_t rank = pos[r] & 255u, num = pos[r] >> 8;
         const uint32_t cur = wcnt[w][d];
@@ -369,7 +370,7 @@ __global__ void __launch_bounds__(THREADS) k_os_pass(const uint32_t *__restrict_
 #pragma unroll
     for (int r = 0; r < ROUNDS; r++) {
         if (pos[r] != 0xFFFFFFFFu) {
-            const uint32_t dd = (key[r] >> shift) & 255u;
+            const uint32_t dd = (key[r] >> shift) & dmask;
             pos[r] += s_lo[dd] + wcnt[w][dd];
         }
     }
@@ -388,7 +389,7 @@ __global__ void __launch_bounds__(THREADS) k_os_pass(const uint32_t *__restrict_
         dst[k] = 0xFFFFFFFFu;
         if (i < m) {
             const uint32_t kk = s_buf[i];
-            const uint32_t dd = (kk >> shift) & 255u;
+            const uint32_t dd = (kk >> shift) & dmask;
             dst[k] = s_gb[dd] + (i - s_lo[dd]);
             kout[dst[k]] = kk;
         }
@@ -473,16 +474,20 @@ int radix_sort(sv_ctx *c, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1
     const uint32_t chunks = div_up(n_max, HIST_CHUNK);
     const uint32_t hb = chunks < HIST_BLOCKS ? chunks : HIST_BLOCKS;
     auto hist = passes == 1 ? k_os_hist<1> : passes == 2 ? k_os_hist<2> : passes == 3 ? k_os_hist<3> : k_os_hist<4>;
-    hipLaunchKernelGGL(hist, dim3(hb), dim3(HIST_THREADS), 0, c->stream, k0, n_dev, n_max, tile_keys, acc, goff, ctl,
-                       status);
+    // the last pass's digit keeps only the bits below key_bits
+    const uint32_t top_bits = (uint32_t)key_bits - 8u * (uint32_t)(passes - 1);
+    const uint32_t top_mask = top_bits >= 8 ? 255u : (1u << top_bits) - 1u;
+    hipLaunchKernelGGL(hist, dim3(hb), dim3(HIST_THREADS), 0, c->stream, k0, n_dev, n_max, top_mask, tile_keys, acc,
+                       goff, ctl, status);
     uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
     for (int p = 0; p < passes; p++) {
+        const uint32_t dmask = p == passes - 1 ? top_mask : 255u;
         if (small)
             hipLaunchKernelGGL((k_os_pass<256, 8>), dim3(nblocks), dim3(256), 0, c->stream, ka, va, kb, vb, n_dev,
-                               n_max, p, goff, ctl, status, counters);
+                               n_max, p, dmask, goff, ctl, status, counters);
         else
             hipLaunchKernelGGL((k_os_pass<512, 16>), dim3(nblocks), dim3(512), 0, c->stream, ka, va, kb, vb, n_dev,
-                               n_max, p, goff, ctl, status, counters);
+                               n_max, p, dmask, goff, ctl, status, counters);
         uint32_t *t = ka; ka = kb; kb = t;
         t = va; va = vb; vb = t;
     }
