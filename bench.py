@@ -5,11 +5,15 @@ Workload: 32,768 bodies (half boxes with half-extents U[0.2,0.6] m, half
 spheres with radius U[0.2,0.5] m, 16 segments) dropped from 1-60 m into a
 30x30 m container (ground + 4 static walls), density 1000, friction 0.5,
 restitution 0.1, gravity -9.81, dt 1/60 s, 10 substeps, 30 PGS iterations,
-FP32.  A step is one Physics.step call: one frame of 10 substeps plus the
-render-vertex update.  The metric is body-substeps/s over the timed frames.
+FP32.  A step is one frame: one Physics.step call (10 substeps plus the
+render-vertex update) and the LBVH rebuilt over the body boxes.  Every run
+first settles the pile for 240 frames, then runs W warmup frames, then times
+K frames, so the timed state does not depend on W.  The metric is
+body-substeps/s over the timed frames.
 
 Prints one JSON line (see the keys below).  Usage:
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-sample-substeps S]
+                  [--window-file PATH]   (timed windows for seevr_amd.profsum)
 """
 from __future__ import annotations
 
@@ -46,12 +50,23 @@ def build_workload(count: int, seed: int):
     return sc
 
 
+SETTLE_FRAMES = 240  # frames every run steps before its warmup and timed frames
+WINDOWS: dict[str, list[int]] = {}  # section -> CLOCK_MONOTONIC ns bracketing its timed frames
+
+
+def window(name: str, t0: int, t1: int) -> None:
+    WINDOWS[name] = [t0, t1]
+
+
 PROFILE = ROOT / "profiles" / "c2_32k_container_kernels.md"
+PROFILE_TRI = ROOT / "profiles" / "c2_triangle_bvh_kernels.md"
 VALU_PEAK = 256 * 4 * 16 * 2.4e9  # non-packed FP32 lane operations per second
 STAGE_FRAMES = 5  # frames after the timed ones whose stages are all timed
 STAGE_KERNEL = {"phys/pgs_sweep": "k_solve_seqp2<64>", "phys/aabb": "k_aabb", "phys/gravity": "k_gravity",
                 "phys/integrate": "k_integrate", "phys/broad_phase": "k_broadphase<false>",
-                "bvh/morton_codes": "k_morton", "bvh/radix_sort": "radix sort (k_sort_hist, k_scan_*, k_sort_scatter)"}
+                "bvh/morton_codes": "k_morton", "bvh/radix_sort": "k_os_hist<4> + 4 x k_os_pass<256, 8>"}
+# kernels (and launches) behind a stage's time, for its profiled traffic
+STAGE_LAUNCHES = {"bvh/radix_sort": {"k_os_hist<4>": 1, "k_os_pass<256, 8>": 4}}
 
 
 def algorithmic_bytes(stage: str, n: int, n_hull: int, n_dyn: int, pairs: int, nc: int) -> float:
@@ -60,8 +75,9 @@ def algorithmic_bytes(stage: str, n: int, n_hull: int, n_dyn: int, pairs: int, n
     aabb: pos 12 + quat 16 + hull range 8 per body read, 12 per hull vertex
     read, 24 per body written.  broadphase: 28 per body read (box + static
     flag) + 8 per pair written.  gravity 52, integrate 84 per dynamic body.
-    LBVH radix sort: 16 per key per pass (key and payload read and written),
-    4 passes of 8-bit digits over 30-bit keys; Morton codes: 12 in + 4 out.
+    LBVH radix sort: one histogram read of the keys (4) and 16 per key per
+    pass (key and payload read and written), 4 passes of 8-bit digits over
+    30-bit keys; Morton codes: 12 in + 4 out.
     Ordered PGS sweep: per item (iteration x contact) the two bodies'
     velocities and angular velocities read and written (96 B), the
     accumulated impulse read and written (8 B) and the contact's 20-word
@@ -78,31 +94,32 @@ def algorithmic_bytes(stage: str, n: int, n_hull: int, n_dyn: int, pairs: int, n
     if stage == "phys/pgs_sweep":
         return 184.0 * PGS_ITERS * nc
     if stage == "bvh/radix_sort":  # 30-bit Morton keys, 8-bit digits: 4 passes of key + index in and out
-        return 16.0 * n * 4
+        return 16.0 * n * 4 + 4.0 * n
     if stage == "bvh/morton_codes":
         return 16.0 * n
     return 0.0
 
 
-def profiled_traffic(kernel: str):
+def profiled_traffic(kernel: str, profile: Path = PROFILE):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary, scaled by the copy-kernel calibration in the same file."""
-    if not PROFILE.exists():
+    WRITE_SIZE summary (dispatches of the same timed frames), scaled by the
+    copy-kernel calibration in the same file."""
+    if not profile.exists():
         return None
     rows = {}
-    for line in PROFILE.read_text().splitlines():
+    for line in profile.read_text().splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cells) == 4 and cells[0].startswith(("k_", "__")):
             try:
                 rows[cells[0]] = (float(cells[2]), float(cells[3]))
             except ValueError:
                 pass
-    if kernel not in rows or "k_copy_calib" not in rows:
+    parts = kernel if isinstance(kernel, dict) else {kernel: 1}  # kernel -> launches per stage
+    if "k_copy_calib" not in rows or any(k not in rows for k in parts):
         return None
     cal_f, cal_w = rows["k_copy_calib"]
     true_kib = (256 << 20) / 1024.0
-    f, w = rows[kernel]
-    return 1024.0 * (f * true_kib / cal_f + w * true_kib / cal_w)
+    return sum(m * 1024.0 * (rows[k][0] * true_kib / cal_f + rows[k][1] * true_kib / cal_w) for k, m in parts.items())
 
 
 def roofline_entry(stage, stages, n, nh, n_dyn, pairs, nc):
@@ -111,8 +128,26 @@ def roofline_entry(stage, stages, n, nh, n_dyn, pairs, nc):
     achieved = alg / (us * 1e-6) / 1e9 if us > 0 else 0.0
     kern = STAGE_KERNEL.get(stage, stage)
     return {"bound": "hbm", "kernel": f"{kern} ({stage})", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": profiled_traffic(kern),
-            "avg_us": round(us, 2), "alg_bytes": int(alg)}
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": profiled_traffic(STAGE_LAUNCHES.get(stage, kern)), "avg_us": round(us, 2), "alg_bytes": int(alg)}
+
+
+def sweep_roofline(sweep_us: float, depth: int, items: int, handoff_us: float, nc: int) -> dict:
+    """The sweep against its latency roofline: the serial order needs `depth`
+    dependent steps (the longest chain of its dependency graph, sv_sweep_depth),
+    and one step cannot cost less than one hand-off between waves through a
+    device-coherent word (sv_debug_handoff, across XCDs).  achieved = levels
+    per microsecond, peak = 1 / hand-off, frac = depth x hand-off / sweep."""
+    lv = depth / sweep_us if sweep_us > 0 else 0.0
+    peak = 1.0 / handoff_us if handoff_us > 0 else 0.0
+    alg = 184.0 * items
+    return {"bound": "latency", "kernel": f"{STAGE_KERNEL['phys/pgs_sweep']} (phys/pgs_sweep)",
+            "achieved": round(lv, 4), "peak": round(peak, 4), "unit": "dependent levels/us",
+            "frac": round(lv / peak, 4) if peak else None, "traffic": profiled_traffic(STAGE_KERNEL["phys/pgs_sweep"]),
+            "avg_us": round(sweep_us, 2), "dag_depth": depth, "items": items, "handoff_us": round(handoff_us, 4),
+            "us_per_level": round(sweep_us / depth, 3) if depth else None,
+            "latency_floor_us": round(depth * handoff_us, 1), "alg_bytes": int(alg),
+            "hbm_view_gbs": round(alg / (sweep_us * 1e-6) / 1e9, 1) if sweep_us > 0 else None}
 
 
 def cpu_baseline(ctx, scene, n, substeps: int, mt: bool = True) -> dict:
@@ -135,6 +170,54 @@ def cpu_baseline(ctx, scene, n, substeps: int, mt: bool = True) -> dict:
     return {"value": n * substeps / el, "unit": "body-substeps/s", "cores": cores, "kind": "port",
             "sample": f"{substeps} substeps of the same scene from the state at the start of the timed "
                       f"frames (N={n}), {how}, {el:.1f} s"}
+
+
+TRI_STAGES = ("bvh/centroids", "bvh/scene_bounds", "bvh/morton_codes", "bvh/radix_sort", "bvh/build_tree",
+              "bvh/leaf_aabbs", "bvh/clear_flags", "bvh/propagate")
+TRI_KERNELS = {"bvh/centroids": {"k_centroids": 1}, "bvh/morton_codes": {"k_morton": 1},
+               "bvh/radix_sort": {"k_os_hist<4>": 1, "k_os_pass<512, 16>": 4}}
+
+
+def triangle_bvh_section(ctx, counts: dict, reps: int = 5) -> dict:
+    """The LBVH over the scene's render triangles (bvh.comp's build, the
+    renderer's tree; 8.57M triangles of the 32k-body container): `reps` timed
+    builds, then each stage timed alone with HIP events over 3 more builds
+    (events around one stage only).  Rooflines for the bandwidth-bound
+    stages: centroids read 3 indices per triangle (12 B) and each vertex once
+    (12 B), and write a centroid and two indices (20 B); Morton codes read a
+    centroid (12 B) and write a code (4 B); the radix sort of 32-bit keys
+    reads the keys once for its histograms (4 B) and reads and writes key and
+    payload in each of 4 passes (64 B)."""
+    nt, nv = counts["num_triangles"], counts["num_vertices"]
+    ctx.bvh_build(nt)
+    w0, p0 = time.monotonic_ns(), time.perf_counter()
+    for _ in range(reps):
+        ctx.bvh_build(nt)
+    el = time.perf_counter() - p0
+    window("tri_bvh", w0, time.monotonic_ns())
+    stages = {}
+    for st in TRI_STAGES:
+        ctx.enable_stage_timing(True, only=st)
+        ctx.stage_times(reset=True)
+        for _ in range(3):
+            ctx.bvh_build(nt)
+        stages[st] = ctx.stage_times(reset=True).get(st, (0.0, 0))[0]
+    ctx.enable_stage_timing(False)
+    alg = {"bvh/centroids": 32.0 * nt + 12.0 * nv, "bvh/morton_codes": 16.0 * nt, "bvh/radix_sort": 68.0 * nt}
+    roof = {}
+    for st, b in alg.items():
+        us = stages[st]
+        a = b / (us * 1e-6) / 1e9 if us > 0 else 0.0
+        roof[st] = {"bound": "hbm", "kernel": " + ".join(f"{m} x {k}" if m > 1 else k for k, m in TRI_KERNELS[st].items()),
+                    "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(a / HBM_PEAK_GBS, 4),
+                    "traffic": profiled_traffic(TRI_KERNELS[st], PROFILE_TRI), "avg_us": round(us, 1),
+                    "alg_bytes": int(b)}
+    log(f"triangle BVH over {nt} triangles: {1000 * el / reps:.2f} ms per build; stages (us) " +
+        ", ".join(f"{k.split('/')[1]} {v:.0f}" for k, v in stages.items()))
+    return {"triangles": nt, "vertices": nv, "ms_per_build": round(1000 * el / reps, 3), "builds": reps,
+            "stage_us": {k: round(v, 1) for k, v in stages.items()}, "roofline_hbm_stages": roof,
+            "note": "stage times from HIP events around one stage per build; they include about 4 us of event "
+                    "and launch overhead, which the kernel durations of the rocprof summary do not"}
 
 
 def batched_section(frames: int, warmup: int, scenes: int = 4096, plain_sample: int = 32) -> dict:
@@ -168,10 +251,11 @@ def batched_section(frames: int, warmup: int, scenes: int = 4096, plain_sample: 
     threads, wgs, us = b.autotune(SUBSTEPS, DT)
     log(f"batched autotune: {threads} threads x {wgs} workgroups/CU, {us:.0f} us/substep")
     b.step(nv, SUBSTEPS, DT)
-    t0 = time.perf_counter()
+    w0, t0 = time.monotonic_ns(), time.perf_counter()
     for i in range(frames):
         b.step(nv, SUBSTEPS, DT)
     el = time.perf_counter() - t0
+    window("batched", w0, time.monotonic_ns())
     c = b.counters()
     env_steps = scenes * frames / el
     # plain path, one scene at a time, on an evenly spaced sample, each
@@ -261,18 +345,19 @@ def differentiable_section(frames: int = 60, scenes: int = 256, bodies: int = 64
     b.diff_forward(frames, SUBSTEPS, DT)
     target = b.packed_state()[: 3 * n] + rng.uniform(-0.5, 0.5, 3 * n).astype(np.float32)
     fwd, bwd = [], []
+    c = None
     for _ in range(reps):
         b.set_packed_state(x0)
         t0 = time.perf_counter()
         b.diff_forward(frames, SUBSTEPS, DT)
         fwd.append(time.perf_counter() - t0)
+        c = b.counters()  # the forward rollout's last substep (backward recomputes earlier frames)
         xf = b.packed_state()
         g = np.zeros(13 * n, np.float32)
         g[: 3 * n] = 2.0 * (xf[: 3 * n] - target)
         t0 = time.perf_counter()
         gstate, gfric, grest = b.diff_backward(frames, g, SUBSTEPS, DT)
         bwd.append(time.perf_counter() - t0)
-    c = b.counters()
     b.close()
     tf, tb = min(fwd), min(bwd)
     gmu = np.zeros((scenes, 2))
@@ -281,7 +366,7 @@ def differentiable_section(frames: int = 60, scenes: int = 256, bodies: int = 64
     return {"metric": "fwd+bwd vs fwd cost", "value": round((tf + tb) / tf, 3), "unit": "x",
             "higher_is_better": False, "scenes": scenes, "bodies_per_scene": bodies + 1, "frames": frames,
             "forward_ms": round(1000 * tf, 2), "backward_ms": round(1000 * tb, 2),
-            "contacts_last_substep": {"mean": round(float(c[:, 1].mean()), 2), "max": int(c[:, 1].max())},
+            "contacts_last_forward_substep": {"mean": round(float(c[:, 1].mean()), 2), "max": int(c[:, 1].max())},
             "grad_norms": {"state": float(np.linalg.norm(gstate)), "friction_per_material": float(np.linalg.norm(gmu))},
             "data": "synthetic: 256 seeded scenes of 64 bodies over a ground, v and w ~ N(0, 1), per-scene "
                     "per-material friction U[0.2, 0.8] and restitution U[0, 0.5]; targets = final positions "
@@ -291,8 +376,9 @@ def differentiable_section(frames: int = 60, scenes: int = 256, bodies: int = 64
 def sweep_variant_section(ctx, n, default_variant: str = "seq") -> dict:
     """The multi-workgroup sweep's hand-off variants from the same state (the
     pile where the timed frames left it): one substep's collision stages,
-    then the sweep timed with HIP events, for each variant; all must leave
-    bit-identical velocities."""
+    then the sweep timed with HIP events, for each variant, with its time per
+    level of the sweep's dependency graph; all must leave bit-identical
+    velocities."""
     state = {k: ctx.download(k, w * n) for k, w in (("BODY_POS", 3), ("BODY_QUAT", 4), ("BODY_VEL", 3),
                                                       ("BODY_OMEGA", 3))}
     dt_sub = float(np.float32(DT) / np.float32(SUBSTEPS))
@@ -314,7 +400,9 @@ def sweep_variant_section(ctx, n, default_variant: str = "seq") -> dict:
         ctx.enable_stage_timing(False)
         vel = np.concatenate([ctx.download("BODY_VEL", 3 * n), ctx.download("BODY_OMEGA", 3 * n)])
         ref = vel if ref is None else ref
-        out[variant] = {"sweep_us": round(st.get("phys/pgs_sweep", (0.0, 0))[0], 1),
+        depth, _ = ctx.sweep_depth()
+        us = st.get("phys/pgs_sweep", (0.0, 0))[0]
+        out[variant] = {"sweep_us": round(us, 1), "us_per_level": round(us / depth, 3) if depth else None,
                         "bit_identical": bool(np.array_equal(vel.view(np.uint32), ref.view(np.uint32)))}
     ctx.set_sweep_variant(default_variant)
     for key, v in state.items():
@@ -395,12 +483,16 @@ def large_scene_section(frames: int = 120, timed: int = 20, compare: bool = True
     ctx.set_sweep_variant(sweep_variant)
     sc.close()
     log(f"large scene: {nb} bodies, {nh} hull verts, built and uploaded in {time.perf_counter() - t0:.1f} s")
-    for _ in range(frames - timed):
+    for i in range(frames - timed):
         ctx.step(nb, nv, nh, SUBSTEPS, DT, GRAVITY)
-    t0 = time.perf_counter()
+        if i % 10 == 9:
+            log(f"large scene: frame {i + 1}/{frames}, counters {ctx.read_counters()}")
+    w0, t0 = time.monotonic_ns(), time.perf_counter()
     for _ in range(timed):
         ctx.step(nb, nv, nh, SUBSTEPS, DT, GRAVITY)
     el = time.perf_counter() - t0
+    window("large", w0, time.monotonic_ns())
+    depth, items = ctx.sweep_depth()
     # stage times in two more frames (events around every stage slow a frame)
     ctx.enable_stage_timing(True)
     ctx.stage_times(reset=True)
@@ -447,6 +539,7 @@ def large_scene_section(frames: int = 120, timed: int = 20, compare: bool = True
             "bodies": nb, "hull_verts": nh, "frames": frames, "timed_frames": timed,
             "ms_per_frame": round(1000 * el / timed, 2), "pairs_last_substep": pairs,
             "contacts_last_substep": contacts, "broadphase": "lbvh", "stage_us_per_launch": top,
+            "sweep_dag_depth": depth, "sweep_items": items,
             "solvers_from_same_state": cmp,
             "data": "synthetic: 1,048,576 seeded bodies, 70% spheres r U[0.1, 0.3] (8 segments), 30% boxes "
                     "half-extents U[0.1, 0.4], spawned in 200x200x50 m"}
@@ -456,7 +549,7 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=240)
+    ap.add_argument("--warmup", type=int, default=20, help="frames after the 240 settling frames, before timing")
     ap.add_argument("--bodies", type=int, default=32768)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-sample-substeps", type=int, default=10)
@@ -479,6 +572,9 @@ def main() -> int:
                     help="hand-off variant of the multi-workgroup sweep")
     ap.add_argument("--broadphase", default="auto", choices=["auto", "all_pairs", "lbvh"],
                     help="broadphase method (both give the all-pairs list bit for bit)")
+    ap.add_argument("--no-tri", action="store_true", help="skip the triangle-BVH section")
+    ap.add_argument("--window-file", type=Path, default=None,
+                    help="write the timed windows (CLOCK_MONOTONIC ns) of each section here, for seevr_amd.profsum")
     ap.add_argument("--calib", action="store_true",
                     help="also run the 256 MiB copy kernel once (byte-counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -507,38 +603,41 @@ def main() -> int:
     copy_gbs = ctx.copy_bandwidth(256 << 20, 1 if args.calib else 20)
     log(f"plain copy kernel (256 MiB): {copy_gbs:.0f} GB/s read+write")
 
-    for i in range(args.warmup):
+    handoff, hx = ctx.handoff_us(20000, same_xcd=False)
+    handoff_same, hxs = ctx.handoff_us(20000, same_xcd=True)
+    log(f"hand-off between two waves: {handoff:.3f} us one way across XCDs (XCC {hx}), "
+        f"{handoff_same:.3f} us within one (XCC {hxs})")
+
+    pre = SETTLE_FRAMES + args.warmup
+    for i in range(pre):
         ctx.step(n, nv, nh, SUBSTEPS, DT, GRAVITY)
         if i % 60 == 59:
-            log(f"warmup frame {i + 1}/{args.warmup}: counters {ctx.read_counters()}")
-    ctx.enable_stage_timing(True)
-    ctx.stage_times(reset=True)
-    ctx.step(n, nv, nh, SUBSTEPS, DT, GRAVITY)  # one timed-stage frame for the breakdown
-    stages = ctx.stage_times(reset=True)
-    ctx.enable_stage_timing(False)
-    log("stage breakdown (us/launch, launches): " +
-        ", ".join(f"{k}={v[0]:.1f}x{v[1]}" for k, v in sorted(stages.items(), key=lambda kv: -kv[1][0] * kv[1][1])))
+            log(f"settle/warmup frame {i + 1}/{pre}: counters {ctx.read_counters()}")
+    depth0, items0 = ctx.sweep_depth()
 
     # Timed region: K frames, device synchronized on both sides (every call is
-    # synchronous).  Each frame is the physics step and the LBVH over the
-    # body boxes; the metric counts the time inside the physics-step calls
-    # only, the LBVH build is timed beside it.  HIP events on the library
-    # stream bracket the sweep (the roofline kernel) in the same frames;
-    # events around every stage cost 1.5 ms per frame, so the other stages
-    # are timed in frames after these.
+    # synchronous).  A frame is the physics step and the LBVH over the body
+    # boxes (the workload rebuilds it every frame).  HIP events on the
+    # library stream bracket the sweep (the roofline kernel) in the same
+    # frames; events around every stage cost 1.5 ms per frame, so the other
+    # stages are timed in frames after these.
     ctx.enable_stage_timing(True, only="phys/pgs_sweep")
     ctx.stage_times(reset=True)
     el = el_bvh = 0.0
+    w0 = time.monotonic_ns()
     for i in range(args.steps):
         t0 = time.perf_counter()
         ctx.step(n, nv, nh, SUBSTEPS, DT, GRAVITY)
         t1 = time.perf_counter()
         ctx.bvh_build_bodies(n)
-        el_bvh += time.perf_counter() - t1
+        t2 = time.perf_counter()
+        el_bvh += t2 - t1
         el += t1 - t0
+    window("plain", w0, time.monotonic_ns())
     stages = ctx.stage_times(reset=True)
     pairs, contacts = ctx.read_counters()
-    ms = 1000.0 * el / args.steps
+    depth, items = ctx.sweep_depth()
+    ms = 1000.0 * (el + el_bvh) / args.steps
     # every stage, in frames after the timed ones (and one LBVH build each)
     ctx.enable_stage_timing(True)
     for i in range(STAGE_FRAMES):
@@ -548,13 +647,19 @@ def main() -> int:
         if k != "phys/pgs_sweep":
             stages[k] = v
     ctx.enable_stage_timing(False)
-    value = n * SUBSTEPS * args.steps / el
-    log(f"timed {args.steps} frames in {el:.2f} s: {ms:.2f} ms/frame, pairs {pairs}, contacts {contacts}")
+    value = n * SUBSTEPS * args.steps / (el + el_bvh)
+    value_phys = n * SUBSTEPS * args.steps / el
+    log(f"timed {args.steps} frames (frames {pre + 1}-{pre + args.steps}) in {el + el_bvh:.2f} s: {ms:.2f} ms/frame "
+        f"({1000 * el / args.steps:.2f} physics + {1000 * el_bvh / args.steps:.3f} LBVH), pairs {pairs}, "
+        f"contacts {contacts}, sweep DAG depth {depth0} -> {depth}")
     if args.solver == "colored":
         log(f"contacts per colour (last class = one-lane remainder): {ctx.color_counts()}")
 
     nc = min(contacts, caps.max_contacts)
-    roof = roofline_entry("phys/pgs_sweep", stages, n, nh, n_dyn, pairs, nc)
+    sweep_us = stages.get("phys/pgs_sweep", (0.0, 0))[0]
+    roof = sweep_roofline(sweep_us, depth, items, handoff, nc)
+    roof["dag_depth_first_timed_frame"] = depth0
+    roof["handoff_us_same_xcd"] = round(handoff_same, 4)
     hbm_stages = {k: roofline_entry(k, stages, n, nh, n_dyn, pairs, nc)
                   for k in ("phys/aabb", "phys/broad_phase", "phys/integrate", "phys/gravity", "bvh/radix_sort",
                             "bvh/morton_codes")}
@@ -570,7 +675,9 @@ def main() -> int:
             "tests_per_launch": int(tests), "tests_per_s": round(tests / (bp_us * 1e-6), 1),
             "vector_ops_per_test": 10, "lane_ops_per_s": round(lane_ops, 1), "peak_lane_ops_per_s": VALU_PEAK,
             "frac": round(lane_ops / VALU_PEAK, 4),
-            "note": "whole stage time (count pass, scan, placement, emit pass) against the test loop's issue"}
+            "note": "whole stage time (count pass, scan, placement, emit pass) against the test loop's issue "
+                    "rate at 10 vector instructions per test (an instruction count from the kernel's source, "
+                    "not a counter)"}
     lbvh = {"ms_per_frame": round(1000.0 * el_bvh / args.steps, 3), "primitives": n,
             "stage_us": {k: round(v[0], 1) for k, v in stages.items() if k.startswith("bvh/")}}
     log(f"LBVH over body boxes: {lbvh['ms_per_frame']:.3f} ms per frame; {lbvh['stage_us']}")
@@ -580,6 +687,7 @@ def main() -> int:
     breakdown = {k: round(100.0 * v[0] * v[1] / total_us, 1) for k, v in phys.items()}
 
     variants = sweep_variant_section(ctx, n, args.sweep_variant)
+    tri = None if args.no_tri else triangle_bvh_section(ctx, counts)
 
     cpu = cpu1 = None
     if not args.no_cpu:
@@ -611,14 +719,21 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded scene generator)",
         "config": {"workload": "32,768 bodies (+5 static) in a 30x30 m walled container, 10 substeps, "
-                               "30 PGS iterations, dt 1/60",
+                               "30 PGS iterations, dt 1/60, LBVH over the body boxes every frame",
                    "bodies": n, "hull_verts": nh, "substeps": SUBSTEPS, "pgs_iterations": PGS_ITERS,
-                   "frames_before_timing": args.warmup, "pairs_last_substep": pairs,
-                   "contacts_last_substep": contacts},
+                   "frames_before_timing": pre, "timed_frames": [pre + 1, pre + args.steps],
+                   "pairs_last_substep": pairs, "contacts_last_substep": contacts},
+        "physics_only": {"value": round(value_phys, 1), "ms_per_frame": round(1000.0 * el / args.steps, 3)},
         "roofline": roof, "roofline_hbm_stages": hbm_stages, "cpu_baseline": cpu, "cpu_baseline_one_core": cpu1,
         "stage_share_pct": breakdown,
-        "lbvh_per_frame": lbvh, "sweep_variants": variants, "warm_start": warm, "batched": batched, "differentiable": diff, "large_scene": large,
+        "lbvh_per_frame": lbvh, "sweep_variants": variants, "triangle_bvh": tri, "warm_start": warm,
+        "batched": batched, "differentiable": diff, "large_scene": large, "timed_windows_ns": WINDOWS,
     }
+    if batched is not None:
+        batched["body_substeps_vs_plain_32k"] = round(batched["body_substeps_per_s"] / value, 3)
+    if args.window_file:
+        args.window_file.parent.mkdir(parents=True, exist_ok=True)
+        args.window_file.write_text(json.dumps(WINDOWS))
     print(json.dumps(out), flush=True)
     return 0
 

@@ -270,6 +270,26 @@ int sv_debug_copy_bandwidth(sv_ctx *ctx, size_t nbytes, uint32_t iters, double *
  * words needed, out[3] allocated.  A sizing check for tests. */
 int sv_debug_broadphase_scratch(uint32_t max_bodies, uint32_t n, uint64_t out[4]);
 
+/* Depth of the last contact sweep's dependency graph and its item count:
+ * items (iteration t, contact c) in the serial sweep order of
+ * physics.comp:500-583, each depending on the previous item of each of its
+ * dynamic bodies; depth = the longest chain.  Every schedule that keeps the
+ * serial result needs at least `depth` dependent hand-offs per sweep.
+ * Synchronous; reads the prepared contacts of the last substep. */
+int sv_sweep_depth(sv_ctx *ctx, uint32_t *depth, uint64_t *items);
+
+/* One-way hand-off latency between two waves through a device-coherent word
+ * (the floor of one dependent step of the sweep): `rounds` round trips
+ * between workgroup 0 and workgroup 1 (another XCD) or 8 (same_xcd != 0,
+ * the same XCD under round-robin placement); xcc[] = the XCC ids the two
+ * waves ran on.  A test and measurement entry point. */
+int sv_debug_handoff(sv_ctx *ctx, uint32_t rounds, int same_xcd, double *us_one_way, uint32_t xcc[2]);
+
+/* Host only: the ticket streams the multi-workgroup sequence sweeps use
+ * with `wgs` workgroups (workgroup w serves stream w mod streams; stream s
+ * hands out the items of 64-item blocks s, s + streams, ... in order). */
+uint32_t sv_debug_seq_streams(uint32_t wgs);
+
 /* XCC (XCD) hardware register read by the first wave of each of `nblocks`
  * one-wave blocks (a placement probe: which XCD runs which workgroup). */
 int sv_debug_xcc_ids(sv_ctx *ctx, uint32_t nblocks, uint32_t *out);

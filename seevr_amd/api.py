@@ -213,6 +213,20 @@ class Context:
                                            vo.ctypes.data_as(u)))
         return ko, vo
 
+    def sweep_depth(self) -> tuple[int, int]:
+        """(depth, items) of the last sweep's dependency graph."""
+        d = C.c_uint32()
+        k = C.c_uint64()
+        check(self.lib.sv_sweep_depth(self.h, C.byref(d), C.byref(k)))
+        return int(d.value), int(k.value)
+
+    def handoff_us(self, rounds: int = 20000, same_xcd: bool = False) -> tuple[float, tuple[int, int]]:
+        """One-way hand-off latency (us) between two waves and their XCC ids."""
+        us = C.c_double()
+        x = (C.c_uint32 * 2)()
+        check(self.lib.sv_debug_handoff(self.h, rounds, int(same_xcd), C.byref(us), x))
+        return float(us.value), (int(x[0]), int(x[1]))
+
     def xcc_ids(self, nblocks: int) -> np.ndarray:
         """XCD each of nblocks one-wave blocks ran on (placement probe)."""
         out = np.zeros(nblocks, np.uint32)

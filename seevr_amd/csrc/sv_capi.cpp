@@ -596,6 +596,35 @@ int sv_stage_times(sv_ctx *c, int reset, const char **names, double *avg_us, uin
 
 // Host-only: the all-pairs broadphase's need at n bodies against what a
 // context of capacity max_bodies allocates.
+uint32_t sv_debug_seq_streams(uint32_t wgs) { return sv::seq_streams(wgs); }
+
+int sv_sweep_depth(sv_ctx *c, uint32_t *depth, uint64_t *items) {
+    if (!c || !depth || !items) return fail(SV_E_INVALID, "sv_sweep_depth: bad argument");
+    return sv::sweep_depth(c, depth, items);
+}
+
+int sv_debug_handoff(sv_ctx *c, uint32_t rounds, int same_xcd, double *us_one_way, uint32_t xcc[2]) {
+    if (!c || !us_one_way || !xcc || rounds == 0 || rounds > (1u << 24))
+        return fail(SV_E_INVALID, "sv_debug_handoff: bad argument");
+    uint32_t *d = nullptr;
+    HIP_OK(hipMalloc(&d, 64 * sizeof(uint32_t)));
+    uint64_t *out = reinterpret_cast<uint64_t *>(d + 16);
+    hipError_t e = hipMemset(d, 0, 64 * sizeof(uint32_t));
+    uint64_t h[5] = {};
+    if (e == hipSuccess) {
+        sv::launch_handoff(c, d, rounds, same_xcd ? 8u : 1u, out);
+        e = hipStreamSynchronize(c->stream);
+    }
+    if (e == hipSuccess) e = hipMemcpy(h, out, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(SV_E_DEVICE, "sv_debug_handoff: %s", hipGetErrorString(e));
+    if (h[4]) return fail(SV_E_TIMEOUT, "sv_debug_handoff: a wait passed 2 s");
+    *us_one_way = (double)(h[1] - h[0]) * 0.01 / (2.0 * rounds - 1.0);
+    xcc[0] = (uint32_t)h[2];
+    xcc[1] = (uint32_t)h[3];
+    return SV_OK;
+}
+
 int sv_debug_broadphase_scratch(uint32_t max_bodies, uint32_t n, uint64_t out[4]) {
     if (!out || n > max_bodies) return fail(SV_E_INVALID, "sv_debug_broadphase_scratch: bad argument");
     const bool all_pairs = n <= sv::BP_ALL_PAIRS_MAX_BODIES;

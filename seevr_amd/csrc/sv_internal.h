@@ -234,6 +234,14 @@ int radix_sort(sv_ctx *c, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1
 // After a synchronisation: whether a sort's look-back wait expired since the
 // last call (the sort's abort word, cleared by the call).
 bool sort_take_abort(sv_ctx *c);
+// ---- sv_pgs.hip ----
+// Ticket streams of the multi-workgroup sequence sweeps for `wgs`
+// workgroups: a power of two, at most 8 and at most wgs.
+uint32_t seq_streams(uint32_t wgs);
+// Depth of the last sweep's dependency DAG (items in sweep order; an item
+// depends on the previous item of each of its dynamic bodies) and its item
+// count, from the prepared contacts on the device (synchronous).
+int sweep_depth(sv_ctx *c, uint32_t *depth, uint64_t *items);
 void fill_u32(sv_ctx *c, uint32_t *p, uint32_t value, size_t n);
 void ensure_u32(uint32_t *&p, uint32_t &cap, uint32_t need);
 // Allocate scan/sort scratch for the largest sizes a context can use.
@@ -260,6 +268,7 @@ void launch_debug_atan2(sv_ctx *c, const float *y, const float *x, float *out, u
 void launch_rot(sv_ctx *c, uint32_t n);
 void launch_copy_calib(sv_ctx *c, const void *src, void *dst, size_t nbytes);
 void launch_xcc_ids(sv_ctx *c, uint32_t *out, uint32_t n);
+void launch_handoff(sv_ctx *c, uint32_t *flag, uint32_t rounds, uint32_t partner, uint64_t *out);
 void launch_pack_boxes(sv_ctx *c, uint32_t n);
 
 // ---- sv_pgs.hip ----

@@ -36,6 +36,8 @@
 // slower on a 32k-body pile).
 #include <type_traits>
 
+#include <vector>
+
 #include "sv_internal.h"
 #include "sv_math.h"
 #include "sv_dev.h"
@@ -1226,6 +1228,41 @@ uint32_t solver_lds_bodies(sv_ctx *c) {
 // some.  With one counter for every wave's refills the 32k-pile sweep took
 // 1.44 ms at 192 workgroups and 1.90 at 256; with 8 it takes 1.30 and 1.28
 // (16 streams: 1.28), and the 1M-body sweep 1.11 ms instead of 1.79.
+int sweep_depth(sv_ctx *c, uint32_t *depth, uint64_t *items) {
+    uint32_t nc = 0;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(&nc, c->s.misc + MISC_NC, sizeof(nc), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return set_error(SV_E_DEVICE, "sv_sweep_depth: device read failed");
+    if (nc > c->caps.max_contacts) nc = c->caps.max_contacts;
+    static_assert(P_B == P_A + 1 && P_FLAGS == P_A + 2, "a, b and flags are adjacent words");
+    std::vector<uint32_t> rows(3 * (size_t)nc);
+    if (nc)
+        e = hipMemcpy2D(rows.data(), 3 * sizeof(uint32_t), c->s.cprep + P_A, CPREP_WORDS * sizeof(float),
+                        3 * sizeof(uint32_t), nc, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return set_error(SV_E_DEVICE, "sv_sweep_depth: device read failed");
+    std::vector<uint32_t> level(c->caps.max_bodies, 0u);  // level of the last item on each body
+    uint32_t d = 0;
+    uint64_t cnt = 0;
+    for (uint32_t t = 0; t < c->params.pgs_iterations; t++) {
+        for (uint32_t k = 0; k < nc; k++) {
+            const uint32_t a = rows[3 * k], b = rows[3 * k + 1], fl = rows[3 * k + 2];
+            if ((fl & F_SKIP) || a >= c->caps.max_bodies || b >= c->caps.max_bodies) continue;
+            const bool da = fl & F_DYN_A, db = fl & F_DYN_B;
+            uint32_t l = 0;
+            if (da) l = level[a];
+            if (db && level[b] > l) l = level[b];
+            ++l;
+            if (da) level[a] = l;
+            if (db) level[b] = l;
+            if (l > d) d = l;
+            ++cnt;
+        }
+    }
+    *depth = d;
+    *items = cnt;
+    return SV_OK;
+}
+
 uint32_t seq_streams(uint32_t wgs) {
     uint32_t s = 8u;
     while (s > 1u && s > wgs) s >>= 1;

@@ -447,6 +447,42 @@ __global__ void k_xcc_ids(uint32_t *out, uint32_t n) {
     if (threadIdx.x == 0 && blockIdx.x < n) out[blockIdx.x] = xcc_id();
 }
 
+// One-way hand-off latency between two waves: workgroups 0 and `partner`
+// (1: the next XCD under round-robin placement; 8: the same XCD) pass a
+// device-coherent counter back and forth `rounds` times; the other
+// workgroups leave.  out[0..1] = wall-clock ticks (100 MHz) of workgroup 0's
+// start and end, out[2..3] = the two waves' XCC ids, out[4] = 1 when a wait
+// passed 2 s (the probe then stops).
+__global__ void k_handoff(uint32_t *flag, uint32_t rounds, uint32_t partner, uint64_t *out) {
+    if (threadIdx.x != 0 || (blockIdx.x != 0 && blockIdx.x != partner)) return;
+    const bool first = blockIdx.x == 0;
+    const uint64_t limit = 200000000ull;
+    out[first ? 2 : 3] = xcc_id();
+    const uint64_t t0 = wall_clock64();
+    bool stalled = false;
+    for (uint32_t k = 0; k < rounds && !stalled; k++) {
+        const uint32_t want = first ? 2u * k : 2u * k + 1u;  // the value that hands this wave the turn
+        const uint64_t w0 = wall_clock64();
+        for (;;) {
+            const uint32_t v = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (v == want) break;
+            if (v == 0xFFFFFFFFu || wall_clock64() - w0 > limit) {  // the other wave stopped, or no turn for 2 s
+                stalled = true;
+                break;
+            }
+        }
+        if (!stalled) __hip_atomic_store(flag, want + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (stalled) {
+        out[4] = 1u;
+        __hip_atomic_store(flag, 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // release the other
+    }
+    if (first) {
+        out[0] = t0;
+        out[1] = wall_clock64();
+    }
+}
+
 __global__ void k_atan2(const float *y, const float *x, float *out, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = sv_atan2f(y[i], x[i]);
@@ -612,6 +648,10 @@ void launch_copy_calib(sv_ctx *c, const void *src, void *dst, size_t nbytes) {
 void launch_xcc_ids(sv_ctx *c, uint32_t *out, uint32_t n) {
     if (!n) return;
     hipLaunchKernelGGL(k_xcc_ids, dim3(n), dim3(64), 0, c->stream, out, n);
+}
+
+void launch_handoff(sv_ctx *c, uint32_t *flag, uint32_t rounds, uint32_t partner, uint64_t *out) {
+    hipLaunchKernelGGL(k_handoff, dim3(partner + 1), dim3(64), 0, c->stream, flag, rounds, partner, out);
 }
 
 void launch_debug_atan2(sv_ctx *c, const float *y, const float *x, float *out, uint32_t n) {
