@@ -134,7 +134,7 @@ SVD int delta(const uint32_t *__restrict__ m, int i, int j, int n) {
 // bvh.comp:299-348 (Karras 2012): leaves n-1+i, internal nodes 0..n-2.
 __global__ void k_hierarchy(uint32_t nt, const uint32_t *__restrict__ m, uint32_t *__restrict__ left,
                             uint32_t *__restrict__ right, uint32_t *__restrict__ count,
-                            uint32_t *__restrict__ parent) {
+                            uint32_t *__restrict__ parent, uint2 *__restrict__ rng) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     int n = (int)nt;
     if (i < nt) {
@@ -172,6 +172,7 @@ __global__ void k_hierarchy(uint32_t nt, const uint32_t *__restrict__ m, uint32_
     count[i] = 0;
     parent[lc] = i;
     parent[rc] = i;
+    if (rng) rng[i] = make_uint2((uint32_t)first, (uint32_t)last);  // for the block-local refit
     if (i == 0) parent[0] = 0xFFFFFFFFu;
 }
 
@@ -273,6 +274,157 @@ __global__ void __launch_bounds__(BLOCK) k_propagate(uint32_t n, const uint32_t 
         node = p;
         p = pp;
     }
+}
+
+// Two-launch refit, the same boxes as the climb above.  Internal node i
+// covers a leaf range with i at one end (Karras).  Launch 1: workgroup k
+// takes leaves [kT, kT + T) (T = REFIT_T); a node whose range lies inside
+// them has its index there too, so its arrival counter and box live in LDS
+// (index i - kT), both its arrivals are threads of this workgroup (an LDS
+// counter and LDS boxes order them), and the workgroup stages its leaves'
+// boxes and parents and the topology and leaf ranges of internal nodes
+// [kT, kT + T) in LDS with one round of loads.  A thread whose node's parent
+// crosses the workgroup's leaves stops there and lists the node in the
+// workgroup's own slots of the list (T slots and a count per workgroup: no
+// shared counter); the workgroup then writes the boxes of the nodes it
+// completed in index order (coalesced).  Launch 2: one thread per listed
+// node climbs the crossing nodes, all of which are ancestors of listed
+// nodes, with the device-coherent hand-off of k_propagate (the listed
+// nodes' boxes are visible to it across the launch boundary).  No thread
+// waits, and the long climbs of launch 2 run side by side instead of holding
+// launch 1's workgroups.  Leaf ranges come from k_hierarchy.
+constexpr int REFIT_T = 512;
+constexpr int REFIT_LANES = 32;  // launch 2: threads per listed workgroup
+// List words for n leaves: T slots and a count per workgroup of launch 1.
+inline size_t refit_list_words(size_t n) { return (n + REFIT_T - 1) / REFIT_T * (REFIT_T + 1) + 1; }
+struct Box3 {
+    f3 mn, mx;
+};
+struct Box4 {
+    float4 lo, hi;
+};
+SVD Box3 box_load(const float *mn, const float *mx, uint32_t i) { return {ld3(mn, i), ld3(mx, i)}; }
+SVD Box3 box_load_coh(const float *mn, const float *mx, uint32_t i) { return {ld3c(mn, i), ld3c(mx, i)}; }
+SVD void box_store(float *mn, float *mx, uint32_t i, const Box3 &b) { st3(mn, i, b.mn); st3(mx, i, b.mx); }
+SVD void box_store_coh(float *mn, float *mx, uint32_t i, const Box3 &b) { st3c(mn, i, b.mn); st3c(mx, i, b.mx); }
+SVD Box3 box_union(const Box3 &l, const Box3 &r) { return {vmin3(l.mn, r.mn), vmax3(l.mx, r.mx)}; }  // left first
+SVD Box4 box_load(const float4 *lo, const float4 *hi, uint32_t i) { return {lo[i], hi[i]}; }
+SVD Box4 box_load_coh(const float4 *lo, const float4 *hi, uint32_t i) { return {ld4c(lo, i), ld4c(hi, i)}; }
+SVD void box_store(float4 *lo, float4 *hi, uint32_t i, const Box4 &b) { lo[i] = b.lo; hi[i] = b.hi; }
+SVD void box_store_coh(float4 *lo, float4 *hi, uint32_t i, const Box4 &b) { st4c(lo, i, b.lo); st4c(hi, i, b.hi); }
+SVD Box4 box_union(const Box4 &l, const Box4 &r) {  // see k_bb_propagate
+    const uint32_t mi = min(__float_as_uint(l.lo.w), __float_as_uint(r.lo.w));
+    const uint32_t st = __float_as_uint(l.hi.w) & __float_as_uint(r.hi.w);
+    return {make_float4(gmin(l.lo.x, r.lo.x), gmin(l.lo.y, r.lo.y), gmin(l.lo.z, r.lo.z), __uint_as_float(mi)),
+            make_float4(gmax(l.hi.x, r.hi.x), gmax(l.hi.y, r.hi.y), gmax(l.hi.z, r.hi.z), __uint_as_float(st))};
+}
+
+template <typename T, typename BoxT>
+__global__ void __launch_bounds__(REFIT_T) k_refit_blocks(uint32_t n, const uint32_t *__restrict__ parent_of,
+                                                          const uint32_t *__restrict__ left,
+                                                          const uint32_t *__restrict__ right,
+                                                          const uint2 *__restrict__ rng, T *na, T *nb,
+                                                          uint32_t *__restrict__ list) {
+    constexpr uint32_t TH = REFIT_T;
+    __shared__ uint32_t lflag[TH];
+    __shared__ BoxT lbox[2 * TH];  // [0, TH): internal nodes b0 + k; [TH, 2 TH): leaves b0 + k
+    __shared__ uint32_t lleft[TH], lright[TH], lpar[TH], lleafpar[TH];
+    __shared__ uint2 lrng[TH];
+    __shared__ uint32_t s_ntop;
+    const uint32_t b0 = blockIdx.x * TH, t = threadIdx.x, i = b0 + t;
+    lflag[t] = 0u;
+    if (t == 0) s_ntop = 0u;
+    BoxT box;
+    if (i < n) {
+        box = box_load(na, nb, n - 1 + i);  // leaf box of the previous launch
+        lbox[TH + t] = box;
+        lleafpar[t] = parent_of[n - 1 + i];
+    }
+    if (i + 1 < n) {
+        lleft[t] = left[i];
+        lright[t] = right[i];
+        lpar[t] = parent_of[i];
+        lrng[t] = rng[i];
+    }
+    __syncthreads();
+    if (i < n) {
+        uint32_t node = n - 1 + i;
+        uint32_t p = lleafpar[t];
+        for (int depth = 0; depth < 80; depth++) {
+            if (p == 0xFFFFFFFFu) break;  // the root, inside this workgroup's leaves (its box is in LDS)
+            bool plocal = false;
+            if (p >= b0 && p - b0 < TH) {
+                const uint2 r = lrng[p - b0];
+                plocal = r.x >= b0 && r.y < b0 + TH;
+            }
+            if (!plocal) {  // p crosses the workgroup's leaves: launch 2 goes on from node
+                list[(size_t)blockIdx.x * (TH + 1) + 1 + atomicAdd(&s_ntop, 1u)] = node;
+                break;
+            }
+            // p is local, so its topology is staged.  LDS only: this
+            // thread's box write has completed before its bump (LDS is one
+            // memory for the workgroup, and a wave's LDS accesses run in
+            // order), and the sibling's box is read after the bump returned.
+            const uint32_t li = lleft[p - b0], ri = lright[p - b0], pp = lpar[p - b0];
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            const uint32_t old = __hip_atomic_fetch_add(&lflag[p - b0], 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            asm volatile("" ::: "memory");
+            if (old == 0u) break;
+            const bool is_left = li == node;
+            const uint32_t sib = is_left ? ri : li;
+            const BoxT sbox = sib >= n - 1 ? lbox[TH + (sib - (n - 1) - b0)] : lbox[sib - b0];
+            box = is_left ? box_union(box, sbox) : box_union(sbox, box);
+            lbox[p - b0] = box;
+            node = p;
+            p = pp;
+        }
+    }
+    __syncthreads();
+    // the internal nodes completed here (both arrivals counted), written out
+    // in index order: consecutive threads store consecutive boxes
+    if (i + 1 < n && lflag[t] == 2u) box_store(na, nb, i, lbox[t]);
+    if (t == 0) list[(size_t)blockIdx.x * (TH + 1)] = s_ntop;
+}
+
+template <typename T, typename BoxT>
+__global__ void __launch_bounds__(BLOCK) k_refit_cross(uint32_t n, const uint32_t *__restrict__ parent_of,
+                                                       const uint32_t *__restrict__ left,
+                                                       const uint32_t *__restrict__ right, uint32_t *flags, T *na,
+                                                       T *nb, const uint32_t *__restrict__ list) {
+    const uint32_t nw = (n + REFIT_T - 1) / REFIT_T;  // workgroups of launch 1
+    const uint32_t w = blockIdx.x * (BLOCK / REFIT_LANES) + threadIdx.x / REFIT_LANES;
+    if (w >= nw) return;
+    const uint32_t *slot = list + (size_t)w * (REFIT_T + 1);
+    const uint32_t m = slot[0];
+    for (uint32_t k = threadIdx.x % REFIT_LANES; k < m; k += REFIT_LANES) {
+        uint32_t node = slot[1 + k];
+        BoxT box = box_load(na, nb, node);  // written by launch 1
+        uint32_t p = parent_of[node];
+        for (int depth = 0; p != 0xFFFFFFFFu && depth < 80; depth++) {
+            const uint32_t li = left[p], ri = right[p], pp = parent_of[p];
+            if (!arrive_second(flags + p, li, ri, pp)) break;
+            const bool is_left = li == node;
+            const uint32_t sib = is_left ? ri : li;
+            const BoxT sbox = box_load_coh(na, nb, sib);
+            box = is_left ? box_union(box, sbox) : box_union(sbox, box);
+            box_store_coh(na, nb, p, box);
+            node = p;
+            p = pp;
+        }
+    }
+}
+
+// Both launches of the two-launch refit, after k_clear_flags (which clears
+// the arrival counters); `list` holds refit_list_words(n) words.
+template <typename T, typename BoxT>
+void launch_refit(hipStream_t st, uint32_t n, const uint32_t *parent_of, const uint32_t *left, const uint32_t *right,
+                  const uint2 *rng, uint32_t *flags, T *na, T *nb, uint32_t *list) {
+    const uint32_t nw = div_up(n, REFIT_T);
+    hipLaunchKernelGGL((k_refit_blocks<T, BoxT>), dim3(nw), dim3(REFIT_T), 0, st, n, parent_of, left, right, rng, na,
+                       nb, list);
+    hipLaunchKernelGGL((k_refit_cross<T, BoxT>), dim3(div_up(nw, BLOCK / REFIT_LANES)), dim3(BLOCK), 0, st, n,
+                       parent_of, left, right, flags, na, nb, (const uint32_t *)list);
 }
 
 __global__ void k_copy_u32(uint32_t n, const uint32_t *src, uint32_t *dst) {
@@ -616,6 +768,8 @@ bool bb_reserve(sv_ctx *c) {
     if (e == hipSuccess) e = lazy_alloc(c, s.bb_fill, N + 1);
     if (e == hipSuccess) e = lazy_alloc(c, s.bb_stage, (size_t)BB_STAGE * N + 1);
     if (e == hipSuccess) e = lazy_alloc(c, s.bb_scnt, N + 1);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_rng, N + 1);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_tops, refit_list_words(N));
     s.bb_ready = e == hipSuccess;
     return s.bb_ready;
 }
@@ -644,11 +798,11 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
     int w = radix_sort(c, s.bb_mk[0], s.bb_mv[0], s.bb_mk[1], s.bb_mv[1], nullptr, n, 30);
     const uint32_t *ids = s.bb_mv[w];
     hipLaunchKernelGGL(k_hierarchy, dim3(g), dim3(BLOCK), 0, st, n, s.bb_mk[w], s.bb_left, s.bb_right, s.bb_count,
-                       s.bb_parent);
+                       s.bb_parent, s.bb_rng);
     hipLaunchKernelGGL(k_bb_leaves, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_lo, s.bb_hi);
     hipLaunchKernelGGL(k_clear_flags, dim3(g), dim3(BLOCK), 0, st, n, s.bb_flags);
-    hipLaunchKernelGGL(k_bb_propagate, dim3(g), dim3(BLOCK), 0, st, n, s.bb_parent, s.bb_left, s.bb_right,
-                       s.bb_flags, s.bb_lo, s.bb_hi);
+    launch_refit<float4, Box4>(st, n, s.bb_parent, s.bb_left, s.bb_right, (const uint2 *)s.bb_rng, s.bb_flags,
+                               reinterpret_cast<float4 *>(s.bb_lo), reinterpret_cast<float4 *>(s.bb_hi), s.bb_tops);
     // count, scan, fill, sort
     const int bits = 32 - __builtin_clz(n - 1);  // body indices < 2^bits
     const bool combined = 2 * bits <= 32;
@@ -720,10 +874,30 @@ void bvh_stage(sv_ctx *c, uint32_t step, uint32_t n, bool bodies) {
         }
         break;
     }
-    case 4:
+    case 4: {
+        // leaf ranges for the block-local refit (none with fixed scratch:
+        // step 7 then climbs with device-coherent hand-offs only)
+        Scratch &sc = c->s;
+        sc.bvh_rng_n = 0;
+        if (!c->scratch_fixed && sc.bvh_rng_cap < n) {
+            void *q = nullptr, *r = nullptr;
+            if (hipMalloc(&q, (size_t)n * sizeof(uint2)) == hipSuccess) sc.lazy.push_back(q);
+            else q = nullptr;
+            if (q && hipMalloc(&r, refit_list_words(n) * sizeof(uint32_t)) == hipSuccess) sc.lazy.push_back(r);
+            else r = nullptr;
+            if (q && r) {
+                sc.bvh_rng = static_cast<uint2 *>(q);
+                sc.bvh_tops = static_cast<uint32_t *>(r);
+                sc.bvh_rng_cap = n;
+            }
+        }
+        uint2 *rng = (!c->scratch_fixed && sc.bvh_rng_cap >= n) ? sc.bvh_rng : nullptr;
         hipLaunchKernelGGL(k_hierarchy, dim3(g), dim3(BLOCK), 0, s, n, morton, B<uint32_t>(c, SV_BVH_LEFT),
-                           B<uint32_t>(c, SV_BVH_RIGHT), B<uint32_t>(c, SV_BVH_COUNT), B<uint32_t>(c, SV_BVH_PARENT));
+                           B<uint32_t>(c, SV_BVH_RIGHT), B<uint32_t>(c, SV_BVH_COUNT), B<uint32_t>(c, SV_BVH_PARENT),
+                           rng);
+        if (rng) sc.bvh_rng_n = n;
         break;
+    }
     case 5:
         if (bodies)
             hipLaunchKernelGGL(k_leaf_bodies, dim3(g), dim3(BLOCK), 0, s, n, sidx, B<float>(c, SV_BODY_AABB_MIN),
@@ -738,9 +912,17 @@ void bvh_stage(sv_ctx *c, uint32_t step, uint32_t n, bool bodies) {
         hipLaunchKernelGGL(k_clear_flags, dim3(g), dim3(BLOCK), 0, s, n, B<uint32_t>(c, SV_BVH_FLAGS));
         break;
     case 7:  // needs step 6's cleared arrival counters
-        hipLaunchKernelGGL(k_propagate, dim3(g), dim3(BLOCK), 0, s, n, B<uint32_t>(c, SV_BVH_PARENT),
-                           B<uint32_t>(c, SV_BVH_LEFT), B<uint32_t>(c, SV_BVH_RIGHT), B<uint32_t>(c, SV_BVH_FLAGS),
-                           B<float>(c, SV_BVH_AABB_MIN), B<float>(c, SV_BVH_AABB_MAX));
+        if (c->s.bvh_rng && c->s.bvh_rng_n == n)  // ranges of this tree (step 4 on this context)
+            launch_refit<float, Box3>(s, n, B<uint32_t>(c, SV_BVH_PARENT), B<uint32_t>(c, SV_BVH_LEFT),
+                                      B<uint32_t>(c, SV_BVH_RIGHT), (const uint2 *)c->s.bvh_rng,
+                                      B<uint32_t>(c, SV_BVH_FLAGS), B<float>(c, SV_BVH_AABB_MIN),
+                                      B<float>(c, SV_BVH_AABB_MAX), c->s.bvh_tops);
+        else
+            hipLaunchKernelGGL(k_propagate, dim3(g), dim3(BLOCK), 0, s, n, B<uint32_t>(c, SV_BVH_PARENT),
+                               B<uint32_t>(c, SV_BVH_LEFT), B<uint32_t>(c, SV_BVH_RIGHT),
+                               B<uint32_t>(c, SV_BVH_FLAGS), B<float>(c, SV_BVH_AABB_MIN),
+                               B<float>(c, SV_BVH_AABB_MAX));
+        c->s.bvh_rng_n = 0;  // ranges serve the refit right after their step 4 only
         break;
     default:
         break;  // 8..11: folded into the stages above

@@ -327,6 +327,7 @@ int sv_buffer_ptr(sv_ctx *c, sv_buf id, void **dev, size_t *bytes) {
 
 int sv_bind_buffer(sv_ctx *c, sv_buf id, void *dev, size_t nbytes) {
     if (!c || !valid_buf(id) || !dev) return fail(SV_E_INVALID, "sv_bind_buffer: bad context, buffer id or pointer");
+    c->s.bvh_rng_n = 0;
     size_t need[SV_BUF_COUNT];
     buffer_sizes(c->caps, need);
     if (nbytes < need[id])
@@ -346,6 +347,7 @@ int sv_bind_buffer(sv_ctx *c, sv_buf id, void *dev, size_t nbytes) {
 
 int sv_upload(sv_ctx *c, sv_buf id, const void *src, size_t off, size_t n) {
     if (!c || !valid_buf(id) || (!src && n)) return fail(SV_E_INVALID, "sv_upload: bad argument");
+    c->s.bvh_rng_n = 0;  // host data may replace the tree the leaf ranges describe
     if (off > c->bytes[id] || n > c->bytes[id] - off)
         return fail(SV_E_RANGE, "sv_upload: [%zu, %zu) outside buffer %d of %zu bytes", off, off + n, (int)id,
                     c->bytes[id]);

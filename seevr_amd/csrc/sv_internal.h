@@ -55,7 +55,14 @@ struct Scratch {
     uint32_t *bb_pk[2] = {nullptr, nullptr}, *bb_pv[2] = {nullptr, nullptr};  // [P + N] pair sort
     uint32_t *bb_stage = nullptr;   // uint[BB_STAGE * N] rows staged by each counting lane
     uint32_t *bb_scnt = nullptr;    // uint[N] rows found by each counting lane
+    uint2 *bb_rng = nullptr;        // uint2[N] leaf range (first, last) of each internal node
+    uint32_t *bb_tops = nullptr;    // refit list (sv_bvh.hip): per workgroup a count and its nodes
     bool bb_ready = false;
+    // leaf ranges of the BVH build's internal nodes (allocated on first use;
+    // bvh_rng_n = the leaf count they were computed for, 0 = none valid)
+    uint2 *bvh_rng = nullptr;
+    uint32_t *bvh_tops = nullptr;  // refit list (sv_bvh.hip): per workgroup a count and its nodes
+    uint32_t bvh_rng_cap = 0, bvh_rng_n = 0;
     // graph-coloured sweep and residual (allocated on first use)
     unsigned long long *col_claim = nullptr;  // [N] colour-round claims per body
     uint32_t *col_color = nullptr;            // [C] colour of each contact
