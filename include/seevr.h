@@ -222,7 +222,7 @@ typedef struct sv_bvh_arrays {
 } sv_bvh_arrays;
 
 /* Largest body count of sv_bvh_build_bodies_noalloc: its scratch is a fixed
- * array of the library's code object (about 15 MB per GPU, held until the
+ * array of the library's code object (about 14 MB per GPU, held until the
  * process ends). */
 #define SV_NOALLOC_MAX_BODIES (1u << 20)
 
