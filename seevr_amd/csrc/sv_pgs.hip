@@ -803,27 +803,35 @@ SVD uint32_t decode_half(Half &h, const RawHalf &r, uint32_t c, uint32_t t, uint
 // item_update (sv_dev.h) split over the pair: l, w are this lane's body's
 // velocity and angular velocity; side 0 is body a, side 1 body b.  Both
 // lanes return the new impulse.
+//
+// Each lane works with u = (other body's contact-point velocity) - (its
+// own): vb - va on body a's lane, va - vb on body b's.  IEEE rounding is
+// odd (RN(-x) = -RN(x)) and x - y = x + (-y), so the reference's relative
+// velocity, tangent and impulses are u's and this lane's quantities up to a
+// sign per lane, and the reference's "a minus, b plus" updates become one
+// addition on both lanes.  (Only the sign of an exact zero can differ in
+// between; it reaches a stored value only when a velocity component is -0,
+// which the step never produces: gravity adds +0 terms and every rounded
+// sum that cancels exactly is +0.)
 SVD float half_update(const Half &h, bool side, f3 &l, f3 &w, float old) {
     f3 vs = l + cross3(w, h.r);
-    f3 vo = pair_swap3(vs);
-    f3 va = side ? vo : vs, vb = side ? vs : vo;
-    const float vn = dot3(vb - va, h.n);
+    f3 u = pair_swap3(vs) - vs;
+    const float du = dot3(u, h.n);
+    const float vn = side ? -du : du;  // dot(vb - va, n)
     float dln = (-vn + h.bias) / h.kn;
     const float nl = gmax(old + dln, 0.0f);
     dln = nl - old;
     const f3 imp = dln * h.n;
     // a: la - ima imp, wa - I(ra x imp); b: lb + imb imp, wb + I(rb x imp)
-    const f3 d = h.im * imp;
-    l = l + (side ? d : -d);
-    const f3 e = apply_inv_I(h.R, h.ii, cross3(h.r, imp));
-    w = w + (side ? e : -e);
+    const f3 ims = sel3(side, imp, -imp);
+    l = l + h.im * ims;
+    w = w + apply_inv_I(h.R, h.ii, cross3(h.r, ims));
 
     vs = l + cross3(w, h.r);
-    vo = pair_swap3(vs);
-    va = side ? vo : vs;
-    vb = side ? vs : vo;
-    const f3 dv = vb - va;
-    const f3 vt = dv - dot3(dv, h.n) * h.n;
+    u = pair_swap3(vs) - vs;
+    // the tangent as this lane sees it: the reference's on body a's lane,
+    // its negation on body b's
+    const f3 vt = u - dot3(u, h.n) * h.n;
     const float vtl = len3(vt);
     if (vtl > 1e-6f) {
         const f3 td = mk(vt.x / vtl, vt.y / vtl, vt.z / vtl);
@@ -833,12 +841,11 @@ SVD float half_update(const Half &h, bool side, f3 &l, f3 &w, float old) {
         const float Ta = side ? To : T, Tb = side ? T : To;
         const float kt = h.ima + h.imb + Ta + Tb;
         const float dlt = gmin(vtl / kt, h.mu * nl);
+        // a: la + ima impt, wa + I(ra x impt); b: lb - imb impt,
+        // wb - I(rb x impt): with this lane's tangent, additions on both
         const f3 impt = dlt * td;
-        // a: la + ima impt, wa + I(ra x impt); b: lb - imb impt, wb - I(rb x impt)
-        const f3 d2 = h.im * impt;
-        l = l + (side ? -d2 : d2);
-        const f3 e2 = apply_inv_I(h.R, h.ii, cross3(h.r, impt));
-        w = w + (side ? -e2 : e2);
+        l = l + h.im * impt;
+        w = w + apply_inv_I(h.R, h.ii, cross3(h.r, impt));
     }
     return nl;
 }
