@@ -175,7 +175,7 @@ def cpu_baseline(ctx, scene, n, substeps: int, mt: bool = True) -> dict:
 TRI_STAGES = ("bvh/centroids", "bvh/scene_bounds", "bvh/morton_codes", "bvh/radix_sort", "bvh/build_tree",
               "bvh/leaf_aabbs", "bvh/clear_flags", "bvh/propagate")
 TRI_KERNELS = {"bvh/centroids": {"k_centroids": 1}, "bvh/morton_codes": {"k_morton": 1},
-               "bvh/radix_sort": {"k_os_hist<4>": 1, "k_os_pass<512, 16>": 4}}
+               "bvh/radix_sort": {"k_os_hist<4>": 1, "k_os_pass<512, 8>": 4}}
 
 
 def triangle_bvh_section(ctx, counts: dict, reps: int = 5) -> dict:
@@ -346,6 +346,7 @@ def differentiable_section(frames: int = 60, scenes: int = 256, bodies: int = 64
     target = b.packed_state()[: 3 * n] + rng.uniform(-0.5, 0.5, 3 * n).astype(np.float32)
     fwd, bwd = [], []
     c = None
+    w0 = time.monotonic_ns()
     for _ in range(reps):
         b.set_packed_state(x0)
         t0 = time.perf_counter()
@@ -358,6 +359,7 @@ def differentiable_section(frames: int = 60, scenes: int = 256, bodies: int = 64
         t0 = time.perf_counter()
         gstate, gfric, grest = b.diff_backward(frames, g, SUBSTEPS, DT)
         bwd.append(time.perf_counter() - t0)
+    window("diff", w0, time.monotonic_ns())
     b.close()
     tf, tb = min(fwd), min(bwd)
     gmu = np.zeros((scenes, 2))

@@ -32,7 +32,7 @@ CALIB = "k_copy_calib"
 
 def short_name(name: str) -> str:
     """'void sv::(anonymous namespace)::k_os_pass<512, 16>(unsigned int const*, ...)' -> 'k_os_pass<512, 16>'."""
-    n = name.replace("(anonymous namespace)", "")
+    n = name.replace("(anonymous namespace)::", "").replace("(anonymous namespace)", "")
     n = re.sub(r"^void\s+", "", n)
     depth = 0
     for i, ch in enumerate(n):  # cut the argument list (the first '(' outside template brackets)
@@ -43,7 +43,16 @@ def short_name(name: str) -> str:
         elif ch == "(" and depth == 0:
             n = n[:i]
             break
-    return n.split("::")[-1].strip()
+    # the last '::' component outside template brackets ('k_refit<sv::Box3>' stays whole)
+    depth, cut = 0, 0
+    for i, ch in enumerate(n):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == ":" and depth == 0 and n[i:i + 2] == "::":
+            cut = i + 2
+    return n[cut:].strip()
 
 
 def load_window(d: Path, window: str) -> tuple[int, int]:
@@ -108,7 +117,9 @@ def main() -> None:
     lines = [f"# {a.title}", "", "Commands (MI355X, one GPU; run from /tmp with TMPDIR=/tmp; R = repo root):", ""]
     lines += [f"    {c}" for c in a.command]
     lines += [""] + [n for n in a.note] + ([""] if a.note else [])
-    lines += [f"Window `{a.window}`: dispatches that start inside the bench's timed frames of that section "
+    lines += [f"Summarized by `python3 -m seevr_amd.profsum --window {a.window} --trace DIR/kt --fetch DIR/fetch "
+              "--write DIR/write`.", "",
+              f"Window `{a.window}`: dispatches that start inside the bench's timed frames of that section "
               "(CLOCK_MONOTONIC bounds the bench writes to window.json).", "",
               "## Kernel time (kernel trace, dispatches in the window)", "",
               "| kernel | calls | total us | avg us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
