@@ -21,6 +21,18 @@
  * per-contact constants, of the rotation matrices (through the quaternion),
  * of gravity and of integration.  Material adjoints are per geom (geom i is
  * body i).
+ *
+ * adj_set_implicit(1) linearizes every reversed pass at the last pass's
+ * inputs instead (the converged sweep, as implicit differentiation at the
+ * Gauss-Seidel fixed point would): the reverse sweep is then the fixed-point
+ * iteration y <- J^T y + g of (I - J)^T y = g.  For this sweep that is not
+ * the derivative of the computation: at the fixed point every item's normal
+ * update is zero, so the linearization there loses the derivative of the
+ * impulse already applied through the lever arms, rotations and k_n, and the
+ * friction update, which is recomputed each pass rather than accumulated,
+ * is switched off there (|v_t| <= 1e-6) although it acted in earlier
+ * passes.  It exists so that tests can show this against finite
+ * differences; the library's adjoint is the unrolled one.
  */
 #include <math.h>
 #include <stdint.h>
@@ -74,6 +86,8 @@ static void inv_I_rbar(m3 *Rb, m3 R_, v3 d, v3 x, v3 yb, R s) {
  * evaluated in double. */
 static int g_f32 = 0;
 void adj_set_forward32(int on) { g_f32 = on != 0; }
+static int g_implicit = 0;
+void adj_set_implicit(int on) { g_implicit = on != 0; }
 static inline R rf(R x) { return g_f32 ? (R)(float)x : x; }
 static inline v3 fadd(v3 a, v3 b) { return V(rf(a.x + b.x), rf(a.y + b.y), rf(a.z + b.z)); }
 static inline v3 fsub(v3 a, v3 b) { return V(rf(a.x - b.x), rf(a.y - b.y), rf(a.z - b.z)); }
@@ -412,7 +426,7 @@ void adj_backward(const adj_scene *s, const R *tape, R *ap, R *aq, R *av, R *aw,
             const cst *k = &K[c];
             if (k->skip) continue;
             uint32_t a = s->ca[c], b = s->cb[c];
-            const R *tp = tape + ((size_t)t * nc + c) * 13;
+            const R *tp = tape + ((size_t)(g_implicit ? (int64_t)s->iters - 1 : t) * nc + c) * 13;
             m3 Ra = fquat_mat(s->quat + 4 * a), Rb = fquat_mat(s->quat + 4 * b);
             v3 zero = V(0, 0, 0);
             item_adj o;

@@ -44,6 +44,8 @@ def lib() -> C.CDLL:
         _lib.adj_backward.argtypes = [C.POINTER(AdjScene), dp, dp, dp, dp, dp, dp, dp]
         _lib.adj_set_forward32.restype = None
         _lib.adj_set_forward32.argtypes = [C.c_int]
+        _lib.adj_set_implicit.restype = None
+        _lib.adj_set_implicit.argtypes = [C.c_int]
     return _lib
 
 
@@ -95,30 +97,38 @@ class Problem:
                 saved.append((start, tape))
         return st, saved, fric, rest
 
-    def grad(self, x0: dict, fric, rest, gout: dict):
-        """Adjoint of the final state gout -> (adjoint of x0, d/dfric, d/drest)."""
+    def grad(self, x0: dict, fric, rest, gout: dict, implicit: bool = False):
+        """Adjoint of the final state gout -> (adjoint of x0, d/dfric, d/drest);
+        implicit: the sweep differentiated at its converged solution
+        (adj_set_implicit) instead of through every pass."""
         _, saved, fric, rest = self.forward(x0, fric, rest, keep=True)
         adj = {k: np.array(gout[k], np.float64) for k in STATE}
         af = np.zeros(self.n)
         ae = np.zeros(self.n)
-        for (start, tape), c in zip(reversed(saved), reversed(self.contacts)):
-            sc = self._scene(start, fric, rest, c)
-            lib().adj_backward(C.byref(sc), _p(tape), _p(adj["pos"]), _p(adj["quat"]), _p(adj["vel"]),
-                               _p(adj["omega"]), _p(af), _p(ae))
+        lib().adj_set_implicit(1 if implicit else 0)
+        try:
+            for (start, tape), c in zip(reversed(saved), reversed(self.contacts)):
+                sc = self._scene(start, fric, rest, c)
+                lib().adj_backward(C.byref(sc), _p(tape), _p(adj["pos"]), _p(adj["quat"]), _p(adj["vel"]),
+                                   _p(adj["omega"]), _p(af), _p(ae))
+        finally:
+            lib().adj_set_implicit(0)
         return adj, af, ae
 
 
-    def grad_along(self, starts: list[dict], fric, rest, gout: dict, forward32: bool = True):
+    def grad_along(self, starts: list[dict], fric, rest, gout: dict, forward32: bool = True,
+                   implicit: bool = False):
         """The adjoint linearised along a given trajectory: starts[s] is the
         state at the start of substep s (for instance an FP32 rollout's, the
         device's own), each substep's Jacobian is taken there, and gout is the
         adjoint of the state after the last substep.  forward32 evaluates the
         forward values and their branches as the FP32 sweep does
-        (adj_set_forward32), the reverse in FP64."""
+        (adj_set_forward32), the reverse in FP64; implicit as in grad."""
         fric = np.ascontiguousarray(fric, np.float64)
         rest = np.ascontiguousarray(rest, np.float64)
         L = lib()
         L.adj_set_forward32(1 if forward32 else 0)
+        L.adj_set_implicit(1 if implicit else 0)
         try:
             tapes = []
             for st0, c in zip(starts, self.contacts):
@@ -138,6 +148,7 @@ class Problem:
             return adj, af, ae
         finally:
             L.adj_set_forward32(0)
+            L.adj_set_implicit(0)
 
 
 def record_fp32(world, substeps: int, dt_sub: float, g) -> tuple[dict, list[dict], dict, list[dict]]:

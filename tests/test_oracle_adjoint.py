@@ -90,3 +90,84 @@ def test_adjoint_matches_finite_differences(seed):
     assert not bad, f"adjoint vs finite differences: {bad[:6]}"
     assert np.count_nonzero(np.abs(np.array([c[1] for c in checks if c[0].startswith('fric')])) > 1e-9) > 0, \
         "some contact should slide (friction gradient non-zero)"
+
+
+def one_contact_problem(mu: float, kick: float):
+    """One dynamic body settled on the ground (one contact per substep),
+    per-geom friction mu, a sideways kick; one substep of the fixed-contact
+    map."""
+    sc = make_scene(4, 1, seed=3, segments=4)
+    w = World.from_scene(sc, max_pairs=4000, max_contacts=2000, pgs_iterations=30, double=True)
+    for _ in range(60):
+        w.oracle_step(SUB, DT, G)
+    n = w.n
+    w.a["GEOM_FRICTION"][:n] = mu
+    w.a["GEOM_RESTITUTION"][:n] = np.linspace(0.1, 0.3, n)
+    dyn = w.a["BODY_INV_MASS"][:n] > 0
+    w.a["BODY_VEL"][: 3 * n].reshape(n, 3)[dyn, 0] += kick
+    dt_sub = float(np.float32(DT) / np.float32(SUB))
+    x0, contacts, statics = record_base(w, 1, dt_sub, G)
+    assert [c["nc"] for c in contacts] == [1]
+    return Problem(statics, contacts, dt_sub, G, iters=30), x0, statics
+
+
+def central_difference(prob, x0, fric, rest, wts, key, i, h=1e-6):
+    if key in STATE:
+        xp = {k: v.copy() for k, v in x0.items()}
+        xm = {k: v.copy() for k, v in x0.items()}
+        xp[key][i] += h
+        xm[key][i] -= h
+        return (loss(prob, xp, fric, rest, wts) - loss(prob, xm, fric, rest, wts)) / (2 * h)
+    fp, fm = fric.copy(), fric.copy()
+    fp[i] += h
+    fm[i] -= h
+    return (loss(prob, x0, fp, rest, wts) - loss(prob, x0, fm, rest, wts)) / (2 * h)
+
+
+def test_linearizing_at_the_converged_sweep_loses_the_applied_impulse():
+    """Implicit differentiation at the Gauss-Seidel fixed point (every
+    reversed pass linearized at the last pass's inputs, adj_set_implicit)
+    against the exact reverse and FP64 central differences, for one
+    frictionless contact that the first pass already solves.  The velocity
+    adjoints agree: the item map is affine in the velocities and the impulse
+    while the projection and the friction branch keep their sides.  The
+    position and rotation adjoints do not: at the fixed point the normal
+    update is zero, so the derivative of the impulse that pass 0 applied
+    through the lever arm, the rotation and k_n is gone.  The unrolled
+    reverse, the library's adjoint, matches the differences."""
+    prob, x0, st = one_contact_problem(mu=0.0, kick=0.0)
+    fric, rest = st["fric"].copy(), st["rest"].copy()
+    rng = np.random.default_rng(1)
+    wts = {k: rng.normal(size=x0[k].size) for k in STATE}
+    gu, _, _ = prob.grad(x0, fric, rest, wts)
+    gi, _, _ = prob.grad(x0, fric, rest, wts, implicit=True)
+    for k in ("vel", "omega"):
+        np.testing.assert_allclose(gi[k], gu[k], rtol=1e-12, atol=1e-12)
+    body = int(np.nonzero(st["inv_mass"] > 0)[0][0])
+    worst_implicit = 0.0
+    for key, width in (("pos", 3), ("quat", 4)):
+        for comp in range(width):
+            i = width * body + comp
+            fd = central_difference(prob, x0, fric, rest, wts, key, i)
+            scale = max(abs(fd), 1e-3)
+            assert abs(gu[key][i] - fd) < 1e-6 * scale + 1e-9, f"unrolled d/d{key}[{i}] {gu[key][i]} vs fd {fd}"
+            worst_implicit = max(worst_implicit, abs(gi[key][i] - fd) / scale)
+    assert worst_implicit > 1e-2, "the fixed-point linearization should miss the applied impulse's derivative"
+
+
+def test_linearizing_at_the_converged_sweep_drops_the_friction():
+    """A sliding contact whose friction (recomputed each pass, never
+    accumulated, physics.comp:560-580) acts in the early passes: the exact
+    reverse's friction gradient matches FP64 central differences, and the
+    fixed-point linearization's is off by more than half of it (the friction
+    branch of the last pass is not what acted in the first)."""
+    prob, x0, st = one_contact_problem(mu=0.5, kick=0.6)
+    fric, rest = st["fric"].copy(), st["rest"].copy()
+    rng = np.random.default_rng(2)
+    wts = {k: rng.normal(size=x0[k].size) for k in STATE}
+    _, af_u, _ = prob.grad(x0, fric, rest, wts)
+    _, af_i, _ = prob.grad(x0, fric, rest, wts, implicit=True)
+    fd = np.array([central_difference(prob, x0, fric, rest, wts, "fric", i) for i in range(prob.n)])
+    assert np.abs(fd).max() > 1e-5, "the contact should slide under friction"
+    np.testing.assert_allclose(af_u, fd, rtol=1e-4)  # central differences of a ~1e-4 gradient: ~1e-5
+    assert np.linalg.norm(af_i - fd) > 0.5 * np.linalg.norm(fd)
