@@ -258,6 +258,26 @@ def batched_section(frames: int, warmup: int, scenes: int = 4096, plain_sample: 
     window("batched", w0, time.monotonic_ns())
     c = b.counters()
     env_steps = scenes * frames / el
+    # where a settled substep goes: the kernel's phases by the device clock
+    # (each workgroup's time per phase, summed), over 3 more frames
+    b.set_phase_timing(True)
+    t1 = time.perf_counter()
+    for _ in range(3):
+        b.step(nv, SUBSTEPS, DT)
+    el_ph = (time.perf_counter() - t1) / 3
+    ph, n_sc = b.phase_times()
+    b.set_phase_timing(False)
+    tot_ph = sum(ph.values())
+    grid = min(scenes, wgs * 256)  # the persistent grid: workgroups per CU x the MI355X's 256 CUs
+    phase = {"share_pct": {k: round(100 * v / tot_ph, 1) for k, v in ph.items()},
+             "us_per_scene_substep": {k: round(v / max(n_sc, 1), 2) for k, v in ph.items()},
+             "workgroup_busy_fraction": round(tot_ph * 1e-6 / (grid * 3 * el_ph), 3),
+             "timed_ms_per_frame": round(1000 * el_ph, 2), "scene_substeps": n_sc,
+             "note": "device wall clock (100 MHz) read by each workgroup's first thread after every phase; "
+                     "busy fraction = summed workgroup time / (workgroups x wall time)"}
+    log("batched phases (share of workgroup time): " +
+        ", ".join(f"{k} {v}%" for k, v in phase["share_pct"].items()) +
+        f"; workgroups busy {phase['workgroup_busy_fraction']:.2f} of the wall time")
     # plain path, one scene at a time, on an evenly spaced sample, each
     # started from that scene's current batched state
     ks = list(range(0, scenes, max(scenes // plain_sample, 1)))[:plain_sample]
@@ -295,7 +315,7 @@ def batched_section(frames: int, warmup: int, scenes: int = 4096, plain_sample: 
             "launch_shape": {"threads": threads, "workgroups_per_cu": wgs, "autotune_us_per_substep": round(us, 1)},
             "plain_one_by_one": {"env_steps_per_s": round(plain_env, 1), "sample_scenes": len(ks),
                                  "sample_mean_bodies": round(bodies / len(ks), 1)},
-            "speedup_vs_plain": round(env_steps / plain_env, 1)}
+            "speedup_vs_plain": round(env_steps / plain_env, 1), "phases": phase}
 
 
 def differentiable_section(frames: int = 60, scenes: int = 256, bodies: int = 64, reps: int = 3) -> dict:

@@ -482,6 +482,16 @@ int sv_batch_set_config(sv_batch *batch, uint32_t threads, uint32_t workgroups_p
  * keeps the fastest; returns it and its time per substep. */
 int sv_batch_autotune(sv_batch *batch, uint32_t substeps, float dt, uint32_t *threads,
                       uint32_t *workgroups_per_cu, double *us_per_substep);
+/* Phase timing of the batched substep kernel (a library extension, for
+ * profiling).  While on, each workgroup's first thread reads the device's
+ * 100 MHz wall clock after every phase and adds the phase's time to a
+ * device total when it leaves.  The phases are: taking the next scene;
+ * gravity and AABB; broadphase; narrowphase; sweep setup (constants and
+ * incidence ranks); the ordered sweep; write-back and integration.
+ * sv_batch_phase_times returns the 7 totals in microseconds summed over
+ * workgroups, and the number of scenes processed, then zeroes them. */
+int sv_batch_set_phase_timing(sv_batch *batch, int on);
+int sv_batch_phase_times(sv_batch *batch, double us[7], uint64_t *scenes);
 
 /* ---- differentiable batched rollout (library extension) ---------------
  * Gradients of `frames` frames of every scene with respect to the initial

@@ -432,6 +432,21 @@ class Batch:
         check(self.lib.sv_batch_autotune(self.h, substeps, C.c_float(dt), C.byref(t), C.byref(w), C.byref(us)))
         return t.value, w.value, us.value
 
+    PHASES = ("next scene", "gravity+aabb", "broadphase", "narrowphase", "sweep setup", "sweep",
+              "write-back+integrate")
+
+    def set_phase_timing(self, on: bool = True) -> None:
+        """Device wall-clock timing of the batched kernel's phases (see phase_times)."""
+        check(self.lib.sv_batch_set_phase_timing(self.h, int(on)))
+
+    def phase_times(self) -> tuple[dict, int]:
+        """Microseconds each phase took, summed over workgroups, since the last
+        call, and the scenes processed; zeroes the totals."""
+        us = (C.c_double * len(self.PHASES))()
+        n = C.c_uint64()
+        check(self.lib.sv_batch_phase_times(self.h, us, C.byref(n)))
+        return {k: us[i] for i, k in enumerate(self.PHASES)}, int(n.value)
+
     # -- differentiable rollout ---------------------------------------------
     def diff_reserve(self, max_frames: int, substeps: int = 10) -> None:
         check(self.lib.sv_batch_diff_reserve(self.h, max_frames, substeps))

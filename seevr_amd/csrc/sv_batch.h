@@ -44,7 +44,15 @@ struct BatchArgs {
     uint32_t *t_nc;   // [S] contacts solved
     float *t_state;   // [13N] state at the substep start: pos | quat | vel | omega
     uint32_t n_all;   // N, the bodies of all scenes (offsets into t_state)
+    // phase timing (sv_batch_set_phase_timing): per phase, the 100 MHz wall
+    // clock ticks every workgroup spent in it, and the scenes processed;
+    // null when off
+    unsigned long long *phase_ticks;  // [BATCH_PHASES + 1]
 };
+// Phases of k_batch_substep, in order: taking the next scene; gravity and
+// AABB; broadphase; narrowphase; sweep setup (constants, incidence ranks);
+// the ordered sweep; write-back and integration.
+constexpr int BATCH_PHASES = 7;
 
 // Exclusive scan of one value per thread over the workgroup; *total gets
 // the sum.  Every thread of the block must call it.
@@ -128,6 +136,8 @@ struct sv_batch {
     uint32_t *sc_body = nullptr, *sc_order = nullptr, *sc_cnt = nullptr, *queue = nullptr, *abort_flag = nullptr;
     float *sc_grav = nullptr;
     float *snap = nullptr;  // pos, quat, vel, omega, counters (autotune)
+    unsigned long long *phase_ticks = nullptr;  // [BATCH_PHASES + 1], allocated when phase timing is on
+    bool phase_timing = false;
     int cus = 0;
     uint32_t threads = 256, wgs_per_cu = 2;
     bool use_graph = true;

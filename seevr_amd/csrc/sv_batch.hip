@@ -40,6 +40,17 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
     __shared__ uint32_t s_q, s_next;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // phase timing: thread 0 stamps the wall clock after each phase's
+    // barrier (every thread has finished the phase) and keeps the sums
+    const bool timing = A.phase_ticks != nullptr;
+    uint64_t ph[BATCH_PHASES] = {}, t_mark = timing && tid == 0 ? wall_clock64() : 0ull, n_done = 0;
+    auto stamp = [&](int k) {
+        if (timing && tid == 0) {
+            const uint64_t now = wall_clock64();
+            ph[k] += now - t_mark;
+            t_mark = now;
+        }
+    };
 
     for (;;) {
         if (tid == 0) {
@@ -50,7 +61,16 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
         __syncthreads();
         const uint32_t q = s_q;
         __syncthreads();
-        if (q >= A.num_scenes) return;
+        stamp(0);
+        if (q >= A.num_scenes) {
+            if (timing && tid == 0) {
+#pragma unroll
+                for (int k = 0; k < BATCH_PHASES; k++) atomicAdd(A.phase_ticks + k, (unsigned long long)ph[k]);
+                atomicAdd(A.phase_ticks + BATCH_PHASES, (unsigned long long)n_done);
+            }
+            return;
+        }
+        n_done++;
         const uint32_t sc = A.sc_order[q];
         const uint32_t bs = A.sc_body[sc], ns = A.sc_body[sc + 1] - bs;
         const float gx = A.sc_grav[3 * sc], gy = A.sc_grav[3 * sc + 1], gz = A.sc_grav[3 * sc + 2];
@@ -89,6 +109,7 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
             }
         }
         __syncthreads();
+        stamp(1);
 
         // broadphase: hits per row i (j > i, not both static), scanned into
         // lexicographic slots, then placed
@@ -137,6 +158,7 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
             }
         }
         __syncthreads();
+        stamp(2);
 
         // narrowphase: MPR per pair, contacts in pair order
         const uint32_t np = min(npt, A.ps);
@@ -176,6 +198,7 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
         const uint32_t nc = min(nct, A.cs);
         if (RECORD && tid == 0) A.t_nc[sc] = nc;
         __syncthreads();
+        stamp(3);
 
         // sweep constants and incidence keys; impulses start at zero (step 4)
         for (uint32_t c = tid; c < nc; c += T) {
@@ -209,6 +232,7 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
         rank_incidences<T>(s_key, s_rank, s_start, s_deg, L);
         if (tid == 0) s_next = 0u;
         __syncthreads();
+        stamp(4);
 
         // ordered sweep (physics.comp:523-582) over items (t, c) in order
         const uint32_t total = A.iters * nc;
@@ -303,6 +327,7 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
             }
         }
         __syncthreads();
+        stamp(5);
         for (uint32_t i = tid; i < ns; i += T) {
             const uint32_t b = bs + i;
 #pragma unroll
@@ -318,6 +343,7 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
         // step 6: integrate
         for (uint32_t i = tid; i < ns; i += T) integrate_body(bs + i, A.dt, A.inv_mass, A.pos, A.quat, A.vel, A.omega);
         __syncthreads();
+        stamp(6);
     }
 }
 
@@ -398,6 +424,7 @@ BatchArgs batch_args(sv_batch *b, float dt, uint32_t *queue) {
     A.t_nc = nullptr;
     A.t_state = nullptr;
     A.n_all = b->num_bodies;
+    A.phase_ticks = b->phase_timing ? b->phase_ticks : nullptr;
     return A;
 }
 
@@ -465,7 +492,7 @@ void free_batch(sv_batch *b) {
     if (!b) return;
     if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
     drop_graph(b);
-    void *ptrs[] = {b->sc_body, b->sc_order, b->sc_cnt, b->queue, b->abort_flag, b->sc_grav, b->snap,
+    void *ptrs[] = {b->sc_body, b->sc_order, b->sc_cnt, b->queue, b->abort_flag, b->sc_grav, b->snap, b->phase_ticks,
                     b->d_ckpt, b->d_state, b->d_w2, b->d_prep, b->d_items, b->d_nc, b->d_adj,
                     b->d_contrib, b->d_amat, b->d_amat32, b->d_adj32};
     for (void *p : ptrs)
@@ -614,6 +641,37 @@ int sv_batch_step(sv_batch *b, uint32_t nv, uint32_t substeps, float dt) {
     hipError_t e = hipGraphLaunch(b->exec, c->stream);
     if (e != hipSuccess) return dev_fail(e, "sv_batch_step: graph launch");
     return sv::batch_finish(b);
+}
+
+int sv_batch_set_phase_timing(sv_batch *b, int on) {
+    if (!b) return set_error(SV_E_INVALID, "sv_batch_set_phase_timing: null batch");
+    if (on && !b->phase_ticks) {
+        hipError_t e = hipMalloc(&b->phase_ticks, (sv::BATCH_PHASES + 1) * sizeof(unsigned long long));
+        if (e == hipSuccess)
+            e = hipMemset(b->phase_ticks, 0, (sv::BATCH_PHASES + 1) * sizeof(unsigned long long));
+        if (e != hipSuccess) {
+            if (b->phase_ticks) (void)hipFree(b->phase_ticks);
+            b->phase_ticks = nullptr;
+            return dev_fail(e, "sv_batch_set_phase_timing");
+        }
+    }
+    if (b->phase_timing != (on != 0)) drop_graph(b);  // the captured launches carry the pointer
+    b->phase_timing = on != 0;
+    return SV_OK;
+}
+
+int sv_batch_phase_times(sv_batch *b, double *us, uint64_t *scenes) {
+    if (!b || !us || !scenes) return set_error(SV_E_INVALID, "sv_batch_phase_times: null argument");
+    unsigned long long t[sv::BATCH_PHASES + 1] = {};
+    if (b->phase_ticks) {
+        hipError_t e = hipStreamSynchronize(b->ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(t, b->phase_ticks, sizeof(t), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemset(b->phase_ticks, 0, sizeof(t));
+        if (e != hipSuccess) return dev_fail(e, "sv_batch_phase_times");
+    }
+    for (int k = 0; k < sv::BATCH_PHASES; k++) us[k] = (double)t[k] * 0.01;  // 100 MHz ticks
+    *scenes = t[sv::BATCH_PHASES];
+    return SV_OK;
 }
 
 int sv_batch_read_counters(sv_batch *b, uint32_t *out) {

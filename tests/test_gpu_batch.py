@@ -124,3 +124,35 @@ def test_batch_overflow_truncates_like_plain_path():
     assert int(b.counters()[k, 2]) & 1, "pair overflow bit"
     ctx.close()
     b.close()
+
+
+def test_phase_timing_changes_nothing_and_counts_every_scene():
+    """The batched kernel's phase timing (device clock per phase) leaves the
+    results bit for bit as they are, counts every scene-substep once, and
+    reports positive phase times."""
+    dt = 1.0 / 60.0
+    sc1, b1, _, _, nv = build_batch(12)
+    sc2, b2, _, _, _ = build_batch(12)
+    sc1.close()
+    sc2.close()
+    for _ in range(10):
+        b1.step(nv, 10, dt)
+        b2.step(nv, 10, dt)
+    b2.set_phase_timing(True)
+    for _ in range(3):
+        b1.step(nv, 10, dt)
+        b2.step(nv, 10, dt)
+    ph, n = b2.phase_times()
+    assert_bitwise(b2.packed_state(), b1.packed_state(), "state with phase timing")
+    assert np.array_equal(b2.counters(), b1.counters())
+    assert n == 12 * 10 * 3
+    assert all(v >= 0 for v in ph.values()) and ph["sweep"] > 0 and ph["narrowphase"] > 0
+    again, n2 = b2.phase_times()  # read and zeroed
+    assert n2 == 0 and sum(again.values()) == 0
+    b2.set_phase_timing(False)
+    b1.step(nv, 10, dt)
+    b2.step(nv, 10, dt)
+    assert b2.phase_times()[1] == 0
+    assert_bitwise(b2.packed_state(), b1.packed_state(), "state after phase timing")
+    b1.close()
+    b2.close()
