@@ -470,6 +470,21 @@ __global__ void k_bb_leaves(uint32_t n, const uint32_t *__restrict__ ids, const 
     nhi[leaf] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(lo.w != 0.0f ? 1u : 0u));
 }
 
+// Each internal node's children's boxes side by side in one 64-byte record
+// (left lo, left hi, right lo, right hi), so that the walk reads a node's
+// children with loads that depend on the node alone: one round trip per
+// node visited instead of two (the child indices, then their boxes).
+__global__ void k_bb_pack(uint32_t n, const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
+                          const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, float4 *__restrict__ rec) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + 1 >= n) return;
+    const uint32_t l = left[i], r = right[i];
+    rec[4 * i] = nlo[l];
+    rec[4 * i + 1] = nhi[l];
+    rec[4 * i + 2] = nlo[r];
+    rec[4 * i + 3] = nhi[r];
+}
+
 // The broadphase tree's refit (see k_propagate): the union of the children,
 // the smaller index minimum and the AND of the all-static flags.
 __global__ void __launch_bounds__(BLOCK) k_bb_propagate(uint32_t n, const uint32_t *__restrict__ parent_of,
@@ -551,8 +566,8 @@ SVD void bb_append(bool found, uint32_t mi, uint32_t j, uint64_t lt, const uint3
 // walk (k_bb_scatter appended the others).
 template <bool EMIT>
 __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *__restrict__ ids,
-                                                    const float4 *__restrict__ box, const float4 *__restrict__ nlo,
-                                                    const float4 *__restrict__ nhi, const uint32_t *__restrict__ left,
+                                                    const float4 *__restrict__ box, const float4 *__restrict__ rec,
+                                                    const uint32_t *__restrict__ left,
                                                     const uint32_t *__restrict__ right, uint32_t *cnt,
                                                     const uint32_t *__restrict__ off, uint32_t *tail,
                                                     uint32_t *__restrict__ pk, uint32_t *__restrict__ pv,
@@ -573,10 +588,12 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
     uint32_t node = 0, nf = 0;
     for (;;) {
         const uint32_t ch[2] = {left[node], right[node]};
+        const float4 *rn = rec + 4 * (size_t)node;
+        const float4 cb[4] = {rn[0], rn[1], rn[2], rn[3]};  // the children's boxes, loaded with their indices
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const uint32_t c = ch[q];
-            const float4 lo = nlo[c], hi = nhi[c];
+            const float4 lo = cb[2 * q], hi = cb[2 * q + 1];
             const uint32_t mi = __float_as_uint(lo.w);
             const bool keep = !(mi >= j || (js && __float_as_uint(hi.w) != 0u) || boxes_apart(jl, jh, lo, hi));
             const bool leaf = keep && c >= n - 1;
@@ -750,6 +767,7 @@ bool bb_reserve(sv_ctx *c) {
     size_t nodes = 2 * N + 1, parts = div_up(N, BLOCK * BOUNDS_ITEMS) + 1;
     hipError_t e = lazy_alloc(c, s.bb_lo, nodes);
     if (e == hipSuccess) e = lazy_alloc(c, s.bb_hi, nodes);
+    if (e == hipSuccess) e = lazy_alloc(c, s.bb_rec, 4 * N + 4);
     if (e == hipSuccess) e = lazy_alloc(c, s.bb_left, nodes);
     if (e == hipSuccess) e = lazy_alloc(c, s.bb_right, nodes);
     if (e == hipSuccess) e = lazy_alloc(c, s.bb_parent, nodes);
@@ -803,11 +821,13 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
     hipLaunchKernelGGL(k_clear_flags, dim3(g), dim3(BLOCK), 0, st, n, s.bb_flags);
     launch_refit<float4, Box4>(st, n, s.bb_parent, s.bb_left, s.bb_right, (const uint2 *)s.bb_rng, s.bb_flags,
                                reinterpret_cast<float4 *>(s.bb_lo), reinterpret_cast<float4 *>(s.bb_hi), s.bb_tops);
+    hipLaunchKernelGGL(k_bb_pack, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, st, n, s.bb_left, s.bb_right, s.bb_lo,
+                       s.bb_hi, s.bb_rec);
     // count, scan, fill, sort
     const int bits = 32 - __builtin_clz(n - 1);  // body indices < 2^bits
     const bool combined = 2 * bits <= 32;
     fill_u32(c, s.bb_cnt, 0u, n);
-    hipLaunchKernelGGL(k_bb_pairs<false>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_lo, s.bb_hi, s.bb_left,
+    hipLaunchKernelGGL(k_bb_pairs<false>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_rec, s.bb_left,
                        s.bb_right, s.bb_cnt, (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr,
                        (uint32_t *)nullptr, mp, bits, combined, s.bb_stage, s.bb_scnt);
     scan_exclusive(c, s.bb_cnt, s.bb_off, nullptr, n, counters);
@@ -815,7 +835,7 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
     hipLaunchKernelGGL(k_bb_rows, dim3(1), dim3(64), 0, st, n, s.bb_cnt, s.bb_off, s.bb_fill, s.misc, mp);
     hipLaunchKernelGGL(k_bb_scatter, dim3(div_up(n, BLOCK * BB_SCATTER_ITEMS)), dim3(BLOCK), 0, st, n, ids, s.bb_stage, s.bb_scnt, s.bb_off, counters,
                        s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits, combined);
-    hipLaunchKernelGGL(k_bb_pairs<true>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_lo, s.bb_hi, s.bb_left,
+    hipLaunchKernelGGL(k_bb_pairs<true>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_rec, s.bb_left,
                        s.bb_right, (uint32_t *)nullptr, s.bb_off, s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits,
                        combined, (uint32_t *)nullptr, s.bb_scnt);
     const uint32_t cap = mp + n;

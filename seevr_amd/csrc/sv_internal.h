@@ -45,6 +45,7 @@ struct Scratch {
     uint32_t *misc = nullptr;       // uint[64] small device scalars
     // LBVH broadphase (allocated on first use, sized from the capacities)
     float4 *bb_lo = nullptr;        // float4[2N-1] node min; w = smallest body index below (bits)
+    float4 *bb_rec = nullptr;       // float4[4(N-1)]: per internal node its children's (lo, hi), as in bb_lo/bb_hi
     float4 *bb_hi = nullptr;        // float4[2N-1] node max; w = 1 if every body below is static (bits)
     uint32_t *bb_left = nullptr, *bb_right = nullptr, *bb_parent = nullptr, *bb_count = nullptr;  // [2N-1]
     uint32_t *bb_flags = nullptr;   // uint[N] refit arrival flags
