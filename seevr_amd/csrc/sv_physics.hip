@@ -373,21 +373,27 @@ __global__ void k_integrate(uint32_t n, float dt, const float *__restrict__ inv_
     integrate_body(i, dt, inv_mass, pos, quat, vel, omega);
 }
 
-// Step 7 (physics.comp:607-627): owner body by binary search on
-// body_vert_start, world = p + R * local.
+// Step 7 (physics.comp:607-627): world = p + R * local for every render
+// vertex, the owner being the last body whose body_vert_start is <= the
+// vertex (the reference's binary search).  body_vert_start is a
+// non-decreasing prefix (data.zig:633-640), so body b owns exactly
+// [start[b], start[b + 1]) (the last body up to nv; body 0 also the
+// vertices below start[0]; a body with no vertices an empty range): one row
+// of 16 lanes per body walks its range, with the body's rotation computed
+// once, instead of one binary search (log2 N dependent loads) per vertex.
+// Same owner, same arithmetic: the same bits.
+constexpr uint32_t VERT_ROW = 16;
 __global__ void __launch_bounds__(BLOCK) k_update_verts(uint32_t nv, uint32_t nb, const uint32_t *__restrict__ vstart,
                                                         const float *__restrict__ pos, const float *__restrict__ quat,
                                                         const float *__restrict__ orig, float *__restrict__ verts) {
-    uint32_t vid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (vid >= nv) return;
-    uint32_t lo = 0, hi = nb;
-    while (lo + 1 < hi) {
-        uint32_t mid = (lo + hi) / 2;
-        if (vstart[mid] <= vid) lo = mid; else hi = mid;
-    }
-    m3 R = quat_mat_at(quat, lo);
-    f3 wp = ld3(pos, lo) + mvec(R, ld3(orig, vid));
-    st3(verts, vid, wp);
+    const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) / VERT_ROW, r = threadIdx.x % VERT_ROW;
+    if (b >= nb) return;
+    const uint32_t v0 = b == 0 ? 0u : vstart[b];
+    const uint32_t v1 = min(b + 1 < nb ? vstart[b + 1] : nv, nv);
+    if (v0 >= v1) return;
+    const m3 R = quat_mat_at(quat, b);
+    const f3 p = ld3(pos, b);
+    for (uint32_t v = v0 + r; v < v1; v += VERT_ROW) st3(verts, v, p + mvec(R, ld3(orig, v)));
 }
 
 // Step 8 (physics.comp:630-634); counters[2] (status bits) accumulates over
@@ -615,7 +621,7 @@ void launch_integrate(sv_ctx *c, uint32_t n, float dt) {
 
 void launch_update_verts(sv_ctx *c, uint32_t nv, uint32_t nb) {
     if (!nv || !nb) return;
-    hipLaunchKernelGGL(k_update_verts, dim3(div_up(nv, BLOCK)), dim3(BLOCK), 0, c->stream, nv, nb,
+    hipLaunchKernelGGL(k_update_verts, dim3(div_up((size_t)nb * VERT_ROW, BLOCK)), dim3(BLOCK), 0, c->stream, nv, nb,
                        B<uint32_t>(c, SV_BODY_VERT_START), B<float>(c, SV_BODY_POS), B<float>(c, SV_BODY_QUAT),
                        B<float>(c, SV_ORIGINAL_VERTICES), B<float>(c, SV_VERTICES));
 }

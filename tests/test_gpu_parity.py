@@ -214,3 +214,30 @@ def test_repeat_runs_identical(settled):
         ctx.close()
     for k in outs[0]:
         assert_bitwise(outs[1][k], outs[0][k], f"repeat {k}")
+
+
+@pytest.mark.parametrize("layout", ["empty_bodies", "gap_before_first", "count_below_last_range"])
+def test_render_vertex_owners_edge_cases(settled, layout):
+    """Step 7's owner of a vertex is the last body whose body_vert_start is
+    <= it (the reference's binary search, physics.comp:611-619); the device
+    walks each body's range instead.  Starts with bodies that own no vertex
+    (repeated starts), vertices below the first body's start, and a vertex
+    count that ends inside the last body's range must give the oracle's
+    values bit for bit."""
+    sc, w = _stage_world(settled)
+    ctx = new_ctx(sc, w.max_pairs, w.max_contacts)
+    w.upload(ctx)
+    n, nv = w.n, w.nv
+    start = w.a["BODY_VERT_START"][:n].copy()
+    if layout == "empty_bodies":  # every third body owns nothing (its start repeats the next one's)
+        for b in range(1, n - 1, 3):
+            start[b] = start[b + 1]
+    elif layout == "gap_before_first":
+        start[0] = start[1] // 2
+    count = nv if layout != "count_below_last_range" else int(start[n - 1]) + 3
+    w.a["BODY_VERT_START"][:n] = start
+    ctx.upload("BODY_VERT_START", start)
+    ctx.dispatch(7, count, num_bodies=n)
+    w.oracle_stage(7, count=count)
+    assert_bitwise(ctx.download("VERTICES", 3 * count), w.a["VERTICES"][: 3 * count], f"render verts ({layout})")
+    ctx.close()
