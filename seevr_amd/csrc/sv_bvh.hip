@@ -312,6 +312,17 @@ SVD Box4 box_load(const float4 *lo, const float4 *hi, uint32_t i) { return {lo[i
 SVD Box4 box_load_coh(const float4 *lo, const float4 *hi, uint32_t i) { return {ld4c(lo, i), ld4c(hi, i)}; }
 SVD void box_store(float4 *lo, float4 *hi, uint32_t i, const Box4 &b) { lo[i] = b.lo; hi[i] = b.hi; }
 SVD void box_store_coh(float4 *lo, float4 *hi, uint32_t i, const Box4 &b) { st4c(lo, i, b.lo); st4c(hi, i, b.hi); }
+// The broadphase tree's node record (k_bb_pairs): the two children's boxes
+// side by side, written by the thread that completes the node.
+SVD void rec_pair(float4 *rec, uint32_t p, const Box4 &l, const Box4 &r) {
+    if (!rec) return;
+    float4 *q = rec + 4 * (size_t)p;
+    q[0] = l.lo;
+    q[1] = l.hi;
+    q[2] = r.lo;
+    q[3] = r.hi;
+}
+SVD void rec_pair(float4 *, uint32_t, const Box3 &, const Box3 &) {}
 SVD Box4 box_union(const Box4 &l, const Box4 &r) {  // see k_bb_propagate
     const uint32_t mi = min(__float_as_uint(l.lo.w), __float_as_uint(r.lo.w));
     const uint32_t st = __float_as_uint(l.hi.w) & __float_as_uint(r.hi.w);
@@ -324,7 +335,7 @@ __global__ void __launch_bounds__(REFIT_T) k_refit_blocks(uint32_t n, const uint
                                                           const uint32_t *__restrict__ left,
                                                           const uint32_t *__restrict__ right,
                                                           const uint2 *__restrict__ rng, T *na, T *nb,
-                                                          uint32_t *__restrict__ list) {
+                                                          uint32_t *__restrict__ list, float4 *__restrict__ rec) {
     constexpr uint32_t TH = REFIT_T;
     __shared__ uint32_t lflag[TH];
     __shared__ BoxT lbox[2 * TH];  // [0, TH): internal nodes b0 + k; [TH, 2 TH): leaves b0 + k
@@ -382,8 +393,18 @@ __global__ void __launch_bounds__(REFIT_T) k_refit_blocks(uint32_t n, const uint
     }
     __syncthreads();
     // the internal nodes completed here (both arrivals counted), written out
-    // in index order: consecutive threads store consecutive boxes
-    if (i + 1 < n && lflag[t] == 2u) box_store(na, nb, i, lbox[t]);
+    // in index order: consecutive threads store consecutive boxes; such a
+    // node covers leaves of this workgroup only, so its children are leaves
+    // or internal nodes of this workgroup and their boxes are in LDS (the
+    // node's record of them goes out the same way)
+    if (i + 1 < n && lflag[t] == 2u) {
+        box_store(na, nb, i, lbox[t]);
+        if (rec) {
+            const uint32_t li = lleft[t], ri = lright[t];
+            rec_pair(rec, i, li >= n - 1 ? lbox[TH + (li - (n - 1) - b0)] : lbox[li - b0],
+                     ri >= n - 1 ? lbox[TH + (ri - (n - 1) - b0)] : lbox[ri - b0]);
+        }
+    }
     if (t == 0) list[(size_t)blockIdx.x * (TH + 1)] = s_ntop;
 }
 
@@ -391,7 +412,7 @@ template <typename T, typename BoxT>
 __global__ void __launch_bounds__(BLOCK) k_refit_cross(uint32_t n, const uint32_t *__restrict__ parent_of,
                                                        const uint32_t *__restrict__ left,
                                                        const uint32_t *__restrict__ right, uint32_t *flags, T *na,
-                                                       T *nb, const uint32_t *__restrict__ list) {
+                                                       T *nb, const uint32_t *__restrict__ list, float4 *__restrict__ rec) {
     const uint32_t nw = (n + REFIT_T - 1) / REFIT_T;  // workgroups of launch 1
     const uint32_t w = blockIdx.x * (BLOCK / REFIT_LANES) + threadIdx.x / REFIT_LANES;
     if (w >= nw) return;
@@ -407,7 +428,9 @@ __global__ void __launch_bounds__(BLOCK) k_refit_cross(uint32_t n, const uint32_
             const bool is_left = li == node;
             const uint32_t sib = is_left ? ri : li;
             const BoxT sbox = box_load_coh(na, nb, sib);
-            box = is_left ? box_union(box, sbox) : box_union(sbox, box);
+            const BoxT own = box;
+            box = is_left ? box_union(own, sbox) : box_union(sbox, own);
+            rec_pair(rec, p, is_left ? own : sbox, is_left ? sbox : own);
             box_store_coh(na, nb, p, box);
             node = p;
             p = pp;
@@ -419,12 +442,12 @@ __global__ void __launch_bounds__(BLOCK) k_refit_cross(uint32_t n, const uint32_
 // the arrival counters); `list` holds refit_list_words(n) words.
 template <typename T, typename BoxT>
 void launch_refit(hipStream_t st, uint32_t n, const uint32_t *parent_of, const uint32_t *left, const uint32_t *right,
-                  const uint2 *rng, uint32_t *flags, T *na, T *nb, uint32_t *list) {
+                  const uint2 *rng, uint32_t *flags, T *na, T *nb, uint32_t *list, float4 *rec = nullptr) {
     const uint32_t nw = div_up(n, REFIT_T);
     hipLaunchKernelGGL((k_refit_blocks<T, BoxT>), dim3(nw), dim3(REFIT_T), 0, st, n, parent_of, left, right, rng, na,
-                       nb, list);
+                       nb, list, rec);
     hipLaunchKernelGGL((k_refit_cross<T, BoxT>), dim3(div_up(nw, BLOCK / REFIT_LANES)), dim3(BLOCK), 0, st, n,
-                       parent_of, left, right, flags, na, nb, (const uint32_t *)list);
+                       parent_of, left, right, flags, na, nb, (const uint32_t *)list, rec);
 }
 
 __global__ void k_copy_u32(uint32_t n, const uint32_t *src, uint32_t *dst) {
@@ -460,29 +483,17 @@ __global__ void k_bb_centroids(uint32_t n, const float4 *__restrict__ box, float
     ids[i] = i;
 }
 
+// Leaf boxes of the broadphase tree; also clears the refit's arrival
+// counters of the internal nodes (k_clear_flags' work).
 __global__ void k_bb_leaves(uint32_t n, const uint32_t *__restrict__ ids, const float4 *__restrict__ box,
-                            float4 *__restrict__ nlo, float4 *__restrict__ nhi) {
+                            float4 *__restrict__ nlo, float4 *__restrict__ nhi, uint32_t *__restrict__ flags) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k + 1 < n) flags[k] = 0;
     if (k >= n) return;
     uint32_t b = ids[k], leaf = n - 1 + k;
     float4 lo = box[2 * b], hi = box[2 * b + 1];
     nlo[leaf] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(b));
     nhi[leaf] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(lo.w != 0.0f ? 1u : 0u));
-}
-
-// Each internal node's children's boxes side by side in one 64-byte record
-// (left lo, left hi, right lo, right hi), so that the walk reads a node's
-// children with loads that depend on the node alone: one round trip per
-// node visited instead of two (the child indices, then their boxes).
-__global__ void k_bb_pack(uint32_t n, const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
-                          const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, float4 *__restrict__ rec) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i + 1 >= n) return;
-    const uint32_t l = left[i], r = right[i];
-    rec[4 * i] = nlo[l];
-    rec[4 * i + 1] = nhi[l];
-    rec[4 * i + 2] = nlo[r];
-    rec[4 * i + 3] = nhi[r];
 }
 
 // The broadphase tree's refit (see k_propagate): the union of the children,
@@ -817,12 +828,11 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
     const uint32_t *ids = s.bb_mv[w];
     hipLaunchKernelGGL(k_hierarchy, dim3(g), dim3(BLOCK), 0, st, n, s.bb_mk[w], s.bb_left, s.bb_right, s.bb_count,
                        s.bb_parent, s.bb_rng);
-    hipLaunchKernelGGL(k_bb_leaves, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_lo, s.bb_hi);
-    hipLaunchKernelGGL(k_clear_flags, dim3(g), dim3(BLOCK), 0, st, n, s.bb_flags);
+    hipLaunchKernelGGL(k_bb_leaves, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_lo, s.bb_hi, s.bb_flags);
+    // the refit also writes each internal node's record of its children's boxes (the walk's)
     launch_refit<float4, Box4>(st, n, s.bb_parent, s.bb_left, s.bb_right, (const uint2 *)s.bb_rng, s.bb_flags,
-                               reinterpret_cast<float4 *>(s.bb_lo), reinterpret_cast<float4 *>(s.bb_hi), s.bb_tops);
-    hipLaunchKernelGGL(k_bb_pack, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, st, n, s.bb_left, s.bb_right, s.bb_lo,
-                       s.bb_hi, s.bb_rec);
+                               reinterpret_cast<float4 *>(s.bb_lo), reinterpret_cast<float4 *>(s.bb_hi), s.bb_tops,
+                               s.bb_rec);
     // count, scan, fill, sort
     const int bits = 32 - __builtin_clz(n - 1);  // body indices < 2^bits
     const bool combined = 2 * bits <= 32;
