@@ -165,12 +165,23 @@ SVD f3 support(const Body &B, f3 d, const Geo &g) {
     float best = -1e10f;
     f3 bv = mk(0.0f, 0.0f, 0.0f);
     if (B.lk == 0) {
-        for (uint32_t i = 0; i < B.vc; i++) {
-            f3 v = ld3(g.hull, B.vs + i);
-            float dd = dot3(v, ld);
-            if (dd > best) {
-                best = dd;
-                bv = v;
+        // chunks of 8 vertices: all 8 loads issued before the first compare
+        // (a box's whole hull in one round trip), then compared in index
+        // order with the same strict '>', so the same vertex wins
+        constexpr uint32_t CH = 8;
+        for (uint32_t i0 = 0; i0 < B.vc; i0 += CH) {
+            f3 v[CH];
+#pragma unroll
+            for (uint32_t k = 0; k < CH; k++) v[k] = ld3(g.hull, B.vs + (i0 + k < B.vc ? i0 + k : i0));
+#pragma unroll
+            for (uint32_t k = 0; k < CH; k++) {
+                if (i0 + k < B.vc) {
+                    float dd = dot3(v[k], ld);
+                    if (dd > best) {
+                        best = dd;
+                        bv = v[k];
+                    }
+                }
             }
         }
         return B.p + mvec(B.R, bv);
