@@ -175,7 +175,7 @@ def cpu_baseline(ctx, scene, n, substeps: int, mt: bool = True) -> dict:
 TRI_STAGES = ("bvh/centroids", "bvh/scene_bounds", "bvh/morton_codes", "bvh/radix_sort", "bvh/build_tree",
               "bvh/leaf_aabbs", "bvh/clear_flags", "bvh/propagate")
 TRI_KERNELS = {"bvh/centroids": {"k_centroids": 1}, "bvh/morton_codes": {"k_morton": 1},
-               "bvh/radix_sort": {"k_os_hist<4>": 1, "k_os_pass<512, 8>": 4}}
+               "bvh/radix_sort": {"k_os_hist<4>": 1, "k_os_pass<1024, 8>": 4}}
 
 
 def triangle_bvh_section(ctx, counts: dict, reps: int = 5) -> dict:

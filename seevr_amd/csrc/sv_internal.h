@@ -93,7 +93,7 @@ enum { SWEEP_SEQ = 0, SWEEP_COHERENT = 1, SWEEP_FENCED = 2, SWEEP_PIPE = 3, SWEE
 // being accumulated, per pass the digit offsets, control words (done
 // counter, tile counter per pass, abort word), and per pass, tile and digit
 // one look-back status word.
-constexpr uint32_t SORT_TILE_SMALL = 2048, SORT_TILE_LARGE = 4096;  // keys per tile, by sort size
+constexpr uint32_t SORT_TILE_SMALL = 2048, SORT_TILE_LARGE = 8192;  // keys per tile, by sort size
 constexpr uint32_t SORT_SMALL_MAX = 1u << 18;                       // largest sort on small tiles
 constexpr uint32_t SCAN_TILE_VALUES = 1024;
 constexpr uint32_t SORT_MAX_PASSES = 4;             // 32-bit keys, 8-bit digits

@@ -18,11 +18,12 @@ static_assert(SCAN_TILE == SCAN_TILE_VALUES, "scan scratch sizing");
 
 // Radix-sort tiles: 256 threads x 8 rounds (2048 keys) up to
 // SORT_SMALL_MAX keys, where short look-back chains and more workgroups
-// matter; 512 threads x 8 rounds (4096 keys) above, for longer write runs
-// and fewer look-backs (8192-key tiles took 22.7 against 21.1 us per pass at
-// 1M keys and 89 against 80 us at 8.57M: each tile's ranking is a longer
-// chain).
-static_assert(256 * 8 == SORT_TILE_SMALL && 512 * 8 == SORT_TILE_LARGE, "sort scratch sizing");
+// matter; 1024 threads x 8 rounds (8192 keys) above: half the tiles of
+// 512-thread tiles for the same keys in flight, so the look-back walks half
+// as far, with each wave's ranking chain as short (8.57M keys: 358 -> 344 us
+// for the 4-pass sort; 512 threads x 16 rounds had been slower, 89 against
+// 80 us per pass, from the longer ranking chain).
+static_assert(256 * 8 == SORT_TILE_SMALL && 1024 * 8 == SORT_TILE_LARGE, "sort scratch sizing");
 constexpr int HIST_THREADS = 256;
 constexpr int HIST_ITEMS = 16;                         // keys per thread and iteration
 constexpr int HIST_CHUNK = HIST_THREADS * HIST_ITEMS;  // 4096
@@ -488,7 +489,7 @@ int radix_sort(sv_ctx *c, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1
             hipLaunchKernelGGL((k_os_pass<256, 8>), dim3(nblocks), dim3(256), 0, c->stream, ka, va, kb, vb, n_dev,
                                n_max, p, dmask, goff, ctl, status, counters);
         else
-            hipLaunchKernelGGL((k_os_pass<512, 8>), dim3(nblocks), dim3(512), 0, c->stream, ka, va, kb, vb, n_dev,
+            hipLaunchKernelGGL((k_os_pass<1024, 8>), dim3(nblocks), dim3(1024), 0, c->stream, ka, va, kb, vb, n_dev,
                                n_max, p, dmask, goff, ctl, status, counters);
         uint32_t *t = ka; ka = kb; kb = t;
         t = va; va = vb; vb = t;
