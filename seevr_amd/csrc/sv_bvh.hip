@@ -132,9 +132,15 @@ SVD int delta(const uint32_t *__restrict__ m, int i, int j, int n) {
 }
 
 // bvh.comp:299-348 (Karras 2012): leaves n-1+i, internal nodes 0..n-2.
+// With `box` given (the broadphase tree) thread i also writes leaf i's float4
+// box, body ids[i]'s (lo.w = the body index, hi.w = 1 if static), and clears
+// internal node i's refit arrival counter.
 __global__ void k_hierarchy(uint32_t nt, const uint32_t *__restrict__ m, uint32_t *__restrict__ left,
                             uint32_t *__restrict__ right, uint32_t *__restrict__ count,
-                            uint32_t *__restrict__ parent, uint2 *__restrict__ rng) {
+                            uint32_t *__restrict__ parent, uint2 *__restrict__ rng,
+                            const uint32_t *__restrict__ ids = nullptr, const float4 *__restrict__ box = nullptr,
+                            float4 *__restrict__ nlo = nullptr, float4 *__restrict__ nhi = nullptr,
+                            uint32_t *__restrict__ flags = nullptr) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     int n = (int)nt;
     if (i < nt) {
@@ -142,6 +148,13 @@ __global__ void k_hierarchy(uint32_t nt, const uint32_t *__restrict__ m, uint32_
         left[leaf] = i;
         count[leaf] = 1;
         if (nt == 1) parent[0] = 0xFFFFFFFFu;  // single leaf is the root
+        if (box) {
+            const uint32_t b = ids[i];
+            const float4 lo = box[2 * b], hi = box[2 * b + 1];
+            nlo[leaf] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(b));
+            nhi[leaf] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(lo.w != 0.0f ? 1u : 0u));
+            if (i + 1 < nt) flags[i] = 0u;
+        }
     }
     if (i + 1 >= nt) return;
     int ii = (int)i;
@@ -472,28 +485,56 @@ __global__ void k_copy_u32(uint32_t n, const uint32_t *src, uint32_t *dst) {
 // order, and that part sorted by (i, j): the first max_pairs entries are the
 // all-pairs kernel's.
 
-__global__ void k_bb_centroids(uint32_t n, const float4 *__restrict__ box, float *__restrict__ cent,
-                               uint32_t *__restrict__ ids) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float4 lo = box[2 * i], hi = box[2 * i + 1];
-    cent[3 * i + 0] = (lo.x + hi.x) * 0.5f;
-    cent[3 * i + 1] = (lo.y + hi.y) * 0.5f;
-    cent[3 * i + 2] = (lo.z + hi.z) * 0.5f;
-    ids[i] = i;
-}
-
-// Leaf boxes of the broadphase tree; also clears the refit's arrival
-// counters of the internal nodes (k_clear_flags' work).
-__global__ void k_bb_leaves(uint32_t n, const uint32_t *__restrict__ ids, const float4 *__restrict__ box,
-                            float4 *__restrict__ nlo, float4 *__restrict__ nhi, uint32_t *__restrict__ flags) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k + 1 < n) flags[k] = 0;
-    if (k >= n) return;
-    uint32_t b = ids[k], leaf = n - 1 + k;
-    float4 lo = box[2 * b], hi = box[2 * b + 1];
-    nlo[leaf] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(b));
-    nhi[leaf] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(lo.w != 0.0f ? 1u : 0u));
+// Box centres, the identity body list, and the centres' bounds in one launch:
+// each workgroup reduces its BOUNDS_TILE centres to a partial as
+// k_bounds_partial does, and the workgroup that arrives last at `done`
+// reduces the partials as k_bounds_final does (same partition, same order)
+// and puts the counter back to zero for the next launch.  Partials go out
+// and come back with device-coherent accesses; the arrival waits for the
+// partial's stores to complete.
+__global__ void __launch_bounds__(BLOCK) k_bb_centroids(uint32_t n, const float4 *__restrict__ box,
+                                                        float *__restrict__ cent, uint32_t *__restrict__ ids,
+                                                        float *part, float *__restrict__ bounds, uint32_t *done) {
+    __shared__ uint32_t s_last;
+    f3 mn = mk(1e30f, 1e30f, 1e30f), mx = mk(-1e30f, -1e30f, -1e30f);
+    const uint32_t base = blockIdx.x * BOUNDS_TILE;
+#pragma unroll
+    for (int k = 0; k < BOUNDS_ITEMS; k++) {
+        const uint32_t i = base + k * BLOCK + threadIdx.x;
+        if (i < n) {
+            const float4 lo = box[2 * i], hi = box[2 * i + 1];
+            const f3 c = mk((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
+            st3(cent, i, c);
+            ids[i] = i;
+            mn = vmin3(mn, c);
+            mx = vmax3(mx, c);
+        }
+    }
+    block_minmax(mn, mx);
+    if (threadIdx.x == 0) {
+        float *p = part + blockIdx.x * 6;
+        stc(p + 0, mn.x); stc(p + 1, mn.y); stc(p + 2, mn.z);
+        stc(p + 3, mx.x); stc(p + 4, mx.y); stc(p + 5, mx.z);
+        __builtin_amdgcn_s_waitcnt(0);
+        const uint32_t old = __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = old + 1u == gridDim.x ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    asm volatile("" ::: "memory");
+    mn = mk(1e30f, 1e30f, 1e30f);
+    mx = mk(-1e30f, -1e30f, -1e30f);
+    for (uint32_t q = threadIdx.x; q < gridDim.x; q += BLOCK) {
+        const float *p = part + q * 6;
+        mn = vmin3(mn, mk(ldc(p + 0), ldc(p + 1), ldc(p + 2)));
+        mx = vmax3(mx, mk(ldc(p + 3), ldc(p + 4), ldc(p + 5)));
+    }
+    block_minmax(mn, mx);
+    if (threadIdx.x == 0) {
+        bounds[0] = mn.x; bounds[1] = mn.y; bounds[2] = mn.z;
+        bounds[3] = mx.x; bounds[4] = mx.y; bounds[5] = mx.z;
+        __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // The broadphase tree's refit (see k_propagate): the union of the children,
@@ -713,15 +754,18 @@ __global__ void __launch_bounds__(BLOCK) k_bb_scatter(uint32_t n, const uint32_t
     }
 }
 
-// Clear the append counter and find where the kept part of the pair list
-// ends: rows that start below max_pairs are kept.  The offsets are an
-// exclusive scan of the counts, so row ends off + cnt do not decrease with
-// the row, and the kept rows are a prefix [0, r): the kept part ends at
-// off[r-1] + cnt[r-1], found by one binary search (off[0] = 0 < max_pairs).
-// One thread.
+// k_bb_post's work (the pair count and the overflow flag), then clear the
+// append counter and find where the kept part of the pair list ends: rows
+// that start below max_pairs are kept.  The offsets are an exclusive scan of
+// the counts, so row ends off + cnt do not decrease with the row, and the
+// kept rows are a prefix [0, r): the kept part ends at off[r-1] + cnt[r-1],
+// found by one binary search (off[0] = 0 < max_pairs).  One thread.
 __global__ void k_bb_rows(uint32_t n, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
-                          uint32_t *__restrict__ tail, uint32_t *misc, uint32_t max_pairs) {
+                          uint32_t *__restrict__ tail, uint32_t *misc, uint32_t max_pairs, uint32_t *counters) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const uint32_t t = counters[0];
+    misc[MISC_NP] = t < max_pairs ? t : max_pairs;
+    if (t > max_pairs) counters[2] |= SV_STATUS_PAIR_OVERFLOW;
     tail[0] = 0u;
     uint32_t lo = 1, hi = n;  // first row in [1, n] whose offset is >= max_pairs (n if none)
     while (lo < hi) {
@@ -819,16 +863,15 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
     Scratch &s = c->s;
     const uint32_t g = div_up(n, BLOCK);
     // tree over the broadphase boxes
-    hipLaunchKernelGGL(k_bb_centroids, dim3(g), dim3(BLOCK), 0, st, n, s.bpbox, s.bb_cent, s.bb_mv[0]);
-    uint32_t parts = div_up(n, BLOCK * BOUNDS_ITEMS);
-    hipLaunchKernelGGL(k_bounds_partial, dim3(parts), dim3(BLOCK), 0, st, n, s.bb_cent, s.bb_bounds + 6);
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(BLOCK), 0, st, parts, s.bb_bounds + 6, s.bb_bounds);
+    const uint32_t parts = div_up(n, BOUNDS_TILE);
+    hipLaunchKernelGGL(k_bb_centroids, dim3(parts), dim3(BLOCK), 0, st, n, s.bpbox, s.bb_cent, s.bb_mv[0],
+                       s.bb_bounds + 6, s.bb_bounds, s.misc + MISC_BOUNDS_DONE);
     hipLaunchKernelGGL(k_morton, dim3(g), dim3(BLOCK), 0, st, n, s.bb_bounds, s.bb_cent, s.bb_mk[0]);
     int w = radix_sort(c, s.bb_mk[0], s.bb_mv[0], s.bb_mk[1], s.bb_mv[1], nullptr, n, 30);
     const uint32_t *ids = s.bb_mv[w];
+    // with the leaf boxes and the refit's cleared arrival counters
     hipLaunchKernelGGL(k_hierarchy, dim3(g), dim3(BLOCK), 0, st, n, s.bb_mk[w], s.bb_left, s.bb_right, s.bb_count,
-                       s.bb_parent, s.bb_rng);
-    hipLaunchKernelGGL(k_bb_leaves, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_lo, s.bb_hi, s.bb_flags);
+                       s.bb_parent, s.bb_rng, ids, s.bpbox, s.bb_lo, s.bb_hi, s.bb_flags);
     // the refit also writes each internal node's record of its children's boxes (the walk's)
     launch_refit<float4, Box4>(st, n, s.bb_parent, s.bb_left, s.bb_right, (const uint2 *)s.bb_rng, s.bb_flags,
                                reinterpret_cast<float4 *>(s.bb_lo), reinterpret_cast<float4 *>(s.bb_hi), s.bb_tops,
@@ -841,8 +884,7 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
                        s.bb_right, s.bb_cnt, (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr,
                        (uint32_t *)nullptr, mp, bits, combined, s.bb_stage, s.bb_scnt);
     scan_exclusive(c, s.bb_cnt, s.bb_off, nullptr, n, counters);
-    hipLaunchKernelGGL(k_bb_post, dim3(1), dim3(1), 0, st, counters, s.misc, mp);
-    hipLaunchKernelGGL(k_bb_rows, dim3(1), dim3(64), 0, st, n, s.bb_cnt, s.bb_off, s.bb_fill, s.misc, mp);
+    hipLaunchKernelGGL(k_bb_rows, dim3(1), dim3(64), 0, st, n, s.bb_cnt, s.bb_off, s.bb_fill, s.misc, mp, counters);
     hipLaunchKernelGGL(k_bb_scatter, dim3(div_up(n, BLOCK * BB_SCATTER_ITEMS)), dim3(BLOCK), 0, st, n, ids, s.bb_stage, s.bb_scnt, s.bb_off, counters,
                        s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits, combined);
     hipLaunchKernelGGL(k_bb_pairs<true>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_rec, s.bb_left,

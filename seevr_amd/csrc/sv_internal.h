@@ -226,6 +226,7 @@ enum : int {
     MISC_NACT = 6,     // contacts the sweep updates (not both inverse masses zero)
     MISC_BPE = 7,      // pairs the LBVH broadphase emits before sorting (rows that start below the cap)
     MISC_WARM_N = 8,   // contacts in the warm-start cache
+    MISC_BOUNDS_DONE = 9,  // workgroups of the broadphase centre bounds that have arrived (zero between launches)
 };
 
 // ---- sv_prims.hip ----
