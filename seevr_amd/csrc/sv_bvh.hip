@@ -491,13 +491,16 @@ __global__ void k_copy_u32(uint32_t n, const uint32_t *src, uint32_t *dst) {
 // reduces the partials as k_bounds_final does (same partition, same order)
 // and puts the counter back to zero for the next launch.  Partials go out
 // and come back with device-coherent accesses; the arrival waits for the
-// partial's stores to complete.
+// partial's stores to complete.  Also zeroes the row counts the counting
+// walk bumps and the append counter of the pair list.
 __global__ void __launch_bounds__(BLOCK) k_bb_centroids(uint32_t n, const float4 *__restrict__ box,
                                                         float *__restrict__ cent, uint32_t *__restrict__ ids,
-                                                        float *part, float *__restrict__ bounds, uint32_t *done) {
+                                                        float *part, float *__restrict__ bounds, uint32_t *done,
+                                                        uint32_t *__restrict__ cnt, uint32_t *__restrict__ tail) {
     __shared__ uint32_t s_last;
     f3 mn = mk(1e30f, 1e30f, 1e30f), mx = mk(-1e30f, -1e30f, -1e30f);
     const uint32_t base = blockIdx.x * BOUNDS_TILE;
+    if (blockIdx.x == 0 && threadIdx.x == 0) tail[0] = 0u;
 #pragma unroll
     for (int k = 0; k < BOUNDS_ITEMS; k++) {
         const uint32_t i = base + k * BLOCK + threadIdx.x;
@@ -506,6 +509,7 @@ __global__ void __launch_bounds__(BLOCK) k_bb_centroids(uint32_t n, const float4
             const f3 c = mk((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
             st3(cent, i, c);
             ids[i] = i;
+            cnt[i] = 0u;
             mn = vmin3(mn, c);
             mx = vmax3(mx, c);
         }
@@ -678,6 +682,40 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
     if (!EMIT && stage != nullptr) scnt[t] = nf;
 }
 
+// k_bb_post's work (the pair count and the overflow flag), and where the
+// kept part of the pair list ends: rows that start below max_pairs are kept.
+// The offsets are an exclusive scan of the counts, so row ends off + cnt do
+// not decrease with the row, and the kept rows are a prefix [0, r): the kept
+// part ends at off[r-1] + cnt[r-1].  r (the first row in [1, n] whose offset
+// is >= max_pairs, n if none; off[0] = 0 < max_pairs) is found by one wave:
+// each round its lanes probe 64 evenly spaced rows of the interval that
+// holds r, and r lies after the last probe below max_pairs and at or before
+// the first one at or above it (a few rounds instead of log2(n)).
+SVD void bb_rows(uint32_t n, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off, uint32_t *misc,
+                 uint32_t max_pairs, uint32_t *counters) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t lo = 1, hi = n;  // r in [lo, hi]; rows below lo start below max_pairs
+    while (lo < hi) {
+        const uint32_t step = (hi - lo + 63u) / 64u, q = lo + lane * step;
+        const bool valid = q < hi;
+        const uint64_t ge = __ballot(valid && off[q] >= max_pairs);
+        if (ge) {
+            const uint32_t k = (uint32_t)__builtin_ctzll(ge);
+            hi = lo + k * step;
+            if (k) lo = lo + (k - 1u) * step + 1u;
+        } else {
+            const uint32_t last = (uint32_t)(63 - __builtin_clzll(__ballot(valid)));
+            lo = lo + last * step + 1u;
+        }
+    }
+    if (lane == 0) {
+        const uint32_t t = counters[0];
+        misc[MISC_NP] = t < max_pairs ? t : max_pairs;
+        if (t > max_pairs) counters[2] |= SV_STATUS_PAIR_OVERFLOW;
+        misc[MISC_BPE] = off[lo - 1] + cnt[lo - 1];
+    }
+}
+
 // Thread x of a workgroup appends the rows staged by lanes t = base + x +
 // k * BLOCK, k < BB_SCATTER_ITEMS (scnt[t] <= BB_STAGE); lanes with more
 // rows are left to the EMIT walk.  One append atomic per workgroup (the
@@ -696,11 +734,13 @@ __global__ void __launch_bounds__(BLOCK) k_bb_scatter(uint32_t n, const uint32_t
                                                       const uint32_t *__restrict__ stage,
                                                       const uint32_t *__restrict__ scnt,
                                                       const uint32_t *__restrict__ off,
-                                                      const uint32_t *__restrict__ counters, uint32_t *tail,
+                                                      uint32_t *counters, uint32_t *tail,
                                                       uint32_t *__restrict__ pk, uint32_t *__restrict__ pv,
-                                                      uint32_t max_pairs, int bits, bool combined) {
+                                                      uint32_t max_pairs, int bits, bool combined,
+                                                      const uint32_t *__restrict__ cnt, uint32_t *misc) {
     __shared__ uint32_t s_wave[BLOCK / 64 + 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (blockIdx.x == 0 && wave == 0) bb_rows(n, cnt, off, misc, max_pairs, counters);
     const uint32_t base = blockIdx.x * (BLOCK * BB_SCATTER_ITEMS) + threadIdx.x;
     const bool all_kept = counters[0] <= max_pairs;
     uint32_t tot = 0;
@@ -752,28 +792,6 @@ __global__ void __launch_bounds__(BLOCK) k_bb_scatter(uint32_t n, const uint32_t
             }
         }
     }
-}
-
-// k_bb_post's work (the pair count and the overflow flag), then clear the
-// append counter and find where the kept part of the pair list ends: rows
-// that start below max_pairs are kept.  The offsets are an exclusive scan of
-// the counts, so row ends off + cnt do not decrease with the row, and the
-// kept rows are a prefix [0, r): the kept part ends at off[r-1] + cnt[r-1],
-// found by one binary search (off[0] = 0 < max_pairs).  One thread.
-__global__ void k_bb_rows(uint32_t n, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
-                          uint32_t *__restrict__ tail, uint32_t *misc, uint32_t max_pairs, uint32_t *counters) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    const uint32_t t = counters[0];
-    misc[MISC_NP] = t < max_pairs ? t : max_pairs;
-    if (t > max_pairs) counters[2] |= SV_STATUS_PAIR_OVERFLOW;
-    tail[0] = 0u;
-    uint32_t lo = 1, hi = n;  // first row in [1, n] whose offset is >= max_pairs (n if none)
-    while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (off[mid] < max_pairs) lo = mid + 1;
-        else hi = mid;
-    }
-    misc[MISC_BPE] = off[lo - 1] + cnt[lo - 1];
 }
 
 __global__ void k_bb_write(const uint32_t *__restrict__ misc, const uint32_t *__restrict__ ki,
@@ -865,7 +883,7 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
     // tree over the broadphase boxes
     const uint32_t parts = div_up(n, BOUNDS_TILE);
     hipLaunchKernelGGL(k_bb_centroids, dim3(parts), dim3(BLOCK), 0, st, n, s.bpbox, s.bb_cent, s.bb_mv[0],
-                       s.bb_bounds + 6, s.bb_bounds, s.misc + MISC_BOUNDS_DONE);
+                       s.bb_bounds + 6, s.bb_bounds, s.misc + MISC_BOUNDS_DONE, s.bb_cnt, s.bb_fill);
     hipLaunchKernelGGL(k_morton, dim3(g), dim3(BLOCK), 0, st, n, s.bb_bounds, s.bb_cent, s.bb_mk[0]);
     int w = radix_sort(c, s.bb_mk[0], s.bb_mv[0], s.bb_mk[1], s.bb_mv[1], nullptr, n, 30);
     const uint32_t *ids = s.bb_mv[w];
@@ -879,14 +897,12 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
     // count, scan, fill, sort
     const int bits = 32 - __builtin_clz(n - 1);  // body indices < 2^bits
     const bool combined = 2 * bits <= 32;
-    fill_u32(c, s.bb_cnt, 0u, n);
     hipLaunchKernelGGL(k_bb_pairs<false>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_rec, s.bb_left,
                        s.bb_right, s.bb_cnt, (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr,
                        (uint32_t *)nullptr, mp, bits, combined, s.bb_stage, s.bb_scnt);
     scan_exclusive(c, s.bb_cnt, s.bb_off, nullptr, n, counters);
-    hipLaunchKernelGGL(k_bb_rows, dim3(1), dim3(64), 0, st, n, s.bb_cnt, s.bb_off, s.bb_fill, s.misc, mp, counters);
     hipLaunchKernelGGL(k_bb_scatter, dim3(div_up(n, BLOCK * BB_SCATTER_ITEMS)), dim3(BLOCK), 0, st, n, ids, s.bb_stage, s.bb_scnt, s.bb_off, counters,
-                       s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits, combined);
+                       s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits, combined, s.bb_cnt, s.misc);
     hipLaunchKernelGGL(k_bb_pairs<true>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_rec, s.bb_left,
                        s.bb_right, (uint32_t *)nullptr, s.bb_off, s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits,
                        combined, (uint32_t *)nullptr, s.bb_scnt);
