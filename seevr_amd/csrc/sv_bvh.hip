@@ -327,15 +327,23 @@ SVD void box_store(float4 *lo, float4 *hi, uint32_t i, const Box4 &b) { lo[i] = 
 SVD void box_store_coh(float4 *lo, float4 *hi, uint32_t i, const Box4 &b) { st4c(lo, i, b.lo); st4c(hi, i, b.hi); }
 // The broadphase tree's node record (k_bb_pairs): the two children's boxes
 // side by side, written by the thread that completes the node.
-SVD void rec_pair(float4 *rec, uint32_t p, const Box4 &l, const Box4 &r) {
+// The walk's 64-byte record of internal node p, per child (left, then
+// right): its box, with lo.w = the smallest body index below it, bit 31 set
+// when every body below is static (indices are < 2^31), and hi.w = the
+// child's node index, so that a visit needs no other load.
+SVD float4 rec_lo(const Box4 &b) {
+    return make_float4(b.lo.x, b.lo.y, b.lo.z,
+                       __uint_as_float(__float_as_uint(b.lo.w) | (__float_as_uint(b.hi.w) << 31)));
+}
+SVD void rec_pair(float4 *rec, uint32_t p, uint32_t li, const Box4 &l, uint32_t ri, const Box4 &r) {
     if (!rec) return;
     float4 *q = rec + 4 * (size_t)p;
-    q[0] = l.lo;
-    q[1] = l.hi;
-    q[2] = r.lo;
-    q[3] = r.hi;
+    q[0] = rec_lo(l);
+    q[1] = make_float4(l.hi.x, l.hi.y, l.hi.z, __uint_as_float(li));
+    q[2] = rec_lo(r);
+    q[3] = make_float4(r.hi.x, r.hi.y, r.hi.z, __uint_as_float(ri));
 }
-SVD void rec_pair(float4 *, uint32_t, const Box3 &, const Box3 &) {}
+SVD void rec_pair(float4 *, uint32_t, uint32_t, const Box3 &, uint32_t, const Box3 &) {}
 SVD Box4 box_union(const Box4 &l, const Box4 &r) {  // see k_bb_propagate
     const uint32_t mi = min(__float_as_uint(l.lo.w), __float_as_uint(r.lo.w));
     const uint32_t st = __float_as_uint(l.hi.w) & __float_as_uint(r.hi.w);
@@ -414,7 +422,7 @@ __global__ void __launch_bounds__(REFIT_T) k_refit_blocks(uint32_t n, const uint
         box_store(na, nb, i, lbox[t]);
         if (rec) {
             const uint32_t li = lleft[t], ri = lright[t];
-            rec_pair(rec, i, li >= n - 1 ? lbox[TH + (li - (n - 1) - b0)] : lbox[li - b0],
+            rec_pair(rec, i, li, li >= n - 1 ? lbox[TH + (li - (n - 1) - b0)] : lbox[li - b0], ri,
                      ri >= n - 1 ? lbox[TH + (ri - (n - 1) - b0)] : lbox[ri - b0]);
         }
     }
@@ -443,7 +451,7 @@ __global__ void __launch_bounds__(BLOCK) k_refit_cross(uint32_t n, const uint32_
             const BoxT sbox = box_load_coh(na, nb, sib);
             const BoxT own = box;
             box = is_left ? box_union(own, sbox) : box_union(sbox, own);
-            rec_pair(rec, p, is_left ? own : sbox, is_left ? sbox : own);
+            rec_pair(rec, p, li, is_left ? own : sbox, ri, is_left ? sbox : own);
             box_store_coh(na, nb, p, box);
             node = p;
             p = pp;
@@ -623,8 +631,7 @@ SVD void bb_append(bool found, uint32_t mi, uint32_t j, uint64_t lt, const uint3
 template <bool EMIT>
 __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *__restrict__ ids,
                                                     const float4 *__restrict__ box, const float4 *__restrict__ rec,
-                                                    const uint32_t *__restrict__ left,
-                                                    const uint32_t *__restrict__ right, uint32_t *cnt,
+                                                    uint32_t *cnt,
                                                     const uint32_t *__restrict__ off, uint32_t *tail,
                                                     uint32_t *__restrict__ pk, uint32_t *__restrict__ pv,
                                                     uint32_t max_pairs, int bits, bool combined,
@@ -635,7 +642,9 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
     if (EMIT && scnt != nullptr && scnt[t] <= (uint32_t)BB_STAGE) return;
     const uint32_t j = ids[t];
     const float4 jl = box[2 * j], jh = box[2 * j + 1];
-    const bool js = jl.w != 0.0f;
+    // a child is skipped when its tag (smallest index, bit 31 = all static)
+    // under this mask is >= j: index >= j, or j and the subtree static
+    const uint32_t jmask = jl.w != 0.0f ? 0xFFFFFFFFu : 0x7FFFFFFFu;
     uint32_t *lstack = s_stack + threadIdx.x;
     uint32_t stack[BB_STACK - BB_LDS];
     const int lane = threadIdx.x & 63;
@@ -643,15 +652,14 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
     int sp = 0;
     uint32_t node = 0, nf = 0;
     for (;;) {
-        const uint32_t ch[2] = {left[node], right[node]};
         const float4 *rn = rec + 4 * (size_t)node;
-        const float4 cb[4] = {rn[0], rn[1], rn[2], rn[3]};  // the children's boxes, loaded with their indices
+        const float4 cb[4] = {rn[0], rn[1], rn[2], rn[3]};  // the children's boxes, tags and indices
 #pragma unroll
         for (int q = 0; q < 2; q++) {
-            const uint32_t c = ch[q];
             const float4 lo = cb[2 * q], hi = cb[2 * q + 1];
-            const uint32_t mi = __float_as_uint(lo.w);
-            const bool keep = !(mi >= j || (js && __float_as_uint(hi.w) != 0u) || boxes_apart(jl, jh, lo, hi));
+            const uint32_t c = __float_as_uint(hi.w), tag = __float_as_uint(lo.w);
+            const uint32_t mi = tag & 0x7FFFFFFFu;
+            const bool keep = !((tag & jmask) >= j || boxes_apart(jl, jh, lo, hi));
             const bool leaf = keep && c >= n - 1;
             if (keep && !leaf) {
                 if (sp < BB_LDS) lstack[sp++ * BLOCK] = c;
@@ -872,7 +880,7 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
     uint32_t *pairs = B<uint32_t>(c, SV_COLLISION_PAIRS);
     const uint32_t mp = c->caps.max_pairs;
     hipStream_t st = c->stream;
-    if (!bb_reserve(c)) return false;
+    if (n > 0x7FFFFFFFu || !bb_reserve(c)) return false;  // the walk's records keep body indices in 31 bits
     if (n < 2) {
         fill_u32(c, counters, 0u, 1);
         hipLaunchKernelGGL(k_bb_post, dim3(1), dim3(1), 0, st, counters, c->s.misc, mp);
@@ -897,14 +905,12 @@ bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
     // count, scan, fill, sort
     const int bits = 32 - __builtin_clz(n - 1);  // body indices < 2^bits
     const bool combined = 2 * bits <= 32;
-    hipLaunchKernelGGL(k_bb_pairs<false>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_rec, s.bb_left,
-                       s.bb_right, s.bb_cnt, (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr,
+    hipLaunchKernelGGL(k_bb_pairs<false>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_rec, s.bb_cnt, (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr,
                        (uint32_t *)nullptr, mp, bits, combined, s.bb_stage, s.bb_scnt);
     scan_exclusive(c, s.bb_cnt, s.bb_off, nullptr, n, counters);
     hipLaunchKernelGGL(k_bb_scatter, dim3(div_up(n, BLOCK * BB_SCATTER_ITEMS)), dim3(BLOCK), 0, st, n, ids, s.bb_stage, s.bb_scnt, s.bb_off, counters,
                        s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits, combined, s.bb_cnt, s.misc);
-    hipLaunchKernelGGL(k_bb_pairs<true>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_rec, s.bb_left,
-                       s.bb_right, (uint32_t *)nullptr, s.bb_off, s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits,
+    hipLaunchKernelGGL(k_bb_pairs<true>, dim3(g), dim3(BLOCK), 0, st, n, ids, s.bpbox, s.bb_rec, (uint32_t *)nullptr, s.bb_off, s.bb_fill, s.bb_pk[0], s.bb_pv[0], mp, bits,
                        combined, (uint32_t *)nullptr, s.bb_scnt);
     const uint32_t cap = mp + n;
     const uint32_t *ki, *kj;
