@@ -654,16 +654,27 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
     for (;;) {
         const float4 *rn = rec + 4 * (size_t)node;
         const float4 cb[4] = {rn[0], rn[1], rn[2], rn[3]};  // the children's boxes, tags and indices
+        // every field is wanted: this keeps the four loads together (the
+        // compiler otherwise loaded a child's box only after its tag passed,
+        // a second dependent round trip per visit)
+        asm volatile("" ::"v"(cb[0].x), "v"(cb[0].y), "v"(cb[0].z), "v"(cb[0].w), "v"(cb[1].x), "v"(cb[1].y),
+                     "v"(cb[1].z), "v"(cb[1].w), "v"(cb[2].x), "v"(cb[2].y), "v"(cb[2].z), "v"(cb[2].w),
+                     "v"(cb[3].x), "v"(cb[3].y), "v"(cb[3].z), "v"(cb[3].w));
+        bool go0 = false, go1 = false;
+        uint32_t ch0 = 0, ch1 = 0;
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const float4 lo = cb[2 * q], hi = cb[2 * q + 1];
             const uint32_t c = __float_as_uint(hi.w), tag = __float_as_uint(lo.w);
             const uint32_t mi = tag & 0x7FFFFFFFu;
-            const bool keep = !((tag & jmask) >= j || boxes_apart(jl, jh, lo, hi));
+            const bool keep = ((tag & jmask) < j) & !boxes_apart(jl, jh, lo, hi);
             const bool leaf = keep && c >= n - 1;
-            if (keep && !leaf) {
-                if (sp < BB_LDS) lstack[sp++ * BLOCK] = c;
-                else if (sp < BB_STACK) stack[sp++ - BB_LDS] = c;
+            if (q == 0) {
+                go0 = keep && !leaf;
+                ch0 = c;
+            } else {
+                go1 = keep && !leaf;
+                ch1 = c;
             }
             if (EMIT) {
                 bb_append(leaf, mi, j, lt, off, tail, pk, pv, max_pairs, bits, combined);
@@ -683,9 +694,27 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
                 }
             }
         }
-        if (sp == 0) break;
-        --sp;
-        node = sp < BB_LDS ? lstack[sp * BLOCK] : stack[sp - BB_LDS];
+        // descend into the right child when it goes on (the left one, if it
+        // also does, waits on the stack), else into the left one, else take
+        // the stack's top: the visiting order of pushing both and popping
+        if (go0 && go1) {
+            if (sp < BB_LDS) lstack[sp++ * BLOCK] = ch0;
+            else if (sp < BB_STACK) stack[sp++ - BB_LDS] = ch0;
+        }
+        node = go1 ? ch1 : ch0;
+        if (!go0 && !go1) {
+            if (sp == 0) break;
+            --sp;
+            // an LDS read, and a private one in a branch of its own whose
+            // value an empty asm takes (else the compiler merged the reads
+            // into one flat load through a selected address)
+            uint32_t top = lstack[(sp < BB_LDS ? sp : BB_LDS - 1) * BLOCK];
+            if (sp >= BB_LDS) {
+                top = stack[sp - BB_LDS];
+                asm volatile("" : "+v"(top));
+            }
+            node = top;
+        }
     }
     if (!EMIT && stage != nullptr) scnt[t] = nf;
 }
