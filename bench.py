@@ -625,10 +625,27 @@ def main() -> int:
     copy_gbs = ctx.copy_bandwidth(256 << 20, 1 if args.calib else 20)
     log(f"plain copy kernel (256 MiB): {copy_gbs:.0f} GB/s read+write")
 
-    handoff, hx = ctx.handoff_us(20000, same_xcd=False)
-    handoff_same, hxs = ctx.handoff_us(20000, same_xcd=True)
-    log(f"hand-off between two waves: {handoff:.3f} us one way across XCDs (XCC {hx}), "
-        f"{handoff_same:.3f} us within one (XCC {hxs})")
+    # The hand-off probe needs its two one-wave workgroups resident together;
+    # when a wait passes its 2 s bound the probe reports a time-out (the
+    # kernel has ended), recorded here instead of ending the bench.
+    probe_notes = []
+
+    def probe(same_xcd):
+        try:
+            return ctx.handoff_us(20000, same_xcd=same_xcd)
+        except Exception as e:  # noqa: BLE001 - a bounded probe's time-out
+            probe_notes.append(f"hand-off probe ({'same' if same_xcd else 'across'} XCD) failed: {e}")
+            log(probe_notes[-1])
+            return None, None
+
+    handoff, hx = probe(False)
+    handoff_same, hxs = probe(True)
+    if handoff is None and handoff_same is not None:
+        probe_notes.append("roofline peak from the same-XCD hand-off")
+        handoff = handoff_same
+    if handoff is not None:
+        log(f"hand-off between two waves: {handoff:.3f} us one way across XCDs (XCC {hx}), "
+            f"{handoff_same if handoff_same is not None else float('nan'):.3f} us within one (XCC {hxs})")
 
     pre = SETTLE_FRAMES + args.warmup
     for i in range(pre):
@@ -679,9 +696,11 @@ def main() -> int:
 
     nc = min(contacts, caps.max_contacts)
     sweep_us = stages.get("phys/pgs_sweep", (0.0, 0))[0]
-    roof = sweep_roofline(sweep_us, depth, items, handoff, nc)
+    roof = sweep_roofline(sweep_us, depth, items, handoff if handoff is not None else 0.0, nc)
     roof["dag_depth_first_timed_frame"] = depth0
-    roof["handoff_us_same_xcd"] = round(handoff_same, 4)
+    roof["handoff_us_same_xcd"] = round(handoff_same, 4) if handoff_same is not None else None
+    if probe_notes:
+        roof["probe_notes"] = probe_notes
     hbm_stages = {k: roofline_entry(k, stages, n, nh, n_dyn, pairs, nc)
                   for k in ("phys/aabb", "phys/broad_phase", "phys/integrate", "phys/gravity", "bvh/radix_sort",
                             "bvh/morton_codes")}

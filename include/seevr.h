@@ -54,7 +54,7 @@ enum {
     SV_STATUS_PAIR_OVERFLOW = 1u << 0,    /* pairs found > max_pairs        */
     SV_STATUS_CONTACT_OVERFLOW = 1u << 1, /* contacts found > max_contacts  */
     SV_STATUS_SOLVER_TIMEOUT = 1u << 2,   /* ordered PGS wait expired       */
-    SV_STATUS_SORT_TIMEOUT = 1u << 3      /* radix-sort look-back expired   */
+    SV_STATUS_SORT_TIMEOUT = 1u << 3      /* sort or scan look-back expired */
 };
 
 /* ---- buffer ids --------------------------------------------------------
@@ -258,6 +258,13 @@ int sv_debug_atan2(sv_ctx *ctx, const float *y, const float *x, float *out, uint
  * 2^30 - 1, 1 <= key_bits <= 32).  A test entry point. */
 int sv_debug_radix_sort(sv_ctx *ctx, const uint32_t *keys, const uint32_t *vals, uint32_t n, uint32_t n_max,
                         uint32_t key_bits, uint32_t *keys_out, uint32_t *vals_out);
+
+/* The library's exclusive scan (the one the broadphase rows, the contact
+ * compaction and the sweep's body incidences use) on a host array: scans
+ * the first n of n_max values, with n read on the device as the pipeline
+ * passes it (n <= n_max <= 2^30); the sum of the n values (mod 2^32) goes to
+ * *total.  A test entry point. */
+int sv_debug_scan(sv_ctx *ctx, const uint32_t *in, uint32_t n, uint32_t n_max, uint32_t *out, uint32_t *total);
 
 /* Copy `nbytes` device to device `iters` times with a plain copy kernel
  * (the calibration kernel of the HBM byte counters) and report the average

@@ -213,6 +213,19 @@ class Context:
                                            vo.ctypes.data_as(u)))
         return ko, vo
 
+    def debug_scan(self, values, n_max: int | None = None):
+        """The library's exclusive scan of `values` (uint32, sums mod 2^32),
+        with the count read on the device from an n_max-sized buffer:
+        (exclusive prefixes, total)."""
+        v = np.ascontiguousarray(values, np.uint32)
+        n = v.size
+        out = np.zeros(n, np.uint32)
+        tot = C.c_uint32()
+        u = C.POINTER(C.c_uint32)
+        check(self.lib.sv_debug_scan(self.h, v.ctypes.data_as(u), n, n if n_max is None else n_max,
+                                     out.ctypes.data_as(u), C.byref(tot)))
+        return out, int(tot.value)
+
     def sweep_depth(self) -> tuple[int, int]:
         """(depth, items) of the last sweep's dependency graph."""
         d = C.c_uint32()

@@ -145,7 +145,7 @@ int check_status(sv_ctx *c) {
         return fail(SV_E_TIMEOUT, "ordered PGS made no progress for 2 s and was aborted");
     if (status & SV_STATUS_SORT_TIMEOUT) {
         (void)sv::sort_take_abort(c);
-        return fail(SV_E_TIMEOUT, "radix sort look-back made no progress for 2 s and was aborted");
+        return fail(SV_E_TIMEOUT, "radix sort or scan look-back made no progress for 2 s and was aborted");
     }
     return SV_OK;
 }
@@ -683,6 +683,31 @@ int sv_debug_radix_sort(sv_ctx *c, const uint32_t *keys, const uint32_t *vals, u
     (void)hipFree(d);
     if (rc != SV_OK) return rc;
     if (e != hipSuccess) return fail(SV_E_DEVICE, "sv_debug_radix_sort: %s", hipGetErrorString(e));
+    return SV_OK;
+}
+
+int sv_debug_scan(sv_ctx *c, const uint32_t *in, uint32_t n, uint32_t n_max, uint32_t *out, uint32_t *total) {
+    if (!c || !total || n > n_max || n_max > (1u << 30) || (n && (!in || !out)))
+        return fail(SV_E_INVALID, "sv_debug_scan: bad argument");
+    *total = 0;
+    if (!n_max) return SV_OK;
+    uint32_t *d = nullptr;
+    HIP_OK(hipMalloc(&d, (2 * (size_t)n_max + 2) * sizeof(uint32_t)));
+    uint32_t *din = d, *dout = d + n_max, *dn = d + 2 * (size_t)n_max, *dtot = dn + 1;
+    hipError_t e = hipMemset(d, 0xFF, 2 * (size_t)n_max * sizeof(uint32_t));  // past n: never read
+    if (e == hipSuccess && n) e = hipMemcpy(din, in, n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dn, &n, sizeof(n), hipMemcpyHostToDevice);
+    int rc = SV_OK;
+    if (e == hipSuccess) {
+        sv::scan_exclusive(c, din, dout, dn, n_max, dtot);
+        rc = sync_and_finish(c);
+        if (rc == SV_OK) rc = check_status(c);
+        if (rc == SV_OK && n) e = hipMemcpy(out, dout, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+        if (rc == SV_OK && e == hipSuccess) e = hipMemcpy(total, dtot, sizeof(uint32_t), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(d);
+    if (rc != SV_OK) return rc;
+    if (e != hipSuccess) return fail(SV_E_DEVICE, "sv_debug_scan: %s", hipGetErrorString(e));
     return SV_OK;
 }
 

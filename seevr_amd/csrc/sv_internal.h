@@ -94,7 +94,7 @@ enum { SWEEP_SEQ = 0, SWEEP_COHERENT = 1, SWEEP_FENCED = 2, SWEEP_PIPE = 3, SWEE
 // counter, tile counter per pass, abort word), and per pass, tile and digit
 // one look-back status word.
 constexpr uint32_t SORT_TILE_SMALL = 2048, SORT_TILE_LARGE = 8192;  // keys per tile, by sort size
-constexpr uint32_t SORT_SMALL_MAX = 1u << 18;                       // largest sort on small tiles
+constexpr uint32_t SORT_SMALL_MAX = 1u << 20;                       // largest sort on small tiles
 constexpr uint32_t SCAN_TILE_VALUES = 1024;
 constexpr uint32_t SORT_MAX_PASSES = 4;             // 32-bit keys, 8-bit digits
 constexpr uint32_t SORT_MAX_KEYS = (1u << 30) - 1;  // counts share a word with 2 flag bits
@@ -111,7 +111,11 @@ constexpr size_t sort_tiles(size_t n) {
     return small > large ? small : large;
 }
 constexpr size_t sort_hist_words(size_t n) { return SORT_STATUS_OFF + (size_t)SORT_MAX_PASSES * 256 * sort_tiles(n); }
-constexpr size_t scan_part_words(size_t n) { return (n + SCAN_TILE_VALUES - 1) / SCAN_TILE_VALUES + 1; }
+// Single-pass scan scratch: a control word, a spare word, then two halves of
+// 64-bit status words, one per tile.  Sized for the most tiles a scan of at
+// most n values uses; a capacity of w words serves scan_tiles_cap(w) tiles.
+constexpr size_t scan_part_words(size_t n) { return 2 + 4 * ((n + SCAN_TILE_VALUES - 1) / SCAN_TILE_VALUES); }
+constexpr uint32_t scan_tiles_cap(uint32_t words) { return words < 2 ? 0u : (words - 2) / 4; }
 // Centroids per partial of the LBVH's bounds reduction (sv_bvh.hip): the
 // partials take 6 floats per tile.
 constexpr uint32_t BOUNDS_TILE = 2048;
@@ -231,9 +235,12 @@ enum : int {
 
 // ---- sv_prims.hip ----
 // Exclusive scan of n = min(*n_dev, n_max) (or n_max when n_dev == null)
-// uint32 values; writes the total to *total_dev when non-null.
+// uint32 values, one launch; writes the total to *total_dev when non-null,
+// and with post_np also min(total, post_cap) to *post_np and
+// SV_STATUS_PAIR_OVERFLOW into *post_status when the total exceeds post_cap.
 void scan_exclusive(sv_ctx *c, const uint32_t *in, uint32_t *out, const uint32_t *n_dev,
-                    uint32_t n_max, uint32_t *total_dev);
+                    uint32_t n_max, uint32_t *total_dev, uint32_t post_cap = 0, uint32_t *post_np = nullptr,
+                    uint32_t *post_status = nullptr);
 // Stable LSD radix sort of (key, val) by the low `key_bits` bits of key, for
 // n = min(*n_dev, n_max) elements (n_max when n_dev == null).  Ping-pongs
 // between (k0, v0) and (k1, v1); returns 0 when the result ends in (k0, v0)

@@ -567,8 +567,9 @@ void launch_broadphase(sv_ctx *c, uint32_t n) {
                        c->s.row_count, (const uint32_t *)nullptr, c->s.bp_stage, c->s.bp_ovf, (uint32_t *)nullptr,
                        c->caps.max_pairs);
     const uint32_t ncnt = n * nseg;
-    scan_exclusive(c, c->s.row_count, c->s.row_off, nullptr, ncnt, counters);
-    hipLaunchKernelGGL(k_post_pairs, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, c->caps.max_pairs);
+    // with k_post_pairs' work (pairs to narrow, the overflow flag)
+    scan_exclusive(c, c->s.row_count, c->s.row_off, nullptr, ncnt, counters, c->caps.max_pairs,
+                   c->s.misc + MISC_NP, counters + 2);
     hipLaunchKernelGGL(k_bp_place, dim3(div_up((uint64_t)ncnt * BP_STAGE, BLOCK)), dim3(BLOCK), 0, c->stream, n,
                        nseg, c->s.row_count, c->s.row_off, c->s.bp_stage, pairs, c->caps.max_pairs);
     const uint32_t emit_grid = nrb * nseg < 512u ? nrb * nseg : 512u;

@@ -22,7 +22,10 @@ static_assert(SCAN_TILE == SCAN_TILE_VALUES, "scan scratch sizing");
 // 512-thread tiles for the same keys in flight, so the look-back walks half
 // as far, with each wave's ranking chain as short (8.57M keys: 358 -> 344 us
 // for the 4-pass sort; 512 threads x 16 rounds had been slower, 89 against
-// 80 us per pass, from the longer ranking chain).
+// 80 us per pass, from the longer ranking chain).  The choice goes by the
+// launch size, which a count read on the device can be far below: the
+// 32k-body pile's incidence sort (capacity 395k keys, ~110k sorted) took
+// 14.0 us per pass on 8192-key tiles, 11.9 on 4096-key ones.
 static_assert(256 * 8 == SORT_TILE_SMALL && 1024 * 8 == SORT_TILE_LARGE, "sort scratch sizing");
 constexpr int HIST_THREADS = 256;
 constexpr int HIST_ITEMS = 16;                         // keys per thread and iteration
@@ -67,58 +70,132 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total)
     return base + inc - v;
 }
 
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_partials(const uint32_t *in, const uint32_t *n_dev,
-                                                                uint32_t n_max, uint32_t *part) {
-    uint32_t n = read_n(n_dev, n_max);
-    uint32_t base = blockIdx.x * SCAN_TILE;
-    uint32_t s = 0;
-    if (base < n) {
+// Single-pass exclusive scan (decoupled look-back).  A workgroup takes the
+// next tile index from the control word (tiles are taken in launch order, so
+// every tile a workgroup waits for is held by a running workgroup), scans its
+// SCAN_TILE values, publishes the tile total as an "aggregate" word, and one
+// wave looks back over the predecessors' words (64 at a time) until an
+// inclusive prefix, which it then publishes for its own tile.  Status words
+// are 64-bit: flag (SCAN_FLAG_A / SCAN_FLAG_P) above a 32-bit value.
+//
+// Scratch `ctl`: word 0 = the half in use (bit 31) | the next tile index,
+// then two halves of `tiles_cap` status words.  A launch uses one half and
+// zeroes the other, the one the next launch uses; the workgroup with the last
+// tile index resets the index and flips the half, so the state lives on the
+// device (graph replays see it too).  With a count read on the device, only
+// the first ceil(n / SCAN_TILE) workgroups take tile indices (one each) and
+// clear; the others leave at once.
+// A wait is bounded (2 s, a broken invariant): it then sets
+// SV_STATUS_SORT_TIMEOUT in *timeout_status and takes the prefix so far.
+//
+// Optional output: the total to *total_dev; with post_np, also k_post_pairs'
+// work: *post_np = min(total, post_cap) and SV_STATUS_PAIR_OVERFLOW in
+// *post_status when the total exceeds post_cap.
+constexpr uint64_t SCAN_FLAG_A = 1ull << 32, SCAN_FLAG_P = 2ull << 32;
+constexpr uint64_t SCAN_STALL_TICKS = 200000000ull;  // 2 s of the 100 MHz wall clock
+
+__device__ __forceinline__ uint64_t ld_agent64(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent64(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                       const uint32_t *n_dev, uint32_t n_max, uint32_t *ctl,
+                                                       uint32_t tiles_cap, uint32_t *total_dev, uint32_t post_cap,
+                                                       uint32_t *post_np, uint32_t *post_status,
+                                                       uint32_t *timeout_status) {
+    __shared__ uint32_t s_word, s_excl;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t n = read_n(n_dev, n_max);
+    const uint32_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;  // tiles with values (<= g)
+    // only the first nt workgroups work (a count read on the device may be
+    // below the launch size): each takes one tile index, and they zero the
+    // other half (with no tile the halves stay as they are: the one in use
+    // is still clean)
+    if (blockIdx.x >= nt) {
+        if (nt == 0 && blockIdx.x == 0 && tid == 0) {
+            if (total_dev) *total_dev = 0u;
+            if (post_np) *post_np = 0u;
+        }
+        return;
+    }
+    if (tid == 0) s_word = atomicAdd(ctl, 1u);
+    __syncthreads();
+    const uint32_t half = s_word >> 31, tile = s_word & 0x7FFFFFFFu;
+    uint64_t *stat = reinterpret_cast<uint64_t *>(ctl + 2) + (size_t)half * tiles_cap;
+    uint64_t *other = reinterpret_cast<uint64_t *>(ctl + 2) + (size_t)(half ^ 1u) * tiles_cap;
+    for (uint32_t i = tile * SCAN_THREADS + tid; i < tiles_cap; i += nt * SCAN_THREADS) other[i] = 0ull;
+    if (tile == nt - 1 && tid == 0)  // every working workgroup has taken its index
+        __hip_atomic_store(ctl, (half ^ 1u) << 31, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // thread t owns elements base + t*ITEMS .. +ITEMS (contiguous per thread)
+    const uint32_t base = tile * SCAN_TILE;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t sum = 0;
 #pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; k++) {
-            uint32_t i = base + k * SCAN_THREADS + threadIdx.x;
-            if (i < n) s += in[i];
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        const uint32_t i = base + tid * SCAN_ITEMS + k;
+        v[k] = i < n ? in[i] : 0u;
+        sum += v[k];
+    }
+    uint32_t agg;
+    uint32_t ex = block_excl_scan(sum, &agg);
+    if (tile == 0) {
+        if (tid == 0) {
+            st_agent64(stat, SCAN_FLAG_P | agg);
+            s_excl = 0u;
+        }
+    } else if (tid < 64) {
+        if (tid == 0) st_agent64(stat + tile, SCAN_FLAG_A | agg);
+        // look back: lane k reads the word of tile t - k
+        uint32_t excl = 0;
+        int32_t t = (int32_t)tile - 1;
+        uint64_t start = 0;
+        for (;;) {
+            const int32_t q = t - (int32_t)lane;
+            const uint64_t w = q >= 0 ? ld_agent64(stat + q) : SCAN_FLAG_P;
+            const uint64_t pm = __ballot((w & SCAN_FLAG_P) != 0ull);
+            const uint64_t em = __ballot((w & (SCAN_FLAG_A | SCAN_FLAG_P)) == 0ull);
+            // the lanes up to the nearest inclusive prefix (or all 64)
+            const uint64_t span = pm ? (pm & (0ull - pm)) * 2ull - 1ull : ~0ull;
+            if ((em & span) == 0ull) {
+                uint32_t x = (span >> lane) & 1ull ? (uint32_t)w : 0u;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+                excl += x;
+                if (pm) break;
+                t -= 64;
+                continue;
+            }
+            // a tile in the span has not published: its workgroup is running
+            if (start == 0) start = wall_clock64();
+            if (wall_clock64() - start > SCAN_STALL_TICKS) {
+                if (lane == 0 && timeout_status) atomicOr(timeout_status, (uint32_t)SV_STATUS_SORT_TIMEOUT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (tid == 0) {
+            st_agent64(stat + tile, SCAN_FLAG_P | (uint64_t)(excl + agg));
+            s_excl = excl;
         }
     }
-    uint32_t tot;
-    block_excl_scan(s, &tot);
-    if (threadIdx.x == 0) part[blockIdx.x] = base < n ? tot : 0u;
-}
-
-// One block scans all partials in place (exclusive) and writes the total.
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_spine(uint32_t *part, uint32_t nparts, uint32_t *total_dev) {
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < nparts; base += SCAN_THREADS) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t v = i < nparts ? part[i] : 0u;
-        uint32_t tot;
-        uint32_t ex = block_excl_scan(v, &tot);
-        if (i < nparts) part[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0 && total_dev) *total_dev = carry;
-}
-
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t *in, uint32_t *out, const uint32_t *n_dev,
-                                                             uint32_t n_max, const uint32_t *part) {
-    uint32_t n = read_n(n_dev, n_max);
-    uint32_t base = blockIdx.x * SCAN_TILE;
-    if (base >= n) return;
-    // thread t owns elements base + t*ITEMS .. +ITEMS (contiguous per thread)
-    uint32_t v[SCAN_ITEMS];
-    uint32_t s = 0;
+    __syncthreads();
+    ex += s_excl;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) {
-        uint32_t i = base + threadIdx.x * SCAN_ITEMS + k;
-        v[k] = i < n ? in[i] : 0u;
-        s += v[k];
-    }
-    uint32_t tot;
-    uint32_t ex = block_excl_scan(s, &tot) + part[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        uint32_t i = base + threadIdx.x * SCAN_ITEMS + k;
+        const uint32_t i = base + tid * SCAN_ITEMS + k;
         if (i < n) out[i] = ex;
         ex += v[k];
+    }
+    if (tile == nt - 1 && tid == 0) {
+        const uint32_t total = s_excl + agg;
+        if (total_dev) *total_dev = total;
+        if (post_np) {
+            *post_np = total < post_cap ? total : post_cap;
+            if (total > post_cap) atomicOr(post_status, (uint32_t)SV_STATUS_PAIR_OVERFLOW);
+        }
     }
 }
 
@@ -445,18 +522,18 @@ void prims_reserve(sv_ctx *c, uint32_t max_scan_n, uint32_t max_sort_n) {
 }
 
 void scan_exclusive(sv_ctx *c, const uint32_t *in, uint32_t *out, const uint32_t *n_dev, uint32_t n_max,
-                    uint32_t *total_dev) {
+                    uint32_t *total_dev, uint32_t post_cap, uint32_t *post_np, uint32_t *post_status) {
     if (n_max == 0) {
         if (total_dev) fill_u32(c, total_dev, 0u, 1);
+        if (post_np) fill_u32(c, post_np, 0u, 1);
         return;
     }
-    uint32_t nparts = div_up(n_max, SCAN_TILE);
-    if (!reserve_fixed(c, c->s.scan_part, c->s.scan_part_cap, nparts)) return;
-    hipLaunchKernelGGL(k_scan_partials, dim3(nparts), dim3(SCAN_THREADS), 0, c->stream, in, n_dev, n_max,
-                       c->s.scan_part);
-    hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(SCAN_THREADS), 0, c->stream, c->s.scan_part, nparts, total_dev);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nparts), dim3(SCAN_THREADS), 0, c->stream, in, out, n_dev, n_max,
-                       c->s.scan_part);
+    const uint32_t tiles = div_up(n_max, SCAN_TILE);
+    if (!reserve_fixed(c, c->s.scan_part, c->s.scan_part_cap, (uint32_t)scan_part_words(n_max))) return;
+    const uint32_t tiles_cap = scan_tiles_cap(c->s.scan_part_cap);
+    uint32_t *counters = reinterpret_cast<uint32_t *>(c->buf[SV_ATOMIC_COUNTERS]);
+    hipLaunchKernelGGL(k_scan, dim3(tiles), dim3(SCAN_THREADS), 0, c->stream, in, out, n_dev, n_max, c->s.scan_part,
+                       tiles_cap, total_dev, post_cap, post_np, post_status, counters ? counters + 2 : nullptr);
 }
 
 int radix_sort(sv_ctx *c, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, const uint32_t *n_dev,
