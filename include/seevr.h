@@ -266,6 +266,15 @@ int sv_debug_radix_sort(sv_ctx *ctx, const uint32_t *keys, const uint32_t *vals,
  * *total.  A test entry point. */
 int sv_debug_scan(sv_ctx *ctx, const uint32_t *in, uint32_t n, uint32_t n_max, uint32_t *out, uint32_t *total);
 
+/* The broadphase's row sort on host arrays: row i < n holds cnt[i] distinct
+ * partners j in (i, n), concatenated in row order in `entries` in any order
+ * within a row; writes the sorted pairs (i, j) that fall below max_pairs to
+ * pairs_out (2 * min(total, max_pairs) values).  window_words = 0 picks the
+ * bitmap window of long rows as the pipeline does, else a multiple of 256 up
+ * to 2^15 (several windows for small values).  A test entry point. */
+int sv_debug_sort_rows(sv_ctx *ctx, const uint32_t *cnt, const uint32_t *entries, uint32_t n, uint32_t max_pairs,
+                       uint32_t window_words, uint32_t *pairs_out);
+
 /* Copy `nbytes` device to device `iters` times with a plain copy kernel
  * (the calibration kernel of the HBM byte counters) and report the average
  * read + write bandwidth in GB/s. */

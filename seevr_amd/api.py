@@ -226,6 +226,19 @@ class Context:
                                      out.ctypes.data_as(u), C.byref(tot)))
         return out, int(tot.value)
 
+    def debug_sort_rows(self, cnt, entries, max_pairs: int, window_words: int = 0) -> np.ndarray:
+        """The broadphase's row sort: rows of cnt[i] partners (entries in row
+        order, any order within a row) to the sorted (i, j) pairs below
+        max_pairs, as an (m, 2) array."""
+        cn = np.ascontiguousarray(cnt, np.uint32)
+        en = np.ascontiguousarray(entries, np.uint32)
+        kept = min(int(cn.sum()), max_pairs)
+        out = np.zeros(2 * max(kept, 1), np.uint32)
+        u = C.POINTER(C.c_uint32)
+        check(self.lib.sv_debug_sort_rows(self.h, cn.ctypes.data_as(u), en.ctypes.data_as(u), cn.size, max_pairs,
+                                          window_words, out.ctypes.data_as(u)))
+        return out[:2 * kept].reshape(-1, 2)
+
     def sweep_depth(self) -> tuple[int, int]:
         """(depth, items) of the last sweep's dependency graph."""
         d = C.c_uint32()

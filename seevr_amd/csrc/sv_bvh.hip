@@ -596,6 +596,29 @@ constexpr int BB_LDS = 16;
 // Rows a counting lane stages for k_bb_scatter (more: that lane walks again).
 constexpr int BB_STAGE = 32;
 
+// Place the pairs (mi, j) of the lanes with `found` set whose row mi starts
+// below max_pairs (the rows the capped list keeps) in their row: at
+// off[mi] + the row's fill count, in no particular order (k_rows_short and
+// k_rows_long sort each row).  The lanes holding the row of the first such
+// lane take their slots with one atomic (the ground's row gathers many
+// lanes of a wave), the others one each.
+SVD void bb_place(bool found, uint32_t mi, uint32_t j, uint64_t lt, const uint32_t *__restrict__ off,
+                  uint32_t *fill, uint32_t *__restrict__ pk, uint32_t max_pairs) {
+    const uint32_t o = found ? off[mi] : 0u;
+    const bool keep = found && o < max_pairs;
+    const uint64_t km = __ballot(keep);
+    if (!km) return;
+    const int first = __builtin_ctzll(km);
+    const uint32_t lead = __shfl(mi, first, 64);
+    const uint64_t same = __ballot(keep && mi == lead);
+    uint32_t base = 0;
+    if (keep && (km & lt) == 0ull) base = atomicAdd(fill + lead, (uint32_t)__popcll(same));
+    base = __shfl(base, first, 64);
+    if (!keep) return;
+    const uint32_t slot = mi == lead ? base + (uint32_t)__popcll(same & lt) : atomicAdd(fill + mi, 1u);
+    pk[o + slot] = j;
+}
+
 // Append the pairs (mi, j) of the lanes with `found` set whose row mi starts
 // below max_pairs (the rows the capped list keeps) at the end of the pair
 // buffer: one atomic on the append counter per wave.  The pairs land in no
@@ -717,6 +740,128 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
         }
     }
     if (!EMIT && stage != nullptr) scnt[t] = nf;
+}
+
+// ---------------------------------------------------------------------------
+// Rows of the pair list in order.  Row i holds cnt[i] distinct partners j > i
+// at pk[off[i] ...] in no particular order (off = exclusive scan of cnt);
+// sorted, they are the pairs (i, j) at slots off[i] + k of the list, which
+// keeps the slots below max_pairs (rows that start at or above it are not
+// there).  A row of at most ROW_SHORT partners is sorted by one lane in
+// registers (Batcher's odd-even merge network); longer rows are listed for
+// k_rows_long: a workgroup ranks up to ROW_RANK_MAX partners in LDS (the
+// rank of j = the partners below it), and marks longer rows in an LDS bitmap
+// over the partner range, window by window, whose bits in order are the
+// row's partners in order.
+constexpr uint32_t ROW_SHORT = 16;
+constexpr uint32_t ROW_RANK_MAX = 2048;
+constexpr uint32_t ROW_WINDOW_WORDS = 1u << 15;  // 2^20 partner indices (128 KiB of LDS) per window
+
+SVD void cmpx(uint32_t &a, uint32_t &b) {
+    const uint32_t lo = min(a, b), hi = max(a, b);
+    a = lo;
+    b = hi;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_rows_short(uint32_t n, const uint32_t *__restrict__ cnt,
+                                                      const uint32_t *__restrict__ off,
+                                                      const uint32_t *__restrict__ pk, uint32_t *__restrict__ pairs,
+                                                      uint32_t max_pairs, uint32_t *list) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = cnt[i];
+    if (c == 0) return;
+    const uint32_t o = off[i];
+    if (o >= max_pairs) return;
+    if (c > ROW_SHORT) {
+        list[1 + atomicAdd(list, 1u)] = i;
+        return;
+    }
+    uint32_t v[ROW_SHORT];
+#pragma unroll
+    for (uint32_t k = 0; k < ROW_SHORT; k++) v[k] = k < c ? pk[o + k] : 0xFFFFFFFFu;
+    // Batcher's odd-even merge sort of 16 (Knuth 5.3.4, algorithm M)
+#pragma unroll
+    for (uint32_t p = 1; p < ROW_SHORT; p <<= 1)
+#pragma unroll
+        for (uint32_t k = p; k >= 1; k >>= 1)
+#pragma unroll
+            for (uint32_t jj = k % p; jj + k < ROW_SHORT; jj += 2 * k)
+#pragma unroll
+                for (uint32_t q = 0; q < k; q++)
+                    if (q + jj + k < ROW_SHORT && (q + jj) / (2 * p) == (q + jj + k) / (2 * p)) cmpx(v[q + jj], v[q + jj + k]);
+    uint2 *out = reinterpret_cast<uint2 *>(pairs);
+#pragma unroll
+    for (uint32_t k = 0; k < ROW_SHORT; k++)
+        if (k < c && o + k < max_pairs) out[o + k] = make_uint2(i, v[k]);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_rows_long(uint32_t n, const uint32_t *__restrict__ cnt,
+                                                     const uint32_t *__restrict__ off,
+                                                     const uint32_t *__restrict__ pk, uint32_t *__restrict__ pairs,
+                                                     uint32_t max_pairs, const uint32_t *__restrict__ list,
+                                                     uint32_t window_words) {
+    extern __shared__ uint32_t lds[];  // max(ROW_RANK_MAX, window_words) words
+    __shared__ uint32_t s_part[BLOCK / 64 + 1];
+    uint2 *out = reinterpret_cast<uint2 *>(pairs);
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t m = list[0];
+    for (uint32_t q = blockIdx.x; q < m; q += gridDim.x) {
+        const uint32_t i = list[1 + q], c = cnt[i], o = off[i];
+        __syncthreads();  // the previous row's LDS reads are done
+        if (c <= ROW_RANK_MAX) {
+            for (uint32_t k = tid; k < c; k += BLOCK) lds[k] = pk[o + k];
+            __syncthreads();
+            for (uint32_t k = tid; k < c; k += BLOCK) {
+                const uint32_t x = lds[k];
+                uint32_t r = 0;
+                for (uint32_t u = 0; u < c; u++) r += lds[u] < x ? 1u : 0u;
+                if (o + r < max_pairs) out[o + r] = make_uint2(i, x);
+            }
+            continue;
+        }
+        // partners lie in [i + 1, n): windows of 32 * window_words indices
+        const uint32_t wbits = window_words * 32u;
+        uint32_t before = 0;  // partners in earlier windows
+        for (uint32_t w0 = i + 1; w0 < n; w0 += wbits) {
+            for (uint32_t k = tid; k < window_words; k += BLOCK) lds[k] = 0u;
+            __syncthreads();
+            for (uint32_t k = tid; k < c; k += BLOCK) {
+                const uint32_t d = pk[o + k] - w0;
+                if (d < wbits) atomicOr(&lds[d >> 5], 1u << (d & 31u));
+            }
+            __syncthreads();
+            // thread t owns words [t * per, (t + 1) * per): its bits' count,
+            // scanned across the workgroup, is the rank of its first bit
+            const uint32_t per = window_words / BLOCK;
+            uint32_t bitsum = 0;
+            for (uint32_t k = 0; k < per; k++) bitsum += (uint32_t)__popc(lds[tid * per + k]);
+            uint32_t x = bitsum;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(x, d, 64);
+                if ((int)lane >= d) x += y;
+            }
+            if (lane == 63) s_part[wave] = x;
+            __syncthreads();
+            uint32_t rank = before + x - bitsum, total = 0;
+            for (uint32_t w = 0; w < BLOCK / 64; w++) {
+                if (w < wave) rank += s_part[w];
+                total += s_part[w];
+            }
+            for (uint32_t k = 0; k < per; k++) {
+                uint32_t bitsw = lds[tid * per + k];
+                while (bitsw) {
+                    const uint32_t b = (uint32_t)__builtin_ctz(bitsw);
+                    bitsw &= bitsw - 1u;
+                    if (o + rank < max_pairs) out[o + rank] = make_uint2(i, w0 + (tid * per + k) * 32u + b);
+                    ++rank;
+                }
+            }
+            before += total;
+            __syncthreads();  // the window's LDS reads are done before the next clear
+        }
+    }
 }
 
 // k_bb_post's work (the pair count and the overflow flag), and where the
@@ -859,6 +1004,26 @@ __global__ void k_bb_post(uint32_t *counters, uint32_t *misc, uint32_t max_pairs
 
 namespace {
 
+// Rows to pairs: k_rows_short over every row, then k_rows_long over the
+// rows it listed (list[0] must be zero before).
+void launch_sort_rows(hipStream_t st, uint32_t n, const uint32_t *cnt, const uint32_t *off, const uint32_t *pk,
+                      uint32_t *pairs, uint32_t max_pairs, uint32_t *list, uint32_t window_words = 0) {
+    uint32_t ww = div_up(div_up(n, 32u), (uint32_t)BLOCK) * BLOCK;  // window words, a multiple of BLOCK
+    if (ww > ROW_WINDOW_WORDS) ww = ROW_WINDOW_WORDS;
+    if (window_words) ww = window_words;
+    const uint32_t lds_bytes = (ww > ROW_RANK_MAX ? ww : ROW_RANK_MAX) * (uint32_t)sizeof(uint32_t);
+    static bool lds_set = false;
+    if (!lds_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_rows_long),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, ROW_WINDOW_WORDS * sizeof(uint32_t));
+        lds_set = true;
+    }
+    hipLaunchKernelGGL(k_rows_short, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, n, cnt, off, pk, pairs, max_pairs,
+                       list);
+    hipLaunchKernelGGL(k_rows_long, dim3(256), dim3(BLOCK), lds_bytes, st, n, cnt, off, pk, pairs, max_pairs,
+                       (const uint32_t *)list, ww);
+}
+
 template <typename T>
 hipError_t lazy_alloc(sv_ctx *c, T *&p, size_t count) {
     void *q = nullptr;
@@ -903,6 +1068,11 @@ bool bb_reserve(sv_ctx *c) {
 }
 
 }  // namespace
+
+void debug_sort_rows(sv_ctx *c, uint32_t n, const uint32_t *cnt, const uint32_t *off, const uint32_t *pk,
+                     uint32_t *pairs, uint32_t max_pairs, uint32_t *list, uint32_t window_words) {
+    launch_sort_rows(c->stream, n, cnt, off, pk, pairs, max_pairs, list, window_words);
+}
 
 bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n) {
     uint32_t *counters = B<uint32_t>(c, SV_ATOMIC_COUNTERS);

@@ -12,6 +12,8 @@
 #include <new>
 #include <string>
 
+#include <vector>
+
 #include "sv_internal.h"
 
 namespace {
@@ -708,6 +710,40 @@ int sv_debug_scan(sv_ctx *c, const uint32_t *in, uint32_t n, uint32_t n_max, uin
     (void)hipFree(d);
     if (rc != SV_OK) return rc;
     if (e != hipSuccess) return fail(SV_E_DEVICE, "sv_debug_scan: %s", hipGetErrorString(e));
+    return SV_OK;
+}
+
+int sv_debug_sort_rows(sv_ctx *c, const uint32_t *cnt, const uint32_t *entries, uint32_t n, uint32_t max_pairs,
+                       uint32_t window_words, uint32_t *pairs_out) {
+    if (!c || n == 0 || !cnt || !pairs_out || (window_words && (window_words % 256 || window_words > (1u << 15))))
+        return fail(SV_E_INVALID, "sv_debug_sort_rows: bad argument");
+    std::vector<uint32_t> off(n);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        if (cnt[i] > n - 1 - i) return fail(SV_E_INVALID, "sv_debug_sort_rows: row %u holds more partners than j > i", i);
+        off[i] = (uint32_t)total;
+        total += cnt[i];
+    }
+    if (total > (1u << 30) || (total && !entries)) return fail(SV_E_INVALID, "sv_debug_sort_rows: bad entries");
+    const size_t words = 3 * (size_t)n + 2 + total + 2 * (size_t)max_pairs + 1;
+    uint32_t *d = nullptr;
+    HIP_OK(hipMalloc(&d, words * sizeof(uint32_t)));
+    uint32_t *dcnt = d, *doff = d + n, *dlist = d + 2 * (size_t)n, *dpk = dlist + n + 2, *dpairs = dpk + total + 1;
+    hipError_t e = hipMemset(d, 0, words * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemcpy(dcnt, cnt, n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(doff, off.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && total) e = hipMemcpy(dpk, entries, total * sizeof(uint32_t), hipMemcpyHostToDevice);
+    int rc = SV_OK;
+    if (e == hipSuccess) {
+        sv::debug_sort_rows(c, n, dcnt, doff, dpk, dpairs, max_pairs, dlist, window_words);
+        rc = sync_and_finish(c);
+        const uint64_t kept = total < max_pairs ? total : max_pairs;
+        if (rc == SV_OK && kept)
+            e = hipMemcpy(pairs_out, dpairs, 2 * kept * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(d);
+    if (rc != SV_OK) return rc;
+    if (e != hipSuccess) return fail(SV_E_DEVICE, "sv_debug_sort_rows: %s", hipGetErrorString(e));
     return SV_OK;
 }
 

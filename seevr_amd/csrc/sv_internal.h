@@ -320,6 +320,12 @@ void bvh_stage(sv_ctx *c, uint32_t step, uint32_t n, bool bodies);
 // order, truncation and counters as the all-pairs kernel.  Returns false
 // (and launches nothing) when its scratch cannot be allocated.
 bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n);
+// The broadphase's row sort alone: rows i < n of cnt[i] partners at
+// pk[off[i]...] (any order) to sorted pairs below max_pairs; list = n + 2
+// words with list[0] = 0; window_words = 0 picks the bitmap window, else a
+// multiple of 256 up to 2^15.
+void debug_sort_rows(sv_ctx *c, uint32_t n, const uint32_t *cnt, const uint32_t *off, const uint32_t *pk,
+                     uint32_t *pairs, uint32_t max_pairs, uint32_t *list, uint32_t window_words);
 // Body count from which the automatic broadphase choice is the LBVH.
 constexpr uint32_t BP_LBVH_MIN_BODIES = 1u << 17;
 
