@@ -592,7 +592,7 @@ def main() -> int:
     ap.add_argument("--solver-wgs", type=int, default=0, help="workgroups of the sweep kernel (0 = default)")
     ap.add_argument("--sweep-variant", default="seq", choices=["seq", "pipelined", "paired", "seq_basic", "coherent", "fenced"],
                     help="hand-off variant of the multi-workgroup sweep")
-    ap.add_argument("--broadphase", default="auto", choices=["auto", "all_pairs", "lbvh"],
+    ap.add_argument("--broadphase", default="auto", choices=["auto", "all_pairs", "lbvh", "sweep"],
                     help="broadphase method (both give the all-pairs list bit for bit)")
     ap.add_argument("--no-tri", action="store_true", help="skip the triangle-BVH section")
     ap.add_argument("--window-file", type=Path, default=None,

@@ -366,8 +366,11 @@ int sv_debug_color_counts(sv_ctx *ctx, uint32_t *out, uint32_t cap);
  * (i, j) order, truncated to max_pairs with counters[0] = the total.  No
  * counterpart in the reference.  SV_BROADPHASE_ALL_PAIRS: the LDS-tiled
  * N^2 test.  SV_BROADPHASE_LBVH: each body walks an LBVH built over the
- * body boxes of this substep.  SV_BROADPHASE_AUTO picks by body count. */
-enum { SV_BROADPHASE_AUTO = 0, SV_BROADPHASE_ALL_PAIRS = 1, SV_BROADPHASE_LBVH = 2 };
+ * body boxes of this substep.  SV_BROADPHASE_SWEEP: sweep and prune along x
+ * (boxes ordered by their lower x bound, each tested against the run of
+ * boxes that start before its upper x bound).  SV_BROADPHASE_AUTO picks by
+ * body count. */
+enum { SV_BROADPHASE_AUTO = 0, SV_BROADPHASE_ALL_PAIRS = 1, SV_BROADPHASE_LBVH = 2, SV_BROADPHASE_SWEEP = 3 };
 int sv_set_broadphase(sv_ctx *ctx, int method);
 
 /* Warm start (library extension beside the reference path; the reference

@@ -187,10 +187,10 @@ class Context:
             check(n)
         return [int(out[i]) for i in range(n)]
 
-    BROADPHASES = {"auto": 0, "all_pairs": 1, "lbvh": 2}
+    BROADPHASES = {"auto": 0, "all_pairs": 1, "lbvh": 2, "sweep": 3}
 
     def set_broadphase(self, method: str = "auto") -> None:
-        """Broadphase method: "auto", "all_pairs" or "lbvh" (identical pair lists)."""
+        """Broadphase method: "auto", "all_pairs", "lbvh" or "sweep" (identical pair lists)."""
         check(self.lib.sv_set_broadphase(self.h, self.BROADPHASES[method]))
 
     SWEEP_VARIANTS = {"seq": 0, "coherent": 1, "fenced": 2, "pipelined": 3, "seq_basic": 4, "paired": 5}

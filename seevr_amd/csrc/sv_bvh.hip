@@ -750,9 +750,9 @@ __global__ void __launch_bounds__(BLOCK) k_bb_pairs(uint32_t n, const uint32_t *
 // there).  A row of at most ROW_SHORT partners is sorted by one lane in
 // registers (Batcher's odd-even merge network); longer rows are listed for
 // k_rows_long: a workgroup ranks up to ROW_RANK_MAX partners in LDS (the
-// rank of j = the partners below it), and marks longer rows in an LDS bitmap
-// over the partner range, window by window, whose bits in order are the
-// row's partners in order.
+// rank of j = the partners below it) when that is cheaper than marking the
+// row in an LDS bitmap over the partner range, window by window, whose bits
+// in order are the row's partners in order.
 constexpr uint32_t ROW_SHORT = 16;
 constexpr uint32_t ROW_RANK_MAX = 2048;
 constexpr uint32_t ROW_WINDOW_WORDS = 1u << 15;  // 2^20 partner indices (128 KiB of LDS) per window
@@ -801,7 +801,8 @@ __global__ void __launch_bounds__(BLOCK) k_rows_long(uint32_t n, const uint32_t 
                                                      const uint32_t *__restrict__ pk, uint32_t *__restrict__ pairs,
                                                      uint32_t max_pairs, const uint32_t *__restrict__ list,
                                                      uint32_t window_words) {
-    extern __shared__ uint32_t lds[];  // max(ROW_RANK_MAX, window_words) words
+    extern __shared__ uint4 lds_raw[];  // max(ROW_RANK_MAX, window_words) words, 16-byte aligned
+    uint32_t *lds = reinterpret_cast<uint32_t *>(lds_raw);
     __shared__ uint32_t s_part[BLOCK / 64 + 1];
     uint2 *out = reinterpret_cast<uint2 *>(pairs);
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -809,13 +810,20 @@ __global__ void __launch_bounds__(BLOCK) k_rows_long(uint32_t n, const uint32_t 
     for (uint32_t q = blockIdx.x; q < m; q += gridDim.x) {
         const uint32_t i = list[1 + q], c = cnt[i], o = off[i];
         __syncthreads();  // the previous row's LDS reads are done
-        if (c <= ROW_RANK_MAX) {
+        // ranks cost about c^2 compares over the workgroup, a bitmap about
+        // three passes over its (n - i) / 32 words plus c
+        if (c <= ROW_RANK_MAX && (uint64_t)c * c <= 3ull * ((n - i) / 32u) * BLOCK / 64u + (uint64_t)c * 64u) {
             for (uint32_t k = tid; k < c; k += BLOCK) lds[k] = pk[o + k];
+            for (uint32_t k = c + tid; k < ((c + 3u) & ~3u); k += BLOCK) lds[k] = 0xFFFFFFFFu;  // pad to 4
             __syncthreads();
+            const uint4 *l4 = lds_raw;
             for (uint32_t k = tid; k < c; k += BLOCK) {
                 const uint32_t x = lds[k];
                 uint32_t r = 0;
-                for (uint32_t u = 0; u < c; u++) r += lds[u] < x ? 1u : 0u;
+                for (uint32_t u = 0; u < (c + 3u) / 4u; u++) {  // four partners per LDS read
+                    const uint4 v = l4[u];
+                    r += (v.x < x ? 1u : 0u) + (v.y < x ? 1u : 0u) + (v.z < x ? 1u : 0u) + (v.w < x ? 1u : 0u);
+                }
                 if (o + r < max_pairs) out[o + r] = make_uint2(i, x);
             }
             continue;
@@ -1000,6 +1008,376 @@ __global__ void k_bb_post(uint32_t *counters, uint32_t *misc, uint32_t max_pairs
     if (t > max_pairs) counters[2] |= SV_STATUS_PAIR_OVERFLOW;
 }
 
+// ---------------------------------------------------------------------------
+// Broadphase by sweep and prune (extension beside physics.comp:451-472, the
+// same pair list).  Boxes are ordered by a 16-bit key of their lower x bound
+// (the top half of the order-preserving integer image of the float, -0
+// taken as +0), which does not decrease with the bound: a box q after box p
+// whose lower x bound is at most p's upper one has a key at most key(p's
+// upper bound), and every box between them too.  So the boxes p must be
+// tested against are a run after it, which ends where the keys pass
+// key(upper x of p); candidates in the run get the exact test (the
+// reference's: touching counts, static-static skipped), and each unordered
+// pair is tested once, from its earlier box.  Pair (b_p, b_q) goes to row
+// min(b_p, b_q); rows are counted, scanned, filled and sorted as in the
+// LBVH broadphase (k_rows_short, k_rows_long).
+//
+// k_sap: one wave per 8 consecutive boxes of the order, 64 candidates a
+// round, at most SAP_SPAN candidates past the first; a box whose run goes
+// on finds where it ends (binary search over the keys) and is listed (box,
+// next candidate, first item) for k_sap_long, which continues the listed
+// runs in items of SAP_ITEM candidates, one wave per item.  One 64-bit
+// atomic counts the entries (high word) and their items (low word), so the
+// entries' first items increase with the entry.
+constexpr uint32_t SAP_ROWS = 8;      // boxes per wave of k_sap
+constexpr uint32_t SAP_SPAN = 4096;   // candidates k_sap takes after a workgroup's first box
+constexpr uint32_t SAP_ITEM = 512;    // candidates per item of k_sap_long
+
+// Keys come from a map set by the previous launch's lower x bounds: with
+// scale > 0, key(x) = floor((x - x0) * scale) clamped to [0, 2^16 - 1]
+// (rounded subtraction and multiplication by a positive constant do not
+// decrease with x, nor does the floor; -0 and +0 give the same key), fine
+// enough that boxes rarely share a key; else (the first launch) the top
+// half of the order-preserving integer image of x (-0 taken as +0).  One map
+// serves every kernel of a launch.
+struct SapMap {
+    float x0, scale;
+};
+
+SVD uint32_t sap_ord(float x) {
+    const uint32_t u = __float_as_uint(x + 0.0f);  // -0 -> +0
+    return u & 0x80000000u ? ~u : u | 0x80000000u;
+}
+SVD float sap_unord(uint32_t u) { return __uint_as_float(u & 0x80000000u ? u & 0x7FFFFFFFu : ~u); }
+
+SVD uint32_t sap_key(float x, SapMap m) {
+    if (m.scale > 0.0f) {
+        const float t = (x - m.x0) * m.scale;
+        return t <= 0.0f ? 0u : (t >= 65535.0f ? 65535u : (uint32_t)t);
+    }
+    return sap_ord(x) >> 16;
+}
+
+// The map for the next launch from this launch's lower-bound extremes
+// (order-preserving images, reset for the next launch).  One thread.
+__global__ void k_sap_map(float *map) {
+    uint32_t *ext = reinterpret_cast<uint32_t *>(map + 2);
+    const uint32_t lo = ext[0], hi = ext[1];
+    float x0 = 0.0f, scale = 0.0f;
+    if (lo <= hi) {
+        x0 = sap_unord(lo);
+        const float span = sap_unord(hi) - x0;
+        if (span > 0.0f && span < 3.0e38f) {
+            scale = 65535.0f / span;
+            if (!(scale < 3.0e38f)) scale = 0.0f;
+        }
+    }
+    map[0] = x0;
+    map[1] = scale;
+    ext[0] = 0xFFFFFFFFu;
+    ext[1] = 0u;
+}
+
+// The boxes are put in key order by counting: k_sap_keys counts the boxes
+// of each of the 2^16 keys, a scan gives each key its first slot, and
+// k_sap_order puts each box at its key's next slot, taken by counting the
+// key's count back down (which leaves the counts at zero for the next
+// launch).  The order within one key is of no account: the runs only need
+// keys that do not decrease.
+constexpr uint32_t SAP_KEYS = 1u << 16;
+
+// Key counts, and the zeroed row counts, fill counts, list heads (long rows
+// of the row sort, continued runs) and the flat list's tail.
+__global__ void __launch_bounds__(BLOCK) k_sap_keys(uint32_t n, const float4 *__restrict__ box,
+                                                    uint32_t *__restrict__ kcount, uint32_t *__restrict__ cnt,
+                                                    uint32_t *__restrict__ fill, uint32_t *__restrict__ rows_list,
+                                                    uint32_t *__restrict__ cont, uint32_t *__restrict__ flat_tail,
+                                                    float *map) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i == 0) {
+        rows_list[0] = 0u;
+        cont[0] = 0u;  // the entry and item counter (64-bit)
+        cont[1] = 0u;
+        flat_tail[0] = 0u;
+    }
+    const SapMap m{map[0], map[1]};
+    // this launch's lower-bound extremes (the next launch's map), one
+    // atomic per wave each
+    uint32_t omin = 0xFFFFFFFFu, omax = 0u;
+    if (i < n) {
+        const float x = box[2 * i].x;
+        atomicAdd(kcount + sap_key(x, m), 1u);
+        cnt[i] = 0u;
+        fill[i] = 0u;
+        omin = omax = sap_ord(x);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        omin = min(omin, (uint32_t)__shfl_xor(omin, d, 64));
+        omax = max(omax, (uint32_t)__shfl_xor(omax, d, 64));
+    }
+    __shared__ uint32_t s_ext[2][BLOCK / 64];
+    if ((threadIdx.x & 63u) == 0) {
+        s_ext[0][threadIdx.x >> 6] = omin;
+        s_ext[1][threadIdx.x >> 6] = omax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // one pair of atomics per workgroup
+#pragma unroll
+        for (int w = 1; w < BLOCK / 64; w++) {
+            omin = min(omin, s_ext[0][w]);
+            omax = max(omax, s_ext[1][w]);
+        }
+        if (omin <= omax) {
+            uint32_t *ext = reinterpret_cast<uint32_t *>(map + 2);
+            atomicMin(ext, omin);
+            atomicMax(ext + 1, omax);
+        }
+    }
+}
+
+// The boxes in key order: lo.w = 1 if static, hi.w = the body index; and
+// their keys.
+__global__ void __launch_bounds__(BLOCK) k_sap_order(uint32_t n, const float4 *__restrict__ box,
+                                                     uint32_t *kcount, const uint32_t *__restrict__ kfirst,
+                                                     uint32_t *__restrict__ skey, float4 *__restrict__ sbox,
+                                                     const float *__restrict__ map) {
+    const uint32_t b = blockIdx.x * BLOCK + threadIdx.x;
+    if (b >= n) return;
+    const float4 lo = box[2 * b], hi = box[2 * b + 1];
+    const uint32_t k = sap_key(lo.x, SapMap{map[0], map[1]});
+    const uint32_t p = kfirst[k] + atomicSub(kcount + k, 1u) - 1u;
+    skey[p] = k;
+    sbox[2 * p] = make_float4(lo.x, lo.y, lo.z, lo.w != 0.0f ? 1.0f : 0.0f);
+    sbox[2 * p + 1] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(b));
+}
+
+// Pairs go out two ways.  The counting passes bump the row counts and stage
+// each wave's pairs in LDS (SAP_STAGE at most), writing a run of them to the
+// flat list at a slot taken with one atomic when the stage is nearly full
+// and at the end (writes stop at the list's capacity; the count goes on).
+// k_sap_place then puts the listed pairs in their rows.  Only when the count
+// passed the capacity do the emit passes sweep again and place every pair
+// in its row themselves (otherwise they leave at once).
+constexpr uint32_t SAP_STAGE = 256;
+
+struct SapOut {
+    uint32_t *cnt;
+    const uint32_t *off;
+    uint32_t *fill;
+    uint32_t *pk;
+    uint32_t max_pairs;
+    uint2 *flat;
+    uint32_t flat_cap;
+    uint32_t *flat_tail;
+    const float *map;
+};
+
+SVD void sap_flush(const uint2 *stage, uint32_t &k, uint32_t lane, const SapOut &o) {
+    __builtin_amdgcn_wave_barrier();  // the wave's stage writes are done
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(o.flat_tail, k);
+    base = __shfl(base, 0, 64);
+    for (uint32_t e = lane; e < k; e += 64)
+        if (base + e < o.flat_cap) o.flat[base + e] = stage[e];
+    __builtin_amdgcn_wave_barrier();  // the reads are done before the stage refills
+    k = 0;
+}
+
+// The pairs of row box rb with the 64 candidates of a round (lane: body qb),
+// m = the lanes with a pair.
+template <bool EMIT>
+SVD void sap_take(uint64_t m, uint32_t rb, uint32_t qb, uint32_t lane, uint64_t lt, uint2 *stage, uint32_t &k,
+                  const SapOut &o) {
+    const bool found = (m >> lane) & 1ull;
+    const uint32_t i = rb < qb ? rb : qb, j = rb < qb ? qb : rb;
+    if (EMIT) {
+        bb_place(found, i, j, lt, o.off, o.fill, o.pk, o.max_pairs);
+        return;
+    }
+    // the lanes whose pair is in the row box's own row: one atomic
+    const uint64_t own = __ballot(found && rb < qb);
+    if (own && (own & lt) == 0ull && found && rb < qb) atomicAdd(o.cnt + rb, (uint32_t)__popcll(own));
+    if (found && qb < rb) atomicAdd(o.cnt + qb, 1u);
+    if (found) stage[k + (uint32_t)__popcll(m & lt)] = make_uint2(i, j);
+    k += (uint32_t)__popcll(m);
+    if (k > SAP_STAGE - 64) sap_flush(stage, k, lane, o);
+}
+
+// k_sap: a workgroup takes SAP_ROWS * 4 consecutive boxes (a wave per
+// SAP_ROWS) and streams the candidates after its first box through LDS, 256
+// a tile (the next tile loaded while one is tested), until the tile's first
+// key is past every row's limit or SAP_SPAN candidates; per (row, 64
+// candidates) one lane mask, as in the all-pairs kernel; a row whose run
+// ended before a chunk skips it.
+template <bool EMIT>
+__global__ void __launch_bounds__(BLOCK) k_sap(uint32_t n, const float4 *__restrict__ sbox,
+                                               const uint32_t *__restrict__ skey, SapOut o, uint32_t *cont) {
+    __shared__ float4 s_lo[BLOCK], s_hi[BLOCK];
+    __shared__ uint32_t s_key[BLOCK], s_kmax[BLOCK / 64];
+    __shared__ uint2 s_stage[BLOCK / 64][SAP_STAGE];
+    if (EMIT && *o.flat_tail <= o.flat_cap) return;  // the flat list holds every pair
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t P0 = blockIdx.x * SAP_ROWS * (BLOCK / 64);
+    if (P0 >= n) return;
+    const uint32_t p0 = P0 + wave * SAP_ROWS;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint2 *stage = s_stage[wave];
+    uint32_t staged = 0;
+    const SapMap km{o.map[0], o.map[1]};
+    float4 rlo[SAP_ROWS], rhi[SAP_ROWS];
+    uint32_t klim[SAP_ROWS];
+    uint32_t kmax = 0, rstatic = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < SAP_ROWS; r++) {
+        const uint32_t p = p0 + r < n ? p0 + r : n - 1;
+        rlo[r] = sbox[2 * p];
+        rhi[r] = sbox[2 * p + 1];
+        klim[r] = sap_key(rhi[r].x, km);
+        if (p0 + r < n) kmax = klim[r] > kmax ? klim[r] : kmax;
+        rstatic |= (rlo[r].w != 0.0f ? 1u : 0u) << r;
+        asm volatile("" : "+v"(rlo[r].x), "+v"(rlo[r].y), "+v"(rlo[r].z), "+v"(rhi[r].x), "+v"(rhi[r].y),
+                     "+v"(rhi[r].z));
+    }
+    rstatic = __builtin_amdgcn_readfirstlane(rstatic);  // wave-uniform: scalar masks and branches
+    kmax = __builtin_amdgcn_readfirstlane(kmax);
+    if (lane == 0) s_kmax[wave] = kmax;
+    __syncthreads();
+    uint32_t kblk = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < BLOCK / 64; w++) kblk = s_kmax[w] > kblk ? s_kmax[w] : kblk;
+    const uint32_t qend = min(n, P0 + 1 + SAP_SPAN);
+    uint32_t t0 = P0 + 1;
+    float4 nlo, nhi;
+    uint32_t nkey;
+    {
+        const uint32_t q = t0 + threadIdx.x, qc = q < n ? q : n - 1;
+        nlo = sbox[2 * qc];
+        nhi = sbox[2 * qc + 1];
+        nkey = q < qend ? skey[qc] : 0xFFFFFFFFu;  // past the span: above every limit
+    }
+    for (; t0 < qend; t0 += BLOCK) {
+        __syncthreads();  // the previous tile's reads are done
+        s_lo[threadIdx.x] = nlo;
+        s_hi[threadIdx.x] = nhi;
+        s_key[threadIdx.x] = nkey;
+        __syncthreads();
+        if (s_key[0] > kblk) break;  // keys do not decrease: every run has ended
+        if (t0 + BLOCK < qend) {
+            const uint32_t q = t0 + BLOCK + threadIdx.x, qc = q < n ? q : n - 1;
+            nlo = sbox[2 * qc];
+            nhi = sbox[2 * qc + 1];
+            nkey = q < qend ? skey[qc] : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (uint32_t ch = 0; ch < BLOCK / 64; ch++) {
+            const uint32_t qi = ch * 64 + lane, qq = t0 + qi;
+            const uint32_t kfirst = s_key[ch * 64];  // the chunk's smallest key
+            if (kfirst > kmax) break;               // this wave's runs have all ended
+            const float4 lo = s_lo[qi], hi = s_hi[qi];
+            const uint32_t kq = s_key[qi], qb = __float_as_uint(hi.w);
+            const uint64_t sq = __builtin_amdgcn_ballot_w64(lo.w != 0.0f);
+#pragma unroll
+            for (uint32_t r = 0; r < SAP_ROWS; r++) {
+                const uint32_t kl = __builtin_amdgcn_readfirstlane(klim[r]);  // wave-uniform: a scalar skip
+                if (p0 + r >= n || kfirst > kl) continue;  // row r's run ended before this chunk
+                // one predicate, no short circuit (each && was a branch)
+                const bool hit = (kq <= kl) & (qq > p0 + r) & !(boxes_gap(rlo[r], rhi[r], lo, hi) > 0.0f);
+                uint64_t m = __builtin_amdgcn_ballot_w64(hit);
+                if ((rstatic >> r) & 1u) m &= ~sq;
+                if (m) sap_take<EMIT>(m, __builtin_amdgcn_readfirstlane(__float_as_uint(rhi[r].w)), qb, lane, lt,
+                                      stage, staged, o);
+            }
+        }
+    }
+    if (EMIT) return;
+    if (staged) sap_flush(stage, staged, lane, o);
+    // runs that go past the span: continued by k_sap_long
+    if (t0 >= qend && qend < n && lane < SAP_ROWS && p0 + lane < n) {
+        uint32_t kl = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < SAP_ROWS; r++)
+            if (r == lane) kl = klim[r];
+        if (skey[qend] <= kl) {
+            uint32_t lo = qend + 1, hi = n;  // the run ends at the first key above kl in (qend, n]
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo) / 2;
+                if (skey[mid] <= kl) lo = mid + 1;
+                else hi = mid;
+            }
+            const uint32_t items = (lo - qend + SAP_ITEM - 1) / SAP_ITEM;
+            const unsigned long long old =
+                atomicAdd(reinterpret_cast<unsigned long long *>(cont), (1ull << 32) | items);
+            const uint32_t e = (uint32_t)(old >> 32);
+            cont[2 + 3 * e] = p0 + lane;
+            cont[3 + 3 * e] = qend;
+            cont[4 + 3 * e] = (uint32_t)old;  // its first item
+        }
+    }
+}
+
+// The flat list's pairs to their rows (when it held them all).
+__global__ void __launch_bounds__(BLOCK) k_sap_place(const uint2 *__restrict__ flat, const uint32_t *flat_tail,
+                                                     uint32_t flat_cap, const uint32_t *__restrict__ off,
+                                                     uint32_t *fill, uint32_t *__restrict__ pk, uint32_t max_pairs) {
+    const uint32_t total = *flat_tail;
+    if (total > flat_cap) return;  // the emit passes place them
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t e0 = blockIdx.x * BLOCK; e0 < total; e0 += gridDim.x * BLOCK) {
+        const uint32_t e = e0 + threadIdx.x;
+        const uint2 pr = e < total ? flat[e] : make_uint2(0u, 0u);
+        bb_place(e < total, pr.x, pr.y, lt, off, fill, pk, max_pairs);
+    }
+}
+
+// The listed runs, one wave per item of SAP_ITEM candidates: item g belongs
+// to the last entry whose first item is <= g (binary search).
+template <bool EMIT>
+__global__ void __launch_bounds__(BLOCK) k_sap_long(uint32_t n, const float4 *__restrict__ sbox,
+                                                    const uint32_t *__restrict__ skey, SapOut o,
+                                                    const uint32_t *__restrict__ cont) {
+    __shared__ uint2 s_stage[BLOCK / 64][SAP_STAGE];
+    if (EMIT && *o.flat_tail <= o.flat_cap) return;  // the flat list holds every pair
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t gw = blockIdx.x * (BLOCK / 64) + wave, nw = gridDim.x * (BLOCK / 64);
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint2 *stage = s_stage[wave];
+    uint32_t staged = 0;
+    const uint32_t items = cont[0], m = cont[1];
+    for (uint32_t g = gw; g < items; g += nw) {
+        uint32_t lo = 0, hi = m;  // last entry with first item <= g, in [0, m)
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (cont[4 + 3 * mid] <= g) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t p = cont[2 + 3 * lo], qbeg = cont[3 + 3 * lo] + (g - cont[4 + 3 * lo]) * SAP_ITEM;
+        const uint32_t qend = min(n, qbeg + SAP_ITEM);
+        const float4 rlo = sbox[2 * p], rhi = sbox[2 * p + 1];
+        const uint32_t klim = sap_key(rhi.x, SapMap{o.map[0], o.map[1]}), rb = __float_as_uint(rhi.w);
+        uint32_t qn = min(qbeg + lane, n - 1);
+        float4 nlo = sbox[2 * qn], nhi = sbox[2 * qn + 1];
+        uint32_t nkq = skey[qn];
+        for (uint32_t q0 = qbeg; q0 < qend; q0 += 64) {
+            const float4 lo4 = nlo, hi4 = nhi;
+            const uint32_t kq = nkq;
+            if (q0 + 64 < qend) {
+                qn = min(q0 + 64 + lane, n - 1);
+                nlo = sbox[2 * qn];
+                nhi = sbox[2 * qn + 1];
+                nkq = skey[qn];
+            }
+            if (__builtin_amdgcn_readfirstlane(kq) > klim) break;
+            const bool found = q0 + lane < qend && kq <= klim && !(lo4.w != 0.0f && rlo.w != 0.0f) &&
+                               !boxes_apart(rlo, rhi, lo4, hi4);
+            const uint64_t mm = __ballot(found);
+            if (mm) sap_take<EMIT>(mm, rb, __float_as_uint(hi4.w), lane, lt, stage, staged, o);
+        }
+    }
+    if (!EMIT && staged) sap_flush(stage, staged, lane, o);
+}
+
 }  // namespace
 
 namespace {
@@ -1067,7 +1445,74 @@ bool bb_reserve(sv_ctx *c) {
     return s.bb_ready;
 }
 
+bool sap_reserve(sv_ctx *c) {
+    if (c->s.sap_ready) return true;
+    Scratch &s = c->s;
+    const size_t N = c->caps.max_bodies, P = c->caps.max_pairs;
+    hipError_t e = hipSuccess;
+    e = lazy_alloc(c, s.sap_kcount, SAP_KEYS);
+    if (e == hipSuccess) e = hipMemset(s.sap_kcount, 0, SAP_KEYS * sizeof(uint32_t));  // zero between launches
+    if (e == hipSuccess) e = lazy_alloc(c, s.sap_kfirst, SAP_KEYS);
+    if (e == hipSuccess) e = lazy_alloc(c, s.sap_map, 4);
+    if (e == hipSuccess) {  // no map yet (x0 = scale = 0), extremes empty
+        const uint32_t init[4] = {0u, 0u, 0xFFFFFFFFu, 0u};
+        e = hipMemcpy(s.sap_map, init, sizeof(init), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = lazy_alloc(c, s.sap_skey, N + 1);
+    if (e == hipSuccess) e = lazy_alloc(c, s.sap_box, 2 * N + 2);
+    if (e == hipSuccess) e = lazy_alloc(c, s.sap_cnt, N + 1);
+    if (e == hipSuccess) e = lazy_alloc(c, s.sap_off, N + 1);
+    if (e == hipSuccess) e = lazy_alloc(c, s.sap_fill, N + 1);
+    if (e == hipSuccess) e = lazy_alloc(c, s.sap_part, P + N + 1);
+    if (e == hipSuccess) e = lazy_alloc(c, s.sap_rows, N + 2);
+    if (e == hipSuccess) e = lazy_alloc(c, s.sap_cont, 3 * N + 4);
+    if (e == hipSuccess) e = lazy_alloc(c, s.sap_flat, P + N + 1);
+    if (e == hipSuccess) e = lazy_alloc(c, s.sap_tail, 4);
+    s.sap_ready = e == hipSuccess;
+    return s.sap_ready;
+}
+
 }  // namespace
+
+bool launch_broadphase_sap(sv_ctx *c, uint32_t n) {
+    uint32_t *counters = B<uint32_t>(c, SV_ATOMIC_COUNTERS);
+    uint32_t *pairs = B<uint32_t>(c, SV_COLLISION_PAIRS);
+    const uint32_t mp = c->caps.max_pairs;
+    hipStream_t st = c->stream;
+    if (n > 0x7FFFFFFFu || !sap_reserve(c)) return false;
+    if (n < 2) {
+        fill_u32(c, counters, 0u, 1);
+        hipLaunchKernelGGL(k_bb_post, dim3(1), dim3(1), 0, st, counters, c->s.misc, mp);
+        return true;
+    }
+    Scratch &s = c->s;
+    const uint32_t g = div_up(n, BLOCK);
+    hipLaunchKernelGGL(k_sap_keys, dim3(g), dim3(BLOCK), 0, st, n, s.bpbox, s.sap_kcount, s.sap_cnt, s.sap_fill,
+                       s.sap_rows, s.sap_cont, s.sap_tail, s.sap_map);
+    scan_exclusive(c, s.sap_kcount, s.sap_kfirst, nullptr, SAP_KEYS, nullptr);
+    hipLaunchKernelGGL(k_sap_order, dim3(g), dim3(BLOCK), 0, st, n, s.bpbox, s.sap_kcount,
+                       (const uint32_t *)s.sap_kfirst, s.sap_skey, s.sap_box, (const float *)s.sap_map);
+    const uint32_t *skey = s.sap_skey;
+    const uint32_t gs = div_up(n, SAP_ROWS * (uint32_t)(BLOCK / 64));  // SAP_ROWS boxes per wave
+    const uint32_t gl = 128;                                          // k_sap_long workgroups
+    const uint32_t flat_cap = mp + n;
+    const SapOut out{s.sap_cnt, s.sap_off, s.sap_fill, s.sap_part, mp, s.sap_flat, flat_cap, s.sap_tail, s.sap_map};
+    // count the rows and list the pairs, scan the rows (with the pair count
+    // and the overflow flag), place the listed pairs in their rows (or sweep
+    // again to place them when the list was too short), sort the rows
+    hipLaunchKernelGGL(k_sap<false>, dim3(gs), dim3(BLOCK), 0, st, n, s.sap_box, skey, out, s.sap_cont);
+    hipLaunchKernelGGL(k_sap_long<false>, dim3(gl), dim3(BLOCK), 0, st, n, s.sap_box, skey, out,
+                       (const uint32_t *)s.sap_cont);
+    scan_exclusive(c, s.sap_cnt, s.sap_off, nullptr, n, counters, mp, s.misc + MISC_NP, counters + 2);
+    hipLaunchKernelGGL(k_sap_place, dim3(256), dim3(BLOCK), 0, st, (const uint2 *)s.sap_flat,
+                       (const uint32_t *)s.sap_tail, flat_cap, s.sap_off, s.sap_fill, s.sap_part, mp);
+    hipLaunchKernelGGL(k_sap<true>, dim3(gs), dim3(BLOCK), 0, st, n, s.sap_box, skey, out, s.sap_cont);
+    hipLaunchKernelGGL(k_sap_long<true>, dim3(gl), dim3(BLOCK), 0, st, n, s.sap_box, skey, out,
+                       (const uint32_t *)s.sap_cont);
+    hipLaunchKernelGGL(k_sap_map, dim3(1), dim3(1), 0, st, s.sap_map);  // after the last use of this launch's map
+    launch_sort_rows(st, n, s.sap_cnt, s.sap_off, s.sap_part, pairs, mp, s.sap_rows);
+    return true;
+}
 
 void debug_sort_rows(sv_ctx *c, uint32_t n, const uint32_t *cnt, const uint32_t *off, const uint32_t *pk,
                      uint32_t *pairs, uint32_t max_pairs, uint32_t *list, uint32_t window_words) {

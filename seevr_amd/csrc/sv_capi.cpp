@@ -577,7 +577,7 @@ int sv_debug_set_sweep_variant(sv_ctx *c, int variant) {
 
 int sv_set_broadphase(sv_ctx *c, int method) {
     if (!c) return fail(SV_E_INVALID, "sv_set_broadphase: null context");
-    if (method < SV_BROADPHASE_AUTO || method > SV_BROADPHASE_LBVH)
+    if (method < SV_BROADPHASE_AUTO || method > SV_BROADPHASE_SWEEP)
         return fail(SV_E_INVALID, "sv_set_broadphase: unknown method %d", method);
     c->broadphase_mode = (uint32_t)method;
     return SV_OK;

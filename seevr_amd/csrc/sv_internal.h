@@ -54,6 +54,18 @@ struct Scratch {
     uint32_t *bb_mk[2] = {nullptr, nullptr}, *bb_mv[2] = {nullptr, nullptr};  // [N] Morton sort
     uint32_t *bb_cnt = nullptr, *bb_off = nullptr, *bb_fill = nullptr;         // [N + 1] per-row pairs
     uint32_t *bb_pk[2] = {nullptr, nullptr}, *bb_pv[2] = {nullptr, nullptr};  // [P + N] pair sort
+    // sweep-and-prune broadphase (lazily allocated)
+    bool sap_ready = false;
+    uint32_t *sap_kcount = nullptr, *sap_kfirst = nullptr;  // [2^16] boxes per key, first slot per key
+    uint32_t *sap_skey = nullptr;                             // [N] keys in key order
+    float *sap_map = nullptr;  // [4] key map (x0, scale), then the lower-bound extremes for the next map
+    float4 *sap_box = nullptr;                                                      // [2N] boxes in key order
+    uint32_t *sap_cnt = nullptr, *sap_off = nullptr, *sap_fill = nullptr;          // [N] per-row pairs
+    uint32_t *sap_part = nullptr;  // [P + N] each kept row's partners
+    uint32_t *sap_rows = nullptr;  // [N + 2] rows sorted by workgroups
+    uint32_t *sap_cont = nullptr;  // [3N + 4] continued runs: items, entries, then (box, next candidate, first item)
+    uint2 *sap_flat = nullptr;     // [P + N] pairs as the sweep finds them
+    uint32_t *sap_tail = nullptr;  // [4] sap_flat's fill
     uint32_t *bb_stage = nullptr;   // uint[BB_STAGE * N] rows staged by each counting lane
     uint32_t *bb_scnt = nullptr;    // uint[N] rows found by each counting lane
     uint2 *bb_rng = nullptr;        // uint2[N] leaf range (first, last) of each internal node
@@ -320,6 +332,7 @@ void bvh_stage(sv_ctx *c, uint32_t step, uint32_t n, bool bodies);
 // order, truncation and counters as the all-pairs kernel.  Returns false
 // (and launches nothing) when its scratch cannot be allocated.
 bool launch_broadphase_lbvh(sv_ctx *c, uint32_t n);
+bool launch_broadphase_sap(sv_ctx *c, uint32_t n);
 // The broadphase's row sort alone: rows i < n of cnt[i] partners at
 // pk[off[i]...] (any order) to sorted pairs below max_pairs; list = n + 2
 // words with list[0] = 0; window_words = 0 picks the bitmap window, else a

@@ -553,8 +553,9 @@ void launch_broadphase(sv_ctx *c, uint32_t n) {
     }
     const bool lbvh = c->broadphase_mode == 2 || (c->broadphase_mode == 0 && n >= BP_LBVH_MIN_BODIES) ||
                       n > BP_ALL_PAIRS_MAX_BODIES;
-    // same pair list either way; without LBVH scratch the all-pairs kernel runs
+    // same pair list every way; without LBVH or sweep scratch the all-pairs kernel runs
     if (lbvh && launch_broadphase_lbvh(c, n)) return;
+    if (!lbvh && c->broadphase_mode == 3 && launch_broadphase_sap(c, n)) return;
     if (n > BP_ALL_PAIRS_MAX_BODIES) {  // no LBVH scratch and too many rows for the cells
         c->launch_err = hipErrorOutOfMemory;
         return;
