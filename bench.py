@@ -62,6 +62,7 @@ PROFILE = ROOT / "profiles" / "c2_32k_container_kernels.md"
 PROFILE_TRI = ROOT / "profiles" / "c2_triangle_bvh_kernels.md"
 VALU_PEAK = 256 * 4 * 16 * 2.4e9  # non-packed FP32 lane operations per second
 STAGE_FRAMES = 5  # frames after the timed ones whose stages are all timed
+BROADPHASE_KERNEL = {"all_pairs": "k_broadphase<false>", "sweep": "k_sap<false>", "lbvh": "k_bb_pairs<false>"}
 STAGE_KERNEL = {"phys/pgs_sweep": "k_solve_seqp2<64>", "phys/aabb": "k_aabb", "phys/gravity": "k_gravity",
                 "phys/integrate": "k_integrate", "phys/broad_phase": "k_broadphase<false>",
                 "bvh/morton_codes": "k_morton", "bvh/radix_sort": "k_os_hist<4> + 4 x k_os_pass<256, 8>"}
@@ -619,6 +620,8 @@ def main() -> int:
     scene.upload(ctx)
     ctx.set_solver(args.solver, args.solver_wgs)
     ctx.set_broadphase(args.broadphase)
+    bp_method = ctx.broadphase_method(n)
+    STAGE_KERNEL["phys/broad_phase"] = BROADPHASE_KERNEL[bp_method]
     ctx.set_sweep_variant(args.sweep_variant)
     name, arch = ctx.device_info()
     log(f"device: {name} {arch}")
@@ -710,7 +713,8 @@ def main() -> int:
     # (six subtractions, three maxima, one compare)
     bp_us = stages.get("phys/broad_phase", (0.0, 0))[0]
     tests = n * (n - 1) / 2.0
-    if bp_us > 0:
+    hbm_stages["phys/broad_phase"]["method"] = bp_method
+    if bp_us > 0 and bp_method == "all_pairs":
         lane_ops = 10.0 * tests / (bp_us * 1e-6)
         hbm_stages["phys/broad_phase"]["valu"] = {
             "tests_per_launch": int(tests), "tests_per_s": round(tests / (bp_us * 1e-6), 1),

@@ -369,9 +369,13 @@ int sv_debug_color_counts(sv_ctx *ctx, uint32_t *out, uint32_t cap);
  * body boxes of this substep.  SV_BROADPHASE_SWEEP: sweep and prune along x
  * (boxes ordered by their lower x bound, each tested against the run of
  * boxes that start before its upper x bound).  SV_BROADPHASE_AUTO picks by
- * body count. */
+ * body count: all-pairs below 2^14 bodies, sweep below 3 * 2^15, the LBVH
+ * from there. */
 enum { SV_BROADPHASE_AUTO = 0, SV_BROADPHASE_ALL_PAIRS = 1, SV_BROADPHASE_LBVH = 2, SV_BROADPHASE_SWEEP = 3 };
 int sv_set_broadphase(sv_ctx *ctx, int method);
+/* The method a broadphase over n bodies takes under the context's setting
+ * (SV_BROADPHASE_ALL_PAIRS, _LBVH or _SWEEP), or a negative error code. */
+int sv_broadphase_method(sv_ctx *ctx, uint32_t n);
 
 /* Warm start (library extension beside the reference path; the reference
  * clears every impulse each substep, physics.comp:494-497, and lists warm

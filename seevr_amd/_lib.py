@@ -77,7 +77,7 @@ EXPORTS = [
     "sv_default_caps", "sv_default_params", "sv_create", "sv_destroy", "sv_get_params", "sv_set_params",
     "sv_buffer_ptr", "sv_bind_buffer", "sv_upload", "sv_download", "sv_physics_step", "sv_read_counters",
     "sv_physics_dispatch", "sv_bvh_build", "sv_bvh_build_bodies", "sv_bvh_build_bodies_noalloc", "sv_bvh_dispatch", "sv_debug_atan2", "sv_debug_radix_sort", "sv_debug_scan", "sv_debug_sort_rows",
-    "sv_debug_copy_bandwidth", "sv_debug_broadphase_scratch", "sv_debug_seq_streams", "sv_sweep_depth", "sv_debug_handoff", "sv_debug_xcc_ids", "sv_debug_set_sweep_variant", "sv_last_error", "sv_device_info", "sv_stage_times", "sv_enable_stage_timing", "sv_stage_timing_only", "sv_set_solver", "sv_solver_residual", "sv_debug_color_counts", "sv_set_broadphase", "sv_set_warm_start", "sv_scene_create",
+    "sv_debug_copy_bandwidth", "sv_debug_broadphase_scratch", "sv_debug_seq_streams", "sv_sweep_depth", "sv_debug_handoff", "sv_debug_xcc_ids", "sv_debug_set_sweep_variant", "sv_last_error", "sv_device_info", "sv_stage_times", "sv_enable_stage_timing", "sv_stage_timing_only", "sv_set_solver", "sv_solver_residual", "sv_debug_color_counts", "sv_set_broadphase", "sv_broadphase_method", "sv_set_warm_start", "sv_scene_create",
     "sv_scene_destroy", "sv_scene_add_material", "sv_scene_add_box", "sv_scene_add_sphere",
     "sv_scene_set_body_state", "sv_scene_get_counts", "sv_scene_array", "sv_scene_upload",
     "sv_scene_build_workload", "sv_scene_build_batch", "sv_scene_serialize", "sv_scene_deserialize",
@@ -145,6 +145,7 @@ def load(path: Path | str | None = None) -> C.CDLL:
         "sv_stage_timing_only": (C.c_int, [vp, C.c_char_p]),
         "sv_set_solver": (C.c_int, [vp, C.c_int, u32]),
         "sv_set_broadphase": (C.c_int, [vp, C.c_int]),
+        "sv_broadphase_method": (C.c_int, [vp, u32]),
         "sv_set_warm_start": (C.c_int, [vp, f32]),
         "sv_solver_residual": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(u32)]),
         "sv_debug_color_counts": (C.c_int, [vp, C.POINTER(u32), u32]),

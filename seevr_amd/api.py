@@ -193,6 +193,12 @@ class Context:
         """Broadphase method: "auto", "all_pairs", "lbvh" or "sweep" (identical pair lists)."""
         check(self.lib.sv_set_broadphase(self.h, self.BROADPHASES[method]))
 
+    def broadphase_method(self, n: int) -> str:
+        """The method a broadphase over n bodies takes under the current setting."""
+        rc = self.lib.sv_broadphase_method(self.h, n)
+        check(min(rc, 0))
+        return {v: k for k, v in self.BROADPHASES.items()}[rc]
+
     SWEEP_VARIANTS = {"seq": 0, "coherent": 1, "fenced": 2, "pipelined": 3, "seq_basic": 4, "paired": 5}
 
     def set_sweep_variant(self, variant: str) -> None:

@@ -583,6 +583,11 @@ int sv_set_broadphase(sv_ctx *c, int method) {
     return SV_OK;
 }
 
+int sv_broadphase_method(sv_ctx *c, uint32_t n) {
+    if (!c) return fail(SV_E_INVALID, "sv_broadphase_method: null context");
+    return sv::broadphase_method(c, n);
+}
+
 int sv_stage_times(sv_ctx *c, int reset, const char **names, double *avg_us, uint64_t *calls, int cap) {
     if (!c) return fail(SV_E_INVALID, "sv_stage_times: null context");
     int n = (int)c->stage_names.size();

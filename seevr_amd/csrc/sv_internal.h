@@ -339,8 +339,17 @@ bool launch_broadphase_sap(sv_ctx *c, uint32_t n);
 // multiple of 256 up to 2^15.
 void debug_sort_rows(sv_ctx *c, uint32_t n, const uint32_t *cnt, const uint32_t *off, const uint32_t *pk,
                      uint32_t *pairs, uint32_t max_pairs, uint32_t *list, uint32_t window_words);
-// Body count from which the automatic broadphase choice is the LBVH.
-constexpr uint32_t BP_LBVH_MIN_BODIES = 1u << 17;
+// The automatic broadphase choice by body count: all-pairs below
+// BP_SWEEP_MIN_BODIES, sweep and prune below BP_LBVH_MIN_BODIES, the LBVH
+// from there (measured on the walled pile, broadphase per substep: 16k bodies
+// all-pairs 129 us, sweep 133; 64k all-pairs 799, sweep 299, LBVH 624; 128k
+// sweep 1636, LBVH 735 - one axis's runs grow with the pile's height).
+constexpr uint32_t BP_SWEEP_MIN_BODIES = 1u << 14;
+constexpr uint32_t BP_LBVH_MIN_BODIES = 3u << 15;
+// The method (SV_BROADPHASE_ALL_PAIRS, _SWEEP or _LBVH) a launch over n
+// bodies takes under the context's setting (without the scratch a method
+// needs, the launch falls back to all-pairs).
+int broadphase_method(const sv_ctx *c, uint32_t n);
 
 // ---- sv_capi.cpp ----
 // BVH.build's stage sequence (bvh.zig:63-172) over n triangles or bodies,

@@ -544,6 +544,13 @@ void launch_aabb(sv_ctx *c, uint32_t n) {
                        c->s.rot, c->s.bpbox);
 }
 
+int broadphase_method(const sv_ctx *c, uint32_t n) {
+    if (n > BP_ALL_PAIRS_MAX_BODIES || c->broadphase_mode == SV_BROADPHASE_LBVH) return SV_BROADPHASE_LBVH;
+    if (c->broadphase_mode != SV_BROADPHASE_AUTO) return (int)c->broadphase_mode;
+    if (n >= BP_LBVH_MIN_BODIES) return SV_BROADPHASE_LBVH;
+    return n >= BP_SWEEP_MIN_BODIES ? SV_BROADPHASE_SWEEP : SV_BROADPHASE_ALL_PAIRS;
+}
+
 void launch_broadphase(sv_ctx *c, uint32_t n) {
     uint32_t *counters = B<uint32_t>(c, SV_ATOMIC_COUNTERS);
     if (n == 0) {
@@ -551,11 +558,10 @@ void launch_broadphase(sv_ctx *c, uint32_t n) {
         hipLaunchKernelGGL(k_post_pairs, dim3(1), dim3(1), 0, c->stream, counters, c->s.misc, c->caps.max_pairs);
         return;
     }
-    const bool lbvh = c->broadphase_mode == 2 || (c->broadphase_mode == 0 && n >= BP_LBVH_MIN_BODIES) ||
-                      n > BP_ALL_PAIRS_MAX_BODIES;
     // same pair list every way; without LBVH or sweep scratch the all-pairs kernel runs
-    if (lbvh && launch_broadphase_lbvh(c, n)) return;
-    if (!lbvh && c->broadphase_mode == 3 && launch_broadphase_sap(c, n)) return;
+    const int method = broadphase_method(c, n);
+    if (method == SV_BROADPHASE_LBVH && launch_broadphase_lbvh(c, n)) return;
+    if (method == SV_BROADPHASE_SWEEP && launch_broadphase_sap(c, n)) return;
     if (n > BP_ALL_PAIRS_MAX_BODIES) {  // no LBVH scratch and too many rows for the cells
         c->launch_err = hipErrorOutOfMemory;
         return;
