@@ -54,7 +54,8 @@ enum {
     SV_STATUS_PAIR_OVERFLOW = 1u << 0,    /* pairs found > max_pairs        */
     SV_STATUS_CONTACT_OVERFLOW = 1u << 1, /* contacts found > max_contacts  */
     SV_STATUS_SOLVER_TIMEOUT = 1u << 2,   /* ordered PGS wait expired       */
-    SV_STATUS_SORT_TIMEOUT = 1u << 3      /* sort or scan look-back expired */
+    SV_STATUS_SORT_TIMEOUT = 1u << 3,     /* radix-sort look-back expired   */
+    SV_STATUS_SCAN_TIMEOUT = 1u << 4      /* scan look-back expired         */
 };
 
 /* ---- buffer ids --------------------------------------------------------

@@ -453,7 +453,7 @@ int batch_finish(sv_batch *b) {
     e = hipMemcpy(&ab, b->abort_flag, sizeof(ab), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return batch_dev_fail(e, "batched step status");
     if (ab) {
-        (void)hipMemset(b->abort_flag, 0, sizeof(uint32_t));
+        (void)zero_device(b->abort_flag, sizeof(uint32_t));
         return set_error(SV_E_TIMEOUT, "batched step: a scene's contact sweep made no progress for 2 s");
     }
     return SV_OK;
@@ -547,8 +547,8 @@ int sv_batch_create(const sv_batch_caps *caps, const sv_params *params, sv_batch
     if (e == hipSuccess) e = hipMalloc(&b->sc_grav, 3 * S * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&b->queue, sv::MAX_SUBSTEPS * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&b->abort_flag, sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(b->abort_flag, 0, sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(b->sc_cnt, 0, 3 * S * sizeof(uint32_t));
+    if (e == hipSuccess) e = sv::zero_device(b->abort_flag, sizeof(uint32_t));
+    if (e == hipSuccess) e = sv::zero_device(b->sc_cnt, 3 * S * sizeof(uint32_t));
     if (e != hipSuccess) {
         free_batch(b);
         return set_error(SV_E_NOMEM, "sv_batch_create: device allocation failed");
@@ -648,7 +648,7 @@ int sv_batch_set_phase_timing(sv_batch *b, int on) {
     if (on && !b->phase_ticks) {
         hipError_t e = hipMalloc(&b->phase_ticks, (sv::BATCH_PHASES + 1) * sizeof(unsigned long long));
         if (e == hipSuccess)
-            e = hipMemset(b->phase_ticks, 0, (sv::BATCH_PHASES + 1) * sizeof(unsigned long long));
+            e = sv::zero_device(b->phase_ticks, (sv::BATCH_PHASES + 1) * sizeof(unsigned long long));
         if (e != hipSuccess) {
             if (b->phase_ticks) (void)hipFree(b->phase_ticks);
             b->phase_ticks = nullptr;
@@ -666,7 +666,7 @@ int sv_batch_phase_times(sv_batch *b, double *us, uint64_t *scenes) {
     if (b->phase_ticks) {
         hipError_t e = hipStreamSynchronize(b->ctx->stream);
         if (e == hipSuccess) e = hipMemcpy(t, b->phase_ticks, sizeof(t), hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemset(b->phase_ticks, 0, sizeof(t));
+        if (e == hipSuccess) e = sv::zero_device(b->phase_ticks, sizeof(t));
         if (e != hipSuccess) return dev_fail(e, "sv_batch_phase_times");
     }
     for (int k = 0; k < sv::BATCH_PHASES; k++) us[k] = (double)t[k] * 0.01;  // 100 MHz ticks

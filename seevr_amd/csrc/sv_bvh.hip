@@ -1451,7 +1451,7 @@ bool sap_reserve(sv_ctx *c) {
     const size_t N = c->caps.max_bodies, P = c->caps.max_pairs;
     hipError_t e = hipSuccess;
     e = lazy_alloc(c, s.sap_kcount, SAP_KEYS);
-    if (e == hipSuccess) e = hipMemset(s.sap_kcount, 0, SAP_KEYS * sizeof(uint32_t));  // zero between launches
+    if (e == hipSuccess) e = zero_device(s.sap_kcount, SAP_KEYS * sizeof(uint32_t));  // zero between launches
     if (e == hipSuccess) e = lazy_alloc(c, s.sap_kfirst, SAP_KEYS);
     if (e == hipSuccess) e = lazy_alloc(c, s.sap_map, 4);
     if (e == hipSuccess) {  // no map yet (x0 = scale = 0), extremes empty

@@ -147,8 +147,10 @@ int check_status(sv_ctx *c) {
         return fail(SV_E_TIMEOUT, "ordered PGS made no progress for 2 s and was aborted");
     if (status & SV_STATUS_SORT_TIMEOUT) {
         (void)sv::sort_take_abort(c);
-        return fail(SV_E_TIMEOUT, "radix sort or scan look-back made no progress for 2 s and was aborted");
+        return fail(SV_E_TIMEOUT, "radix sort look-back made no progress for 2 s and was aborted");
     }
+    if (status & SV_STATUS_SCAN_TIMEOUT)
+        return fail(SV_E_TIMEOUT, "scan look-back made no progress for 2 s and was aborted");
     return SV_OK;
 }
 
@@ -288,6 +290,12 @@ int sv_create(const sv_caps *caps, const sv_params *params, sv_ctx **out) {
     if (!s.scan_part || !s.sort_hist) {
         free_ctx(c);
         return fail(SV_E_NOMEM, "sv_create: scratch allocation failed");
+    }
+    // the zeroing above ran on the null stream; the context's stream does not wait for it
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        free_ctx(c);
+        return fail(SV_E_DEVICE, "sv_create: %s", hipGetErrorString(e));
     }
     *out = c;
     return SV_OK;
@@ -618,7 +626,7 @@ int sv_debug_handoff(sv_ctx *c, uint32_t rounds, int same_xcd, double *us_one_wa
     uint32_t *d = nullptr;
     HIP_OK(hipMalloc(&d, 64 * sizeof(uint32_t)));
     uint64_t *out = reinterpret_cast<uint64_t *>(d + 16);
-    hipError_t e = hipMemset(d, 0, 64 * sizeof(uint32_t));
+    hipError_t e = sv::zero_device(d, 64 * sizeof(uint32_t));  // before the launch on the context stream
     uint64_t h[5] = {};
     if (e == hipSuccess) {
         sv::launch_handoff(c, d, rounds, same_xcd ? 8u : 1u, out);

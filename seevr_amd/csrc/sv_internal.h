@@ -245,6 +245,17 @@ enum : int {
     MISC_BOUNDS_DONE = 9,  // workgroups of the broadphase centre bounds that have arrived (zero between launches)
 };
 
+// Zero device memory and wait for it: the context streams are non-blocking,
+// so a hipMemset on the null stream is not ordered before their kernels (a
+// fresh allocation often reuses freed memory; its old contents must not be
+// seen).  For allocation-time and host-side resets only (it synchronizes the
+// device).
+inline hipError_t zero_device(void *p, size_t bytes) {
+    hipError_t e = hipMemset(p, 0, bytes);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e;
+}
+
 // ---- sv_prims.hip ----
 // Exclusive scan of n = min(*n_dev, n_max) (or n_max when n_dev == null)
 // uint32 values, one launch; writes the total to *total_dev when non-null,

@@ -86,7 +86,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total)
 // the first ceil(n / SCAN_TILE) workgroups take tile indices (one each) and
 // clear; the others leave at once.
 // A wait is bounded (2 s, a broken invariant): it then sets
-// SV_STATUS_SORT_TIMEOUT in *timeout_status and takes the prefix so far.
+// SV_STATUS_SCAN_TIMEOUT in *timeout_status and takes the prefix so far.
 //
 // Optional output: the total to *total_dev; with post_np, also k_post_pairs'
 // work: *post_np = min(total, post_cap) and SV_STATUS_PAIR_OVERFLOW in
@@ -124,6 +124,10 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan(const uint32_t *__restric
     if (tid == 0) s_word = atomicAdd(ctl, 1u);
     __syncthreads();
     const uint32_t half = s_word >> 31, tile = s_word & 0x7FFFFFFFu;
+    if (tile >= nt || tile >= tiles_cap) {  // a broken invariant (the counter was not at zero): fail, in bounds
+        if (tid == 0 && timeout_status) atomicOr(timeout_status, (uint32_t)SV_STATUS_SCAN_TIMEOUT);
+        return;
+    }
     uint64_t *stat = reinterpret_cast<uint64_t *>(ctl + 2) + (size_t)half * tiles_cap;
     uint64_t *other = reinterpret_cast<uint64_t *>(ctl + 2) + (size_t)(half ^ 1u) * tiles_cap;
     for (uint32_t i = tile * SCAN_THREADS + tid; i < tiles_cap; i += nt * SCAN_THREADS) other[i] = 0ull;
@@ -171,7 +175,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan(const uint32_t *__restric
             // a tile in the span has not published: its workgroup is running
             if (start == 0) start = wall_clock64();
             if (wall_clock64() - start > SCAN_STALL_TICKS) {
-                if (lane == 0 && timeout_status) atomicOr(timeout_status, (uint32_t)SV_STATUS_SORT_TIMEOUT);
+                if (lane == 0 && timeout_status) atomicOr(timeout_status, (uint32_t)SV_STATUS_SCAN_TIMEOUT);
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
@@ -490,12 +494,16 @@ __global__ void __launch_bounds__(THREADS) k_os_pass(const uint32_t *__restrict_
 // never inside graph capture because prims_reserve sizes everything up front).
 void ensure_u32(uint32_t *&p, uint32_t &cap, uint32_t need) {
     if (need <= cap) return;
-    if (p) (void)hipFree(p);
+    if (p) {
+        (void)hipDeviceSynchronize();  // launches queued on a context stream may still use it
+        (void)hipFree(p);
+    }
     p = nullptr;
     cap = 0;
-    // zeroed: the sort's control words must start at zero
+    // zeroed before any launch uses it: the sort's and the scan's control
+    // words must start at zero
     if (hipMalloc(&p, (size_t)need * sizeof(uint32_t)) == hipSuccess) {
-        if (hipMemset(p, 0, (size_t)need * sizeof(uint32_t)) == hipSuccess) {
+        if (zero_device(p, (size_t)need * sizeof(uint32_t)) == hipSuccess) {
             cap = need;
         } else {
             (void)hipFree(p);
