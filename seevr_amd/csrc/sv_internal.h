@@ -113,7 +113,8 @@ constexpr uint32_t SORT_MAX_KEYS = (1u << 30) - 1;  // counts share a word with 
 constexpr size_t SORT_ACC_OFF = 0, SORT_GOFF_OFF = SORT_MAX_PASSES * 256, SORT_CTL_OFF = 2 * SORT_MAX_PASSES * 256;
 constexpr size_t SORT_CTL_WORDS = 64;
 constexpr size_t SORT_STATUS_OFF = SORT_CTL_OFF + SORT_CTL_WORDS;
-enum : uint32_t { SORT_CTL_DONE = 0, SORT_CTL_TILE = 1, SORT_CTL_ABORT = 8 };
+enum : uint32_t { SORT_CTL_DONE = 0, SORT_CTL_TILE = 1, SORT_CTL_ABORT = 8, SORT_CTL_BAR = 9, SORT_CTL_EXIT = 10 };
+constexpr uint32_t SORT_ONE_MAX = 1u << 19;  // largest sort done in one launch (256 resident tiles)
 // Status words are sized for the most tiles any sort of at most n keys
 // uses (small tiles up to SORT_SMALL_MAX keys, large above), so that
 // scratch sized for a capacity serves every smaller sort.

@@ -263,6 +263,86 @@ __device__ __forceinline__ uint32_t digit_excl_scan(uint32_t v) {
     return base + inc - v;
 }
 
+// Stable ranks of a wave's ROUNDS x 64 keys (element base + r * 64 + lane)
+// by their digit: the ballots of every round first (no dependence between
+// rounds), then the wave's running digit counters `wc` (zeroed by the
+// caller).  pos[r] = rank of the key among the wave's keys of its digit,
+// all ones past the n keys; wc[d] ends as the wave's count of digit d.
+template <int ROUNDS>
+__device__ __forceinline__ void os_rank(const uint32_t (&key)[ROUNDS], uint32_t (&pos)[ROUNDS], uint32_t base,
+                                        uint32_t n, int shift, uint32_t dmask, uint32_t *wc) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const bool valid = base + r * 64 + lane < n;
+        const uint32_t d = (key[r] >> shift) & dmask;
+        // lanes whose digit bits all equal this lane's: bal XNOR (bit ? ~0 : 0)
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            const uint32_t b = (d >> bit) & 1u;
+            const uint64_t bal = __ballot(b);
+            const uint64_t m = (uint64_t)(int64_t)(-(int32_t)b);
+            peers &= ~(bal ^ m);
+        }
+        // rank among the round's lanes with this digit | their number << 8;
+        // all ones for a lane past the keys
+        pos[r] = valid ? (uint32_t)__popcll(peers & lt) | ((uint32_t)__popcll(peers) << 8) : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t d = (key[r] >> shift) & dmask;
+        const bool valid = pos[r] != 0xFFFFFFFFu;
+        const uint32_t rank = pos[r] & 255u, num = pos[r] >> 8;
+        const uint32_t cur = wc[d];
+        if (valid) pos[r] = cur + rank;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) wc[d] = cur + num;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Look-back for one digit: the sum of the counts of tiles 0..tile-1 from
+// their status words col[t * 256] (an inclusive prefix ends the walk).
+__device__ __forceinline__ uint32_t os_lookback(const uint32_t *col, uint32_t tile, uint32_t *ctl, uint32_t *counters) {
+    uint32_t excl = 0;
+    int32_t t = (int32_t)tile - 1;  // nearest predecessor not yet added
+    uint64_t start = 0;
+    while (t >= 0) {
+        uint32_t v[LB_WINDOW];
+#pragma unroll
+        for (int j = 0; j < LB_WINDOW; j++) v[j] = (t - j >= 0) ? ld_agent(col + (size_t)(t - j) * 256) : OS_FLAG_P;
+        int j = 0;
+        bool done = false;
+#pragma unroll
+        for (int q = 0; q < LB_WINDOW; q++) {
+            if (j == q) {
+                const uint32_t x = v[q];
+                if (x & OS_FLAG_P) {
+                    excl += x & OS_VALUE;
+                    done = true;
+                } else if (x & OS_FLAG_A) {
+                    excl += x & OS_VALUE;
+                    j = q + 1;
+                }
+            }
+        }
+        if (done) break;
+        t -= j;
+        if (j == LB_WINDOW) continue;
+        // tile t has not published: the workgroup holding it is running
+        if (start == 0) start = wall_clock64();
+        if (wall_clock64() - start > OS_STALL_TICKS) {
+            st_agent(ctl + SORT_CTL_ABORT, 1u);
+            if (counters) atomicOr(counters + 2, (uint32_t)SV_STATUS_SORT_TIMEOUT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return excl;
+}
+
 template <int PASSES>
 __global__ void __launch_bounds__(HIST_THREADS) k_os_hist(const uint32_t *__restrict__ keys, const uint32_t *n_dev,
                                                           uint32_t n_max, uint32_t top_mask, uint32_t tile_keys,
@@ -365,35 +445,7 @@ __global__ void __launch_bounds__(THREADS) k_os_pass(const uint32_t *__restrict_
         key[r] = valid ? kin[i] : 0u;
         val[r] = valid ? vin[i] : 0u;
     }
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-#pragma unroll
-    for (int r = 0; r < ROUNDS; r++) {
-        const bool valid = base + r * 64 + lane < n;
-        const uint32_t d = (key[r] >> shift) & dmask;
-        // lanes whose digit bits all equal this lane's: bal XNOR (bit ? ~0 : 0)
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int bit = 0; bit < 8; bit++) {
-            const uint32_t b = (d >> bit) & 1u;
-            const uint64_t bal = __ballot(b);
-            const uint64_t m = (uint64_t)(int64_t)(-(int32_t)b);
-            peers &= ~(bal ^ m);
-        }
-        // rank among the round's lanes with this digit | their number << 8;
-        // all ones for a lane past the keys
-        pos[r] = valid ? (uint32_t)__popcll(peers & lt) | ((uint32_t)__popcll(peers) << 8) : 0xFFFFFFFFu;
-    }
-#pragma unroll
-    for (int r = 0; r < ROUNDS; r++) {
-        const uint32_t d = (key[r] >> shift) & dmask;
-        const bool valid = pos[r] != 0xFFFFFFFFu;
-        const uint32_t rank = pos[r] & 255u, num = pos[r] >> 8;
-        const uint32_t cur = wcnt[w][d];
-        if (valid) pos[r] = cur + rank;
-        __builtin_amdgcn_wave_barrier();
-        if (valid && rank == 0) wcnt[w][d] = cur + num;
-        __builtin_amdgcn_wave_barrier();
-    }
+    os_rank<ROUNDS>(key, pos, base, n, shift, dmask, wcnt[w]);
     __syncthreads();
     // digit d = tid (< 256): the waves' offsets and the tile's count
     const uint32_t d = (uint32_t)tid;
@@ -411,42 +463,7 @@ __global__ void __launch_bounds__(THREADS) k_os_pass(const uint32_t *__restrict_
     const uint32_t lo = digit_excl_scan<THREADS>(cnt);
     if (tid < 256) {
         s_lo[d] = lo;
-        // look back over the predecessors' words of digit d
-        uint32_t excl = 0;
-        int32_t t = (int32_t)tile - 1;  // nearest predecessor not yet added
-        uint64_t start = 0;
-        const uint32_t *col = status + (size_t)pass * nb * 256 + d;
-        while (t >= 0) {
-            uint32_t v[LB_WINDOW];
-#pragma unroll
-            for (int j = 0; j < LB_WINDOW; j++) v[j] = (t - j >= 0) ? ld_agent(col + (size_t)(t - j) * 256) : OS_FLAG_P;
-            int j = 0;
-            bool done = false;
-#pragma unroll
-            for (int q = 0; q < LB_WINDOW; q++) {
-                if (j == q) {
-                    const uint32_t x = v[q];
-                    if (x & OS_FLAG_P) {
-                        excl += x & OS_VALUE;
-                        done = true;
-                    } else if (x & OS_FLAG_A) {
-                        excl += x & OS_VALUE;
-                        j = q + 1;
-                    }
-                }
-            }
-            if (done) break;
-            t -= j;
-            if (j == LB_WINDOW) continue;
-            // tile t has not published: the workgroup holding it is running
-            if (start == 0) start = wall_clock64();
-            if (wall_clock64() - start > OS_STALL_TICKS) {
-                st_agent(ctl + SORT_CTL_ABORT, 1u);
-                if (counters) atomicOr(counters + 2, (uint32_t)SV_STATUS_SORT_TIMEOUT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
+        const uint32_t excl = os_lookback(status + (size_t)pass * nb * 256 + d, tile, ctl, counters);
         if (tile > 0) st_agent(mine, OS_FLAG_P | (excl + cnt));
         s_gb[d] = goff[pass * 256 + d] + excl;
     }
@@ -486,6 +503,217 @@ __global__ void __launch_bounds__(THREADS) k_os_pass(const uint32_t *__restrict_
 #pragma unroll
     for (int k = 0; k < ROUNDS; k++)
         if (dst[k] != 0xFFFFFFFFu) vout[dst[k]] = s_buf[k * THREADS + tid];
+}
+
+
+// Grid barrier of k_os_one: every thread's stores have completed (its own
+// wait, then the workgroup barrier) before thread 0 adds the workgroup's
+// arrival, and the workgroup goes on once `target` arrivals are counted
+// (bounded like the look-back: 2 s, then the abort word and status bit).
+__device__ __forceinline__ void os_grid_sync(uint32_t *ctl, uint32_t target, uint32_t *counters) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(ctl + SORT_CTL_BAR, 1u);
+        uint64_t start = 0;
+        while (ld_agent(ctl + SORT_CTL_BAR) < target) {
+            if (start == 0) start = wall_clock64();
+            if (wall_clock64() - start > OS_STALL_TICKS) {
+                st_agent(ctl + SORT_CTL_ABORT, 1u);
+                if (counters) atomicOr(counters + 2, (uint32_t)SV_STATUS_SORT_TIMEOUT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+}
+
+// k_os_one: the whole sort in one launch, for at most SORT_ONE_MAX keys,
+// where the launches of the multi-launch form (a histogram launch, then one
+// per pass, each a few microseconds from dispatch to its end) cost more than
+// the work.  Workgroup t sorts tile t (2048 keys over eight waves of four
+// rounds: a shorter ranking and write-out chain per wave than four waves of
+// eight), and the at most 256 working workgroups are resident together (two
+// fit on a CU): they count their tile's digits of every pass into the
+// accumulators, meet at a grid barrier, scan the totals into each pass's
+// digit offsets, and run the passes (ranking, look-back, digit-ordered
+// write-out as in k_os_pass) with a grid barrier after each but the last.  What one pass writes for the
+// next (keys, values) and the look-back words go to the point of coherence
+// and are read from there (device-scope relaxed stores and loads), so no
+// cache write-back or invalidation is needed across the XCDs' L2 caches.
+// Each workgroup zeroes its own look-back words before the first barrier;
+// the workgroup that leaves last zeroes the accumulators and the barrier
+// counters for the next sort.  Output in (k1, v1) for an odd pass count,
+// else (k0, v0), as with the multi-launch form.
+constexpr int ONE_THREADS = 512, ONE_ROUNDS = 4;  // 2048-key tiles over eight waves
+template <int PASSES>
+__global__ void __launch_bounds__(ONE_THREADS) k_os_one(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1,
+                                                        const uint32_t *n_dev, uint32_t n_max, uint32_t top_mask,
+                                                        uint32_t *acc, uint32_t *ctl, uint32_t *status,
+                                                        uint32_t *counters) {
+    constexpr int THREADS = ONE_THREADS, ROUNDS = ONE_ROUNDS, WAVES = THREADS / 64, TILE = THREADS * ROUNDS;
+    static_assert(TILE == SORT_TILE_SMALL, "tile of the look-back scratch");
+    __shared__ uint32_t s_buf[TILE];
+    __shared__ uint32_t wcnt[WAVES][256];
+    __shared__ uint32_t s_goff[PASSES][256];
+    __shared__ uint32_t s_lo[256], s_gb[256];
+    __shared__ uint32_t s_last;
+    const uint32_t n = read_n(n_dev, n_max), nb = (n + TILE - 1) / TILE;
+    const uint32_t tile = blockIdx.x;
+    if (tile >= nb) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const bool digit = tid < 256;  // threads 0..255 each hold digit tid
+    const uint32_t base = tile * TILE + w * (ROUNDS * 64);
+    if (digit) {
+        for (int p = 0; p < PASSES; p++) {
+            st_agent(status + ((size_t)p * nb + tile) * 256 + tid, 0u);
+            s_goff[p][tid] = 0u;
+        }
+    }
+    uint32_t key[ROUNDS], val[ROUNDS], pos[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t i = base + r * 64 + lane;
+        const bool valid = i < n;
+        key[r] = valid ? k0[i] : 0u;
+        val[r] = valid ? v0[i] : 0u;
+    }
+    __syncthreads();
+    // the tile's digit counts of every pass (one addition per run of equal
+    // digits in a round; the lanes holding keys are a prefix of the wave)
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const bool valid = base + r * 64 + lane < n;
+        const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
+#pragma unroll
+        for (int p = 0; p < PASSES; p++) {
+            const uint32_t d = (key[r] >> (8 * p)) & (p == PASSES - 1 ? top_mask : 255u);
+            const uint32_t prev = __shfl_up(d, 1, 64);
+            const bool head = valid && (lane == 0 || d != prev);
+            const uint64_t heads = __ballot(head);
+            if (head) {
+                const uint64_t above = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+                const uint32_t next = above ? (uint32_t)__builtin_ctzll(above) : nvalid;
+                atomicAdd(&s_goff[p][d], next - (uint32_t)lane);
+            }
+        }
+    }
+    __syncthreads();
+    if (digit) {
+#pragma unroll
+        for (int p = 0; p < PASSES; p++) {
+            const uint32_t c = s_goff[p][tid];
+            if (c) atomicAdd(acc + p * 256 + tid, c);
+        }
+    }
+    os_grid_sync(ctl, nb, counters);
+    uint32_t tot[PASSES];
+#pragma unroll
+    for (int p = 0; p < PASSES; p++) tot[p] = digit ? ld_agent(acc + p * 256 + tid) : 0u;  // one round trip
+#pragma unroll
+    for (int p = 0; p < PASSES; p++) {
+        const uint32_t ex = digit_excl_scan<THREADS>(tot[p]);
+        if (digit) s_goff[p][tid] = ex;
+    }
+
+#pragma unroll 1
+    for (int p = 0; p < PASSES; p++) {
+        const bool last = p == PASSES - 1;
+        const uint32_t *kin = (p & 1) ? k1 : k0, *vin = (p & 1) ? v1 : v0;
+        uint32_t *kout = (p & 1) ? k0 : k1, *vout = (p & 1) ? v0 : v1;
+        const int shift = 8 * p;
+        const uint32_t dmask = last ? top_mask : 255u;
+        if (p > 0) {
+#pragma unroll
+            for (int r = 0; r < ROUNDS; r++) {
+                const uint32_t i = base + r * 64 + lane;
+                const bool valid = i < n;
+                key[r] = valid ? ld_agent(kin + i) : 0u;
+                val[r] = valid ? ld_agent(vin + i) : 0u;
+            }
+        }
+        for (int i = tid; i < WAVES * 256; i += THREADS) (&wcnt[0][0])[i] = 0u;
+        __syncthreads();
+        os_rank<ROUNDS>(key, pos, base, n, shift, dmask, wcnt[w]);
+        __syncthreads();
+        // digit d = tid: the waves' offsets, the tile's count, its base
+        const uint32_t d = (uint32_t)tid;
+        uint32_t cnt = 0;
+        uint32_t *mine = status + ((size_t)p * nb + tile) * 256 + d;
+        if (digit) {
+#pragma unroll
+            for (int q = 0; q < WAVES; q++) {
+                const uint32_t t = wcnt[q][d];
+                wcnt[q][d] = cnt;
+                cnt += t;
+            }
+            st_agent(mine, (tile == 0 ? OS_FLAG_P : OS_FLAG_A) | cnt);
+        }
+        const uint32_t lo = digit_excl_scan<THREADS>(cnt);
+        if (digit) {
+            s_lo[d] = lo;
+            const uint32_t excl = os_lookback(status + (size_t)p * nb * 256 + d, tile, ctl, counters);
+            if (tile > 0) st_agent(mine, OS_FLAG_P | (excl + cnt));
+            s_gb[d] = s_goff[p][d] + excl;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < ROUNDS; r++) {
+            if (pos[r] != 0xFFFFFFFFu) {
+                const uint32_t dd = (key[r] >> shift) & dmask;
+                pos[r] += s_lo[dd] + wcnt[w][dd];
+            }
+        }
+        // keys, then values, staged digit-sorted in LDS and written out in
+        // runs; device-scope stores for the next pass, plain ones at the end
+        const uint32_t left = n - tile * TILE;
+        const uint32_t m = left < (uint32_t)TILE ? left : (uint32_t)TILE;
+#pragma unroll
+        for (int r = 0; r < ROUNDS; r++)
+            if (pos[r] != 0xFFFFFFFFu) s_buf[pos[r]] = key[r];
+        __syncthreads();
+        uint32_t dst[ROUNDS];
+#pragma unroll
+        for (int k = 0; k < ROUNDS; k++) {
+            const uint32_t i = k * THREADS + tid;
+            dst[k] = 0xFFFFFFFFu;
+            if (i < m) {
+                const uint32_t kk = s_buf[i];
+                const uint32_t dd = (kk >> shift) & dmask;
+                dst[k] = s_gb[dd] + (i - s_lo[dd]);
+                if (last) kout[dst[k]] = kk;
+                else st_agent(kout + dst[k], kk);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < ROUNDS; r++)
+            if (pos[r] != 0xFFFFFFFFu) s_buf[pos[r]] = val[r];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ROUNDS; k++) {
+            if (dst[k] != 0xFFFFFFFFu) {
+                if (last) vout[dst[k]] = s_buf[k * THREADS + tid];
+                else st_agent(vout + dst[k], s_buf[k * THREADS + tid]);
+            }
+        }
+        if (!last) os_grid_sync(ctl, (uint32_t)(p + 2) * nb, counters);
+    }
+    // every workgroup has passed its last barrier and read the accumulators
+    // once the exit count reaches nb: the last one leaves them at zero
+    if (tid == 0) s_last = atomicAdd(ctl + SORT_CTL_EXIT, 1u) == nb - 1;
+    __syncthreads();
+    if (s_last) {
+        if (digit) {
+#pragma unroll
+            for (int p = 0; p < PASSES; p++) st_agent(acc + p * 256 + tid, 0u);
+        }
+        if (tid == 0) {
+            st_agent(ctl + SORT_CTL_BAR, 0u);
+            st_agent(ctl + SORT_CTL_EXIT, 0u);
+        }
+    }
 }
 
 }  // namespace
@@ -559,12 +787,18 @@ int radix_sort(sv_ctx *c, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1
              *ctl = c->s.sort_hist + SORT_CTL_OFF, *status = c->s.sort_hist + SORT_STATUS_OFF;
     uint32_t *counters = reinterpret_cast<uint32_t *>(c->buf[SV_ATOMIC_COUNTERS]);
     const int passes = (key_bits + 7) / 8;
-    const uint32_t chunks = div_up(n_max, HIST_CHUNK);
-    const uint32_t hb = chunks < HIST_BLOCKS ? chunks : HIST_BLOCKS;
-    auto hist = passes == 1 ? k_os_hist<1> : passes == 2 ? k_os_hist<2> : passes == 3 ? k_os_hist<3> : k_os_hist<4>;
     // the last pass's digit keeps only the bits below key_bits
     const uint32_t top_bits = (uint32_t)key_bits - 8u * (uint32_t)(passes - 1);
     const uint32_t top_mask = top_bits >= 8 ? 255u : (1u << top_bits) - 1u;
+    if (n_max <= SORT_ONE_MAX) {
+        auto one = passes == 1 ? k_os_one<1> : passes == 2 ? k_os_one<2> : passes == 3 ? k_os_one<3> : k_os_one<4>;
+        hipLaunchKernelGGL(one, dim3(nblocks), dim3(ONE_THREADS), 0, c->stream, k0, v0, k1, v1, n_dev, n_max, top_mask, acc,
+                           ctl, status, counters);
+        return passes & 1;
+    }
+    const uint32_t chunks = div_up(n_max, HIST_CHUNK);
+    const uint32_t hb = chunks < HIST_BLOCKS ? chunks : HIST_BLOCKS;
+    auto hist = passes == 1 ? k_os_hist<1> : passes == 2 ? k_os_hist<2> : passes == 3 ? k_os_hist<3> : k_os_hist<4>;
     hipLaunchKernelGGL(hist, dim3(hb), dim3(HIST_THREADS), 0, c->stream, k0, n_dev, n_max, top_mask, tile_keys, acc,
                        goff, ctl, status);
     uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;

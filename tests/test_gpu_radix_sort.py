@@ -4,10 +4,12 @@ sweep's body incidences), called alone through sv_debug_radix_sort and
 checked against numpy's stable argsort of the masked keys: the same
 permutation, element for element, so equal keys keep their input order.
 
-Sizes cross the tile boundaries of both tile shapes (2048 keys up to 2^18,
-8192 above), the device count is below the buffer capacity (the idle
-workgroups' path), keys carry bits above key_bits, and one context runs
-many sorts in a row (the counters each sort leaves at zero)."""
+Sizes cross the tile boundaries of both tile shapes (2048 keys up to 2^20,
+8192 above) and the bound of the one-launch form (capacity up to 2^19 keys,
+grid barriers between passes), the device count is below the buffer
+capacity (the idle workgroups' path), keys carry bits above key_bits, and
+one context runs many sorts of both forms in a row (the counters each sort
+leaves at zero)."""
 import numpy as np
 import pytest
 
@@ -32,7 +34,8 @@ def check(ctx, keys, bits, n_max=None, vals=None):
     assert np.array_equal(vo, vals[order]), f"values n={keys.size} bits={bits}"
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 5, 2047, 2048, 2049, 40000, 1 << 18, (1 << 18) + 1, 8192 * 65 + 3])
+@pytest.mark.parametrize("n", [0, 1, 2, 5, 2047, 2048, 2049, 40000, 1 << 18, (1 << 18) + 1, 1 << 19,
+                               (1 << 19) + 1, 8192 * 65 + 3])
 @pytest.mark.parametrize("bits", [32, 30, 21, 16, 8, 3])
 def test_random_keys(ctx, n, bits):
     rng = np.random.default_rng(n * 37 + bits)
@@ -60,7 +63,8 @@ def test_structured_keys(ctx, kind):
     check(ctx, keys.astype(np.uint32), 32)
 
 
-@pytest.mark.parametrize("n,n_max", [(0, 5000), (1000, 1_000_000), (300000, 4_000_000), (2049, 2_000_000)])
+@pytest.mark.parametrize("n,n_max", [(0, 5000), (1000, 1_000_000), (300000, 4_000_000), (2049, 2_000_000),
+                                     (110000, 395000), (4095, 1 << 19), (1 << 19, 1 << 19)])
 def test_device_count_below_capacity(ctx, n, n_max):
     rng = np.random.default_rng(n + n_max)
     keys = rng.integers(0, 1 << 21, n).astype(np.uint32)
@@ -69,7 +73,7 @@ def test_device_count_below_capacity(ctx, n, n_max):
 
 def test_many_sorts_in_a_row(ctx):
     rng = np.random.default_rng(5)
-    for i, n in enumerate([3, 9000000, 100, 70000, 2048 * 128, 0, 5_000_000, 17]):
+    for i, n in enumerate([3, 9000000, 100, 70000, 2048 * 128, 0, 5_000_000, 17, 600000, 32773, 1 << 19]):
         keys = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
         check(ctx, keys, [32, 30, 16, 24][i % 4] if n else 32)
 
