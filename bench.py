@@ -65,9 +65,9 @@ STAGE_FRAMES = 5  # frames after the timed ones whose stages are all timed
 BROADPHASE_KERNEL = {"all_pairs": "k_broadphase<false>", "sweep": "k_sap<false>", "lbvh": "k_bb_pairs<false>"}
 STAGE_KERNEL = {"phys/pgs_sweep": "k_solve_seqp2<64>", "phys/aabb": "k_aabb", "phys/gravity": "k_gravity",
                 "phys/integrate": "k_integrate", "phys/broad_phase": "k_broadphase<false>",
-                "bvh/morton_codes": "k_morton", "bvh/radix_sort": "k_os_hist<4> + 4 x k_os_pass<256, 8>"}
+                "bvh/morton_codes": "k_morton", "bvh/radix_sort": "k_os_one<4>"}
 # kernels (and launches) behind a stage's time, for its profiled traffic
-STAGE_LAUNCHES = {"bvh/radix_sort": {"k_os_hist<4>": 1, "k_os_pass<256, 8>": 4}}
+STAGE_LAUNCHES = {"bvh/radix_sort": {"k_os_one<4>": 1}}
 
 
 def algorithmic_bytes(stage: str, n: int, n_hull: int, n_dyn: int, pairs: int, nc: int) -> float:

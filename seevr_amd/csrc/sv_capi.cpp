@@ -389,18 +389,27 @@ int sv_physics_step(sv_ctx *c, uint32_t nb, uint32_t nv, uint32_t nh, uint32_t s
     sv::launch_hull_ranges(c, nb);
     sv::ensure_support_vertices(c, nb);
     sv::fill_u32(c, counters + 2, 0u, 1);
+    // with stage timing off, gravity rides on the AABB launch and the
+    // integration on the sweep's write-back (same arithmetic, fewer launches);
+    // with it on, every stage keeps its own launch and timer
+    const bool fuse = !c->timing;
     for (uint32_t s = 0; s < substeps; s++) {
-        { sv::StageTimer t(c, "phys/gravity"); sv::launch_gravity(c, nb, sub_dt, gravity, true); }
-        { sv::StageTimer t(c, "phys/aabb"); sv::launch_aabb(c, nb); }
+        if (fuse) {
+            sv::launch_gravity_aabb(c, nb, sub_dt, gravity);
+        } else {
+            { sv::StageTimer t(c, "phys/gravity"); sv::launch_gravity(c, nb, sub_dt, gravity, true); }
+            { sv::StageTimer t(c, "phys/aabb"); sv::launch_aabb(c, nb); }
+        }
         { sv::StageTimer t(c, "phys/broad_phase"); sv::launch_broadphase(c, nb); }
         // without warm start the impulse clear rides on the contact compaction
         const bool warm = c->warm_factor > 0.0f;
         { sv::StageTimer t(c, "phys/narrow_phase"); sv::launch_narrowphase(c, true, !warm, nb); }
         if (warm) { sv::StageTimer t(c, "phys/clear_impulse"); sv::launch_clear_lambda(c); }
         c->rot_fresh = true;
-        { sv::StageTimer t(c, "phys/pgs_solve"); sv::launch_solve(c, nb, sub_dt); }
+        bool integrated;
+        { sv::StageTimer t(c, "phys/pgs_solve"); integrated = sv::launch_solve(c, nb, sub_dt, fuse); }
         c->rot_fresh = false;
-        { sv::StageTimer t(c, "phys/integrate"); sv::launch_integrate(c, nb, sub_dt); }
+        if (!integrated) { sv::StageTimer t(c, "phys/integrate"); sv::launch_integrate(c, nb, sub_dt); }
     }
     { sv::StageTimer t(c, "phys/render_verts"); sv::launch_update_verts(c, nv, nb); }
     int rc = sync_and_finish(c);

@@ -294,6 +294,8 @@ void ensure_support_vertices(sv_ctx *c, uint32_t n);
 // clear_counters: also clear counters[0..1] (the full step's first dispatch)
 void launch_gravity(sv_ctx *c, uint32_t n, float dt, const float g[3], bool clear_counters = false);
 void launch_aabb(sv_ctx *c, uint32_t n);
+// launch_gravity (counters cleared) and launch_aabb in one launch
+void launch_gravity_aabb(sv_ctx *c, uint32_t n, float dt, const float g[3]);
 void launch_broadphase(sv_ctx *c, uint32_t n);
 // after_broadphase: the same substep's broadphase has set the pair count;
 // clear_lambda: also clear the contact impulses (the next step's work);
@@ -312,7 +314,9 @@ void launch_handoff(sv_ctx *c, uint32_t *flag, uint32_t rounds, uint32_t partner
 void launch_pack_boxes(sv_ctx *c, uint32_t n);
 
 // ---- sv_pgs.hip ----
-void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt);
+// integrate = true lets the sweep's write-back launch also integrate the
+// bodies (launch_integrate's update); returns whether it did.
+bool launch_solve(sv_ctx *c, uint32_t n_bodies, float dt, bool integrate = false);
 
 // ---- sv_warm.hip ----
 // Allocates the warm-start cache (false on failure) and empties it.

@@ -344,10 +344,14 @@ __global__ void __launch_bounds__(BLOCK) k_seq_pack(const uint32_t *__restrict__
     }
 }
 
+// With pos set, each body is also integrated from the velocities just
+// written (k_integrate's update, one launch fewer in the full step).
 __global__ void __launch_bounds__(BLOCK) k_seq_unpack(const uint32_t *__restrict__ misc, uint32_t n_bodies,
                                                       const float4 *__restrict__ sv, const uint2 *__restrict__ sc,
                                                       float *__restrict__ vel, float *__restrict__ omega,
-                                                      float *__restrict__ lambda) {
+                                                      float *__restrict__ lambda, float dt,
+                                                      const float *__restrict__ inv_mass, float *__restrict__ pos,
+                                                      float *__restrict__ quat) {
     const uint32_t nc = misc[MISC_NC];
     const uint32_t m = n_bodies > nc ? n_bodies : nc;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
@@ -359,6 +363,7 @@ __global__ void __launch_bounds__(BLOCK) k_seq_unpack(const uint32_t *__restrict
             omega[3 * i] = w.x;
             omega[3 * i + 1] = w.y;
             omega[3 * i + 2] = w.z;
+            if (pos) integrate_body(i, dt, inv_mass, pos, quat, vel, omega);
         }
         if (i < nc) lambda[i] = __uint_as_float(sc[i].x);
     }
@@ -1279,12 +1284,12 @@ uint32_t seq_streams(uint32_t wgs) {
 // Step 5 (physics.comp:500-583) for the contacts counters[1] names.
 // Reads lambda_n as the starting impulses (step 4 clears them, or, with
 // warm start, carries them over).
-void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
+static bool launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt, bool integrate) {
     uint32_t *counters = B<uint32_t>(c, SV_ATOMIC_COUNTERS);
     uint32_t mc = c->caps.max_contacts;
     const bool setup_done = c->sweep_setup_done;  // by the full step's contact compaction
     c->sweep_setup_done = false;
-    if (mc == 0 || n_bodies == 0) return;
+    if (mc == 0 || n_bodies == 0) return false;
     if (!setup_done)
         hipLaunchKernelGGL(k_sweep_setup, dim3(div_up(n_bodies, 256)), dim3(256), 0, c->stream, counters, c->s.misc,
                            mc, c->s.deg, n_bodies, c->s.solve_ctl);
@@ -1312,7 +1317,7 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
     const bool warm = c->warm_factor > 0.0f;
     if (c->solver_colored && !warm) {
         if (!launch_solve_colored(c, n_bodies)) c->launch_err = hipErrorLaunchFailure;
-        return;
+        return false;
     }
     int key_bits = 32 - __builtin_clz(n_bodies);  // keys are in [0, n_bodies]
     int where = radix_sort(c, c->s.inc_key[0], c->s.inc_val[0], c->s.inc_key[1], c->s.inc_val[1],
@@ -1321,7 +1326,7 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
     if (warm) launch_warm_apply(c, n_bodies, c->s.inc_val[where]);
     if (c->solver_colored) {
         if (!launch_solve_colored(c, n_bodies)) c->launch_err = hipErrorLaunchFailure;
-        return;
+        return false;
     }
     uint32_t rgrid = div_up(2u * (uint64_t)mc, BLOCK);
     if (rgrid > 8192) rgrid = 8192;
@@ -1341,7 +1346,7 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
         hipLaunchKernelGGL(c->sweep_variant != SWEEP_FENCED ? k_flow<true> : k_flow<false>, dim3(wgs), dim3(MC_THREADS), 0, c->stream,
                            c->s.misc, iters, c->s.cprep, vel,
                            omega, c->s.rot, inv_i, lam, c->s.flow_count, c->s.flow_queue, c->s.solve_ctl, counters);
-        return;
+        return false;
     }
     hipLaunchKernelGGL(k_rank, dim3(rgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.inc_key[where],
                        c->s.inc_val[where], c->s.deg_off, c->s.rank);
@@ -1394,8 +1399,9 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
                                c->s.solve_ctl, counters, seq_streams(wgs));
         }
         hipLaunchKernelGGL(k_seq_unpack, dim3(pgrid), dim3(BLOCK), 0, c->stream, c->s.misc, n_bodies, c->s.seq_v,
-                           c->s.seq_c, vel, omega, lam);
-        return;
+                           c->s.seq_c, vel, omega, lam, dt, B<float>(c, SV_BODY_INV_MASS),
+                           integrate ? B<float>(c, SV_BODY_POS) : (float *)nullptr, B<float>(c, SV_BODY_QUAT));
+        return integrate;
     }
     StageTimer sweep_timer(c, "phys/pgs_sweep");
     if (wgs > 1) {
@@ -1420,11 +1426,14 @@ void launch_sweep(sv_ctx *c, uint32_t n_bodies, float dt) {
                            c->s.misc, iters, n_bodies, c->s.cprep, c->s.rank, c->s.deg, c->s.ticket, vel, omega,
                            c->s.rot, inv_i, lam, c->s.solve_ctl, counters);
     }
+    return false;
 }
 
-void launch_solve(sv_ctx *c, uint32_t n_bodies, float dt) {
-    launch_sweep(c, n_bodies, dt);
+bool launch_solve(sv_ctx *c, uint32_t n_bodies, float dt, bool integrate) {
+    const bool done = launch_sweep(c, n_bodies, dt, integrate);
+    // (the save reads impulses and contact ids only: after the integration is fine)
     if (c->warm_factor > 0.0f) launch_warm_save(c);
+    return done;
 }
 
 }  // namespace sv

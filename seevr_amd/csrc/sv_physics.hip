@@ -53,14 +53,26 @@ __global__ void k_gravity(uint32_t n, float dt, float gx, float gy, float gz, fl
 // loads in flight per lane, then a row reduction.
 // Also caches the body's rotation columns and a packed broadphase box
 // (min.xyz, static flag), (max.xyz, 0) for the later stages.
+// With ga.vel set, the launch also applies k_gravity's update to the body
+// (the full step's substeps, one launch fewer): the box depends on position
+// and orientation only, so the two commute.
+struct GravityArgs {
+    float dt, gx, gy, gz, ld, ad;
+    float *vel, *omega;
+    uint32_t *counters;  // pair and contact counts cleared when non-null
+};
 __global__ void __launch_bounds__(BLOCK) k_aabb(uint32_t n, const float *__restrict__ pos,
                                                 const float *__restrict__ quat, const uint32_t *__restrict__ hrange,
                                                 const float *__restrict__ hull, const float *__restrict__ inv_mass,
                                                 float *__restrict__ amin_out, float *__restrict__ amax_out,
-                                                float *__restrict__ rot, float4 *__restrict__ bpbox) {
+                                                float *__restrict__ rot, float4 *__restrict__ bpbox, GravityArgs ga) {
     // a DPP row per body: rows only read lanes of their own row
     const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) / AABB_ROW;
     const int g = threadIdx.x & (AABB_ROW - 1);
+    if (ga.counters && blockIdx.x == 0 && threadIdx.x == 0) {
+        ga.counters[0] = 0;
+        ga.counters[1] = 0;
+    }
     if (b >= n) return;
     m3 R;
     f3 mn, mx;
@@ -72,6 +84,7 @@ __global__ void __launch_bounds__(BLOCK) k_aabb(uint32_t n, const float *__restr
         float st = inv_mass[b] <= 0.0f ? 1.0f : 0.0f;
         bpbox[2 * b] = make_float4(mn.x, mn.y, mn.z, st);
         bpbox[2 * b + 1] = make_float4(mx.x, mx.y, mx.z, 0.0f);
+        if (ga.vel) gravity_body(b, ga.dt, ga.gx, ga.gy, ga.gz, ga.ld, ga.ad, inv_mass, ga.vel, ga.omega);
     }
 }
 
@@ -536,12 +549,26 @@ void launch_gravity(sv_ctx *c, uint32_t n, float dt, const float g[3], bool clea
                        clear_counters ? B<uint32_t>(c, SV_ATOMIC_COUNTERS) : (uint32_t *)nullptr);
 }
 
-void launch_aabb(sv_ctx *c, uint32_t n) {
-    if (!n) return;
+static void launch_aabb_with(sv_ctx *c, uint32_t n, GravityArgs ga) {
     hipLaunchKernelGGL(k_aabb, dim3(div_up((uint64_t)n * AABB_ROW, BLOCK)), dim3(BLOCK), 0, c->stream, n,
                        B<float>(c, SV_BODY_POS), B<float>(c, SV_BODY_QUAT), c->s.hrange, B<float>(c, SV_HULL_VERTS),
                        B<float>(c, SV_BODY_INV_MASS), B<float>(c, SV_BODY_AABB_MIN), B<float>(c, SV_BODY_AABB_MAX),
-                       c->s.rot, c->s.bpbox);
+                       c->s.rot, c->s.bpbox, ga);
+}
+
+void launch_aabb(sv_ctx *c, uint32_t n) {
+    if (!n) return;
+    launch_aabb_with(c, n, GravityArgs{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, nullptr, nullptr, nullptr});
+}
+
+void launch_gravity_aabb(sv_ctx *c, uint32_t n, float dt, const float g[3]) {
+    if (!n) {
+        launch_clear_counters(c);
+        return;
+    }
+    launch_aabb_with(c, n, GravityArgs{dt, g[0], g[1], g[2], c->params.linear_damping, c->params.angular_damping,
+                                       B<float>(c, SV_BODY_VEL), B<float>(c, SV_BODY_OMEGA),
+                                       B<uint32_t>(c, SV_ATOMIC_COUNTERS)});
 }
 
 int broadphase_method(const sv_ctx *c, uint32_t n) {
