@@ -389,12 +389,13 @@ int sv_physics_step(sv_ctx *c, uint32_t nb, uint32_t nv, uint32_t nh, uint32_t s
     sv::launch_hull_ranges(c, nb);
     sv::ensure_support_vertices(c, nb);
     sv::fill_u32(c, counters + 2, 0u, 1);
-    // with stage timing off, gravity rides on the AABB launch and the
-    // integration on the sweep's write-back (same arithmetic, fewer launches);
-    // with it on, every stage keeps its own launch and timer
-    const bool fuse = !c->timing;
+    // unless those stages are being timed, gravity rides on the AABB launch
+    // and the integration on the sweep's write-back (same arithmetic, fewer
+    // launches); timed, every stage keeps its own launch and timer
+    auto timed = [c](const char *stage) { return c->timing && (c->timing_only.empty() || c->timing_only == stage); };
+    const bool fuse_gravity = !timed("phys/gravity") && !timed("phys/aabb"), fuse_integrate = !timed("phys/integrate");
     for (uint32_t s = 0; s < substeps; s++) {
-        if (fuse) {
+        if (fuse_gravity) {
             sv::launch_gravity_aabb(c, nb, sub_dt, gravity);
         } else {
             { sv::StageTimer t(c, "phys/gravity"); sv::launch_gravity(c, nb, sub_dt, gravity, true); }
@@ -407,7 +408,7 @@ int sv_physics_step(sv_ctx *c, uint32_t nb, uint32_t nv, uint32_t nh, uint32_t s
         if (warm) { sv::StageTimer t(c, "phys/clear_impulse"); sv::launch_clear_lambda(c); }
         c->rot_fresh = true;
         bool integrated;
-        { sv::StageTimer t(c, "phys/pgs_solve"); integrated = sv::launch_solve(c, nb, sub_dt, fuse); }
+        { sv::StageTimer t(c, "phys/pgs_solve"); integrated = sv::launch_solve(c, nb, sub_dt, fuse_integrate); }
         c->rot_fresh = false;
         if (!integrated) { sv::StageTimer t(c, "phys/integrate"); sv::launch_integrate(c, nb, sub_dt); }
     }
