@@ -763,11 +763,14 @@ SVD void cmpx(uint32_t &a, uint32_t &b) {
     b = hi;
 }
 
+__device__ void sap_map_next(float *map);
+
 __global__ void __launch_bounds__(BLOCK) k_rows_short(uint32_t n, const uint32_t *__restrict__ cnt,
                                                       const uint32_t *__restrict__ off,
                                                       const uint32_t *__restrict__ pk, uint32_t *__restrict__ pairs,
-                                                      uint32_t max_pairs, uint32_t *list) {
+                                                      uint32_t max_pairs, uint32_t *list, float *map) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (map && i == 0) sap_map_next(map);  // the sweep's map for its next launch
     if (i >= n) return;
     const uint32_t c = cnt[i];
     if (c == 0) return;
@@ -1059,8 +1062,9 @@ SVD uint32_t sap_key(float x, SapMap m) {
 }
 
 // The map for the next launch from this launch's lower-bound extremes
-// (order-preserving images, reset for the next launch).  One thread.
-__global__ void k_sap_map(float *map) {
+// (order-preserving images, reset for the next launch).  One thread, after
+// the launch's last use of its map (the row sort's first kernel).
+__device__ void sap_map_next(float *map) {
     uint32_t *ext = reinterpret_cast<uint32_t *>(map + 2);
     const uint32_t lo = ext[0], hi = ext[1];
     float x0 = 0.0f, scale = 0.0f;
@@ -1211,12 +1215,26 @@ SVD void sap_take(uint64_t m, uint32_t rb, uint32_t qb, uint32_t lane, uint64_t 
 // candidates) one lane mask, as in the all-pairs kernel; a row whose run
 // ended before a chunk skips it.
 template <bool EMIT>
+__device__ __forceinline__ void sap_long_items(uint32_t n, const float4 *__restrict__ sbox,
+                                               const uint32_t *__restrict__ skey, const SapOut &o,
+                                               const uint32_t *__restrict__ cont, uint32_t block, uint32_t nblocks);
+
+// The emit pass also sweeps the continuation items, in its workgroups from
+// row_blocks on (emit mode lists no items, so the two parts are independent).
+template <bool EMIT>
 __global__ void __launch_bounds__(BLOCK) k_sap(uint32_t n, const float4 *__restrict__ sbox,
-                                               const uint32_t *__restrict__ skey, SapOut o, uint32_t *cont) {
+                                               const uint32_t *__restrict__ skey, SapOut o, uint32_t *cont,
+                                               uint32_t row_blocks) {
     __shared__ float4 s_lo[BLOCK], s_hi[BLOCK];
     __shared__ uint32_t s_key[BLOCK], s_kmax[BLOCK / 64];
     __shared__ uint2 s_stage[BLOCK / 64][SAP_STAGE];
-    if (EMIT && *o.flat_tail <= o.flat_cap) return;  // the flat list holds every pair
+    if constexpr (EMIT) {
+        if (*o.flat_tail <= o.flat_cap) return;  // the flat list holds every pair
+        if (blockIdx.x >= row_blocks) {
+            sap_long_items<true>(n, sbox, skey, o, cont, blockIdx.x - row_blocks, gridDim.x - row_blocks);
+            return;
+        }
+    }
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t P0 = blockIdx.x * SAP_ROWS * (BLOCK / 64);
     if (P0 >= n) return;
@@ -1333,14 +1351,15 @@ __global__ void __launch_bounds__(BLOCK) k_sap_place(const uint2 *__restrict__ f
 
 // The listed runs, one wave per item of SAP_ITEM candidates: item g belongs
 // to the last entry whose first item is <= g (binary search).
+// Workgroup `block` of `nblocks` sweeping continuation items (k_sap_long,
+// and the emit pass's workgroups past its row blocks).
 template <bool EMIT>
-__global__ void __launch_bounds__(BLOCK) k_sap_long(uint32_t n, const float4 *__restrict__ sbox,
-                                                    const uint32_t *__restrict__ skey, SapOut o,
-                                                    const uint32_t *__restrict__ cont) {
+__device__ __forceinline__ void sap_long_items(uint32_t n, const float4 *__restrict__ sbox,
+                                               const uint32_t *__restrict__ skey, const SapOut &o,
+                                               const uint32_t *__restrict__ cont, uint32_t block, uint32_t nblocks) {
     __shared__ uint2 s_stage[BLOCK / 64][SAP_STAGE];
-    if (EMIT && *o.flat_tail <= o.flat_cap) return;  // the flat list holds every pair
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t gw = blockIdx.x * (BLOCK / 64) + wave, nw = gridDim.x * (BLOCK / 64);
+    const uint32_t gw = block * (BLOCK / 64) + wave, nw = nblocks * (BLOCK / 64);
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     uint2 *stage = s_stage[wave];
     uint32_t staged = 0;
@@ -1378,6 +1397,12 @@ __global__ void __launch_bounds__(BLOCK) k_sap_long(uint32_t n, const float4 *__
     if (!EMIT && staged) sap_flush(stage, staged, lane, o);
 }
 
+__global__ void __launch_bounds__(BLOCK) k_sap_long(uint32_t n, const float4 *__restrict__ sbox,
+                                                    const uint32_t *__restrict__ skey, SapOut o,
+                                                    const uint32_t *__restrict__ cont) {
+    sap_long_items<false>(n, sbox, skey, o, cont, blockIdx.x, gridDim.x);
+}
+
 }  // namespace
 
 namespace {
@@ -1385,7 +1410,8 @@ namespace {
 // Rows to pairs: k_rows_short over every row, then k_rows_long over the
 // rows it listed (list[0] must be zero before).
 void launch_sort_rows(hipStream_t st, uint32_t n, const uint32_t *cnt, const uint32_t *off, const uint32_t *pk,
-                      uint32_t *pairs, uint32_t max_pairs, uint32_t *list, uint32_t window_words = 0) {
+                      uint32_t *pairs, uint32_t max_pairs, uint32_t *list, uint32_t window_words = 0,
+                      float *map = nullptr) {
     uint32_t ww = div_up(div_up(n, 32u), (uint32_t)BLOCK) * BLOCK;  // window words, a multiple of BLOCK
     if (ww > ROW_WINDOW_WORDS) ww = ROW_WINDOW_WORDS;
     if (window_words) ww = window_words;
@@ -1397,7 +1423,7 @@ void launch_sort_rows(hipStream_t st, uint32_t n, const uint32_t *cnt, const uin
         lds_set = true;
     }
     hipLaunchKernelGGL(k_rows_short, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, n, cnt, off, pk, pairs, max_pairs,
-                       list);
+                       list, map);
     hipLaunchKernelGGL(k_rows_long, dim3(256), dim3(BLOCK), lds_bytes, st, n, cnt, off, pk, pairs, max_pairs,
                        (const uint32_t *)list, ww);
 }
@@ -1500,17 +1526,15 @@ bool launch_broadphase_sap(sv_ctx *c, uint32_t n) {
     // count the rows and list the pairs, scan the rows (with the pair count
     // and the overflow flag), place the listed pairs in their rows (or sweep
     // again to place them when the list was too short), sort the rows
-    hipLaunchKernelGGL(k_sap<false>, dim3(gs), dim3(BLOCK), 0, st, n, s.sap_box, skey, out, s.sap_cont);
-    hipLaunchKernelGGL(k_sap_long<false>, dim3(gl), dim3(BLOCK), 0, st, n, s.sap_box, skey, out,
+    hipLaunchKernelGGL(k_sap<false>, dim3(gs), dim3(BLOCK), 0, st, n, s.sap_box, skey, out, s.sap_cont, gs);
+    hipLaunchKernelGGL(k_sap_long, dim3(gl), dim3(BLOCK), 0, st, n, s.sap_box, skey, out,
                        (const uint32_t *)s.sap_cont);
     scan_exclusive(c, s.sap_cnt, s.sap_off, nullptr, n, counters, mp, s.misc + MISC_NP, counters + 2);
     hipLaunchKernelGGL(k_sap_place, dim3(256), dim3(BLOCK), 0, st, (const uint2 *)s.sap_flat,
                        (const uint32_t *)s.sap_tail, flat_cap, s.sap_off, s.sap_fill, s.sap_part, mp);
-    hipLaunchKernelGGL(k_sap<true>, dim3(gs), dim3(BLOCK), 0, st, n, s.sap_box, skey, out, s.sap_cont);
-    hipLaunchKernelGGL(k_sap_long<true>, dim3(gl), dim3(BLOCK), 0, st, n, s.sap_box, skey, out,
-                       (const uint32_t *)s.sap_cont);
-    hipLaunchKernelGGL(k_sap_map, dim3(1), dim3(1), 0, st, s.sap_map);  // after the last use of this launch's map
-    launch_sort_rows(st, n, s.sap_cnt, s.sap_off, s.sap_part, pairs, mp, s.sap_rows);
+    hipLaunchKernelGGL(k_sap<true>, dim3(gs + gl), dim3(BLOCK), 0, st, n, s.sap_box, skey, out, s.sap_cont, gs);
+    // (the row sort's first kernel also sets the map for the next launch)
+    launch_sort_rows(st, n, s.sap_cnt, s.sap_off, s.sap_part, pairs, mp, s.sap_rows, 0, s.sap_map);
     return true;
 }
 
