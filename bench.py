@@ -250,7 +250,9 @@ def batched_section(frames: int, warmup: int, scenes: int = 4096, plain_sample: 
     for i in range(warmup):
         b.step(nv, SUBSTEPS, DT)
     threads, wgs, us = b.autotune(SUBSTEPS, DT)
-    log(f"batched autotune: {threads} threads x {wgs} workgroups/CU, {us:.0f} us/substep")
+    pair_tile = b.pair_tile
+    log(f"batched autotune: {threads} threads x {wgs} workgroups/CU, broadphase pair tile {pair_tile} "
+        f"lanes per row, {us:.0f} us/substep")
     b.step(nv, SUBSTEPS, DT)
     w0, t0 = time.monotonic_ns(), time.perf_counter()
     for i in range(frames):
@@ -313,7 +315,8 @@ def batched_section(frames: int, warmup: int, scenes: int = 4096, plain_sample: 
             "scenes": scenes, "bodies": cnt["num_bodies"],
             "body_substeps_per_s": round(cnt["num_bodies"] * SUBSTEPS * frames / el, 1),
             "contacts_last_substep": {"mean": round(float(c[:, 1].mean()), 2), "max": int(c[:, 1].max())},
-            "launch_shape": {"threads": threads, "workgroups_per_cu": wgs, "autotune_us_per_substep": round(us, 1)},
+            "launch_shape": {"threads": threads, "workgroups_per_cu": wgs, "pair_tile_lanes_per_row": pair_tile,
+                             "autotune_us_per_substep": round(us, 1)},
             "plain_one_by_one": {"env_steps_per_s": round(plain_env, 1), "sample_scenes": len(ks),
                                  "sample_mean_bodies": round(bodies / len(ks), 1)},
             "speedup_vs_plain": round(env_steps / plain_env, 1), "phases": phase}

@@ -502,8 +502,17 @@ int sv_batch_read_counters(sv_batch *batch, uint32_t *out);
 /* Launch shape: threads per workgroup (64, 128, 256 or 512), workgroups per
  * CU of the persistent grid, and whether frames replay a hipGraph. */
 int sv_batch_set_config(sv_batch *batch, uint32_t threads, uint32_t workgroups_per_cu, int use_graph);
+/* Broadphase pair tile of the batched substep (a library extension; the
+ * reference has no batched engine): lanes_per_row = 1 gives each lane a row
+ * of the scene's all-pairs test, 2..64 (a power of two) give a group of that
+ * many lanes one row, testing that many candidates a step.  The pair list
+ * is the same for every tile.  Default 1. */
+int sv_batch_set_pair_tile(sv_batch *batch, uint32_t lanes_per_row);
+int sv_batch_pair_tile(sv_batch *batch, uint32_t *lanes_per_row);
 /* Times every launch shape on the current scenes (state is restored) and
- * keeps the fastest; returns it and its time per substep. */
+ * keeps the fastest; then times the pair tiles 1, 4, 16 and 64 at that
+ * shape and keeps the fastest (read it with sv_batch_pair_tile); returns
+ * the shape and its time per substep. */
 int sv_batch_autotune(sv_batch *batch, uint32_t substeps, float dt, uint32_t *threads,
                       uint32_t *workgroups_per_cu, double *us_per_substep);
 /* Phase timing of the batched substep kernel (a library extension, for

@@ -83,6 +83,7 @@ EXPORTS = [
     "sv_scene_build_workload", "sv_scene_build_batch", "sv_scene_serialize", "sv_scene_deserialize",
     "sv_scene_save", "sv_scene_load", "sv_scene_read_state", "sv_batch_create", "sv_batch_destroy", "sv_batch_context",
     "sv_batch_set_scenes", "sv_batch_step", "sv_batch_read_counters", "sv_batch_set_config", "sv_batch_autotune",
+    "sv_batch_set_pair_tile", "sv_batch_pair_tile",
     "sv_batch_set_phase_timing", "sv_batch_phase_times",
     "sv_batch_diff_reserve", "sv_batch_diff_forward", "sv_batch_diff_backward",
 ]
@@ -173,6 +174,8 @@ def load(path: Path | str | None = None) -> C.CDLL:
         "sv_batch_read_counters": (C.c_int, [vp, C.POINTER(u32)]),
         "sv_batch_set_config": (C.c_int, [vp, u32, u32, C.c_int]),
         "sv_batch_autotune": (C.c_int, [vp, u32, f32, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_double)]),
+        "sv_batch_set_pair_tile": (C.c_int, [vp, u32]),
+        "sv_batch_pair_tile": (C.c_int, [vp, C.POINTER(u32)]),
         "sv_batch_set_phase_timing": (C.c_int, [vp, C.c_int]),
         "sv_batch_phase_times": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
         "sv_batch_diff_reserve": (C.c_int, [vp, u32, u32]),

@@ -459,6 +459,16 @@ class Batch:
     def set_config(self, threads: int = 256, workgroups_per_cu: int = 2, use_graph: bool = True) -> None:
         check(self.lib.sv_batch_set_config(self.h, threads, workgroups_per_cu, int(use_graph)))
 
+    def set_pair_tile(self, lanes_per_row: int = 1) -> None:
+        """Broadphase lanes per row (1, 2, 4, ..., 64); the pairs are the same for every tile."""
+        check(self.lib.sv_batch_set_pair_tile(self.h, lanes_per_row))
+
+    @property
+    def pair_tile(self) -> int:
+        g = C.c_uint32()
+        check(self.lib.sv_batch_pair_tile(self.h, C.byref(g)))
+        return g.value
+
     def autotune(self, substeps: int = 10, dt: float = 1.0 / 60.0) -> tuple[int, int, float]:
         t, w, us = C.c_uint32(), C.c_uint32(), C.c_double()
         check(self.lib.sv_batch_autotune(self.h, substeps, C.c_float(dt), C.byref(t), C.byref(w), C.byref(us)))

@@ -32,6 +32,7 @@ struct BatchArgs {
     float *amin, *amax, *rot;
     uint32_t *pairs;
     uint32_t ps;  // pair slots per scene
+    uint32_t pair_cols;  // broadphase pair tile: lanes per row (1: a lane per row; 4, 16, 64: a lane group)
     float *cpos, *cnrm, *cpen, *lambda;
     uint32_t *cga, *cgb;
     uint32_t cs;  // contact slots per scene
@@ -140,6 +141,7 @@ struct sv_batch {
     bool phase_timing = false;
     int cus = 0;
     uint32_t threads = 256, wgs_per_cu = 2;
+    uint32_t pair_cols = 1;  // broadphase lanes per row (sv_batch_set_pair_tile)
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;

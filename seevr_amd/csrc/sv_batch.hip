@@ -112,16 +112,40 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
         stamp(1);
 
         // broadphase: hits per row i (j > i, not both static), scanned into
-        // lexicographic slots, then placed
-        for (uint32_t i = tid; i < ns; i += T) {
-            const float4 lo = s_lo[i], hi = s_hi[i];
-            const bool si = lo.w != 0.0f;
-            uint32_t c = 0;
-            for (uint32_t j = i + 1; j < ns; j++) {
-                const float4 l2 = s_lo[j], h2 = s_hi[j];
-                if (!(si && l2.w != 0.0f) && !boxes_apart(lo, hi, l2, h2)) c++;
+        // lexicographic slots, then placed.  Pair tile: a lane per row, or a
+        // group of G lanes (G | 64) per row taking G candidates a step, its
+        // ballot's prefix counts keeping the row's order.
+        const uint32_t G = A.pair_cols;
+        const uint32_t lane = tid & 63u, gl = lane % G;
+        const uint64_t gmask = G >= 64 ? ~0ull : ((1ull << G) - 1ull) << (lane - gl);
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        if (G <= 1) {
+            for (uint32_t i = tid; i < ns; i += T) {
+                const float4 lo = s_lo[i], hi = s_hi[i];
+                const bool si = lo.w != 0.0f;
+                uint32_t c = 0;
+                for (uint32_t j = i + 1; j < ns; j++) {
+                    const float4 l2 = s_lo[j], h2 = s_hi[j];
+                    if (!(si && l2.w != 0.0f) && !boxes_apart(lo, hi, l2, h2)) c++;
+                }
+                s_row[i] = c;
             }
-            s_row[i] = c;
+        } else {
+            for (uint32_t i = tid / G; i < ns; i += T / G) {
+                const float4 lo = s_lo[i], hi = s_hi[i];
+                const bool si = lo.w != 0.0f;
+                uint32_t c = 0;
+                for (uint32_t j0 = i + 1; j0 < ns; j0 += G) {
+                    const uint32_t j = j0 + gl;
+                    bool hit = false;
+                    if (j < ns) {
+                        const float4 l2 = s_lo[j], h2 = s_hi[j];
+                        hit = !(si && l2.w != 0.0f) && !boxes_apart(lo, hi, l2, h2);
+                    }
+                    c += (uint32_t)__popcll(__ballot(hit) & gmask);
+                }
+                if (gl == 0) s_row[i] = c;
+            }
         }
         __syncthreads();
         uint32_t npt;
@@ -143,17 +167,41 @@ __global__ void __launch_bounds__(T) k_batch_substep(BatchArgs A) {
             if (npt > A.ps) cnt[2] |= SV_STATUS_PAIR_OVERFLOW;
         }
         uint32_t *pairs = A.pairs + 2 * (size_t)sc * A.ps;
-        for (uint32_t i = tid; i < ns; i += T) {
-            uint32_t o = s_row[i];
-            if (o >= A.ps) continue;
-            const float4 lo = s_lo[i], hi = s_hi[i];
-            const bool si = lo.w != 0.0f;
-            for (uint32_t j = i + 1; j < ns && o < A.ps; j++) {
-                const float4 l2 = s_lo[j], h2 = s_hi[j];
-                if (!(si && l2.w != 0.0f) && !boxes_apart(lo, hi, l2, h2)) {
-                    pairs[2 * o] = bs + i;
-                    pairs[2 * o + 1] = bs + j;
-                    o++;
+        if (G <= 1) {
+            for (uint32_t i = tid; i < ns; i += T) {
+                uint32_t o = s_row[i];
+                if (o >= A.ps) continue;
+                const float4 lo = s_lo[i], hi = s_hi[i];
+                const bool si = lo.w != 0.0f;
+                for (uint32_t j = i + 1; j < ns && o < A.ps; j++) {
+                    const float4 l2 = s_lo[j], h2 = s_hi[j];
+                    if (!(si && l2.w != 0.0f) && !boxes_apart(lo, hi, l2, h2)) {
+                        pairs[2 * o] = bs + i;
+                        pairs[2 * o + 1] = bs + j;
+                        o++;
+                    }
+                }
+            }
+        } else {
+            for (uint32_t i = tid / G; i < ns; i += T / G) {
+                uint32_t o = s_row[i];
+                if (o >= A.ps) continue;
+                const float4 lo = s_lo[i], hi = s_hi[i];
+                const bool si = lo.w != 0.0f;
+                for (uint32_t j0 = i + 1; j0 < ns && o < A.ps; j0 += G) {
+                    const uint32_t j = j0 + gl;
+                    bool hit = false;
+                    if (j < ns) {
+                        const float4 l2 = s_lo[j], h2 = s_hi[j];
+                        hit = !(si && l2.w != 0.0f) && !boxes_apart(lo, hi, l2, h2);
+                    }
+                    const uint64_t m = __ballot(hit) & gmask;
+                    const uint32_t q = o + (uint32_t)__popcll(m & below);
+                    if (hit && q < A.ps) {
+                        pairs[2 * q] = bs + i;
+                        pairs[2 * q + 1] = bs + j;
+                    }
+                    o += (uint32_t)__popcll(m);
                 }
             }
         }
@@ -425,6 +473,7 @@ BatchArgs batch_args(sv_batch *b, float dt, uint32_t *queue) {
     A.t_state = nullptr;
     A.n_all = b->num_bodies;
     A.phase_ticks = b->phase_timing ? b->phase_ticks : nullptr;
+    A.pair_cols = b->pair_cols;
     return A;
 }
 
@@ -681,6 +730,21 @@ int sv_batch_read_counters(sv_batch *b, uint32_t *out) {
     return e == hipSuccess ? SV_OK : dev_fail(e, "sv_batch_read_counters");
 }
 
+int sv_batch_set_pair_tile(sv_batch *b, uint32_t lanes_per_row) {
+    if (!b) return set_error(SV_E_INVALID, "sv_batch_set_pair_tile: null batch");
+    if (lanes_per_row < 1 || lanes_per_row > 64 || (lanes_per_row & (lanes_per_row - 1)))
+        return set_error(SV_E_INVALID, "sv_batch_set_pair_tile: lanes per row must be 1, 2, 4, ..., 64");
+    if (b->pair_cols != lanes_per_row) drop_graph(b);  // the captured launches carry it
+    b->pair_cols = lanes_per_row;
+    return SV_OK;
+}
+
+int sv_batch_pair_tile(sv_batch *b, uint32_t *lanes_per_row) {
+    if (!b || !lanes_per_row) return set_error(SV_E_INVALID, "sv_batch_pair_tile: null argument");
+    *lanes_per_row = b->pair_cols;
+    return SV_OK;
+}
+
 int sv_batch_autotune(sv_batch *b, uint32_t substeps, float dt, uint32_t *threads_out, uint32_t *wgs_out,
                       double *us_out) {
     if (!b) return set_error(SV_E_INVALID, "sv_batch_autotune: null batch");
@@ -715,27 +779,36 @@ int sv_batch_autotune(sv_batch *b, uint32_t substeps, float dt, uint32_t *thread
     double best = 1e30;
     uint32_t bt = b->threads, bw = b->wgs_per_cu;
     const uint32_t cand_t[] = {64, 128, 256, 512};
+    const uint32_t cand_g[] = {1, 4, 16, 64};  // broadphase pair tiles: lanes per row
+    const uint32_t g_user = b->pair_cols;
+    uint32_t bg = g_user;
     int rc = SV_OK;
+    // one trial: `substeps` substeps of shape (t, w) with pair tile g, timed, state restored
+    auto trial = [&](uint32_t t, uint32_t w, uint32_t g, double *us) -> int {
+        b->pair_cols = g;
+        sv::enqueue_substeps(b, 1, sub_dt, t, w);  // warm the shape once
+        (void)hipEventRecord(e0, c->stream);
+        sv::enqueue_substeps(b, substeps, sub_dt, t, w);
+        (void)hipEventRecord(e1, c->stream);
+        e = copy_all(false);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return dev_fail(e, "sv_batch_autotune: trial");
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *us = 1000.0 * ms / substeps;
+        return (int)SV_OK;
+    };
+    // the launch shape first (threads x workgroups per CU, the pair tile as
+    // set), then the pair tile at the fastest shape
     for (uint32_t t : cand_t) {
         int occ = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(sv::batch_kernel(t, false)),
                                                           (int)t, 0) != hipSuccess || occ < 1)
             continue;
-        for (uint32_t w = 1; w <= (uint32_t)occ && w <= 8; w++) {
-            sv::enqueue_substeps(b, 1, sub_dt, t, w);  // warm the shape once
-            (void)hipEventRecord(e0, c->stream);
-            sv::enqueue_substeps(b, substeps, sub_dt, t, w);
-            (void)hipEventRecord(e1, c->stream);
-            e = copy_all(false);
-            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-            if (e != hipSuccess) {
-                rc = dev_fail(e, "sv_batch_autotune: trial");
-                break;
-            }
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            const double us = 1000.0 * ms / substeps;
-            if (us < best) {
+        for (uint32_t w = 1; w <= (uint32_t)occ && w <= 8 && !rc; w++) {
+            double us = 0.0;
+            rc = trial(t, w, g_user, &us);
+            if (!rc && us < best) {
                 best = us;
                 bt = t;
                 bw = w;
@@ -743,6 +816,16 @@ int sv_batch_autotune(sv_batch *b, uint32_t substeps, float dt, uint32_t *thread
         }
         if (rc) break;
     }
+    for (uint32_t g : cand_g) {
+        if (rc || g == g_user) continue;
+        double us = 0.0;
+        rc = trial(bt, bw, g, &us);
+        if (!rc && us < best) {
+            best = us;
+            bg = g;
+        }
+    }
+    b->pair_cols = rc ? g_user : bg;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;

@@ -98,6 +98,7 @@ def test_autotune_restores_state_and_keeps_results():
     before = {k: b.ctx.download(k, w * n_all) for k, w in STATE}
     t, w, us = b.autotune(10, 1 / 60)
     assert t in (64, 128, 256, 512) and w >= 1 and us > 0
+    assert b.pair_tile in (1, 4, 16, 64)
     after = {k: b.ctx.download(k, wd * n_all) for k, wd in STATE}
     for k in before:
         assert_bitwise(after[k], before[k], f"{k} restored after autotune")
@@ -110,6 +111,30 @@ def test_autotune_restores_state_and_keeps_results():
         compare_scene(b, start, k, ctx, n, f)
     ctx.close()
     b.close()
+
+
+@pytest.mark.parametrize("lanes", [2, 4, 16, 64])
+def test_pair_tiles_give_the_plain_path(lanes):
+    """Every broadphase pair tile (a group of `lanes` lanes per row) gives a
+    scene the plain path's pairs, contacts and state bit for bit, also when
+    its pair slots overflow (the lexicographic prefix is kept)."""
+    for ps, cs, seed in ((PAIRS, CONTACTS, 3), (40, 15, 9)):
+        sc, b, start, grav, nv = build_batch(8, seed=seed, ps=ps, cs=cs)
+        b.set_pair_tile(lanes)
+        assert b.pair_tile == lanes
+        k = int(np.argmax(np.diff(start)))
+        ctx, n, pnv, nh = plain_scene(k, grav, seed=seed, ps=ps, cs=cs)
+        for f in range(4):
+            b.step(nv, 10, 1 / 60)
+            ctx.step(n, pnv, nh, 10, 1 / 60, tuple(float(x) for x in grav[k]))
+            compare_scene(b, start, k, ctx, n, f, ps=ps, cs=cs)
+        if ps == 40:
+            assert int(b.counters()[k, 2]) & 1, "pair overflow bit"
+        for bad in (0, 3, 128):
+            with pytest.raises(Exception):
+                b.set_pair_tile(bad)
+        ctx.close()
+        b.close()
 
 
 def test_batch_overflow_truncates_like_plain_path():
