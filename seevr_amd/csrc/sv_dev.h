@@ -95,7 +95,7 @@ SVD void body_aabb_wave(uint32_t b, int lane, const float *__restrict__ pos, con
 // right shifts of 1, 2, 4, 8 lanes, so lane 15 of the row ends with the
 // box.  min and max select one of their operands, so any order of
 // reduction gives the serial loop's box.
-constexpr int AABB_ROW = 16, AABB_UNROLL = 6;
+constexpr int AABB_ROW = 16, AABB_UNROLL = 10;
 
 template <int CTRL>
 SVD float dpp_keep(float v) {
@@ -110,12 +110,18 @@ SVD void body_aabb_row(uint32_t b, int g, const float *__restrict__ pos, const f
     const uint32_t vs = hrange[2 * b], vc = hrange[2 * b + 1];
     mn = mk(1e10f, 1e10f, 1e10f);
     mx = mk(-1e10f, -1e10f, -1e10f);
+    // loads past the hull's end are not issued (a small hull needs few per
+    // lane); the lane's vertices are still taken in index order
     for (uint32_t v0 = (uint32_t)g; v0 < vc; v0 += AABB_ROW * AABB_UNROLL) {
         f3 hv[AABB_UNROLL];
 #pragma unroll
-        for (int k = 0; k < AABB_UNROLL; k++) hv[k] = ld3(hull, vs + min(v0 + (uint32_t)(AABB_ROW * k), vc - 1u));
+        for (int k = 0; k < AABB_UNROLL; k++) {
+            const uint32_t v = v0 + (uint32_t)(AABB_ROW * k);
+            hv[k] = v < vc ? ld3(hull, vs + v) : mk(0.0f, 0.0f, 0.0f);
+        }
 #pragma unroll
         for (int k = 0; k < AABB_UNROLL; k++) {
+            if (v0 + (uint32_t)(AABB_ROW * k) >= vc) break;
             const f3 wv = p + mvec(R, hv[k]);
             mn = vmin3(mn, wv);
             mx = vmax3(mx, wv);
