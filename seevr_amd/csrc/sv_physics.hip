@@ -395,7 +395,7 @@ __global__ void k_integrate(uint32_t n, float dt, const float *__restrict__ inv_
 // of 16 lanes per body walks its range, with the body's rotation computed
 // once, instead of one binary search (log2 N dependent loads) per vertex.
 // Same owner, same arithmetic: the same bits.
-constexpr uint32_t VERT_ROW = 16;
+constexpr uint32_t VERT_ROW = 16, VERT_UNROLL = 8;
 __global__ void __launch_bounds__(BLOCK) k_update_verts(uint32_t nv, uint32_t nb, const uint32_t *__restrict__ vstart,
                                                         const float *__restrict__ pos, const float *__restrict__ quat,
                                                         const float *__restrict__ orig, float *__restrict__ verts) {
@@ -406,7 +406,18 @@ __global__ void __launch_bounds__(BLOCK) k_update_verts(uint32_t nv, uint32_t nb
     if (v0 >= v1) return;
     const m3 R = quat_mat_at(quat, b);
     const f3 p = ld3(pos, b);
-    for (uint32_t v = v0 + r; v < v1; v += VERT_ROW) st3(verts, v, p + mvec(R, ld3(orig, v)));
+    // VERT_UNROLL loads in flight per lane (none past the range's end)
+    for (uint32_t v = v0 + r; v < v1; v += VERT_ROW * VERT_UNROLL) {
+        f3 o[VERT_UNROLL];
+#pragma unroll
+        for (uint32_t k = 0; k < VERT_UNROLL; k++)
+            o[k] = v + VERT_ROW * k < v1 ? ld3(orig, v + VERT_ROW * k) : mk(0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (uint32_t k = 0; k < VERT_UNROLL; k++) {
+            if (v + VERT_ROW * k >= v1) break;
+            st3(verts, v + VERT_ROW * k, p + mvec(R, o[k]));
+        }
+    }
 }
 
 // Step 8 (physics.comp:630-634); counters[2] (status bits) accumulates over
